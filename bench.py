@@ -1,0 +1,234 @@
+#!/usr/bin/env python3
+"""LES steps/s at 256^3 on one MI355X: FP64 vs mixed-precision PCG to 1e-8.
+
+Workload (mixed-precision configuration of the reference's LES step):
+  256^3 unit cube, spectral velocity field (E(k) ~ k^-5/3, rms 1, seed 0),
+  5% of cells solid (Bernoulli, seed 0), p = 0, nu = 1e-3, rho = 1.2,
+  Cs = 0.17, dt = 1e-3 for steps <= 20 then the reference's adaptive rule,
+  pressure by PCG-Chebyshev degree 8 on bounds from 20 Lanczos steps,
+  stopping at ||r|| <= 1e-8 ||b||; mixed precision replaces the residual in
+  FP64 every 50 iterations.
+One step = Smagorinsky nu_t -> momentum (+ lagged divergence/RHS) -> PCG ->
+projection, all on the GPU with fields resident in HBM.
+
+`value` is the mixed-precision run's steps/s; the FP64 run on the same
+inputs is reported beside it.  Prints one JSON line on stdout; progress goes
+to stderr.
+"""
+import argparse
+import json
+import os
+import platform
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "les-model_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print("[bench]", *a, file=sys.stderr, flush=True)
+
+
+def cheb_bytes_per_cell(degree, elem):
+    """Algorithmic bytes per cell per Chebyshev inner-step launch, averaged
+    over the degree-1 launches of one application.  First step: r (stencil),
+    7 coefficients -> p, z.  Later steps: z (stencil), r, p, 7 coefficients
+    -> p, z."""
+    first = (1 + 7 + 2) * elem
+    later = (3 + 7 + 2) * elem
+    m = degree - 1
+    return (first + (m - 1) * later) / m
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor() or "unknown"
+
+
+def make_workload(n, seed=0):
+    import les_workloads as wl
+    log(f"generating {n}^3 spectral field and 5% mask")
+    u, v, w = wl.spectral_velocity(n, seed)
+    solid = wl.bernoulli_mask(n, 0.05, seed)
+    return wl.mesh(n), (u, v, w), solid
+
+
+def run_mode(g, st0, precision, bounds, steps, warmup, kt_class, pcg_kw):
+    import torch
+    import lesgpu
+    g.set_state(st0)
+    its, info = [], []
+    lmin, lmax = bounds
+    kw = dict(nu=1e-3, rho=1.2, Cs=0.17, precond=lesgpu.PC_CHEBYSHEV, lmin=lmin, lmax=lmax,
+              degree=8, max_iter=20000, tol=1e-8, tol_rel=True, precision=precision,
+              reliable_every=50, **pcg_kw)
+    for s in range(warmup):
+        r = g.step(step=s, **kw)
+        log(f"  warmup step {s}: iters {r['iters']} conv {r['converged']}")
+    g.synchronize()
+    torch.cuda.synchronize()
+    g.kernel_timing(kt_class)
+    t0 = time.perf_counter()
+    for s in range(warmup, warmup + steps):
+        r = g.step(step=s, **kw)
+        its.append(r["iters"])
+        info.append(r)
+    g.synchronize()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    kt = g.kernel_timing_read()
+    g.kernel_timing(lesgpu.KT_NONE)
+    for r in info:
+        if not r["converged"]:
+            raise RuntimeError(f"PCG did not converge: {r}")
+    return dict(seconds=t1 - t0, iters=its, last=info[-1], kt=kt, state=g.get_state())
+
+
+def cpu_baseline(m, st0, solid, bounds, threads):
+    """One LES step of the same workload by the reference solver built from
+    its own sources (oracle/_ref), else by the oracle; wall time on the
+    host cores."""
+    from cpu_checkers import Reference, Oracle, REF_SO, PC_CHEBYSHEV
+    lmin, lmax = bounds
+    kw = dict(lmin=lmin, lmax=lmax, degree=8, max_iter=20000, tol=1e-8, tol_rel=True, nu=1e-3,
+              rho=1.2)
+    if os.path.exists(REF_SO):
+        chk, kind = Reference(), "reference"
+    else:
+        chk, kind = Oracle(), "port"
+    chk.set_threads(threads)
+    A = None if kind == "reference" else chk.build_stencil(solid, m)
+    t0 = time.perf_counter()
+    if kind == "reference":
+        # reference Cs is fixed at 0.18 (main.cpp:425); the cost is the same
+        _, its, _ = chk.run_steps(st0, solid, m, None, 1, step0=0, solver=4, **kw)
+    else:
+        _, its, _ = chk.run_steps(st0, solid, m, A, 1, step0=0, precond=PC_CHEBYSHEV, Cs=0.17,
+                                  **kw)
+    t = time.perf_counter() - t0
+    return dict(value=1.0 / t, unit="steps/s", cores=threads, kind=kind,
+                sample=f"1 step of the {m['nx']}^3 workload from the same initial state, "
+                       f"PCG-Chebyshev degree 8 FP64 to 1e-8 rel ({its[0]} iterations), "
+                       f"{t:.1f} s on {threads} threads of {cpu_model()}")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--check-every", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    import lesgpu
+    assert torch.cuda.is_available(), "no GPU"
+    torch.cuda.set_device(0)
+
+    m, (u, v, w), solid = make_workload(args.n)
+    N = m["nx"] * m["ny"] * m["nz"]
+    g = lesgpu.LesGpu.from_mesh(m)
+    g.set_mask(solid)
+    st0 = {k: np.zeros(N) for k in lesgpu.FIELDS[:9]}
+    st0["u"], st0["v"], st0["w"] = u, v, w
+    g.set_state(st0)
+    g.velocity_bcs()          # main.cpp:102
+    st0 = g.get_state()
+    g.build_stencil()
+    t0 = time.perf_counter()
+    rmin, rmax = g.lanczos_bounds(20, seed=0)
+    bounds = (rmin, 1.05 * rmax)
+    log(f"Lanczos(20): ritz [{rmin:.4g}, {rmax:.4g}] -> Chebyshev [{bounds[0]:.4g}, "
+        f"{bounds[1]:.4g}] in {time.perf_counter() - t0:.2f} s")
+
+    modes = args.modes.split(",")
+    res = {}
+    pcg_kw = dict(check_every=args.check_every)
+    for mode in modes:
+        prec = lesgpu.PREC_MIXED if mode == "mixed" else lesgpu.PREC_FP64
+        log(f"mode {mode}: {args.warmup} warmup + {args.steps} timed steps")
+        res[mode] = run_mode(g, st0, prec, bounds, args.steps, args.warmup, lesgpu.KT_CHEB_STEP,
+                             pcg_kw)
+        r = res[mode]
+        log(f"  {mode}: {args.steps / r['seconds']:.3f} steps/s, iters/step "
+            f"{np.mean(r['iters']):.1f}, cheb launches {r['kt'][0]} total {r['kt'][1]:.1f} ms")
+    head = "mixed" if "mixed" in res else modes[0]
+    R = res[head]
+    steps_per_s = args.steps / R["seconds"]
+    n_launch, kt_ms, elem = R["kt"]
+    bpc = cheb_bytes_per_cell(8, elem)
+    achieved = (bpc * N) / (kt_ms / n_launch * 1e-3) / 1e9 if n_launch else 0.0
+
+    out = {
+        "metric": "LES steps/s at 256^3, FP64 vs mixed PCG to 1e-8 rel.; PCG HBM GB/s vs peak",
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * R["seconds"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "data": "synthetic (seeded spectral velocity, seeded 5% solid mask)",
+        "config": {"workload": f"LES step {m['nx']}^3, PCG-Chebyshev(8) on 20-step Lanczos "
+                               f"bounds to 1e-8 rel, {head} precision",
+                   "grid": [m["nx"], m["ny"], m["nz"]], "solid_fraction": 0.05,
+                   "precond": "chebyshev", "degree": 8, "reliable_every": 50,
+                   "lanczos_bounds": list(bounds)},
+        "pcg_iters_per_step": float(np.mean(R["iters"])),
+        "pcg_iters_per_s": float(np.sum(R["iters"]) / R["seconds"]),
+        "roofline": {"bound": "hbm", "kernel": f"k_cheb_step<{'float' if elem == 4 else 'double'}>",
+                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_cell": bpc, "cells": N, "launches": n_launch,
+                     "avg_us": 1e3 * kt_ms / n_launch if n_launch else None},
+    }
+    if "fp64" in res:
+        F = res["fp64"]
+        out["fp64_steps_per_s"] = args.steps / F["seconds"]
+        out["fp64_pcg_iters_per_step"] = float(np.mean(F["iters"]))
+        n64, ms64, e64 = F["kt"]
+        a64 = (cheb_bytes_per_cell(8, e64) * N) / (ms64 / n64 * 1e-3) / 1e9 if n64 else 0.0
+        out["fp64_roofline_frac"] = a64 / HBM_PEAK_GBS
+        if "mixed" in res:
+            out["mixed_vs_fp64_speedup"] = steps_per_s / out["fp64_steps_per_s"]
+            err = {}
+            for k in ("u", "v", "w", "p"):
+                a, b = res["mixed"]["state"][k], F["state"][k]
+                err[k] = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+            out["mixed_max_rel_err_vs_fp64"] = err
+    if not args.no_cpu_baseline:
+        log(f"CPU baseline: 1 reference step on {args.cpu_threads} threads")
+        try:
+            out["cpu_baseline"] = cpu_baseline(m, st0, solid, bounds, args.cpu_threads)
+        except Exception as e:  # reported, not hidden
+            out["cpu_baseline"] = {"error": repr(e)}
+    g.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,212 @@
+/*
+ * les_gpu.h -- C ABI of the MI355X (gfx950) LES hot-path library
+ *              (les-model_amd/liblesgpu.so).
+ *
+ * The reference (GabrieLeitao/les-model) has no FFI: its boundary is the set
+ * of free functions in src/pressure.h:31-63 and the physics functions that
+ * src/main.cpp:20-29 declares.  Each entry point below replaces one of them
+ * (cited per function) or a composition of them; the C++ drop-in layer
+ * (include/les/pressure.h, include/les/physics.h) maps the reference's
+ * signatures onto these calls.
+ *
+ * Conventions
+ *   - Grids are flat, idx = i*ny*nz + j*nz + k (src/field.h:40), FP64.
+ *   - Masks are uint8 0/1 per cell, the byte encoding the reference's
+ *     checkpoint uses for Field::is_solid (src/io.cpp:71-78).
+ *   - Stencils are 7 arrays in StencilMatrix member order c,n,s,e,w,t,b
+ *     (src/pressure.h:22).
+ *   - A context owns all device memory; the caller owns host buffers.
+ *     ptr_kind says whether a caller pointer is host (0) or device (1).
+ *   - Every function returns 0 on success or a negative LES_ERR_* code;
+ *     les_last_error() gives the message.  Non-convergence of a solver is
+ *     not an error (it is reported in the result struct, mirroring
+ *     pressure.cpp:411-413).
+ *   - Calls are ordered on the context's stream.  Calls that return host
+ *     scalars or copy to host memory synchronise that stream.
+ *   - A context is not thread-safe: one host thread per context.
+ */
+#ifndef LES_GPU_H
+#define LES_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LES_OK 0
+#define LES_ERR_ARG (-1)
+#define LES_ERR_HIP (-2)
+#define LES_ERR_STATE (-3)
+#define LES_ERR_NOMEM (-4)
+
+typedef struct les_ctx les_ctx;
+
+enum { LES_PTR_HOST = 0, LES_PTR_DEVICE = 1 };
+
+/* Field slots of a context, named after the Field members (src/field.h:13-20)
+ * plus the pressure right-hand side b (src/main.cpp:135). */
+enum {
+    LES_F_U = 0, LES_F_V, LES_F_W, LES_F_U_NEW, LES_F_V_NEW, LES_F_W_NEW,
+    LES_F_P, LES_F_P_RHS, LES_F_NU_T, LES_F_B, LES_F_COUNT
+};
+
+enum { LES_STENCIL_A = 0, LES_STENCIL_L = 1 };
+
+/* Preconditioners of the PCG solve. */
+enum {
+    LES_PC_NONE = 0,
+    LES_PC_JACOBI = 1,      /* apply_jacobi_preconditioner, pressure.cpp:420 */
+    LES_PC_IC0 = 2,         /* apply_ic_preconditioner with L from build_ic_preconditioner */
+    LES_PC_LEGACY_IC_A = 3, /* apply_ic_preconditioner(., ., A) as shipped, pressure.cpp:363,396 */
+    LES_PC_CHEBYSHEV = 4    /* apply_chebyshev_preconditioner, pressure.cpp:851 */
+};
+
+enum { LES_PREC_FP64 = 0, LES_PREC_MIXED = 1 };
+
+enum {
+    LES_SOLVER_PCG = 0,       /* calculate_divergence + build_b_vector + PCG (main.cpp:134-137) */
+    LES_SOLVER_SOR_LEX = 1,   /* solve_pressure_GS, exact lexicographic order (main.cpp:133) */
+    LES_SOLVER_SOR_RB = 2,    /* red-black SOR (no reference counterpart) */
+    LES_SOLVER_JACOBI = 3     /* solve_pressure_jacobi (main.cpp:132) */
+};
+
+/* ---- context ---- */
+int les_ctx_create(int nx, int ny, int nz, double dx, double dy, double dz, les_ctx** out);
+int les_ctx_destroy(les_ctx* ctx);
+/* Message of the last failed call on ctx (or of the last failed
+ * les_ctx_create when ctx is NULL). */
+const char* les_last_error(const les_ctx* ctx);
+/* Use the caller's hipStream_t for all further work (NULL: the context's own). */
+int les_set_stream(les_ctx* ctx, void* hip_stream);
+int les_synchronize(les_ctx* ctx);
+
+/* ---- fields (Field, src/field.h:9-41) ---- */
+int les_set_field(les_ctx* ctx, int field, const double* src, int ptr_kind);
+int les_get_field(les_ctx* ctx, int field, double* dst, int ptr_kind);
+/* Device address of a field slot (valid until the next call that swaps
+ * buffers: les_momentum, les_step). */
+int les_field_ptr(les_ctx* ctx, int field, double** dev_ptr);
+int les_set_mask(les_ctx* ctx, const uint8_t* solid, int ptr_kind);
+int les_get_mask(les_ctx* ctx, uint8_t* solid, int ptr_kind);
+
+/* ---- operator ---- */
+/* build_stencil (pressure.cpp:608).  ignore_mask = 1 builds A as the
+ * shipped driver does, before the obstacle mask exists (main.cpp:66, 81). */
+int les_build_stencil(les_ctx* ctx, int ignore_mask);
+int les_set_stencil(les_ctx* ctx, int which, double* const coef[7], int ptr_kind);
+int les_get_stencil(les_ctx* ctx, int which, double* const coef[7], int ptr_kind);
+/* y = A x: fast_apply_laplacian (pressure.cpp:709).  which selects A or L. */
+int les_matvec(les_ctx* ctx, int which, const double* x, double* y, int ptr_kind);
+/* parallel_dot (pressure.cpp:962), deterministic order. */
+int les_dot(les_ctx* ctx, const double* a, const double* b, int ptr_kind, double* out);
+
+/* ---- preconditioners ---- */
+/* build_ic_preconditioner (pressure.cpp:495): L from the context's A. */
+int les_ic0_factor(les_ctx* ctx);
+/* z = M^-1 r for one of LES_PC_*: apply_jacobi_preconditioner,
+ * apply_ic_preconditioner (pressure.cpp:442; L or A per precond) or
+ * apply_chebyshev_preconditioner (pressure.cpp:851). */
+int les_precond_apply(les_ctx* ctx, int precond, const double* r, double* z, int ptr_kind,
+                      double lambda_min, double lambda_max, int degree);
+/* estimate_lambda_max (pressure.cpp:917): power iteration from the
+ * mt19937(1337) U(-1,1) start vector drawn in sequence. */
+int les_power_lambda_max(les_ctx* ctx, int n_iter, double* out);
+/* Lanczos extreme Ritz values of A after `steps` steps from a seeded start
+ * vector (no reference counterpart; bounds for the Chebyshev polynomial). */
+int les_lanczos_bounds(les_ctx* ctx, int steps, uint64_t seed, double* ritz_min,
+                       double* ritz_max);
+
+/* ---- PCG (solve_pressure_pcg, pressure.cpp:342; and
+ *      solve_pressure_pcg_chebyshev, pressure.cpp:762) ---- */
+typedef struct les_pcg_params {
+    int precond;          /* LES_PC_* */
+    int max_iter;
+    double tol;           /* stop when ||r|| < tol (pressure.cpp:389) ... */
+    int tol_is_relative;  /* ... or when ||r|| < tol*||b|| */
+    double lambda_min, lambda_max;  /* Chebyshev interval */
+    int degree;           /* Chebyshev degree (pressure.h:49: default 5) */
+    int precision;        /* LES_PREC_FP64 or LES_PREC_MIXED */
+    int reliable_every;   /* mixed: FP64 residual replacement period (0: 50) */
+    int check_every;      /* host polls convergence every this many iterations (0: auto) */
+} les_pcg_params;
+
+typedef struct les_pcg_result {
+    int iters;            /* reference convention: 0 if converged at start, max_iter if not */
+    int converged;
+    double res_norm;      /* ||r|| of the recursive residual at exit */
+    double res0_norm;     /* ||b - A x0|| */
+    double b_norm;
+    double true_res_norm; /* ||b - A x|| in FP64 at exit (mixed precision; FP64: = res_norm) */
+    int replacements;     /* mixed: FP64 residual replacements performed */
+    int fault;            /* a non-finite scalar stopped the iteration */
+} les_pcg_result;
+
+/* Solves A p = b in place: x0 and the result are the context's LES_F_P,
+ * the right-hand side its LES_F_B. */
+int les_pcg(les_ctx* ctx, const les_pcg_params* prm, les_pcg_result* res);
+/* ||r_k|| for k = 0..iters of the last FP64 les_pcg (n entries at most). */
+int les_pcg_trace(les_ctx* ctx, double* trace, int n);
+
+/* ---- stationary pressure solvers (pressure.cpp:14-247) ---- */
+int les_pressure_bcs(les_ctx* ctx);                       /* apply_pressure_BCs */
+/* solve_pressure_jacobi (pressure.cpp:106): RHS from u,v,w; updates p. */
+int les_jacobi_solve(les_ctx* ctx, double dt, double rho, int* sweeps_out);
+/* solve_pressure_GS (pressure.cpp:186) with ordering LES_SOLVER_SOR_LEX
+ * (bit-exact, i+j+k hyperplane order) or LES_SOLVER_SOR_RB (red-black). */
+int les_sor_solve(les_ctx* ctx, int ordering, double dt, double rho, double omega, int max_sweeps,
+                  int* sweeps_out);
+
+/* ---- physics (main.cpp:208-714) ---- */
+/* compute_sgs_stress (main.cpp:423) with Cs as a parameter (the reference
+ * fixes 0.18).  apply_bcs = 1 also applies apply_nut_BCs (main.cpp:483).
+ * tau (6 arrays xx,xy,xz,yy,yz,zz; may be NULL) receives the stresses. */
+int les_sgs(les_ctx* ctx, double Cs, int apply_bcs, double* const tau[6], int ptr_kind);
+int les_nut_bcs(les_ctx* ctx);                            /* apply_nut_BCs */
+/* solve_momentum (main.cpp:563) including its u<->u_new buffer swap.
+ * apply_bcs = 1 also applies apply_velocity_BCs to the new velocity. */
+int les_momentum(les_ctx* ctx, double dt, double nu, int apply_bcs);
+int les_velocity_bcs(les_ctx* ctx);                       /* apply_velocity_BCs */
+/* calculate_divergence (pressure.cpp:256) of u_new into p_rhs, then
+ * build_b_vector (pressure.cpp:566) into b. */
+int les_divergence_b(les_ctx* ctx, double dt, double rho);
+/* correct_velocity (main.cpp:671); apply_bcs = 1 also applies
+ * apply_velocity_BCs. */
+int les_correct(les_ctx* ctx, double dt, double rho, int apply_bcs);
+/* compute_adaptive_dt (main.cpp:208). */
+int les_adaptive_dt(les_ctx* ctx, double nu, double* dt_out);
+
+/* ---- one step of the driver loop (main.cpp:115-144) ---- */
+typedef struct les_step_params {
+    int step;          /* dt = 1e-3 while step <= 20, else adaptive (main.cpp:115-119) */
+    double dt_fixed;   /* > 0 overrides that schedule */
+    double nu, rho, Cs;
+    int solver;        /* LES_SOLVER_* */
+    les_pcg_params pcg;
+    double omega;      /* SOR relaxation (pressure.cpp:195: 1.7) */
+    int sor_sweeps;    /* SOR max sweeps (main.cpp:133) */
+} les_step_params;
+
+typedef struct les_step_stats {
+    double dt;
+    les_pcg_result pcg;
+    int sweeps;
+} les_step_stats;
+
+int les_step(les_ctx* ctx, const les_step_params* prm, les_step_stats* stats);
+
+/* ---- diagnostics (no reference counterpart) ----
+ * Brackets every launch of one kernel class with HIP events on the
+ * context's stream so its average duration can be read back.  Starting
+ * resets the record; LES_KT_NONE stops. */
+enum { LES_KT_NONE = 0, LES_KT_CHEB_STEP = 1, LES_KT_PCG_MATVEC = 2, LES_KT_PCG_UPDATE = 3,
+       LES_KT_MOMENTUM = 4, LES_KT_IC_SOLVE = 5 };
+int les_kernel_timing(les_ctx* ctx, int kernel_class);
+/* launches recorded, their summed duration in ms, and the bytes per cell the
+ * class is charged (element size of the run) are returned; synchronises. */
+int les_kernel_timing_read(les_ctx* ctx, int* launches, double* total_ms, int* elem_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

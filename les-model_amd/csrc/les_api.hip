@@ -1,0 +1,1290 @@
+// les_api.hip -- the C ABI (include/les_gpu.h) over a device-resident context.
+//
+// The context owns every device buffer: the Field members, the mask, the
+// stencil A (FP64, plus an FP32 copy for mixed precision), the IC(0) factor
+// L, Krylov workspaces, and the device-resident solver scalars.  Host
+// scalars the reference forms with its own expressions (1/(dx*dx), rho/dt,
+// the Chebyshev alpha/beta sequence, the mt19937 start vector ...) are
+// formed here on the host the same way, so device results match the
+// reference's bit for bit wherever no reduction is involved.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "../../include/les_gpu.h"
+#include "les_physics.hpp"
+#include "les_mixed.hpp"
+
+using namespace les;
+
+struct les_ctx {
+    int device = 0;
+    hipStream_t own = nullptr;
+    hipStream_t stream = nullptr;
+    Dims d{};
+    double dx = 0, dy = 0, dz = 0;
+    double* f[LES_F_COUNT] = {};
+    uint8_t* solid = nullptr;
+    double* A[7] = {};
+    bool have_A = false;
+    bool A_from_mask = false;  // A was built by les_build_stencil from the mask
+    double* L[7] = {};
+    bool have_L = false;
+    float* A32[7] = {};
+    bool have_A32 = false;
+    float* L32[7] = {};
+    bool have_L32 = false;
+    // FP64 Krylov workspaces
+    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *za = nullptr, *zb = nullptr,
+           *pc = nullptr, *t1 = nullptr, *t2 = nullptr;
+    MixedWork mw{};
+    RedScratch red{};
+    SolveScalars* sc = nullptr;       // device
+    SolveScalars* sc_host = nullptr;  // pinned, two slots
+    StatScalars* st = nullptr;        // device
+    StatScalars* st_host = nullptr;   // pinned
+    double* dsc = nullptr;            // device scalar slots
+    double* hsc = nullptr;            // pinned scalar slots
+    double* trace = nullptr;          // device, trace_cap entries
+    int trace_cap = 0;
+    int last_trace_n = 0;
+    hipEvent_t ev[2] = {};
+    // kernel timing diagnostics
+    int kt_class = 0;
+    int kt_elem = 0;
+    std::vector<hipEvent_t> kt_events;  // pairs (start, stop)
+    size_t kt_used = 0;
+    std::string err;
+};
+
+static std::string g_create_err;
+
+namespace {
+
+constexpr size_t kScalarSlots = 64;
+
+int fail(les_ctx* c, int code, const std::string& msg)
+{
+    if (c) c->err = msg;
+    else g_create_err = msg;
+    return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                     \
+    do {                                                                                      \
+        hipError_t e_ = (expr);                                                               \
+        if (e_ != hipSuccess)                                                                 \
+            return fail(ctx, LES_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+size_t nbytes(const les_ctx* c, size_t elem) { return (size_t)c->d.N * elem; }
+
+template <typename T>
+int alloc(les_ctx* c, T** p, size_t count, bool zero = true)
+{
+    if (*p) return LES_OK;
+    hipError_t e = hipMalloc((void**)p, count * sizeof(T));
+    if (e != hipSuccess) {
+        *p = nullptr;
+        return fail(c, LES_ERR_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
+    }
+    if (zero) {
+        e = hipMemsetAsync(*p, 0, count * sizeof(T), c->stream);
+        if (e != hipSuccess) return fail(c, LES_ERR_HIP, hipGetErrorString(e));
+    }
+    return LES_OK;
+}
+
+int ensure_krylov(les_ctx* c)
+{
+    int rc;
+    double** bufs[] = {&c->r, &c->z, &c->p, &c->q, &c->za, &c->zb, &c->pc, &c->t1, &c->t2};
+    for (double** b : bufs)
+        if ((rc = alloc(c, b, c->d.N)) != LES_OK) return rc;
+    return LES_OK;
+}
+
+int ensure_trace(les_ctx* c, int n)
+{
+    if (c->trace_cap >= n) return LES_OK;
+    if (c->trace) hipFree(c->trace);
+    c->trace = nullptr;
+    c->trace_cap = 0;
+    int rc = alloc(c, &c->trace, (size_t)n);
+    if (rc == LES_OK) c->trace_cap = n;
+    return rc;
+}
+
+hipMemcpyKind kind_in(int ptr_kind) { return ptr_kind ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice; }
+hipMemcpyKind kind_out(int ptr_kind) { return ptr_kind ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost; }
+
+Stencil<double> stencil_of(double* const a[7])
+{
+    Stencil<double> s;
+    for (int q = 0; q < 7; ++q) s.a[q] = a[q];
+    return s;
+}
+Stencil<float> stencil_of(float* const a[7])
+{
+    Stencil<float> s;
+    for (int q = 0; q < 7; ++q) s.a[q] = a[q];
+    return s;
+}
+
+PhysConsts make_consts(const les_ctx* c, double dt, double nu, double rho, double Cs)
+{
+    PhysConsts k{};
+    k.dt = dt;
+    k.nu = nu;
+    k.rho = rho;
+    k.inv_dx2 = 1.0 / (c->dx * c->dx);
+    k.inv_dy2 = 1.0 / (c->dy * c->dy);
+    k.inv_dz2 = 1.0 / (c->dz * c->dz);
+    k.inv_2dx = 1.0 / (2.0 * c->dx);
+    k.inv_2dy = 1.0 / (2.0 * c->dy);
+    k.inv_2dz = 1.0 / (2.0 * c->dz);
+    k.two_dx = 2 * c->dx;
+    k.two_dy = 2 * c->dy;
+    k.two_dz = 2 * c->dz;
+    const double delta = std::pow(c->dx * c->dy * c->dz, 1.0 / 3.0);
+    k.csd2 = std::pow(Cs * delta, 2);
+    k.div_scale = rho / dt;
+    k.div_ix = 1.0 / (2.0 * c->dx);
+    k.div_iy = 1.0 / (2.0 * c->dy);
+    k.div_iz = 1.0 / (2.0 * c->dz);
+    k.div_ix1 = 1.0 / c->dx;
+    k.div_iy1 = 1.0 / c->dy;
+    k.div_iz1 = 1.0 / c->dz;
+    k.dor = dt / rho;
+    return k;
+}
+
+// Chebyshev scalars (pressure.cpp:863-898): alpha_0, then (beta_k, alpha_k)
+// for k = 1..degree-1, formed with the reference's expressions.
+void cheb_coeffs(double lmin, double lmax, int degree, double& a0, std::vector<double>& beta,
+                 std::vector<double>& alpha)
+{
+    const double d = (lmax + lmin) / 2.0;
+    const double c = (lmax - lmin) / 2.0;
+    double a = 2.0 / d;
+    a0 = a;
+    beta.clear();
+    alpha.clear();
+    for (int k = 1; k < degree; ++k) {
+        const double b = (c * a / 2.0) * (c * a / 2.0);
+        a = 1.0 / (d - b);
+        beta.push_back(b);
+        alpha.push_back(a);
+    }
+}
+
+int last_level(const Dims& d) { return d.nx + d.ny + d.nz - 3; }
+
+// Event bracket around the launches of the timed kernel class.
+struct KtScope {
+    les_ctx* c;
+    bool on;
+    hipEvent_t stop = nullptr;
+    KtScope(les_ctx* ctx, int cls, int elem) : c(ctx), on(ctx->kt_class == cls)
+    {
+        if (!on) return;
+        if (c->kt_used + 2 > c->kt_events.size()) {
+            for (int q = 0; q < 64; ++q) {
+                hipEvent_t e;
+                if (hipEventCreate(&e) != hipSuccess) { on = false; return; }
+                c->kt_events.push_back(e);
+            }
+        }
+        c->kt_elem = elem;
+        (void)hipEventRecord(c->kt_events[c->kt_used], c->stream);
+        stop = c->kt_events[c->kt_used + 1];
+        c->kt_used += 2;
+    }
+    ~KtScope()
+    {
+        if (on) (void)hipEventRecord(stop, c->stream);
+    }
+};
+
+double tridiag_eig(const std::vector<double>& a, const std::vector<double>& b, int which);
+
+// z = M^-1 r for FP64 preconditioners.  When epi >= 0 the final kernel also
+// reduces r.z into the solve scalars with that epilogue.
+int precond_fp64(les_ctx* c, int pc, const double* r, double* zout, double lmin, double lmax,
+                 int degree, int epi, SolveScalars* sc)
+{
+    const Dims& d = c->d;
+    hipStream_t s = c->stream;
+    switch (pc) {
+    case LES_PC_NONE:
+        HIPCHK(c, hipMemcpyAsync(zout, r, nbytes(c, 8), hipMemcpyDeviceToDevice, s));
+        break;
+    case LES_PC_JACOBI:
+        HIPCHK(c, launch_pcg_p_jacobi<double>(s, d, r, c->A[SC], zout, 1, epi, c->red, sc));
+        return LES_OK;
+    case LES_PC_IC0:
+    case LES_PC_LEGACY_IC_A: {
+        double* const* L = (pc == LES_PC_IC0) ? c->L : c->A;
+        if (pc == LES_PC_IC0 && !c->have_L) return fail(c, LES_ERR_STATE, "IC(0) factor not built");
+        KtScope kt(c, LES_KT_IC_SOLVE, 8);
+        for (int l = 0; l <= last_level(d); ++l)
+            HIPCHK(c, launch_ic_fwd_level<double>(s, d, l, r, zout, L[SC], L[SW], L[SS], L[SB], sc));
+        for (int l = last_level(d); l >= 0; --l)
+            HIPCHK(c, launch_ic_bwd_level<double>(s, d, l, zout, L[SC], L[SW], L[SS], L[SB], sc));
+        break;
+    }
+    case LES_PC_CHEBYSHEV: {
+        double a0;
+        std::vector<double> be, al;
+        cheb_coeffs(lmin, lmax, degree, a0, be, al);
+        Stencil<double> A = stencil_of(c->A);
+        if (degree <= 1) {
+            HIPCHK(c, launch_cheb_scale<double>(s, d, r, zout, a0, epi, c->red, sc));
+            return LES_OK;
+        }
+        double* buf[2] = {c->za, c->zb};
+        for (int k = 1; k < degree; ++k) {
+            const bool last = (k == degree - 1);
+            const double* zin = (k == 1) ? nullptr : buf[k & 1];
+            double* zo = last ? zout : buf[(k + 1) & 1];
+            KtScope kt(c, LES_KT_CHEB_STEP, 8);
+            HIPCHK(c, launch_cheb_step<double>(s, d, A, r, zin, zo, c->pc, a0, be[k - 1],
+                                               al[k - 1], k == 1, last ? epi : -1, c->red, sc));
+        }
+        return LES_OK;
+    }
+    default:
+        return fail(c, LES_ERR_ARG, "unknown preconditioner");
+    }
+    if (epi >= 0) HIPCHK(c, launch_pcg_rz<double>(s, d, r, zout, epi, c->red, sc));
+    return LES_OK;
+}
+
+}  // namespace
+
+// ====================================================================
+// C ABI
+// ====================================================================
+extern "C" {
+
+int les_ctx_create(int nx, int ny, int nz, double dx, double dy, double dz, les_ctx** out)
+{
+    if (!out) return fail(nullptr, LES_ERR_ARG, "out is NULL");
+    *out = nullptr;
+    if (nx < 3 || ny < 3 || nz < 3)
+        return fail(nullptr, LES_ERR_ARG, "grid must be at least 3 cells in each direction");
+    const long long N = (long long)nx * ny * nz;
+    if (N >= (1LL << 31)) return fail(nullptr, LES_ERR_ARG, "grid exceeds 2^31 cells (int indices)");
+    if (!(dx > 0 && dy > 0 && dz > 0)) return fail(nullptr, LES_ERR_ARG, "spacings must be > 0");
+    les_ctx* c = new (std::nothrow) les_ctx();
+    if (!c) return fail(nullptr, LES_ERR_NOMEM, "host allocation failed");
+    c->d = Dims{nx, ny, nz, ny * nz, (int)N};
+    c->dx = dx;
+    c->dy = dy;
+    c->dz = dz;
+    auto bail = [&](hipError_t e, const char* what) {
+        g_create_err = std::string(what) + ": " + hipGetErrorString(e);
+        les_ctx_destroy(c);
+        return LES_ERR_HIP;
+    };
+    hipError_t e;
+    if ((e = hipGetDevice(&c->device)) != hipSuccess) return bail(e, "hipGetDevice");
+    if ((e = hipStreamCreateWithFlags(&c->own, hipStreamNonBlocking)) != hipSuccess)
+        return bail(e, "hipStreamCreate");
+    c->stream = c->own;
+    for (int q = 0; q < LES_F_COUNT; ++q)
+        if (alloc(c, &c->f[q], N) != LES_OK) { g_create_err = c->err; les_ctx_destroy(c); return LES_ERR_NOMEM; }
+    if (alloc(c, &c->solid, N) != LES_OK) { g_create_err = c->err; les_ctx_destroy(c); return LES_ERR_NOMEM; }
+    if (alloc(c, &c->red.partials, (size_t)kMaxGrid * kMaxRed) != LES_OK ||
+        alloc(c, &c->red.counter, 1) != LES_OK || alloc(c, &c->sc, 1) != LES_OK ||
+        alloc(c, &c->st, 1) != LES_OK || alloc(c, &c->dsc, kScalarSlots) != LES_OK) {
+        g_create_err = c->err;
+        les_ctx_destroy(c);
+        return LES_ERR_NOMEM;
+    }
+    if ((e = hipHostMalloc((void**)&c->sc_host, 2 * sizeof(SolveScalars), 0)) != hipSuccess)
+        return bail(e, "hipHostMalloc");
+    if ((e = hipHostMalloc((void**)&c->st_host, 2 * sizeof(StatScalars), 0)) != hipSuccess)
+        return bail(e, "hipHostMalloc");
+    if ((e = hipHostMalloc((void**)&c->hsc, kScalarSlots * sizeof(double), 0)) != hipSuccess)
+        return bail(e, "hipHostMalloc");
+    for (int q = 0; q < 2; ++q)
+        if ((e = hipEventCreateWithFlags(&c->ev[q], hipEventDisableTiming)) != hipSuccess)
+            return bail(e, "hipEventCreate");
+    if ((e = hipStreamSynchronize(c->stream)) != hipSuccess) return bail(e, "hipStreamSynchronize");
+    *out = c;
+    return LES_OK;
+}
+
+int les_ctx_destroy(les_ctx* c)
+{
+    if (!c) return LES_OK;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    auto fr = [](void* p) {
+        if (p) hipFree(p);
+    };
+    for (auto* p : c->f) fr(p);
+    fr(c->solid);
+    for (int q = 0; q < 7; ++q) {
+        fr(c->A[q]);
+        fr(c->L[q]);
+        fr(c->A32[q]);
+        fr(c->L32[q]);
+    }
+    for (double* p : {c->r, c->z, c->p, c->q, c->za, c->zb, c->pc, c->t1, c->t2}) fr(p);
+    free_mixed(c->mw);
+    fr(c->red.partials);
+    fr(c->red.counter);
+    fr(c->sc);
+    fr(c->st);
+    fr(c->dsc);
+    fr(c->trace);
+    if (c->sc_host) hipHostFree(c->sc_host);
+    if (c->st_host) hipHostFree(c->st_host);
+    if (c->hsc) hipHostFree(c->hsc);
+    for (auto ev : c->ev)
+        if (ev) hipEventDestroy(ev);
+    for (auto ev : c->kt_events) hipEventDestroy(ev);
+    if (c->own) hipStreamDestroy(c->own);
+    delete c;
+    return LES_OK;
+}
+
+const char* les_last_error(const les_ctx* c) { return c ? c->err.c_str() : g_create_err.c_str(); }
+
+int les_set_stream(les_ctx* c, void* stream)
+{
+    if (!c) return LES_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->stream = stream ? (hipStream_t)stream : c->own;
+    return LES_OK;
+}
+
+int les_synchronize(les_ctx* c)
+{
+    if (!c) return LES_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+// ---------------- fields ----------------
+int les_set_field(les_ctx* c, int field, const double* src, int ptr_kind)
+{
+    if (!c || !src || field < 0 || field >= LES_F_COUNT) return fail(c, LES_ERR_ARG, "bad field/src");
+    HIPCHK(c, hipMemcpyAsync(c->f[field], src, nbytes(c, 8), kind_in(ptr_kind), c->stream));
+    if (!ptr_kind) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+int les_get_field(les_ctx* c, int field, double* dst, int ptr_kind)
+{
+    if (!c || !dst || field < 0 || field >= LES_F_COUNT) return fail(c, LES_ERR_ARG, "bad field/dst");
+    HIPCHK(c, hipMemcpyAsync(dst, c->f[field], nbytes(c, 8), kind_out(ptr_kind), c->stream));
+    if (!ptr_kind) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+int les_field_ptr(les_ctx* c, int field, double** dev_ptr)
+{
+    if (!c || !dev_ptr || field < 0 || field >= LES_F_COUNT) return fail(c, LES_ERR_ARG, "bad field");
+    *dev_ptr = c->f[field];
+    return LES_OK;
+}
+
+int les_set_mask(les_ctx* c, const uint8_t* solid, int ptr_kind)
+{
+    if (!c || !solid) return fail(c, LES_ERR_ARG, "mask is NULL");
+    HIPCHK(c, hipMemcpyAsync(c->solid, solid, nbytes(c, 1), kind_in(ptr_kind), c->stream));
+    if (!ptr_kind) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+int les_get_mask(les_ctx* c, uint8_t* solid, int ptr_kind)
+{
+    if (!c || !solid) return fail(c, LES_ERR_ARG, "mask is NULL");
+    HIPCHK(c, hipMemcpyAsync(solid, c->solid, nbytes(c, 1), kind_out(ptr_kind), c->stream));
+    if (!ptr_kind) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+// ---------------- operator ----------------
+int les_build_stencil(les_ctx* c, int ignore_mask)
+{
+    if (!c) return LES_ERR_ARG;
+    int rc;
+    for (int q = 0; q < 7; ++q)
+        if ((rc = alloc(c, &c->A[q], c->d.N, false)) != LES_OK) return rc;
+    const uint8_t* m = c->solid;
+    uint8_t* zeros = nullptr;
+    if (ignore_mask) {
+        if ((rc = alloc(c, &zeros, c->d.N)) != LES_OK) return rc;
+        m = zeros;
+    }
+    HIPCHK(c, launch_build_stencil(c->stream, c->d, m, c->A, 1.0 / (c->dx * c->dx),
+                                   1.0 / (c->dy * c->dy), 1.0 / (c->dz * c->dz)));
+    if (zeros) {
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        hipFree(zeros);
+    }
+    c->have_A = true;
+    c->A_from_mask = !ignore_mask;
+    c->have_A32 = false;
+    c->have_L = false;
+    c->have_L32 = false;
+    return LES_OK;
+}
+
+int les_set_stencil(les_ctx* c, int which, double* const coef[7], int ptr_kind)
+{
+    if (!c || !coef || (which != LES_STENCIL_A && which != LES_STENCIL_L))
+        return fail(c, LES_ERR_ARG, "bad stencil arguments");
+    double** dst = which == LES_STENCIL_A ? c->A : c->L;
+    int rc;
+    for (int q = 0; q < 7; ++q) {
+        if ((rc = alloc(c, &dst[q], c->d.N, false)) != LES_OK) return rc;
+        if (!coef[q]) return fail(c, LES_ERR_ARG, "stencil array is NULL");
+        HIPCHK(c, hipMemcpyAsync(dst[q], coef[q], nbytes(c, 8), kind_in(ptr_kind), c->stream));
+    }
+    if (!ptr_kind) HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (which == LES_STENCIL_A) {
+        c->have_A = true;
+        c->A_from_mask = false;
+        c->have_A32 = false;
+    } else {
+        c->have_L = true;
+        c->have_L32 = false;
+    }
+    return LES_OK;
+}
+
+int les_get_stencil(les_ctx* c, int which, double* const coef[7], int ptr_kind)
+{
+    if (!c || !coef) return fail(c, LES_ERR_ARG, "bad stencil arguments");
+    const bool isA = which == LES_STENCIL_A;
+    if ((isA && !c->have_A) || (!isA && !c->have_L)) return fail(c, LES_ERR_STATE, "stencil not set");
+    double* const* src = isA ? c->A : c->L;
+    for (int q = 0; q < 7; ++q)
+        HIPCHK(c, hipMemcpyAsync(coef[q], src[q], nbytes(c, 8), kind_out(ptr_kind), c->stream));
+    if (!ptr_kind) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+int les_matvec(les_ctx* c, int which, const double* x, double* y, int ptr_kind)
+{
+    if (!c || !x || !y) return fail(c, LES_ERR_ARG, "NULL vector");
+    const bool isA = which == LES_STENCIL_A;
+    if ((isA && !c->have_A) || (!isA && !c->have_L)) return fail(c, LES_ERR_STATE, "stencil not set");
+    int rc;
+    if ((rc = ensure_krylov(c)) != LES_OK) return rc;
+    const double* xd = x;
+    double* yd = y;
+    if (!ptr_kind) {
+        HIPCHK(c, hipMemcpyAsync(c->t1, x, nbytes(c, 8), hipMemcpyHostToDevice, c->stream));
+        xd = c->t1;
+        yd = c->t2;
+    }
+    HIPCHK(c, launch_matvec<double>(c->stream, c->d, stencil_of(isA ? c->A : c->L), xd, yd));
+    if (!ptr_kind) {
+        HIPCHK(c, hipMemcpyAsync(y, yd, nbytes(c, 8), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return LES_OK;
+}
+
+int les_dot(les_ctx* c, const double* a, const double* b, int ptr_kind, double* out)
+{
+    if (!c || !a || !b || !out) return fail(c, LES_ERR_ARG, "NULL argument");
+    int rc;
+    if ((rc = ensure_krylov(c)) != LES_OK) return rc;
+    const double *ad = a, *bd = b;
+    if (!ptr_kind) {
+        HIPCHK(c, hipMemcpyAsync(c->t1, a, nbytes(c, 8), hipMemcpyHostToDevice, c->stream));
+        HIPCHK(c, hipMemcpyAsync(c->t2, b, nbytes(c, 8), hipMemcpyHostToDevice, c->stream));
+        ad = c->t1;
+        bd = c->t2;
+    }
+    HIPCHK(c, launch_dot(c->stream, c->d, ad, bd, c->red, c->dsc));
+    HIPCHK(c, hipMemcpyAsync(c->hsc, c->dsc, 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = c->hsc[0];
+    return LES_OK;
+}
+
+// ---------------- preconditioners ----------------
+int les_ic0_factor(les_ctx* c)
+{
+    if (!c) return LES_ERR_ARG;
+    if (!c->have_A) return fail(c, LES_ERR_STATE, "stencil A not built");
+    int rc;
+    for (int q = 0; q < 7; ++q) {
+        if ((rc = alloc(c, &c->L[q], c->d.N, false)) != LES_OK) return rc;
+        // StencilMatrix(n) starts at zero; guarded entries keep that value
+        HIPCHK(c, hipMemsetAsync(c->L[q], 0, nbytes(c, 8), c->stream));
+    }
+    const Stencil<double> A = stencil_of(c->A);
+    for (int l = 0; l <= last_level(c->d); ++l)
+        HIPCHK(c, launch_ic_factor_level(c->stream, c->d, l, A, c->L[SC], c->L[SW], c->L[SS],
+                                         c->L[SB]));
+    c->have_L = true;
+    c->have_L32 = false;
+    return LES_OK;
+}
+
+int les_precond_apply(les_ctx* c, int pc, const double* r, double* z, int ptr_kind, double lmin,
+                      double lmax, int degree)
+{
+    if (!c || !r || !z) return fail(c, LES_ERR_ARG, "NULL vector");
+    if (!c->have_A) return fail(c, LES_ERR_STATE, "stencil A not built");
+    int rc;
+    if ((rc = ensure_krylov(c)) != LES_OK) return rc;
+    const double* rd = r;
+    double* zd = z;
+    if (!ptr_kind) {
+        HIPCHK(c, hipMemcpyAsync(c->t1, r, nbytes(c, 8), hipMemcpyHostToDevice, c->stream));
+        rd = c->t1;
+        zd = c->t2;
+    }
+    if ((rc = precond_fp64(c, pc, rd, zd, lmin, lmax, degree, -1, nullptr)) != LES_OK) return rc;
+    if (!ptr_kind) {
+        HIPCHK(c, hipMemcpyAsync(z, zd, nbytes(c, 8), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+    }
+    return LES_OK;
+}
+
+int les_power_lambda_max(les_ctx* c, int n_iter, double* out)
+{
+    if (!c || !out) return fail(c, LES_ERR_ARG, "NULL argument");
+    if (!c->have_A) return fail(c, LES_ERR_STATE, "stencil A not built");
+    int rc;
+    if ((rc = ensure_krylov(c)) != LES_OK) return rc;
+    // start vector: pressure.cpp:926-931, drawn in sequence
+    std::vector<double> v0(c->d.N);
+    std::mt19937 gen(1337);
+    std::uniform_real_distribution<> dis(-1.0, 1.0);
+    for (int i = 0; i < c->d.N; ++i) v0[i] = dis(gen);
+    double *v = c->t1, *w = c->t2;
+    HIPCHK(c, hipMemcpyAsync(v, v0.data(), nbytes(c, 8), hipMemcpyHostToDevice, c->stream));
+    const Stencil<double> A = stencil_of(c->A);
+    for (int k = 0; k < n_iter; ++k) {
+        HIPCHK(c, launch_matvec<double>(c->stream, c->d, A, v, w));
+        HIPCHK(c, launch_dot(c->stream, c->d, v, w, c->red, c->dsc + 0));
+        HIPCHK(c, launch_dot(c->stream, c->d, w, w, c->red, c->dsc + 1));
+        HIPCHK(c, launch_scale_by_invnorm(c->stream, c->d.N, w, v, c->dsc + 1));
+    }
+    HIPCHK(c, hipMemcpyAsync(c->hsc, c->dsc, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *out = n_iter > 0 ? c->hsc[0] : 0.0;
+    return LES_OK;
+}
+
+
+int les_lanczos_bounds(les_ctx* c, int steps, uint64_t seed, double* ritz_min, double* ritz_max)
+{
+    if (!c || !ritz_min || !ritz_max || steps < 1) return fail(c, LES_ERR_ARG, "bad arguments");
+    if (!c->have_A) return fail(c, LES_ERR_STATE, "stencil A not built");
+    int rc;
+    if ((rc = ensure_krylov(c)) != LES_OK) return rc;
+    const int N = c->d.N;
+    // seeded start vector (SplitMix64 per cell), normalised on the device
+    std::vector<double> v0(N);
+    for (int i = 0; i < N; ++i) {
+        uint64_t z = seed * 0x9E3779B97F4A7C15ull + (uint64_t)i + 1;
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+        z ^= z >> 31;
+        v0[i] = (double)(z >> 11) * (1.0 / 9007199254740992.0) - 0.5;
+    }
+    if (c->A_from_mask) {
+        // Solid and outlet rows of A are decoupled identity rows (eigenvalue
+        // 1, pressure.cpp:634-642); start in the fluid subspace so the Ritz
+        // values bound the fluid block the Chebyshev polynomial acts on.
+        std::vector<uint8_t> m(N);
+        HIPCHK(c, hipMemcpyAsync(m.data(), c->solid, (size_t)N, hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (int i = 0; i < N; ++i)
+            if (m[i] || i / c->d.sij == c->d.nx - 1) v0[i] = 0.0;
+    }
+    double *v = c->t1, *w = c->t2, *vp = c->za;
+    hipStream_t s = c->stream;
+    HIPCHK(c, hipMemcpyAsync(w, v0.data(), (size_t)N * 8, hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemsetAsync(vp, 0, (size_t)N * 8, s));
+    HIPCHK(c, launch_dot(s, c->d, w, w, c->red, c->dsc + 1));
+    HIPCHK(c, launch_scale_by_invnorm(s, N, w, v, c->dsc + 1));
+    const Stencil<double> A = stencil_of(c->A);
+    std::vector<double> al, be;
+    double beta_prev = 0.0;
+    for (int j = 0; j < steps; ++j) {
+        HIPCHK(c, launch_matvec<double>(s, c->d, A, v, w));
+        HIPCHK(c, launch_dot(s, c->d, w, v, c->red, c->dsc + 0));
+        HIPCHK(c, hipMemcpyAsync(c->hsc, c->dsc, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        const double alpha = c->hsc[0];
+        al.push_back(alpha);
+        HIPCHK(c, launch_lanczos_axpy(s, N, w, v, vp, alpha, beta_prev));
+        if (j == steps - 1) break;
+        HIPCHK(c, launch_dot(s, c->d, w, w, c->red, c->dsc + 1));
+        HIPCHK(c, hipMemcpyAsync(c->hsc + 1, c->dsc + 1, 8, hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipStreamSynchronize(s));
+        const double beta = std::sqrt(c->hsc[1]);
+        if (!(beta > 1e-300)) break;
+        be.push_back(beta);
+        // vp <- v ; v <- w / beta
+        HIPCHK(c, hipMemcpyAsync(vp, v, (size_t)N * 8, hipMemcpyDeviceToDevice, s));
+        HIPCHK(c, launch_scale_by_invnorm(s, N, w, v, c->dsc + 1));
+        beta_prev = beta;
+    }
+    be.resize(al.size() > 0 ? al.size() - 1 : 0);
+    *ritz_min = tridiag_eig(al, be, 0);
+    *ritz_max = tridiag_eig(al, be, (int)al.size() - 1);
+    return LES_OK;
+}
+
+// ---------------- PCG ----------------
+}  // extern "C"
+
+namespace {
+
+int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
+{
+    const Dims& d = c->d;
+    hipStream_t s = c->stream;
+    const int pc = prm->precond;
+    int rc;
+    if ((rc = ensure_krylov(c)) != LES_OK) return rc;
+    if ((rc = ensure_trace(c, prm->max_iter + 1)) != LES_OK) return rc;
+    if (pc == LES_PC_IC0 && !c->have_L) return fail(c, LES_ERR_STATE, "IC(0) factor not built");
+
+    SolveScalars init{};
+    init.tol = prm->tol_is_relative ? 0.0 : prm->tol;
+    init.tol_rel = prm->tol_is_relative ? prm->tol : 0.0;
+    init.max_iter = prm->max_iter;
+    init.trace = c->trace;
+    c->sc_host[0] = init;
+    HIPCHK(c, hipMemcpyAsync(c->sc, &c->sc_host[0], sizeof(SolveScalars), hipMemcpyHostToDevice, s));
+
+    const Stencil<double> A = stencil_of(c->A);
+    double* x = c->f[LES_F_P];
+    const double* b = c->f[LES_F_B];
+    double *r = c->r, *p = c->p, *q = c->q, *z = c->z;
+    SolveScalars* sc = c->sc;
+
+    // r = b - A x, ||r||, stop if converged (pressure.cpp:355-359)
+    HIPCHK(c, launch_pcg_init<double>(s, d, A, b, x, r, c->red, sc));
+    // p = M^-1 r, rho = r.p (pressure.cpp:363-369)
+    if (pc == LES_PC_JACOBI)
+        HIPCHK(c, launch_pcg_p_jacobi<double>(s, d, r, c->A[SC], p, 1, EPI_RHO, c->red, sc));
+    else if ((rc = precond_fp64(c, pc, r, p, prm->lambda_min, prm->lambda_max, prm->degree, EPI_RHO,
+                                sc)) != LES_OK)
+        return rc;
+
+    int check = prm->check_every > 0 ? prm->check_every : (d.N >= (1 << 22) ? 4 : 16);
+    int issued = 0, batch = 0;
+    bool finished = false;
+    while (!finished) {
+        const int n = std::min(check, prm->max_iter - issued);
+        for (int it = 0; it < n; ++it) {
+            {
+                KtScope kt(c, LES_KT_PCG_MATVEC, 8);
+                HIPCHK(c, launch_pcg_matvec<double>(s, d, A, p, q, c->red, sc));
+            }
+            if (pc == LES_PC_JACOBI) {
+                HIPCHK(c, (launch_pcg_update<double, double>(s, d, x, r, p, q, c->A[SC], c->red, sc)));
+                HIPCHK(c, launch_pcg_p_jacobi<double>(s, d, r, c->A[SC], p, 0, -1, c->red, sc));
+            } else {
+                HIPCHK(c, (launch_pcg_update<double, double>(s, d, x, r, p, q, nullptr, c->red, sc)));
+                if ((rc = precond_fp64(c, pc, r, z, prm->lambda_min, prm->lambda_max, prm->degree,
+                                       EPI_BETA, sc)) != LES_OK)
+                    return rc;
+                HIPCHK(c, launch_pcg_p_update<double>(s, d, z, p, sc));
+            }
+        }
+        issued += n;
+        const int slot = batch & 1;
+        HIPCHK(c, hipMemcpyAsync(&c->sc_host[slot], sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipEventRecord(c->ev[slot], s));
+        if (issued >= prm->max_iter) {
+            HIPCHK(c, hipEventSynchronize(c->ev[slot]));
+            finished = true;
+        } else if (batch > 0) {
+            // check the previous batch while this one runs
+            HIPCHK(c, hipEventSynchronize(c->ev[slot ^ 1]));
+            if (c->sc_host[slot ^ 1].done) finished = true;
+        }
+        if (prm->max_iter <= 0) finished = true;
+        ++batch;
+    }
+    HIPCHK(c, hipMemcpyAsync(&c->sc_host[0], sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    const SolveScalars& h = c->sc_host[0];
+    res->iters = h.converged ? h.iter : (h.err0 < h.tol ? 0 : prm->max_iter);
+    if (h.converged && h.iter == 0) res->iters = 0;
+    res->converged = h.converged;
+    res->res_norm = h.err;
+    res->res0_norm = h.err0;
+    res->b_norm = std::sqrt(h.bb);
+    res->true_res_norm = h.err;
+    res->replacements = 0;
+    res->fault = h.fault;
+    c->last_trace_n = h.iter + 1;
+    return LES_OK;
+}
+
+
+int ensure_fp32_ops(les_ctx* c, int pc)
+{
+    int rc;
+    if (!c->have_A32) {
+        for (int q = 0; q < 7; ++q) {
+            if ((rc = alloc(c, &c->A32[q], c->d.N, false)) != LES_OK) return rc;
+            HIPCHK(c, (launch_convert<float, double>(c->stream, c->d.N, c->A[q], c->A32[q])));
+        }
+        c->have_A32 = true;
+    }
+    if (pc == LES_PC_IC0 && !c->have_L32) {
+        if (!c->have_L) return fail(c, LES_ERR_STATE, "IC(0) factor not built");
+        for (int q = 0; q < 7; ++q) {
+            if ((rc = alloc(c, &c->L32[q], c->d.N, false)) != LES_OK) return rc;
+            HIPCHK(c, (launch_convert<float, double>(c->stream, c->d.N, c->L[q], c->L32[q])));
+        }
+        c->have_L32 = true;
+    }
+    MixedWork& w = c->mw;
+    for (float** b : {&w.r, &w.p, &w.q, &w.z, &w.za, &w.zb, &w.pc, &w.x})
+        if ((rc = alloc(c, b, c->d.N)) != LES_OK) return rc;
+    return LES_OK;
+}
+
+// z = M^-1 r in FP32 (operator and vectors FP32, r.z accumulated in FP64)
+int precond_fp32(les_ctx* c, int pc, const float* r, float* zout, double lmin, double lmax,
+                 int degree, int epi, SolveScalars* sc)
+{
+    const Dims& d = c->d;
+    hipStream_t s = c->stream;
+    switch (pc) {
+    case LES_PC_NONE:
+        HIPCHK(c, hipMemcpyAsync(zout, r, nbytes(c, 4), hipMemcpyDeviceToDevice, s));
+        break;
+    case LES_PC_JACOBI:
+        HIPCHK(c, launch_pcg_p_jacobi<float>(s, d, r, c->A32[SC], zout, 1, epi, c->red, sc));
+        return LES_OK;
+    case LES_PC_IC0:
+    case LES_PC_LEGACY_IC_A: {
+        float* const* L = (pc == LES_PC_IC0) ? c->L32 : c->A32;
+        KtScope kt(c, LES_KT_IC_SOLVE, 4);
+        for (int l = 0; l <= last_level(d); ++l)
+            HIPCHK(c, launch_ic_fwd_level<float>(s, d, l, r, zout, L[SC], L[SW], L[SS], L[SB], sc));
+        for (int l = last_level(d); l >= 0; --l)
+            HIPCHK(c, launch_ic_bwd_level<float>(s, d, l, zout, L[SC], L[SW], L[SS], L[SB], sc));
+        break;
+    }
+    case LES_PC_CHEBYSHEV: {
+        double a0;
+        std::vector<double> be, al;
+        cheb_coeffs(lmin, lmax, degree, a0, be, al);
+        Stencil<float> A = stencil_of(c->A32);
+        if (degree <= 1) {
+            HIPCHK(c, launch_cheb_scale<float>(s, d, r, zout, a0, epi, c->red, sc));
+            return LES_OK;
+        }
+        float* buf[2] = {c->mw.za, c->mw.zb};
+        for (int k = 1; k < degree; ++k) {
+            const bool last = (k == degree - 1);
+            const float* zin = (k == 1) ? nullptr : buf[k & 1];
+            float* zo = last ? zout : buf[(k + 1) & 1];
+            KtScope kt(c, LES_KT_CHEB_STEP, 4);
+            HIPCHK(c, launch_cheb_step<float>(s, d, A, r, zin, zo, c->mw.pc, a0, be[k - 1],
+                                              al[k - 1], k == 1, last ? epi : -1, c->red, sc));
+        }
+        return LES_OK;
+    }
+    default:
+        return fail(c, LES_ERR_ARG, "unknown preconditioner");
+    }
+    if (epi >= 0) HIPCHK(c, launch_pcg_rz<float>(s, d, r, zout, epi, c->red, sc));
+    return LES_OK;
+}
+
+// Mixed-precision PCG: the Krylov iteration runs on the FP32 copy of A with
+// FP32 vectors and FP64 dot accumulation; the iterate is held as
+// x64 + x32.  Every `reliable_every` iterations, whenever the FP32
+// recursive residual meets the tolerance, and at max_iter, x32 is folded
+// into x64 and r is replaced by the FP64 residual b - A x64 (FP64 stencil).
+// Convergence is declared only on that FP64 residual, so the result meets
+// the tolerance in FP64 (outer iterative refinement to the requested tol).
+int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
+{
+    const Dims& d = c->d;
+    hipStream_t s = c->stream;
+    const int pc = prm->precond;
+    int rc;
+    if ((rc = ensure_krylov(c)) != LES_OK) return rc;
+    if ((rc = ensure_fp32_ops(c, pc)) != LES_OK) return rc;
+    if ((rc = ensure_trace(c, prm->max_iter + 1)) != LES_OK) return rc;
+    MixedWork& w = c->mw;
+
+    SolveScalars init{};
+    init.tol = prm->tol_is_relative ? 0.0 : prm->tol;
+    init.tol_rel = prm->tol_is_relative ? prm->tol : 0.0;
+    init.max_iter = prm->max_iter;
+    init.reliable_every = prm->reliable_every > 0 ? prm->reliable_every : 50;
+    init.trace = c->trace;
+    c->sc_host[0] = init;
+    HIPCHK(c, hipMemcpyAsync(c->sc, &c->sc_host[0], sizeof(SolveScalars), hipMemcpyHostToDevice, s));
+    HIPCHK(c, hipMemsetAsync(w.x, 0, nbytes(c, 4), s));
+
+    const Stencil<double> A64 = stencil_of(c->A);
+    const Stencil<float> A32 = stencil_of(c->A32);
+    double* x64 = c->f[LES_F_P];
+    const double* b = c->f[LES_F_B];
+    SolveScalars* sc = c->sc;
+
+    // r = b - A x0 in FP64, rounded to FP32; ||r||, ||b||
+    HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.r, 1, 0, c->red, sc, nullptr));
+    if ((rc = precond_fp32(c, pc, w.r, w.p, prm->lambda_min, prm->lambda_max, prm->degree, EPI_RHO,
+                           sc)) != LES_OK)
+        return rc;
+
+    int check = prm->check_every > 0 ? prm->check_every : (d.N >= (1 << 22) ? 4 : 16);
+    int issued = 0, batch = 0;
+    bool finished = prm->max_iter <= 0;
+    while (!finished) {
+        const int n = std::min(check, prm->max_iter - issued);
+        for (int it = 0; it < n; ++it) {
+            {
+                KtScope kt(c, LES_KT_PCG_MATVEC, 4);
+                HIPCHK(c, launch_pcg_matvec<float>(s, d, A32, w.p, w.q, c->red, sc));
+            }
+            {
+                KtScope kt(c, LES_KT_PCG_UPDATE, 4);
+                HIPCHK(c, launch_pcg_update_mixed(s, d, w.x, w.r, w.p, w.q, c->red, sc));
+            }
+            HIPCHK(c, launch_mixed_fold(s, d, x64, w.x, 0, sc));
+            HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.r, 0, 0, c->red, sc, nullptr));
+            if ((rc = precond_fp32(c, pc, w.r, w.z, prm->lambda_min, prm->lambda_max, prm->degree,
+                                   EPI_BETA, sc)) != LES_OK)
+                return rc;
+            HIPCHK(c, launch_pcg_p_update<float>(s, d, w.z, w.p, sc));
+        }
+        issued += n;
+        const int slot = batch & 1;
+        HIPCHK(c, hipMemcpyAsync(&c->sc_host[slot], sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
+        HIPCHK(c, hipEventRecord(c->ev[slot], s));
+        if (issued >= prm->max_iter) {
+            HIPCHK(c, hipEventSynchronize(c->ev[slot]));
+            finished = true;
+        } else if (batch > 0) {
+            HIPCHK(c, hipEventSynchronize(c->ev[slot ^ 1]));
+            if (c->sc_host[slot ^ 1].done) finished = true;
+        }
+        ++batch;
+    }
+    // fold whatever correction is left and measure the FP64 residual
+    HIPCHK(c, launch_mixed_fold(s, d, x64, w.x, 1, sc));
+    HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.q, 0, 1, c->red, sc, c->dsc));
+    HIPCHK(c, hipMemcpyAsync(&c->sc_host[0], sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipMemcpyAsync(c->hsc, c->dsc, 8, hipMemcpyDeviceToHost, s));
+    HIPCHK(c, hipStreamSynchronize(s));
+    const SolveScalars& h = c->sc_host[0];
+    res->iters = h.converged ? h.iter : prm->max_iter;
+    res->converged = h.converged;
+    res->res_norm = h.err;
+    res->res0_norm = h.err0;
+    res->b_norm = std::sqrt(h.bb);
+    res->true_res_norm = c->hsc[0];
+    res->replacements = h.replacements;
+    res->fault = h.fault;
+    c->last_trace_n = h.iter + 1;
+    return LES_OK;
+}
+
+// eigenvalue of a symmetric tridiagonal matrix (diag a, off-diag b) with
+// index `which` (0 = smallest) by Sturm-sequence bisection
+double tridiag_eig(const std::vector<double>& a, const std::vector<double>& b, int which)
+{
+    const int m = (int)a.size();
+    double lo = a[0], hi = a[0];
+    for (int i = 0; i < m; ++i) {
+        const double r = (i > 0 ? std::fabs(b[i - 1]) : 0.0) + (i < m - 1 ? std::fabs(b[i]) : 0.0);
+        lo = std::min(lo, a[i] - r);
+        hi = std::max(hi, a[i] + r);
+    }
+    auto count_below = [&](double x) {
+        int cnt = 0;
+        double q = 1.0;
+        for (int i = 0; i < m; ++i) {
+            const double bb = i > 0 ? b[i - 1] * b[i - 1] : 0.0;
+            q = a[i] - x - (i > 0 ? bb / q : 0.0);
+            if (q == 0.0) q = -1e-300;
+            if (q < 0) ++cnt;
+        }
+        return cnt;
+    };
+    for (int it = 0; it < 200; ++it) {
+        const double mid = 0.5 * (lo + hi);
+        if (mid == lo || mid == hi) break;
+        if (count_below(mid) > which) hi = mid;
+        else lo = mid;
+    }
+    return 0.5 * (lo + hi);
+}
+
+}  // namespace
+
+extern "C" {
+
+int les_pcg(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
+{
+    if (!c || !prm || !res) return fail(c, LES_ERR_ARG, "NULL argument");
+    if (!c->have_A) return fail(c, LES_ERR_STATE, "stencil A not built");
+    if (prm->precond < LES_PC_NONE || prm->precond > LES_PC_CHEBYSHEV)
+        return fail(c, LES_ERR_ARG, "unknown preconditioner");
+    if (prm->max_iter < 0) return fail(c, LES_ERR_ARG, "max_iter < 0");
+    std::memset(res, 0, sizeof(*res));
+    if (prm->precision == LES_PREC_MIXED) return pcg_mixed(c, prm, res);
+    if (prm->precision != LES_PREC_FP64) return fail(c, LES_ERR_ARG, "unknown precision");
+    return pcg_fp64(c, prm, res);
+}
+
+int les_pcg_trace(les_ctx* c, double* trace, int n)
+{
+    if (!c || !trace) return fail(c, LES_ERR_ARG, "NULL argument");
+    const int m = std::min(n, c->last_trace_n);
+    if (m <= 0) return LES_OK;
+    HIPCHK(c, hipMemcpyAsync(trace, c->trace, (size_t)m * 8, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+// ---------------- stationary solvers ----------------
+int les_pressure_bcs(les_ctx* c)
+{
+    if (!c) return LES_ERR_ARG;
+    HIPCHK(c, launch_pressure_bcs(c->stream, c->d, c->f[LES_F_P], c->solid, nullptr, 0));
+    return LES_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+int stat_begin(les_ctx* c, int max_sweeps)
+{
+    StatScalars init{};
+    init.thresh = 1e-6;
+    init.max_sweeps = max_sweeps;
+    c->st_host[0] = init;
+    HIPCHK(c, hipMemcpyAsync(c->st, &c->st_host[0], sizeof(StatScalars), hipMemcpyHostToDevice,
+                             c->stream));
+    return LES_OK;
+}
+
+// Runs `sweep(k)` for k = 0..max_sweeps-1 with device-side early exit,
+// polling the host every few sweeps.
+template <typename F>
+int stat_loop(les_ctx* c, int max_sweeps, F sweep)
+{
+    const int check = 8;
+    int issued = 0, batch = 0;
+    bool finished = max_sweeps <= 0;
+    while (!finished) {
+        const int n = std::min(check, max_sweeps - issued);
+        for (int k = 0; k < n; ++k) {
+            int rc = sweep(issued + k);
+            if (rc != LES_OK) return rc;
+        }
+        issued += n;
+        const int slot = batch & 1;
+        HIPCHK(c, hipMemcpyAsync(&c->st_host[slot], c->st, sizeof(StatScalars),
+                                 hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipEventRecord(c->ev[slot], c->stream));
+        if (issued >= max_sweeps) {
+            HIPCHK(c, hipEventSynchronize(c->ev[slot]));
+            finished = true;
+        } else if (batch > 0) {
+            HIPCHK(c, hipEventSynchronize(c->ev[slot ^ 1]));
+            if (c->st_host[slot ^ 1].done) finished = true;
+        }
+        ++batch;
+    }
+    HIPCHK(c, hipMemcpyAsync(&c->st_host[0], c->st, sizeof(StatScalars), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int les_jacobi_solve(les_ctx* c, double dt, double rho, int* sweeps_out)
+{
+    if (!c) return LES_ERR_ARG;
+    int rc;
+    if ((rc = ensure_krylov(c)) != LES_OK) return rc;
+    const Dims& d = c->d;
+    // pressure.cpp:111-132
+    const double coef = rho / dt;
+    const double dx2 = c->dx * c->dx, dy2 = c->dy * c->dy, dz2 = c->dz * c->dz;
+    const double denom = 2 * (1.0 / dx2 + 1.0 / dy2 + 1.0 / dz2);
+    double* rhs = c->t1;
+    HIPCHK(c, launch_rhs_central(c->stream, d, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W], rhs,
+                                 coef, 2 * c->dx, 2 * c->dy, 2 * c->dz));
+    double* buf[2] = {c->f[LES_F_P], c->t2};
+    HIPCHK(c, hipMemsetAsync(c->t2, 0, nbytes(c, 8), c->stream));
+    if ((rc = stat_begin(c, 100)) != LES_OK) return rc;
+    rc = stat_loop(c, 100, [&](int k) -> int {
+        HIPCHK(c, launch_jacobi_sweep(c->stream, d, buf[k & 1], buf[(k + 1) & 1], rhs, dx2, dy2,
+                                      dz2, denom, c->st));
+        HIPCHK(c, launch_pressure_bcs(c->stream, d, buf[(k + 1) & 1], c->solid, c->st, k));
+        return LES_OK;
+    });
+    if (rc != LES_OK) return rc;
+    const int sw = c->st_host[0].sweeps;
+    if (sw & 1)
+        HIPCHK(c, hipMemcpyAsync(c->f[LES_F_P], c->t2, nbytes(c, 8), hipMemcpyDeviceToDevice,
+                                 c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (sweeps_out) *sweeps_out = sw;
+    return LES_OK;
+}
+
+int les_sor_solve(les_ctx* c, int ordering, double dt, double rho, double omega, int max_sweeps,
+                  int* sweeps_out)
+{
+    if (!c) return LES_ERR_ARG;
+    if (ordering != LES_SOLVER_SOR_LEX && ordering != LES_SOLVER_SOR_RB)
+        return fail(c, LES_ERR_ARG, "ordering must be LES_SOLVER_SOR_LEX or LES_SOLVER_SOR_RB");
+    int rc;
+    if ((rc = ensure_krylov(c)) != LES_OK) return rc;
+    const Dims& d = c->d;
+    // pressure.cpp:193-210
+    const double dx2 = c->dx * c->dx, dy2 = c->dy * c->dy, dz2 = c->dz * c->dz;
+    const double coef = 1.0 / (2.0 * (1.0 / dx2 + 1.0 / dy2 + 1.0 / dz2));
+    double* rhs = c->t1;
+    HIPCHK(c, launch_rhs_central(c->stream, d, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W], rhs,
+                                 rho / dt, 2 * c->dx, 2 * c->dy, 2 * c->dz));
+    const double omc = 1 - omega, oc = omega * coef;
+    double* p = c->f[LES_F_P];
+    if ((rc = stat_begin(c, max_sweeps)) != LES_OK) return rc;
+    const int lmax = (d.nx - 2) + (d.ny - 2) + (d.nz - 2);
+    rc = stat_loop(c, max_sweeps, [&](int k) -> int {
+        if (ordering == LES_SOLVER_SOR_LEX) {
+            for (int l = 3; l <= lmax; ++l)
+                HIPCHK(c, launch_sor_level(c->stream, d, l, p, rhs, omc, oc, dx2, dy2, dz2, c->st));
+        } else {
+            for (int colour = 0; colour < 2; ++colour)
+                HIPCHK(c, launch_sor_colour(c->stream, d, p, rhs, colour, omc, oc, dx2, dy2, dz2,
+                                            c->st));
+        }
+        HIPCHK(c, launch_pressure_bcs(c->stream, d, p, c->solid, c->st, k));
+        return LES_OK;
+    });
+    if (rc != LES_OK) return rc;
+    if (sweeps_out) *sweeps_out = c->st_host[0].sweeps;
+    return LES_OK;
+}
+
+// ---------------- physics ----------------
+int les_sgs(les_ctx* c, double Cs, int apply_bcs, double* const tau[6], int ptr_kind)
+{
+    if (!c) return LES_ERR_ARG;
+    Tau t{};
+    std::vector<double*> tmp;
+    if (tau) {
+        for (int q = 0; q < 6; ++q) {
+            if (!tau[q]) return fail(c, LES_ERR_ARG, "tau array is NULL");
+            if (ptr_kind) {
+                t.t[q] = tau[q];
+            } else {
+                double* dptr = nullptr;
+                int rc = alloc(c, &dptr, c->d.N);
+                if (rc != LES_OK) {
+                    for (double* p : tmp) hipFree(p);
+                    return rc;
+                }
+                HIPCHK(c, hipMemcpyAsync(dptr, tau[q], nbytes(c, 8), hipMemcpyHostToDevice, c->stream));
+                tmp.push_back(dptr);
+                t.t[q] = dptr;
+            }
+        }
+    }
+    const PhysConsts k = make_consts(c, 1.0, 0.0, 1.0, Cs);
+    HIPCHK(c, launch_sgs(c->stream, c->d, k, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W], c->solid,
+                         c->f[LES_F_NU_T], t, apply_bcs, c->red, nullptr));
+    if (tau && !ptr_kind) {
+        for (int q = 0; q < 6; ++q)
+            HIPCHK(c, hipMemcpyAsync(tau[q], tmp[q], nbytes(c, 8), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        for (double* p : tmp) hipFree(p);
+    }
+    return LES_OK;
+}
+
+int les_nut_bcs(les_ctx* c)
+{
+    if (!c) return LES_ERR_ARG;
+    HIPCHK(c, launch_nut_bcs(c->stream, c->d, c->solid, c->f[LES_F_NU_T]));
+    return LES_OK;
+}
+
+static void swap_velocity(les_ctx* c)
+{
+    std::swap(c->f[LES_F_U], c->f[LES_F_U_NEW]);
+    std::swap(c->f[LES_F_V], c->f[LES_F_V_NEW]);
+    std::swap(c->f[LES_F_W], c->f[LES_F_W_NEW]);
+}
+
+int les_momentum(les_ctx* c, double dt, double nu, int apply_bcs)
+{
+    if (!c) return LES_ERR_ARG;
+    const PhysConsts k = make_consts(c, dt, nu, 1.0, 0.18);
+    if (apply_bcs) {
+        // provisional velocity with BCs; the lagged divergence is not wanted here
+        HIPCHK(c, launch_momentum(c->stream, c->d, k, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W],
+                                  c->f[LES_F_NU_T], c->solid, c->f[LES_F_U_NEW], c->f[LES_F_V_NEW],
+                                  c->f[LES_F_W_NEW], nullptr, nullptr, MOM_STEP));
+    } else {
+        HIPCHK(c, launch_momentum(c->stream, c->d, k, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W],
+                                  c->f[LES_F_NU_T], c->solid, c->f[LES_F_U_NEW], c->f[LES_F_V_NEW],
+                                  c->f[LES_F_W_NEW], nullptr, nullptr, MOM_RAW));
+    }
+    swap_velocity(c);
+    return LES_OK;
+}
+
+int les_velocity_bcs(les_ctx* c)
+{
+    if (!c) return LES_ERR_ARG;
+    HIPCHK(c, launch_velocity_bcs(c->stream, c->d, c->solid, c->f[LES_F_U], c->f[LES_F_V],
+                                  c->f[LES_F_W]));
+    return LES_OK;
+}
+
+int les_divergence_b(les_ctx* c, double dt, double rho)
+{
+    if (!c) return LES_ERR_ARG;
+    const PhysConsts k = make_consts(c, dt, 0.0, rho, 0.18);
+    HIPCHK(c, launch_momentum(c->stream, c->d, k, c->f[LES_F_U_NEW], c->f[LES_F_V_NEW],
+                              c->f[LES_F_W_NEW], c->f[LES_F_NU_T], c->solid, nullptr, nullptr,
+                              nullptr, c->f[LES_F_P_RHS], c->f[LES_F_B], MOM_DIV_ONLY));
+    return LES_OK;
+}
+
+int les_correct(les_ctx* c, double dt, double rho, int apply_bcs)
+{
+    if (!c) return LES_ERR_ARG;
+    const PhysConsts k = make_consts(c, dt, 0.0, rho, 0.18);
+    HIPCHK(c, launch_correct(c->stream, c->d, k, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W],
+                             c->f[LES_F_P], c->solid, apply_bcs));
+    return LES_OK;
+}
+
+static double dt_from_max(const les_ctx* c, double nu, double umax2, double ntmax)
+{
+    // compute_adaptive_dt (main.cpp:227-249)
+    const double CFL_CONV = 0.4, CFL_VISC = 0.2, DT_MAX = 1e-4;
+    const double u_max = std::sqrt(umax2);
+    const double min_dx = std::min({c->dx, c->dy, c->dz});
+    double dt_conv = std::numeric_limits<double>::max();
+    if (u_max > 1e-12) dt_conv = CFL_CONV * min_dx / u_max;
+    const double min_dx_sq = min_dx * min_dx;
+    const double dt_visc = CFL_VISC * min_dx_sq / (nu + ntmax);
+    return std::min({dt_conv, dt_visc, DT_MAX});
+}
+
+int les_adaptive_dt(les_ctx* c, double nu, double* dt_out)
+{
+    if (!c || !dt_out) return fail(c, LES_ERR_ARG, "NULL argument");
+    HIPCHK(c, launch_dt_max(c->stream, c->d, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W],
+                            c->f[LES_F_NU_T], c->red, c->dsc));
+    HIPCHK(c, hipMemcpyAsync(c->hsc, c->dsc, 16, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *dt_out = dt_from_max(c, nu, c->hsc[0], c->hsc[1]);
+    return LES_OK;
+}
+
+// ---------------- one step (main.cpp:115-144) ----------------
+int les_step(les_ctx* c, const les_step_params* prm, les_step_stats* stats)
+{
+    if (!c || !prm) return fail(c, LES_ERR_ARG, "NULL argument");
+    les_step_stats st{};
+    int rc;
+    double dt;
+    if (prm->dt_fixed > 0) dt = prm->dt_fixed;
+    else if (prm->step <= 20) dt = 1e-3;
+    else if ((rc = les_adaptive_dt(c, prm->nu, &dt)) != LES_OK) return rc;
+    st.dt = dt;
+    const PhysConsts k = make_consts(c, dt, prm->nu, prm->rho, prm->Cs);
+    hipStream_t s = c->stream;
+    // compute_sgs_stress + apply_nut_BCs
+    HIPCHK(c, launch_sgs(s, c->d, k, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W], c->solid,
+                         c->f[LES_F_NU_T], Tau{}, 1, c->red, nullptr));
+    const bool pcg = prm->solver == LES_SOLVER_PCG;
+    // solve_momentum + apply_velocity_BCs, and (PCG path) calculate_divergence
+    // + build_b_vector of the step's starting velocity
+    KtScope* ktm = new KtScope(c, LES_KT_MOMENTUM, 8);
+    HIPCHK(c, launch_momentum(s, c->d, k, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W],
+                              c->f[LES_F_NU_T], c->solid, c->f[LES_F_U_NEW], c->f[LES_F_V_NEW],
+                              c->f[LES_F_W_NEW], pcg ? c->f[LES_F_P_RHS] : nullptr,
+                              pcg ? c->f[LES_F_B] : nullptr, MOM_STEP));
+    delete ktm;
+    swap_velocity(c);
+    switch (prm->solver) {
+    case LES_SOLVER_PCG:
+        if ((rc = les_pcg(c, &prm->pcg, &st.pcg)) != LES_OK) return rc;
+        break;
+    case LES_SOLVER_SOR_LEX:
+    case LES_SOLVER_SOR_RB:
+        if ((rc = les_sor_solve(c, prm->solver, dt, prm->rho, prm->omega, prm->sor_sweeps,
+                                &st.sweeps)) != LES_OK)
+            return rc;
+        break;
+    case LES_SOLVER_JACOBI:
+        if ((rc = les_jacobi_solve(c, dt, prm->rho, &st.sweeps)) != LES_OK) return rc;
+        break;
+    default:
+        return fail(c, LES_ERR_ARG, "unknown solver");
+    }
+    // correct_velocity + apply_velocity_BCs
+    HIPCHK(c, launch_correct(s, c->d, k, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W],
+                             c->f[LES_F_P], c->solid, 1));
+    if (stats) *stats = st;
+    return LES_OK;
+}
+
+
+int les_kernel_timing(les_ctx* c, int kernel_class)
+{
+    if (!c) return LES_ERR_ARG;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->kt_class = kernel_class;
+    c->kt_used = 0;
+    c->kt_elem = 0;
+    return LES_OK;
+}
+
+int les_kernel_timing_read(les_ctx* c, int* launches, double* total_ms, int* elem_bytes)
+{
+    if (!c || !launches || !total_ms) return fail(c, LES_ERR_ARG, "NULL argument");
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    double tot = 0.0;
+    for (size_t q = 0; q + 1 < c->kt_used; q += 2) {
+        float ms = 0.f;
+        HIPCHK(c, hipEventElapsedTime(&ms, c->kt_events[q], c->kt_events[q + 1]));
+        tot += ms;
+    }
+    *launches = (int)(c->kt_used / 2);
+    *total_ms = tot;
+    if (elem_bytes) *elem_bytes = c->kt_elem;
+    return LES_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,158 @@
+// les_kernels.hpp -- host-callable launchers of the LES HIP kernels.
+//
+// Types here are shared by the kernel translation units and by the C-ABI
+// layer (les_api.hip).  Every launcher enqueues on the given stream and
+// returns the hipError_t of the launch.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "les_device.hpp"
+
+namespace les {
+
+// Seven stencil coefficient arrays in the member order of StencilMatrix
+// (src/pressure.h:22): c, n, s, e, w, t, b.
+enum { SC = 0, SN = 1, SS = 2, SE = 3, SW = 4, ST = 5, SB = 6 };
+
+template <typename T>
+struct Stencil {
+    const T* a[7];
+};
+
+// Scalars of one Krylov solve, resident in device memory so that the
+// iteration never needs the host.
+struct SolveScalars {
+    double rho;      // r.z of the current iterate
+    double alpha;
+    double beta;
+    double rr;       // ||r||^2 of the latest residual
+    double bb;       // ||b||^2
+    double tol;      // absolute stopping tolerance on ||r||
+    double tol_rel;  // > 0: tol = tol_rel * ||b|| (set by the init kernel)
+    double err;      // ||r|| of the latest residual
+    double err0;     // ||r|| at iteration 0
+    double pq;
+    int iter;        // iterations completed
+    int done;        // 1 once converged or out of iterations
+    int converged;
+    int max_iter;
+    int fault;       // 1 if a non-finite scalar was met
+    // mixed precision (FP32 iteration, FP64 reliable residual replacement)
+    int reliable_every;  // replace r by the FP64 residual every this many iterations
+    int reliable_now;    // 1: fold x32 into x64 and replace r in this iteration
+    int stop_pending;    // 1: stop after the pending replacement (max_iter reached)
+    int replacements;
+    double true_err;     // ||b - A x64|| of the last replacement
+    double* trace;       // optional: ||r_k|| for k = 0..max_iter
+};
+
+// Scratch used by every fused reduction kernel.
+struct RedScratch {
+    double* partials;      // kMaxGrid * kMaxRed
+    unsigned int* counter; // one counter (zero between kernels)
+};
+
+// ---- scalar epilogues run by the last workgroup of a reduction ----
+enum Epilogue {
+    EPI_STORE = 0,       // out[r] = tot[r]
+    EPI_INIT = 1,        // tot = {rr, bb}: err0, tol, done if err0 < tol
+    EPI_RHO = 2,         // tot = {rz}: rho = rz
+    EPI_ALPHA = 3,       // tot = {pq}: alpha = rho / pq
+    EPI_CONV = 4,        // tot = {rr}: iter++, err, done if err < tol or iter == max_iter
+    EPI_CONV_RZ = 5,     // tot = {rr, rz}: EPI_CONV, then beta = rz/rho, rho = rz
+    EPI_BETA = 6,        // tot = {rz}: beta = rz/rho, rho = rz
+    EPI_CONV_MIXED = 7,  // tot = {rr}: iter++, err; schedule an FP64 replacement when
+                         // err < tol, iter % reliable_every == 0 or iter == max_iter
+    EPI_RELIABLE = 8,    // tot = {rr64}: true residual; done if below tol
+    EPI_INIT_MIXED = 9,  // tot = {rr64, bb}: as EPI_INIT, also true_err
+};
+
+// ---- operator ----
+hipError_t launch_build_stencil(hipStream_t s, const Dims& d, const uint8_t* solid,
+                                double* const A[7], double idx2, double idy2, double idz2);
+template <typename T>
+hipError_t launch_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T* x, T* y);
+template <typename TO, typename TI>
+hipError_t launch_convert(hipStream_t s, int n, const TI* in, TO* out);
+hipError_t launch_dot(hipStream_t s, const Dims& d, const double* a, const double* b,
+                      RedScratch red, double* out);
+
+// ---- PCG building blocks (pressure.cpp:342-414), fused ----
+// r = b - A x; ||r||^2, ||b||^2  (pressure.cpp:355-359)
+template <typename T>
+hipError_t launch_pcg_init(hipStream_t s, const Dims& d, Stencil<T> A, const T* b, const T* x,
+                           T* r, RedScratch red, SolveScalars* sc);
+// q = A p; alpha = rho / (p.q)  (pressure.cpp:375-378)
+template <typename T>
+hipError_t launch_pcg_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T* p, T* q,
+                             RedScratch red, SolveScalars* sc);
+// x += alpha p; r -= alpha q; ||r|| test (pressure.cpp:381-392).  With
+// jacobi_diag != nullptr also r.(r/A_c) and beta (pressure.cpp:396-402).
+template <typename T, typename TX>
+hipError_t launch_pcg_update(hipStream_t s, const Dims& d, TX* x, T* r, const T* p, const T* q,
+                             const T* jacobi_diag, RedScratch red, SolveScalars* sc);
+// p = M^-1 r + beta p for Jacobi (beta ignored and p overwritten when init);
+// init also reduces rho = r.p (pressure.cpp:363-369, 395-405).
+// When init, z = r / A_c is written into p and r.z reduced with epilogue
+// epi (< 0: no reduction; sc may be nullptr then).
+template <typename T>
+hipError_t launch_pcg_p_jacobi(hipStream_t s, const Dims& d, const T* r, const T* diag, T* p,
+                               int init, int epi, RedScratch red, SolveScalars* sc);
+// p = z + beta p (pressure.cpp:405)
+template <typename T>
+hipError_t launch_pcg_p_update(hipStream_t s, const Dims& d, const T* z, T* p, SolveScalars* sc);
+// rho / beta from r.z (pressure.cpp:369, 399-402)
+template <typename T>
+hipError_t launch_pcg_rz(hipStream_t s, const Dims& d, const T* r, const T* z, int epilogue,
+                         RedScratch red, SolveScalars* sc);
+
+// ---- Chebyshev preconditioner (pressure.cpp:851-906) ----
+// One inner step k >= 1 of the recurrence:
+//   res = r - A z_k ; p = res + beta p ; z_{k+1} = z_k + alpha p
+// first: z_1 = 0 + alpha0 r and p_0 = r are formed on the fly from r.
+// degree 1: z = 0 + alpha0 r only (launch_cheb_scale).
+// When epi >= 0 the kernel also reduces r.z_{k+1} with that epilogue.
+template <typename T>
+hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
+                            const T* z_in, T* z_out, T* p, double alpha0, double beta,
+                            double alpha, int first, int epi, RedScratch red, SolveScalars* sc);
+template <typename T>
+hipError_t launch_cheb_scale(hipStream_t s, const Dims& d, const T* r, T* z, double alpha0,
+                             int epi, RedScratch red, SolveScalars* sc);
+
+// ---- IC(0) (pressure.cpp:442-556), i+j+k hyperplane order ----
+hipError_t launch_ic_factor_level(hipStream_t s, const Dims& d, int level, Stencil<double> A,
+                                  double* Lc, double* Lw, double* Ls, double* Lb);
+template <typename T>
+hipError_t launch_ic_fwd_level(hipStream_t s, const Dims& d, int level, const T* r, T* z,
+                               const T* Lc, const T* Lw, const T* Ls, const T* Lb,
+                               const SolveScalars* sc);
+template <typename T>
+hipError_t launch_ic_bwd_level(hipStream_t s, const Dims& d, int level, T* z, const T* Lc,
+                               const T* Lw, const T* Ls, const T* Lb, const SolveScalars* sc);
+
+// ---- mixed precision (no reference counterpart) ----
+// x64 += x32; x32 = 0  (when force or sc->reliable_now)
+hipError_t launch_mixed_fold(hipStream_t s, const Dims& d, double* x64, float* x32, int force,
+                             SolveScalars* sc);
+// r64 = b - A64 x64 -> r32 = (float) r64, ||r64||^2 (and ||b||^2 when init);
+// runs when force, init or sc->reliable_now.  Epilogue EPI_INIT_MIXED when
+// init, else EPI_RELIABLE (force && !init: EPI_STORE of ||r64|| into out).
+hipError_t launch_mixed_residual(hipStream_t s, const Dims& d, Stencil<double> A, const double* b,
+                                 const double* x64, float* r32, int init, int force,
+                                 RedScratch red, SolveScalars* sc, double* out);
+// FP32 update with the mixed convergence epilogue
+hipError_t launch_pcg_update_mixed(hipStream_t s, const Dims& d, float* x, float* r, const float* p,
+                                   const float* q, RedScratch red, SolveScalars* sc);
+
+// ---- power iteration helpers (pressure.cpp:917-955) ----
+// v = w / sqrt(tot) where tot = ||w||^2 is read from device memory
+hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double* v,
+                                   const double* nrm2);
+// w = w - a v - b vp
+hipError_t launch_lanczos_axpy(hipStream_t s, int n, double* w, const double* v, const double* vp,
+                               double a, double b);
+
+}  // namespace les

@@ -1,0 +1,20 @@
+// les_mixed.hpp -- FP32 workspaces of the mixed-precision PCG.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace les {
+
+struct MixedWork {
+    float *r = nullptr, *p = nullptr, *q = nullptr, *z = nullptr, *za = nullptr, *zb = nullptr,
+          *pc = nullptr, *x = nullptr;
+};
+
+inline void free_mixed(MixedWork& w)
+{
+    for (float* p : {w.r, w.p, w.q, w.z, w.za, w.zb, w.pc, w.x})
+        if (p) hipFree(p);
+    w = MixedWork{};
+}
+
+}  // namespace les

@@ -1,0 +1,622 @@
+// les_physics.hip -- per-step physics kernels for gfx950: Smagorinsky nu_t,
+// explicit momentum fused with the divergence/RHS of the lagged projection,
+// the velocity correction, boundary conditions, the stationary pressure
+// solvers and the adaptive time-step reduction.
+//
+// Per-cell operand order follows src/main.cpp and src/pressure.cpp so FP64
+// results are bit-identical to the reference when built with
+// -ffp-contract=off.  Boundary conditions that the reference applies as
+// separate ordered passes are folded into the producing kernel as the final
+// value each boundary cell ends up with after all of those passes.
+#include "les_physics.hpp"
+
+#include <math.h>
+
+namespace les {
+
+namespace {
+
+#define FOR_CELLS(d, ch, lb)                                                         \
+    for (int id = (lb) * (ch).chunk + (int)threadIdx.x,                              \
+             id_end_ = min((d).N, ((lb) + 1) * (ch).chunk);                          \
+         id < id_end_; id += kBlock)
+
+__device__ __forceinline__ bool on_wall(const Dims& d, int j, int k)
+{
+    return j == 0 || j == d.ny - 1 || k == 0 || k == d.nz - 1;
+}
+
+// nu_t at an interior cell (main.cpp:436-461).  Gradients divide by 2*dx.
+__device__ __forceinline__ double sgs_at(const Dims& d, const PhysConsts& c, const double* u,
+                                         const double* v, const double* w, int id, double* S)
+{
+    const int si = d.sij, sj = d.nz;
+    const double dudx = (u[id + si] - u[id - si]) / c.two_dx;
+    const double dudy = (u[id + sj] - u[id - sj]) / c.two_dy;
+    const double dudz = (u[id + 1] - u[id - 1]) / c.two_dz;
+    const double dvdx = (v[id + si] - v[id - si]) / c.two_dx;
+    const double dvdy = (v[id + sj] - v[id - sj]) / c.two_dy;
+    const double dvdz = (v[id + 1] - v[id - 1]) / c.two_dz;
+    const double dwdx = (w[id + si] - w[id - si]) / c.two_dx;
+    const double dwdy = (w[id + sj] - w[id - sj]) / c.two_dy;
+    const double dwdz = (w[id + 1] - w[id - 1]) / c.two_dz;
+    const double Sxx = dudx, Syy = dvdy, Szz = dwdz;
+    const double Sxy = 0.5 * (dudy + dvdx);
+    const double Sxz = 0.5 * (dudz + dwdx);
+    const double Syz = 0.5 * (dvdz + dwdy);
+    const double Smag = sqrt(2.0 * (Sxx * Sxx + Syy * Syy + Szz * Szz) +
+                             4.0 * (Sxy * Sxy + Sxz * Sxz + Syz * Syz));
+    if (S) {
+        S[0] = Sxx; S[1] = Sxy; S[2] = Sxz; S[3] = Syy; S[4] = Syz; S[5] = Szz;
+    }
+    return c.csd2 * Smag;
+}
+
+// compute_sgs_stress + apply_nut_BCs (main.cpp:423-547) in one pass.
+// Final nu_t: walls and inlet 0; outlet = raw interior value at i = nx-2;
+// interior solids 0; other interior cells the Smagorinsky value.  tau (the
+// six stresses the reference writes and never reads) is optional.
+__global__ __launch_bounds__(kBlock) void k_sgs(Dims d, Chunking ch, PhysConsts c, const double* u,
+                                                const double* v, const double* w,
+                                                const uint8_t* solid, double* nu_t, Tau tau,
+                                                int apply_bcs, RedScratch red, double* nut_max)
+{
+    const int lb = logical_block(ch.grid);
+    double mx[1] = {0.0};
+    FOR_CELLS(d, ch, lb)
+    {
+        int i, j, k;
+        decode(d, id, i, j, k);
+        double out;
+        if (!apply_bcs) {
+            // compute_sgs_stress alone: interior cells, raw values
+            if (on_wall(d, j, k) || i == 0 || i == d.nx - 1) continue;
+            double S[6];
+            const double nt = sgs_at(d, c, u, v, w, id, tau.t[0] ? S : nullptr);
+            if (tau.t[0]) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) tau.t[q][id] = -2.0 * nt * S[q];
+            }
+            nu_t[id] = nt;
+            continue;
+        }
+        if (on_wall(d, j, k) || i == 0) {
+            out = 0.0;
+        } else if (i == d.nx - 1) {
+            out = sgs_at(d, c, u, v, w, id - d.sij, nullptr);
+        } else {
+            double S[6];
+            const double nt = sgs_at(d, c, u, v, w, id, tau.t[0] ? S : nullptr);
+            if (tau.t[0]) {
+#pragma unroll
+                for (int q = 0; q < 6; ++q) tau.t[q][id] = -2.0 * nt * S[q];
+            }
+            out = solid[id] ? 0.0 : nt;
+        }
+        nu_t[id] = out;
+        mx[0] = mx[0] > out ? mx[0] : out;
+    }
+    if (nut_max) {
+        block_max<1>(mx);
+        double tot[1];
+        if (grid_reduce<1, true>(mx, red.partials, red.counter, ch.grid, lb, tot) &&
+            threadIdx.x == 0)
+            nut_max[0] = tot[0];
+    }
+}
+
+// one component of solve_momentum at an interior cell (main.cpp:584-600)
+__device__ __forceinline__ double mom_at(const Dims& d, const PhysConsts& c, const double* f,
+                                         const double* u, const double* v, const double* w,
+                                         double nu_eff, int id)
+{
+    const int si = d.sij, sj = d.nz;
+    const double fc = f[id];
+    const double fe = f[id + si], fw = f[id - si], fn = f[id + sj], fs = f[id - sj];
+    const double ft = f[id + 1], fb = f[id - 1];
+    const double ddx = (fe - fw) * c.inv_2dx;
+    const double ddy = (fn - fs) * c.inv_2dy;
+    const double ddz = (ft - fb) * c.inv_2dz;
+    const double adv = u[id] * ddx + v[id] * ddy + w[id] * ddz;
+    const double lap = (fe - 2 * fc + fw) * c.inv_dx2 + (fn - 2 * fc + fs) * c.inv_dy2 +
+                       (ft - 2 * fc + fb) * c.inv_dz2;
+    return fc + c.dt * (-adv + nu_eff * lap);
+}
+
+// solve_momentum + apply_velocity_BCs (main.cpp:563-659, 323-410) writing
+// the provisional velocity into (un, vn, wn), plus calculate_divergence and
+// build_b_vector (pressure.cpp:256-329, 566-599) of the step's starting
+// velocity (u, v, w) -- the reference's lagged RHS reads exactly that field
+// (Field.u_new after its buffer swap).
+// Final provisional velocity: walls 0; inlet (1,0,0) except solid inlet
+// cells, which keep the buffer's previous content; outlet = raw momentum
+// value at i = nx-2; interior solids 0.
+__global__ __launch_bounds__(kBlock) void k_momentum(Dims d, Chunking ch, PhysConsts c,
+                                                     const double* u, const double* v,
+                                                     const double* w, const double* nu_t,
+                                                     const uint8_t* solid, double* un, double* vn,
+                                                     double* wn, double* p_rhs, double* b,
+                                                     int mode)
+{
+    const int lb = logical_block(ch.grid);
+    FOR_CELLS(d, ch, lb)
+    {
+        int i, j, k;
+        decode(d, id, i, j, k);
+        const bool sol = solid[id];
+        // ---- provisional velocity ----
+        if (mode == MOM_RAW) {
+            // solve_momentum alone: interior cells, no boundary conditions
+            if (!(on_wall(d, j, k) || i == 0 || i == d.nx - 1)) {
+                const double nu_eff = c.nu + nu_t[id];
+                un[id] = mom_at(d, c, u, u, v, w, nu_eff, id);
+                vn[id] = mom_at(d, c, v, u, v, w, nu_eff, id);
+                wn[id] = mom_at(d, c, w, u, v, w, nu_eff, id);
+            }
+            continue;
+        }
+        if (mode == MOM_DIV_ONLY) {
+            // fall through to the divergence only
+        } else if (on_wall(d, j, k)) {
+            un[id] = 0.0; vn[id] = 0.0; wn[id] = 0.0;
+        } else if (i == 0) {
+            if (!sol) { un[id] = 1.0; vn[id] = 0.0; wn[id] = 0.0; }
+        } else {
+            const int src = (i == d.nx - 1) ? id - d.sij : id;
+            if (i != d.nx - 1 && sol) {
+                un[id] = 0.0; vn[id] = 0.0; wn[id] = 0.0;
+            } else {
+                const double nu_eff = c.nu + nu_t[src];
+                un[id] = mom_at(d, c, u, u, v, w, nu_eff, src);
+                vn[id] = mom_at(d, c, v, u, v, w, nu_eff, src);
+                wn[id] = mom_at(d, c, w, u, v, w, nu_eff, src);
+            }
+        }
+        // ---- lagged divergence and b ----
+        double rhs = 0.0, bv = 0.0;
+        if (!sol && i != d.nx - 1) {
+            double dudx, dvdy, dwdz;
+            if (i == 0) dudx = (u[id + d.sij] - u[id]) * c.div_ix1;
+            else dudx = (u[id + d.sij] - u[id - d.sij]) * c.div_ix;
+            if (j == 0) dvdy = (v[id + d.nz] - v[id]) * c.div_iy1;
+            else if (j == d.ny - 1) dvdy = (v[id] - v[id - d.nz]) * c.div_iy1;
+            else dvdy = (v[id + d.nz] - v[id - d.nz]) * c.div_iy;
+            if (k == 0) dwdz = (w[id + 1] - w[id]) * c.div_iz1;
+            else if (k == d.nz - 1) dwdz = (w[id] - w[id - 1]) * c.div_iz1;
+            else dwdz = (w[id + 1] - w[id - 1]) * c.div_iz;
+            rhs = c.div_scale * (dudx + dvdy + dwdz);
+            bv = -rhs;
+        }
+        if (p_rhs) p_rhs[id] = rhs;
+        if (b) b[id] = bv;
+    }
+}
+
+// correct_velocity (main.cpp:671-714) on interior cells, with the interior
+// solid zeroing of apply_velocity_BCs folded in for i <= nx-3 (solids at
+// i = nx-2 are zeroed by k_vel_faces after the outlet copy reads them).
+__global__ __launch_bounds__(kBlock) void k_correct(Dims d, Chunking ch, PhysConsts c, double* u,
+                                                    double* v, double* w, const double* p,
+                                                    const uint8_t* solid, int apply_bcs)
+{
+    const int lb = logical_block(ch.grid);
+    FOR_CELLS(d, ch, lb)
+    {
+        int i, j, k;
+        decode(d, id, i, j, k);
+        if (i == 0 || i == d.nx - 1 || on_wall(d, j, k)) continue;
+        if (apply_bcs && solid[id] && i <= d.nx - 3) {
+            u[id] = 0.0; v[id] = 0.0; w[id] = 0.0;
+            continue;
+        }
+        u[id] -= c.dor * ((p[id + d.sij] - p[id - d.sij]) * c.inv_2dx);
+        v[id] -= c.dor * ((p[id + d.nz] - p[id - d.nz]) * c.inv_2dy);
+        w[id] -= c.dor * ((p[id + 1] - p[id - 1]) * c.inv_2dz);
+    }
+}
+
+// zero the given arrays at interior solid cells with 1 <= i <= nx-3
+__global__ __launch_bounds__(kBlock) void k_zero_solids(Dims d, Chunking ch, const uint8_t* solid,
+                                                        double* a0, double* a1, double* a2)
+{
+    const int lb = logical_block(ch.grid);
+    FOR_CELLS(d, ch, lb)
+    {
+        if (!solid[id]) continue;
+        int i, j, k;
+        decode(d, id, i, j, k);
+        if (i == 0 || i > d.nx - 3 || on_wall(d, j, k)) continue;
+        a0[id] = 0.0;
+        if (a1) a1[id] = 0.0;
+        if (a2) a2[id] = 0.0;
+    }
+}
+
+// Boundary cells, each owned by exactly one thread: set X (i = 0 or nx-1,
+// all j,k), set Y (j = 0 or ny-1, 1 <= i <= nx-2), set Z (k = 0 or nz-1,
+// 1 <= i <= nx-2, 1 <= j <= ny-2).
+__device__ __forceinline__ bool face_cell(const Dims& d, int t, int& i, int& j, int& k)
+{
+    const int nX = 2 * d.ny * d.nz;
+    const int nY = 2 * (d.nx - 2) * d.nz;
+    const int nZ = 2 * (d.nx - 2) * (d.ny - 2);
+    if (t < nX) {
+        const int f = t / (d.ny * d.nz), r = t - f * (d.ny * d.nz);
+        i = f ? d.nx - 1 : 0;
+        j = r / d.nz;
+        k = r - j * d.nz;
+        return true;
+    }
+    t -= nX;
+    if (t < nY) {
+        const int f = t / ((d.nx - 2) * d.nz), r = t - f * ((d.nx - 2) * d.nz);
+        j = f ? d.ny - 1 : 0;
+        i = 1 + r / d.nz;
+        k = r - (i - 1) * d.nz;
+        return true;
+    }
+    t -= nY;
+    if (t < nZ) {
+        const int f = t / ((d.nx - 2) * (d.ny - 2)), r = t - f * ((d.nx - 2) * (d.ny - 2));
+        k = f ? d.nz - 1 : 0;
+        i = 1 + r / (d.ny - 2);
+        j = 1 + r - (i - 1) * (d.ny - 2);
+        return true;
+    }
+    return false;
+}
+
+__host__ __device__ inline int face_count(const Dims& d)
+{
+    return 2 * d.ny * d.nz + 2 * (d.nx - 2) * d.nz + 2 * (d.nx - 2) * (d.ny - 2);
+}
+
+// Final boundary values of apply_velocity_BCs (main.cpp:323-410) /
+// apply_nut_BCs (main.cpp:483-547).  For the outlet the thread also zeroes
+// the solid cell at i = nx-2 after reading it.
+template <bool VEL>
+__global__ __launch_bounds__(kBlock) void k_faces(Dims d, const uint8_t* solid, double* a0,
+                                                  double* a1, double* a2)
+{
+    int i, j, k;
+    if (!face_cell(d, blockIdx.x * kBlock + threadIdx.x, i, j, k)) return;
+    const int id = i * d.sij + j * d.nz + k;
+    if (on_wall(d, j, k)) {
+        a0[id] = 0.0;
+        if (VEL) { a1[id] = 0.0; a2[id] = 0.0; }
+    } else if (i == 0) {
+        if (VEL) {
+            if (!solid[id]) { a0[id] = 1.0; a1[id] = 0.0; a2[id] = 0.0; }
+        } else {
+            a0[id] = 0.0;
+        }
+    } else {
+        const int src = id - d.sij;
+        a0[id] = a0[src];
+        if (VEL) { a1[id] = a1[src]; a2[id] = a2[src]; }
+        if (solid[src]) {
+            a0[src] = 0.0;
+            if (VEL) { a1[src] = 0.0; a2[src] = 0.0; }
+        }
+    }
+}
+
+// max(u^2+v^2+w^2) and max nu_t over all cells (main.cpp:215-245)
+__global__ __launch_bounds__(kBlock) void k_dt_max(Dims d, Chunking ch, const double* u,
+                                                   const double* v, const double* w,
+                                                   const double* nu_t, RedScratch red,
+                                                   double* out)
+{
+    const int lb = logical_block(ch.grid);
+    double mx[2] = {0.0, 0.0};
+    FOR_CELLS(d, ch, lb)
+    {
+        const double s = u[id] * u[id] + v[id] * v[id] + w[id] * w[id];
+        mx[0] = mx[0] > s ? mx[0] : s;
+        mx[1] = mx[1] > nu_t[id] ? mx[1] : nu_t[id];
+    }
+    block_max<2>(mx);
+    double tot[2];
+    if (grid_reduce<2, true>(mx, red.partials, red.counter, ch.grid, lb, tot) && threadIdx.x == 0) {
+        out[0] = tot[0];
+        out[1] = tot[1];
+    }
+}
+
+// ------------------------------------------------------------------
+// apply_pressure_BCs (pressure.cpp:14-93).  The ordered face passes (inlet
+// copy, outlet 0, Y walls, Z walls) leave each boundary cell equal to the
+// original value at (i', clamp(j,1,ny-2), clamp(k,1,nz-2)) with i' = 1 at
+// the inlet, or 0 at the outlet; the solid pass then copies the first fluid
+// neighbour in the order +i,-i,+j,-j,+k,-k.
+// ------------------------------------------------------------------
+__global__ __launch_bounds__(kBlock) void k_pbc_faces(Dims d, double* p, const StatScalars* st)
+{
+    if (st && st->done) return;
+    int i, j, k;
+    if (!face_cell(d, blockIdx.x * kBlock + threadIdx.x, i, j, k)) return;
+    const int id = i * d.sij + j * d.nz + k;
+    if (i == d.nx - 1) {
+        p[id] = 0.0;
+        return;
+    }
+    const int i2 = (i == 0) ? 1 : i;
+    const int j2 = j < 1 ? 1 : (j > d.ny - 2 ? d.ny - 2 : j);
+    const int k2 = k < 1 ? 1 : (k > d.nz - 2 ? d.nz - 2 : k);
+    p[id] = p[i2 * d.sij + j2 * d.nz + k2];
+}
+
+__global__ __launch_bounds__(kBlock) void k_pbc_solids(Dims d, Chunking ch, double* p,
+                                                       const uint8_t* solid, StatScalars* st,
+                                                       int sweep)
+{
+    if (st && st->done) return;
+    const int lb = logical_block(ch.grid);
+    FOR_CELLS(d, ch, lb)
+    {
+        if (!solid[id]) continue;
+        int i, j, k;
+        decode(d, id, i, j, k);
+        if (i == 0 || i == d.nx - 1 || on_wall(d, j, k)) continue;
+        if (!solid[id + d.sij]) p[id] = p[id + d.sij];
+        else if (!solid[id - d.sij]) p[id] = p[id - d.sij];
+        else if (!solid[id + d.nz]) p[id] = p[id + d.nz];
+        else if (!solid[id - d.nz]) p[id] = p[id - d.nz];
+        else if (!solid[id + 1]) p[id] = p[id + 1];
+        else if (!solid[id - 1]) p[id] = p[id - 1];
+    }
+}
+
+// After a sweep's boundary pass: record the sweep and stop when the max
+// update is below the threshold (pressure.cpp:165, 240).
+__global__ void k_stat_check(StatScalars* st, int sweep)
+{
+    if (st->done) return;
+    st->sweeps = sweep + 1;
+    st->last_max = st->maxr;
+    if (st->maxr < st->thresh || sweep + 1 >= st->max_sweeps) st->done = 1;
+    st->maxr = 0.0;
+}
+
+// rhs = coef * central div(u) on interior cells (pressure.cpp:119-132)
+__global__ __launch_bounds__(kBlock) void k_rhs_central(Dims d, Chunking ch, const double* u,
+                                                        const double* v, const double* w,
+                                                        double* rhs, double coef, double two_dx,
+                                                        double two_dy, double two_dz)
+{
+    const int lb = logical_block(ch.grid);
+    FOR_CELLS(d, ch, lb)
+    {
+        int i, j, k;
+        decode(d, id, i, j, k);
+        if (i == 0 || i == d.nx - 1 || on_wall(d, j, k)) {
+            rhs[id] = 0.0;
+            continue;
+        }
+        const double div = (u[id + d.sij] - u[id - d.sij]) / two_dx +
+                           (v[id + d.nz] - v[id - d.nz]) / two_dy +
+                           (w[id + 1] - w[id - 1]) / two_dz;
+        rhs[id] = coef * div;
+    }
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(double* addr, double v)
+{
+    // non-negative doubles order like their bit patterns
+    __hip_atomic_fetch_max(reinterpret_cast<unsigned long long*>(addr),
+                           (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One Jacobi sweep (pressure.cpp:138-157) from pc into pn.
+__global__ __launch_bounds__(kBlock) void k_jacobi_sweep(Dims d, Chunking ch, const double* pc,
+                                                         double* pn, const double* rhs, double dx2,
+                                                         double dy2, double dz2, double denom,
+                                                         StatScalars* st)
+{
+    if (st->done) return;
+    const int lb = logical_block(ch.grid);
+    double mx[1] = {0.0};
+    FOR_CELLS(d, ch, lb)
+    {
+        int i, j, k;
+        decode(d, id, i, j, k);
+        if (i == 0 || i == d.nx - 1 || on_wall(d, j, k)) continue;
+        const double pold = pc[id];
+        const double val = ((pc[id + d.sij] + pc[id - d.sij]) / dx2 +
+                            (pc[id + d.nz] + pc[id - d.nz]) / dy2 +
+                            (pc[id + 1] + pc[id - 1]) / dz2 - rhs[id]) /
+                           denom;
+        pn[id] = val;
+        const double r = fabs(val - pold);
+        mx[0] = mx[0] > r ? mx[0] : r;
+    }
+    block_max<1>(mx);
+    if (threadIdx.x == 0 && mx[0] > 0.0) atomic_max_nonneg(&st->maxr, mx[0]);
+}
+
+// One i+j+k hyperplane of the lexicographic SOR sweep (pressure.cpp:216-234):
+// west/south/bottom neighbours already hold this sweep's values and
+// east/north/top the previous sweep's, as in the serial loop.
+__global__ __launch_bounds__(kBlock) void k_sor_level(Dims d, int level, double* p,
+                                                      const double* rhs, double one_m_omega,
+                                                      double omega_coef, double dx2, double dy2,
+                                                      double dz2, StatScalars* st)
+{
+    if (st->done) return;
+    const int t = blockIdx.x * kBlock + threadIdx.x;
+    const int mj = d.ny - 2;
+    double r = 0.0;
+    if (t < (d.nx - 2) * mj) {
+        const int i = 1 + t / mj;
+        const int j = 1 + t - (i - 1) * mj;
+        const int k = level - i - j;
+        if (k >= 1 && k <= d.nz - 2) {
+            const int id = i * d.sij + j * d.nz + k;
+            const double pold = p[id];
+            const double val = one_m_omega * p[id] +
+                               omega_coef * ((p[id + d.sij] + p[id - d.sij]) / dx2 +
+                                             (p[id + d.nz] + p[id - d.nz]) / dy2 +
+                                             (p[id + 1] + p[id - 1]) / dz2 - rhs[id]);
+            p[id] = val;
+            r = fabs(val - pold);
+        }
+    }
+    double mx[1] = {r};
+    block_max<1>(mx);
+    if (threadIdx.x == 0 && mx[0] > 0.0) atomic_max_nonneg(&st->maxr, mx[0]);
+}
+
+// One colour of red-black SOR: cells with (i+j+k)%2 == colour.
+__global__ __launch_bounds__(kBlock) void k_sor_colour(Dims d, Chunking ch, double* p,
+                                                       const double* rhs, int colour,
+                                                       double one_m_omega, double omega_coef,
+                                                       double dx2, double dy2, double dz2,
+                                                       StatScalars* st)
+{
+    if (st->done) return;
+    const int lb = logical_block(ch.grid);
+    double mx[1] = {0.0};
+    FOR_CELLS(d, ch, lb)
+    {
+        int i, j, k;
+        decode(d, id, i, j, k);
+        if (i == 0 || i == d.nx - 1 || on_wall(d, j, k) || ((i + j + k) & 1) != colour) continue;
+        const double pold = p[id];
+        const double val = one_m_omega * p[id] +
+                           omega_coef * ((p[id + d.sij] + p[id - d.sij]) / dx2 +
+                                         (p[id + d.nz] + p[id - d.nz]) / dy2 +
+                                         (p[id + 1] + p[id - 1]) / dz2 - rhs[id]);
+        p[id] = val;
+        const double r = fabs(val - pold);
+        mx[0] = mx[0] > r ? mx[0] : r;
+    }
+    block_max<1>(mx);
+    if (threadIdx.x == 0 && mx[0] > 0.0) atomic_max_nonneg(&st->maxr, mx[0]);
+}
+
+int face_grid(const Dims& d) { return (face_count(d) + kBlock - 1) / kBlock; }
+
+}  // namespace
+
+// ------------------------------------------------------------------
+// launchers
+// ------------------------------------------------------------------
+hipError_t launch_sgs(hipStream_t s, const Dims& d, const PhysConsts& c, const double* u,
+                      const double* v, const double* w, const uint8_t* solid, double* nu_t,
+                      Tau tau, int apply_bcs, RedScratch red, double* nut_max)
+{
+    const Chunking ch = make_chunking(d.N);
+    hipLaunchKernelGGL(k_sgs, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, c, u, v, w, solid, nu_t,
+                       tau, apply_bcs, red, apply_bcs ? nut_max : nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_momentum(hipStream_t s, const Dims& d, const PhysConsts& c, const double* u,
+                           const double* v, const double* w, const double* nu_t,
+                           const uint8_t* solid, double* un, double* vn, double* wn,
+                           double* p_rhs, double* b, int mode)
+{
+    const Chunking ch = make_chunking(d.N);
+    hipLaunchKernelGGL(k_momentum, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, c, u, v, w, nu_t,
+                       solid, un, vn, wn, p_rhs, b, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_correct(hipStream_t s, const Dims& d, const PhysConsts& c, double* u, double* v,
+                          double* w, const double* p, const uint8_t* solid, int apply_bcs)
+{
+    const Chunking ch = make_chunking(d.N);
+    hipLaunchKernelGGL(k_correct, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, c, u, v, w, p, solid,
+                       apply_bcs);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    if (!apply_bcs) return hipSuccess;
+    hipLaunchKernelGGL(k_faces<true>, dim3(face_grid(d)), dim3(kBlock), 0, s, d, solid, u, v, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_velocity_bcs(hipStream_t s, const Dims& d, const uint8_t* solid, double* u,
+                               double* v, double* w)
+{
+    const Chunking ch = make_chunking(d.N);
+    hipLaunchKernelGGL(k_zero_solids, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, solid, u, v, w);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    hipLaunchKernelGGL(k_faces<true>, dim3(face_grid(d)), dim3(kBlock), 0, s, d, solid, u, v, w);
+    return hipGetLastError();
+}
+
+hipError_t launch_nut_bcs(hipStream_t s, const Dims& d, const uint8_t* solid, double* nu_t)
+{
+    const Chunking ch = make_chunking(d.N);
+    hipLaunchKernelGGL(k_zero_solids, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, solid, nu_t,
+                       (double*)nullptr, (double*)nullptr);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    hipLaunchKernelGGL(k_faces<false>, dim3(face_grid(d)), dim3(kBlock), 0, s, d, solid, nu_t,
+                       (double*)nullptr, (double*)nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_dt_max(hipStream_t s, const Dims& d, const double* u, const double* v,
+                         const double* w, const double* nu_t, RedScratch red, double* out2)
+{
+    const Chunking ch = make_chunking(d.N);
+    hipLaunchKernelGGL(k_dt_max, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, u, v, w, nu_t, red,
+                       out2);
+    return hipGetLastError();
+}
+
+hipError_t launch_pressure_bcs(hipStream_t s, const Dims& d, double* p, const uint8_t* solid,
+                               StatScalars* st, int sweep)
+{
+    const Chunking ch = make_chunking(d.N);
+    hipLaunchKernelGGL(k_pbc_faces, dim3(face_grid(d)), dim3(kBlock), 0, s, d, p,
+                       (const StatScalars*)st);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    hipLaunchKernelGGL(k_pbc_solids, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, p, solid, st,
+                       sweep);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    if (st) hipLaunchKernelGGL(k_stat_check, dim3(1), dim3(1), 0, s, st, sweep);
+    return hipGetLastError();
+}
+
+hipError_t launch_rhs_central(hipStream_t s, const Dims& d, const double* u, const double* v,
+                              const double* w, double* rhs, double coef, double two_dx,
+                              double two_dy, double two_dz)
+{
+    const Chunking ch = make_chunking(d.N);
+    hipLaunchKernelGGL(k_rhs_central, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, u, v, w, rhs, coef,
+                       two_dx, two_dy, two_dz);
+    return hipGetLastError();
+}
+
+hipError_t launch_jacobi_sweep(hipStream_t s, const Dims& d, const double* pc, double* pn,
+                               const double* rhs, double dx2, double dy2, double dz2,
+                               double denom, StatScalars* st)
+{
+    const Chunking ch = make_chunking(d.N);
+    hipLaunchKernelGGL(k_jacobi_sweep, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, pc, pn, rhs, dx2,
+                       dy2, dz2, denom, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_sor_level(hipStream_t s, const Dims& d, int level, double* p, const double* rhs,
+                            double one_m_omega, double omega_coef, double dx2, double dy2,
+                            double dz2, StatScalars* st)
+{
+    const int g = ((d.nx - 2) * (d.ny - 2) + kBlock - 1) / kBlock;
+    hipLaunchKernelGGL(k_sor_level, dim3(g), dim3(kBlock), 0, s, d, level, p, rhs, one_m_omega,
+                       omega_coef, dx2, dy2, dz2, st);
+    return hipGetLastError();
+}
+
+hipError_t launch_sor_colour(hipStream_t s, const Dims& d, double* p, const double* rhs,
+                             int colour, double one_m_omega, double omega_coef, double dx2,
+                             double dy2, double dz2, StatScalars* st)
+{
+    const Chunking ch = make_chunking(d.N);
+    hipLaunchKernelGGL(k_sor_colour, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, p, rhs, colour,
+                       one_m_omega, omega_coef, dx2, dy2, dz2, st);
+    return hipGetLastError();
+}
+
+}  // namespace les

@@ -1,0 +1,806 @@
+/*
+ * les_oracle.c -- CPU oracle for the LES per-step hot path (TEST
+ * INFRASTRUCTURE ONLY; see les_oracle.h).
+ *
+ * Each function restates one reference function, cited file:line, keeping
+ * the reference's loop order and operand order.  Build with -O2
+ * -ffp-contract=off and no -march so no fused multiply-add is formed, as the
+ * reference's own build (Makefile:3) forms none.
+ *
+ * How it is pinned (tests/test_oracle_*.py):
+ *   - against the unmodified reference compiled from its sources
+ *     (oracle/Makefile -> oracle/_ref/libref.so), bit for bit on one thread,
+ *     function by function and for whole steps;
+ *   - against the committed fixtures in tests/golden/ written by that build;
+ *   - against closed forms: the separable cosine eigenvectors of the
+ *     obstacle-free stencil, exact central differences of quadratic fields,
+ *     LL^T = A on the lower pattern for IC(0), dense solves on tiny grids.
+ *   orc_solve_pressure_rbsor has no reference counterpart; it is pinned by
+ *   convergence to the same fixed point as orc_solve_pressure_gs and by a
+ *   brute-force numpy restatement on a tiny grid.
+ */
+#include "les_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define IDX(i, j, k) ((i) * sij + (j) * nz + (k))
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+#else
+    (void)n;
+#endif
+}
+
+/* ------------------------------------------------------------------ */
+/* BLAS-1: pressure.cpp:962-1013                                       */
+/* ------------------------------------------------------------------ */
+double orc_dot(const double* a, const double* b, long n)
+{
+    double sum = 0.0;
+#pragma omp parallel for reduction(+ : sum)
+    for (long i = 0; i < n; ++i) sum += a[i] * b[i];
+    return sum;
+}
+
+double orc_norm(const double* a, long n) { return sqrt(orc_dot(a, a, n)); }
+
+void orc_axpy(double* out, double a, const double* in, long n)
+{
+#pragma omp parallel for
+    for (long i = 0; i < n; ++i) out[i] += a * in[i];
+}
+
+void orc_saxpy(double* out, double a, const double* in1, double b, const double* in2, long n)
+{
+#pragma omp parallel for
+    for (long i = 0; i < n; ++i) out[i] = a * in1[i] + b * in2[i];
+}
+
+void orc_copy(double* out, const double* in, long n)
+{
+#pragma omp parallel for
+    for (long i = 0; i < n; ++i) out[i] = in[i];
+}
+
+/* ------------------------------------------------------------------ */
+/* build_stencil: pressure.cpp:608-699                                 */
+/* ------------------------------------------------------------------ */
+void orc_build_stencil(double* const A[7], const uint8_t* solid, const orc_mesh* m)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const double idx2 = 1.0 / (m->dx * m->dx);
+    const double idy2 = 1.0 / (m->dy * m->dy);
+    const double idz2 = 1.0 / (m->dz * m->dz);
+    const double e_b = -idx2, w_b = -idx2, n_b = -idy2, s_b = -idy2, t_b = -idz2, b_b = -idz2;
+    const double c_b = (2.0 * idx2 + 2.0 * idy2 + 2.0 * idz2);
+    const long N = (long)nx * ny * nz;
+    for (int a = 0; a < 7; ++a) memset(A[a], 0, sizeof(double) * N);
+#pragma omp parallel for
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                if (solid[id]) { A[ORC_C][id] = 1.0; continue; }
+                if (i == nx - 1) { A[ORC_C][id] = 1.0; continue; }
+                double c = c_b, e = e_b, w = w_b, n = n_b, s = s_b, t = t_b, b = b_b;
+                if (i == 0) { e += w_b; w = 0.0; }
+                else if (solid[IDX(i - 1, j, k)]) { e += w_b; w = 0.0; }
+                if (i == nx - 2) { e = 0.0; }
+                else if (solid[IDX(i + 1, j, k)]) { w += e_b; e = 0.0; }
+                if (j == 0) { n += s_b; s = 0.0; }
+                else if (solid[IDX(i, j - 1, k)]) { n += s_b; s = 0.0; }
+                if (j == ny - 1) { s += n_b; n = 0.0; }
+                else if (solid[IDX(i, j + 1, k)]) { s += n_b; n = 0.0; }
+                if (k == 0) { t += b_b; b = 0.0; }
+                else if (solid[IDX(i, j, k - 1)]) { t += b_b; b = 0.0; }
+                if (k == nz - 1) { b += t_b; t = 0.0; }
+                else if (solid[IDX(i, j, k + 1)]) { b += t_b; t = 0.0; }
+                A[ORC_C][id] = c; A[ORC_E][id] = e; A[ORC_W][id] = w; A[ORC_N][id] = n;
+                A[ORC_S][id] = s; A[ORC_T][id] = t; A[ORC_B][id] = b;
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* fast_apply_laplacian: pressure.cpp:709-746 (order c,w,e,s,n,b,t)    */
+/* ------------------------------------------------------------------ */
+void orc_matvec(double* q, const double* p, double* const A[7], int nx, int ny, int nz)
+{
+    const int sij = ny * nz;
+#pragma omp parallel for
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                double val = A[ORC_C][id] * p[id];
+                if (i > 0) val += A[ORC_W][id] * p[id - sij];
+                if (i < nx - 1) val += A[ORC_E][id] * p[id + sij];
+                if (j > 0) val += A[ORC_S][id] * p[id - nz];
+                if (j < ny - 1) val += A[ORC_N][id] * p[id + nz];
+                if (k > 0) val += A[ORC_B][id] * p[id - 1];
+                if (k < nz - 1) val += A[ORC_T][id] * p[id + 1];
+                q[id] = val;
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* calculate_divergence: pressure.cpp:256-329                          */
+/* ------------------------------------------------------------------ */
+void orc_calculate_divergence(double* rhs, const double* un, const double* vn, const double* wn,
+                              const uint8_t* solid, const orc_mesh* m, double dt, double rho)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const double scale = rho / dt;
+    const double ix = 1.0 / (2.0 * m->dx), iy = 1.0 / (2.0 * m->dy), iz = 1.0 / (2.0 * m->dz);
+    const double ix1 = 1.0 / m->dx, iy1 = 1.0 / m->dy, iz1 = 1.0 / m->dz;
+#pragma omp parallel for
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                if (solid[id] || i == nx - 1) { rhs[id] = 0.0; continue; }
+                double dudx, dvdy, dwdz;
+                if (i == 0) dudx = (un[IDX(i + 1, j, k)] - un[id]) * ix1;
+                else dudx = (un[IDX(i + 1, j, k)] - un[IDX(i - 1, j, k)]) * ix;
+                if (j == 0) dvdy = (vn[IDX(i, j + 1, k)] - vn[id]) * iy1;
+                else if (j == ny - 1) dvdy = (vn[id] - vn[IDX(i, j - 1, k)]) * iy1;
+                else dvdy = (vn[IDX(i, j + 1, k)] - vn[IDX(i, j - 1, k)]) * iy;
+                if (k == 0) dwdz = (wn[IDX(i, j, k + 1)] - wn[id]) * iz1;
+                else if (k == nz - 1) dwdz = (wn[id] - wn[IDX(i, j, k - 1)]) * iz1;
+                else dwdz = (wn[IDX(i, j, k + 1)] - wn[IDX(i, j, k - 1)]) * iz;
+                rhs[id] = scale * (dudx + dvdy + dwdz);
+            }
+}
+
+/* build_b_vector: pressure.cpp:566-599 */
+void orc_build_b_vector(double* b, const double* div, const uint8_t* solid, const orc_mesh* m)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+#pragma omp parallel for
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                if (solid[id]) b[id] = 0.0;
+                else if (i == nx - 1) b[id] = 0.0;
+                else b[id] = -div[id];
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* preconditioners                                                     */
+/* ------------------------------------------------------------------ */
+/* apply_jacobi_preconditioner: pressure.cpp:420-428 */
+void orc_jacobi_precond(double* z, const double* r, const double* Ac, long n)
+{
+#pragma omp parallel for
+    for (long i = 0; i < n; ++i) z[i] = r[i] / Ac[i];
+}
+
+/* apply_ic_preconditioner: pressure.cpp:442-482 (serial, lexicographic) */
+void orc_ic_apply(double* z, const double* r, double* const L[7], int nx, int ny, int nz)
+{
+    const int sij = ny * nz;
+    const double *Lc = L[ORC_C], *Lw = L[ORC_W], *Ls = L[ORC_S], *Lb = L[ORC_B];
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                double sum = r[id];
+                if (i > 0) sum -= Lw[id] * z[id - sij];
+                if (j > 0) sum -= Ls[id] * z[id - nz];
+                if (k > 0) sum -= Lb[id] * z[id - 1];
+                z[id] = sum / Lc[id];
+            }
+    for (int i = nx - 1; i >= 0; --i)
+        for (int j = ny - 1; j >= 0; --j)
+            for (int k = nz - 1; k >= 0; --k) {
+                const int id = IDX(i, j, k);
+                double sum = z[id];
+                if (i < nx - 1) sum -= Lw[id + sij] * z[id + sij];
+                if (j < ny - 1) sum -= Ls[id + nz] * z[id + nz];
+                if (k < nz - 1) sum -= Lb[id + 1] * z[id + 1];
+                z[id] = sum / Lc[id];
+            }
+}
+
+/* build_ic_preconditioner: pressure.cpp:495-556 (serial).  Like the
+ * reference, entries of L that the guards skip keep their previous value;
+ * the caller passes L zero-initialised as StencilMatrix(n) is. */
+void orc_ic_build(double* const L[7], double* const A[7], int nx, int ny, int nz)
+{
+    const int sij = ny * nz;
+    const double small_val = 1e-30;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                double sum_sq = 0.0;
+                if (i > 0) {
+                    double lc = L[ORC_C][id - sij];
+                    if (fabs(lc) > small_val) {
+                        L[ORC_W][id] = A[ORC_W][id] / lc;
+                        sum_sq += L[ORC_W][id] * L[ORC_W][id];
+                    }
+                }
+                if (j > 0) {
+                    double lc = L[ORC_C][id - nz];
+                    if (fabs(lc) > small_val) {
+                        L[ORC_S][id] = A[ORC_S][id] / lc;
+                        sum_sq += L[ORC_S][id] * L[ORC_S][id];
+                    }
+                }
+                if (k > 0) {
+                    double lc = L[ORC_C][id - 1];
+                    if (fabs(lc) > small_val) {
+                        L[ORC_B][id] = A[ORC_B][id] / lc;
+                        sum_sq += L[ORC_B][id] * L[ORC_B][id];
+                    }
+                }
+                double diag = A[ORC_C][id] - sum_sq;
+                L[ORC_C][id] = (diag <= small_val) ? 1.0 : sqrt(diag);
+            }
+}
+
+/* apply_chebyshev_preconditioner: pressure.cpp:851-906 */
+void orc_cheb_apply(double* z, const double* r, double* const A[7], int nx, int ny, int nz,
+                    double lmin, double lmax, int degree)
+{
+    const long n = (long)nx * ny * nz;
+    double* p = (double*)calloc(n, sizeof(double));
+    double* res = (double*)calloc(n, sizeof(double));
+    const double d = (lmax + lmin) / 2.0;
+    const double c = (lmax - lmin) / 2.0;
+    double alpha, beta;
+#pragma omp parallel for
+    for (long i = 0; i < n; ++i) z[i] = 0.0;
+    orc_copy(res, r, n);
+    alpha = 2.0 / d;
+    orc_copy(p, res, n);
+    orc_axpy(z, alpha, p, n);
+    for (int k = 1; k < degree; ++k) {
+        orc_matvec(res, z, A, nx, ny, nz);
+        orc_saxpy(res, 1.0, r, -1.0, res, n);
+        beta = (c * alpha / 2.0) * (c * alpha / 2.0);
+        alpha = 1.0 / (d - beta);
+        orc_saxpy(p, 1.0, res, beta, p, n);
+        orc_axpy(z, alpha, p, n);
+    }
+    free(p);
+    free(res);
+}
+
+/* --- std::mt19937 and libstdc++'s uniform_real_distribution<double> ---
+ * MT19937 is the published Matsumoto-Nishimura generator (seed by the
+ * 1812433253 recurrence).  libstdc++ (g++ 11) draws a double in [0,1) by
+ * generate_canonical<double,53>: two 32-bit outputs x0 + x1*2^32, divided by
+ * 2^64, clamped below 1; the distribution returns u*(b-a)+a. */
+typedef struct { uint32_t mt[624]; int idx; } orc_mt;
+
+static void mt_seed(orc_mt* g, uint32_t s)
+{
+    g->mt[0] = s;
+    for (int i = 1; i < 624; ++i)
+        g->mt[i] = 1812433253u * (g->mt[i - 1] ^ (g->mt[i - 1] >> 30)) + (uint32_t)i;
+    g->idx = 624;
+}
+
+static uint32_t mt_next(orc_mt* g)
+{
+    if (g->idx >= 624) {
+        for (int i = 0; i < 624; ++i) {
+            uint32_t y = (g->mt[i] & 0x80000000u) | (g->mt[(i + 1) % 624] & 0x7fffffffu);
+            g->mt[i] = g->mt[(i + 397) % 624] ^ (y >> 1) ^ ((y & 1u) ? 0x9908b0dfu : 0u);
+        }
+        g->idx = 0;
+    }
+    uint32_t y = g->mt[g->idx++];
+    y ^= (y >> 11);
+    y ^= (y << 7) & 0x9d2c5680u;
+    y ^= (y << 15) & 0xefc60000u;
+    y ^= (y >> 18);
+    return y;
+}
+
+static double mt_uniform(orc_mt* g, double a, double b)
+{
+    double sum = 0.0, tmp = 1.0;
+    const double r = 4294967296.0;
+    for (int m = 0; m < 2; ++m) {
+        sum += (double)mt_next(g) * tmp;
+        tmp *= r;
+    }
+    double ret = sum / tmp;
+    if (ret >= 1.0) ret = nextafter(1.0, 0.0);
+    return ret * (b - a) + a;
+}
+
+/* estimate_lambda_max: pressure.cpp:917-955 */
+double orc_estimate_lambda_max(double* const A[7], int nx, int ny, int nz, int n_iter)
+{
+    const long n = (long)nx * ny * nz;
+    double* v = (double*)malloc(n * sizeof(double));
+    double* w = (double*)malloc(n * sizeof(double));
+    orc_mt g;
+    mt_seed(&g, 1337u);
+    for (long i = 0; i < n; ++i) v[i] = mt_uniform(&g, -1.0, 1.0);
+    orc_copy(w, v, n);
+    double lam = 0.0;
+    for (int k = 0; k < n_iter; ++k) {
+        orc_matvec(w, v, A, nx, ny, nz);
+        lam = orc_dot(v, w, n);
+        double nrm = orc_norm(w, n);
+        orc_copy(v, w, n);
+#pragma omp parallel for
+        for (long i = 0; i < n; ++i) v[i] /= nrm;
+    }
+    free(v);
+    free(w);
+    return lam;
+}
+
+/* ------------------------------------------------------------------ */
+/* PCG: pressure.cpp:342-414 (and 762-835 for the Chebyshev variant)   */
+/* ------------------------------------------------------------------ */
+static void orc_precond(int pc, double* z, const double* r, double* const A[7],
+                        double* const L[7], int nx, int ny, int nz, double lmin, double lmax,
+                        int degree)
+{
+    const long n = (long)nx * ny * nz;
+    switch (pc) {
+    case ORC_PC_JACOBI: orc_jacobi_precond(z, r, A[ORC_C], n); break;
+    case ORC_PC_IC0: orc_ic_apply(z, r, L, nx, ny, nz); break;
+    case ORC_PC_LEGACY_IC_A: orc_ic_apply(z, r, A, nx, ny, nz); break;
+    case ORC_PC_CHEBYSHEV: orc_cheb_apply(z, r, A, nx, ny, nz, lmin, lmax, degree); break;
+    default: orc_copy(z, r, n); break;
+    }
+}
+
+int orc_pcg(double* const A[7], const double* b, double* x, int nx, int ny, int nz,
+            int precond, double* const L[7], double lmin, double lmax, int degree,
+            int max_iter, double tol, double* res_trace)
+{
+    const long n = (long)nx * ny * nz;
+    double* r = (double*)calloc(n, sizeof(double));
+    double* z = (double*)calloc(n, sizeof(double));
+    double* p = (double*)calloc(n, sizeof(double));
+    double* q = (double*)calloc(n, sizeof(double));
+    int ret = max_iter;
+    orc_matvec(q, x, A, nx, ny, nz);
+    orc_saxpy(r, 1.0, b, -1.0, q, n);
+    double err0 = orc_norm(r, n);
+    if (res_trace) res_trace[0] = err0;
+    if (err0 < tol) { ret = 0; goto out; }
+    orc_precond(precond, p, r, A, L, nx, ny, nz, lmin, lmax, degree);
+    double rho = orc_dot(r, p, n);
+    for (int k = 1; k <= max_iter; ++k) {
+        orc_matvec(q, p, A, nx, ny, nz);
+        double alpha = rho / orc_dot(p, q, n);
+        orc_axpy(x, alpha, p, n);
+        orc_axpy(r, -alpha, q, n);
+        double err = orc_norm(r, n);
+        if (res_trace) res_trace[k] = err;
+        if (err < tol) { ret = k; goto out; }
+        orc_precond(precond, z, r, A, L, nx, ny, nz, lmin, lmax, degree);
+        double rho_new = orc_dot(r, z, n);
+        double beta = rho_new / rho;
+        orc_saxpy(p, 1.0, z, beta, p, n);
+        rho = rho_new;
+    }
+out:
+    free(r); free(z); free(p); free(q);
+    return ret;
+}
+
+/* ------------------------------------------------------------------ */
+/* stationary solvers                                                  */
+/* ------------------------------------------------------------------ */
+/* apply_pressure_BCs: pressure.cpp:14-93 */
+void orc_apply_pressure_bcs(double* p, const uint8_t* solid, const orc_mesh* m)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) p[IDX(0, j, k)] = p[IDX(1, j, k)];
+    for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) p[IDX(nx - 1, j, k)] = 0.0;
+    for (int i = 0; i < nx; ++i)
+        for (int k = 0; k < nz; ++k) {
+            p[IDX(i, 0, k)] = p[IDX(i, 1, k)];
+            p[IDX(i, ny - 1, k)] = p[IDX(i, ny - 2, k)];
+        }
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            p[IDX(i, j, 0)] = p[IDX(i, j, 1)];
+            p[IDX(i, j, nz - 1)] = p[IDX(i, j, nz - 2)];
+        }
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 1; j < ny - 1; ++j)
+            for (int k = 1; k < nz - 1; ++k) {
+                const int id = IDX(i, j, k);
+                if (!solid[id]) continue;
+                if (i < nx - 1 && !solid[id + sij]) p[id] = p[id + sij];
+                else if (i > 0 && !solid[id - sij]) p[id] = p[id - sij];
+                else if (j < ny - 1 && !solid[id + nz]) p[id] = p[id + nz];
+                else if (j > 0 && !solid[id - nz]) p[id] = p[id - nz];
+                else if (k < nz - 1 && !solid[id + 1]) p[id] = p[id + 1];
+                else if (k > 0 && !solid[id - 1]) p[id] = p[id - 1];
+            }
+}
+
+/* RHS of the stationary solvers: pressure.cpp:119-132 and 197-210 */
+static void orc_rhs_central(double* rhs, const double* u, const double* v, const double* w,
+                            const orc_mesh* m, double coef)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+#pragma omp parallel for
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 1; j < ny - 1; ++j)
+            for (int k = 1; k < nz - 1; ++k) {
+                double div = (u[IDX(i + 1, j, k)] - u[IDX(i - 1, j, k)]) / (2 * m->dx)
+                           + (v[IDX(i, j + 1, k)] - v[IDX(i, j - 1, k)]) / (2 * m->dy)
+                           + (w[IDX(i, j, k + 1)] - w[IDX(i, j, k - 1)]) / (2 * m->dz);
+                rhs[IDX(i, j, k)] = coef * div;
+            }
+}
+
+/* solve_pressure_jacobi: pressure.cpp:106-172.  p_new is a fresh zeroed
+ * buffer per call, swapped with p every sweep, exactly as the reference. */
+int orc_solve_pressure_jacobi(double* p, const double* u, const double* v, const double* w,
+                              const uint8_t* solid, const orc_mesh* m, double dt, double rho)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const long n = (long)nx * ny * nz;
+    double* pn = (double*)calloc(n, sizeof(double));
+    double* rhs = (double*)calloc(n, sizeof(double));
+    double* cur = p;  /* the buffer currently holding f.p */
+    const double coef = rho / dt;
+    const double dx2 = m->dx * m->dx, dy2 = m->dy * m->dy, dz2 = m->dz * m->dz;
+    const double denom = 2 * (1.0 / dx2 + 1.0 / dy2 + 1.0 / dz2);
+    orc_rhs_central(rhs, u, v, w, m, coef);
+    int sweeps = 0;
+    for (int it = 0; it < 100; ++it) {
+        double maxr = 0.0;
+#pragma omp parallel for reduction(max : maxr)
+        for (int i = 1; i < nx - 1; ++i)
+            for (int j = 1; j < ny - 1; ++j)
+                for (int k = 1; k < nz - 1; ++k) {
+                    const int id = IDX(i, j, k);
+                    double pold = cur[id];
+                    pn[id] = ((cur[IDX(i + 1, j, k)] + cur[IDX(i - 1, j, k)]) / dx2 +
+                              (cur[IDX(i, j + 1, k)] + cur[IDX(i, j - 1, k)]) / dy2 +
+                              (cur[IDX(i, j, k + 1)] + cur[IDX(i, j, k - 1)]) / dz2 - rhs[id]) /
+                             denom;
+                    double res = fabs(pn[id] - pold);
+                    if (res > maxr) maxr = res;
+                }
+        double* t = cur; cur = pn; pn = t;
+        orc_apply_pressure_bcs(cur, solid, m);
+        sweeps = it + 1;
+        if (maxr < 1e-6) break;
+    }
+    if (cur != p) {
+        memcpy(p, cur, n * sizeof(double));
+        free(cur);
+    } else {
+        free(pn);
+    }
+    free(rhs);
+    return sweeps;
+}
+
+/* solve_pressure_GS: pressure.cpp:186-247 (lexicographic, in place) */
+int orc_solve_pressure_gs(double* p, const double* u, const double* v, const double* w,
+                          const uint8_t* solid, const orc_mesh* m, double dt, double rho,
+                          int max_iter, double omega)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const long n = (long)nx * ny * nz;
+    double* rhs = (double*)calloc(n, sizeof(double));
+    const double dx2 = m->dx * m->dx, dy2 = m->dy * m->dy, dz2 = m->dz * m->dz;
+    const double coef = 1.0 / (2.0 * (1.0 / dx2 + 1.0 / dy2 + 1.0 / dz2));
+    orc_rhs_central(rhs, u, v, w, m, rho / dt);
+    int sweeps = 0;
+    for (int it = 0; it < max_iter; ++it) {
+        double maxr = 0.0;
+        for (int i = 1; i < nx - 1; ++i)
+            for (int j = 1; j < ny - 1; ++j)
+                for (int k = 1; k < nz - 1; ++k) {
+                    const int id = IDX(i, j, k);
+                    double pold = p[id];
+                    p[id] = (1 - omega) * p[id] + omega * coef *
+                            ((p[IDX(i + 1, j, k)] + p[IDX(i - 1, j, k)]) / dx2 +
+                             (p[IDX(i, j + 1, k)] + p[IDX(i, j - 1, k)]) / dy2 +
+                             (p[IDX(i, j, k + 1)] + p[IDX(i, j, k - 1)]) / dz2 - rhs[id]);
+                    double res = fabs(p[id] - pold);
+                    if (res > maxr) maxr = res;
+                }
+        orc_apply_pressure_bcs(p, solid, m);
+        sweeps = it + 1;
+        if (maxr < 1e-6) break;
+    }
+    free(rhs);
+    return sweeps;
+}
+
+/* Red-black SOR (new): the GS update of pressure.cpp:224-229 applied to the
+ * cells with (i+j+k) even, then to those with (i+j+k) odd. */
+int orc_solve_pressure_rbsor(double* p, const double* u, const double* v, const double* w,
+                             const uint8_t* solid, const orc_mesh* m, double dt, double rho,
+                             int max_iter, double omega)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const long n = (long)nx * ny * nz;
+    double* rhs = (double*)calloc(n, sizeof(double));
+    const double dx2 = m->dx * m->dx, dy2 = m->dy * m->dy, dz2 = m->dz * m->dz;
+    const double coef = 1.0 / (2.0 * (1.0 / dx2 + 1.0 / dy2 + 1.0 / dz2));
+    orc_rhs_central(rhs, u, v, w, m, rho / dt);
+    int sweeps = 0;
+    for (int it = 0; it < max_iter; ++it) {
+        double maxr = 0.0;
+        for (int colour = 0; colour < 2; ++colour) {
+#pragma omp parallel for reduction(max : maxr)
+            for (int i = 1; i < nx - 1; ++i)
+                for (int j = 1; j < ny - 1; ++j)
+                    for (int k = 1; k < nz - 1; ++k) {
+                        if (((i + j + k) & 1) != colour) continue;
+                        const int id = IDX(i, j, k);
+                        double pold = p[id];
+                        p[id] = (1 - omega) * p[id] + omega * coef *
+                                ((p[IDX(i + 1, j, k)] + p[IDX(i - 1, j, k)]) / dx2 +
+                                 (p[IDX(i, j + 1, k)] + p[IDX(i, j - 1, k)]) / dy2 +
+                                 (p[IDX(i, j, k + 1)] + p[IDX(i, j, k - 1)]) / dz2 - rhs[id]);
+                        double res = fabs(p[id] - pold);
+                        if (res > maxr) maxr = res;
+                    }
+        }
+        orc_apply_pressure_bcs(p, solid, m);
+        sweeps = it + 1;
+        if (maxr < 1e-6) break;
+    }
+    free(rhs);
+    return sweeps;
+}
+
+/* ------------------------------------------------------------------ */
+/* physics: main.cpp                                                   */
+/* ------------------------------------------------------------------ */
+/* compute_adaptive_dt: main.cpp:208-250 */
+double orc_adaptive_dt(const double* u, const double* v, const double* w, const double* nu_t,
+                       const orc_mesh* m, double nu)
+{
+    const long n = (long)m->nx * m->ny * m->nz;
+    double umax2 = 0.0, ntmax = 0.0;
+#pragma omp parallel for reduction(max : umax2)
+    for (long i = 0; i < n; ++i) {
+        double s = u[i] * u[i] + v[i] * v[i] + w[i] * w[i];
+        umax2 = umax2 > s ? umax2 : s;
+    }
+    double umax = sqrt(umax2);
+    double mind = m->dx;
+    if (m->dy < mind) mind = m->dy;
+    if (m->dz < mind) mind = m->dz;
+    double dt_conv = 1.7976931348623157e308;
+    if (umax > 1e-12) dt_conv = 0.4 * mind / umax;
+#pragma omp parallel for reduction(max : ntmax)
+    for (long i = 0; i < n; ++i) ntmax = ntmax > nu_t[i] ? ntmax : nu_t[i];
+    double dt_visc = 0.2 * (mind * mind) / (nu + ntmax);
+    double r = dt_conv;
+    if (dt_visc < r) r = dt_visc;
+    if (1e-4 < r) r = 1e-4;
+    return r;
+}
+
+/* (Cs*Delta)^2: main.cpp:426 and 461 */
+double orc_cs_delta_sq(const orc_mesh* m, double Cs)
+{
+    double delta = pow(m->dx * m->dy * m->dz, 1.0 / 3.0);
+    return pow(Cs * delta, 2);
+}
+
+/* compute_sgs_stress: main.cpp:423-472.  tau may be NULL (the reference
+ * writes the six stresses and never reads them). */
+void orc_sgs(double* nu_t, double* const tau[6], const double* u, const double* v, const double* w,
+             const orc_mesh* m, double Cs)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const double csd2 = orc_cs_delta_sq(m, Cs);
+#pragma omp parallel for
+    for (int i = 1; i < nx - 1; i++)
+        for (int j = 1; j < ny - 1; j++)
+            for (int k = 1; k < nz - 1; k++) {
+                const int id = IDX(i, j, k);
+                double dudx = (u[IDX(i + 1, j, k)] - u[IDX(i - 1, j, k)]) / (2 * m->dx);
+                double dudy = (u[IDX(i, j + 1, k)] - u[IDX(i, j - 1, k)]) / (2 * m->dy);
+                double dudz = (u[IDX(i, j, k + 1)] - u[IDX(i, j, k - 1)]) / (2 * m->dz);
+                double dvdx = (v[IDX(i + 1, j, k)] - v[IDX(i - 1, j, k)]) / (2 * m->dx);
+                double dvdy = (v[IDX(i, j + 1, k)] - v[IDX(i, j - 1, k)]) / (2 * m->dy);
+                double dvdz = (v[IDX(i, j, k + 1)] - v[IDX(i, j, k - 1)]) / (2 * m->dz);
+                double dwdx = (w[IDX(i + 1, j, k)] - w[IDX(i - 1, j, k)]) / (2 * m->dx);
+                double dwdy = (w[IDX(i, j + 1, k)] - w[IDX(i, j - 1, k)]) / (2 * m->dy);
+                double dwdz = (w[IDX(i, j, k + 1)] - w[IDX(i, j, k - 1)]) / (2 * m->dz);
+                double Sxx = dudx, Syy = dvdy, Szz = dwdz;
+                double Sxy = 0.5 * (dudy + dvdx);
+                double Sxz = 0.5 * (dudz + dwdx);
+                double Syz = 0.5 * (dvdz + dwdy);
+                double Smag = sqrt(2.0 * (Sxx * Sxx + Syy * Syy + Szz * Szz) +
+                                   4.0 * (Sxy * Sxy + Sxz * Sxz + Syz * Syz));
+                double nt = csd2 * Smag;
+                nu_t[id] = nt;
+                if (tau) {
+                    tau[0][id] = -2.0 * nt * Sxx;
+                    tau[1][id] = -2.0 * nt * Sxy;
+                    tau[2][id] = -2.0 * nt * Sxz;
+                    tau[3][id] = -2.0 * nt * Syy;
+                    tau[4][id] = -2.0 * nt * Syz;
+                    tau[5][id] = -2.0 * nt * Szz;
+                }
+            }
+}
+
+/* apply_nut_BCs: main.cpp:483-547 */
+void orc_nut_bcs(double* nu_t, const uint8_t* solid, const orc_mesh* m)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) nu_t[IDX(0, j, k)] = 0.0;
+    for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) nu_t[IDX(nx - 1, j, k)] = nu_t[IDX(nx - 2, j, k)];
+    for (int i = 0; i < nx; ++i)
+        for (int k = 0; k < nz; ++k) {
+            nu_t[IDX(i, 0, k)] = 0.0;
+            nu_t[IDX(i, ny - 1, k)] = 0.0;
+        }
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            nu_t[IDX(i, j, 0)] = 0.0;
+            nu_t[IDX(i, j, nz - 1)] = 0.0;
+        }
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 1; j < ny - 1; ++j)
+            for (int k = 1; k < nz - 1; ++k)
+                if (solid[IDX(i, j, k)]) nu_t[IDX(i, j, k)] = 0.0;
+}
+
+/* solve_momentum: main.cpp:563-659 (three sweeps, interior only) */
+void orc_momentum(double* un, double* vn, double* wn, const double* u, const double* v,
+                  const double* w, const double* nu_t, const orc_mesh* m, double dt, double nu)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const double idx2 = 1.0 / (m->dx * m->dx), idy2 = 1.0 / (m->dy * m->dy),
+                 idz2 = 1.0 / (m->dz * m->dz);
+    const double i2x = 1.0 / (2.0 * m->dx), i2y = 1.0 / (2.0 * m->dy), i2z = 1.0 / (2.0 * m->dz);
+    const double* src[3] = {u, v, w};
+    double* dst[3] = {un, vn, wn};
+    for (int c = 0; c < 3; ++c) {
+        const double* f = src[c];
+        double* o = dst[c];
+#pragma omp parallel for
+        for (int i = 1; i < nx - 1; ++i)
+            for (int j = 1; j < ny - 1; ++j)
+                for (int k = 1; k < nz - 1; ++k) {
+                    const int id = IDX(i, j, k);
+                    double nu_eff = nu + nu_t[id];
+                    double ddx = (f[IDX(i + 1, j, k)] - f[IDX(i - 1, j, k)]) * i2x;
+                    double ddy = (f[IDX(i, j + 1, k)] - f[IDX(i, j - 1, k)]) * i2y;
+                    double ddz = (f[IDX(i, j, k + 1)] - f[IDX(i, j, k - 1)]) * i2z;
+                    double adv = u[id] * ddx + v[id] * ddy + w[id] * ddz;
+                    double lap = (f[IDX(i + 1, j, k)] - 2 * f[id] + f[IDX(i - 1, j, k)]) * idx2
+                               + (f[IDX(i, j + 1, k)] - 2 * f[id] + f[IDX(i, j - 1, k)]) * idy2
+                               + (f[IDX(i, j, k + 1)] - 2 * f[id] + f[IDX(i, j, k - 1)]) * idz2;
+                    o[id] = f[id] + dt * (-adv + nu_eff * lap);
+                }
+    }
+}
+
+/* apply_velocity_BCs: main.cpp:323-410 */
+void orc_velocity_bcs(double* u, double* v, double* w, const uint8_t* solid, const orc_mesh* m)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) {
+            const int id = IDX(0, j, k);
+            if (solid[id]) continue;
+            u[id] = 1.0; v[id] = 0.0; w[id] = 0.0;
+        }
+    for (int j = 0; j < ny; ++j)
+        for (int k = 0; k < nz; ++k) {
+            const int a = IDX(nx - 1, j, k), b = IDX(nx - 2, j, k);
+            u[a] = u[b]; v[a] = v[b]; w[a] = w[b];
+        }
+    for (int i = 0; i < nx; ++i)
+        for (int k = 0; k < nz; ++k) {
+            const int a = IDX(i, 0, k), b = IDX(i, ny - 1, k);
+            u[a] = 0.0; v[a] = 0.0; w[a] = 0.0;
+            u[b] = 0.0; v[b] = 0.0; w[b] = 0.0;
+        }
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j) {
+            const int a = IDX(i, j, 0), b = IDX(i, j, nz - 1);
+            u[a] = 0.0; v[a] = 0.0; w[a] = 0.0;
+            u[b] = 0.0; v[b] = 0.0; w[b] = 0.0;
+        }
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 1; j < ny - 1; ++j)
+            for (int k = 1; k < nz - 1; ++k) {
+                const int id = IDX(i, j, k);
+                if (solid[id]) { u[id] = 0.0; v[id] = 0.0; w[id] = 0.0; }
+            }
+}
+
+/* correct_velocity: main.cpp:671-714 */
+void orc_correct(double* u, double* v, double* w, const double* p, const orc_mesh* m,
+                 double dt, double rho)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const double dor = dt / rho;
+    const double i2x = 1.0 / (2.0 * m->dx), i2y = 1.0 / (2.0 * m->dy), i2z = 1.0 / (2.0 * m->dz);
+#pragma omp parallel for
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 1; j < ny - 1; ++j)
+            for (int k = 1; k < nz - 1; ++k) {
+                const int id = IDX(i, j, k);
+                u[id] -= dor * ((p[IDX(i + 1, j, k)] - p[IDX(i - 1, j, k)]) * i2x);
+            }
+#pragma omp parallel for
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 1; j < ny - 1; ++j)
+            for (int k = 1; k < nz - 1; ++k) {
+                const int id = IDX(i, j, k);
+                v[id] -= dor * ((p[IDX(i, j + 1, k)] - p[IDX(i, j - 1, k)]) * i2y);
+            }
+#pragma omp parallel for
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 1; j < ny - 1; ++j)
+            for (int k = 1; k < nz - 1; ++k) {
+                const int id = IDX(i, j, k);
+                w[id] -= dor * ((p[IDX(i, j, k + 1)] - p[IDX(i, j, k - 1)]) * i2z);
+            }
+}
+
+/* ------------------------------------------------------------------ */
+/* one step: main.cpp:115-144                                          */
+/* ------------------------------------------------------------------ */
+void orc_step(orc_fields* f, const orc_mesh* m, const orc_step_params* prm, orc_step_stats* st)
+{
+    const long n = (long)m->nx * m->ny * m->nz;
+    double dt;
+    if (prm->dt_fixed > 0) dt = prm->dt_fixed;
+    else if (prm->step <= 20) dt = 1e-3;
+    else dt = orc_adaptive_dt(f->u, f->v, f->w, f->nu_t, m, prm->nu);
+
+    orc_sgs(f->nu_t, NULL, f->u, f->v, f->w, m, prm->Cs);
+    orc_nut_bcs(f->nu_t, f->solid, m);
+    orc_momentum(f->u_new, f->v_new, f->w_new, f->u, f->v, f->w, f->nu_t, m, dt, prm->nu);
+    double* t;
+    t = f->u; f->u = f->u_new; f->u_new = t;
+    t = f->v; f->v = f->v_new; f->v_new = t;
+    t = f->w; f->w = f->w_new; f->w_new = t;
+    orc_velocity_bcs(f->u, f->v, f->w, f->solid, m);
+
+    orc_calculate_divergence(f->p_rhs, f->u_new, f->v_new, f->w_new, f->solid, m, dt, prm->rho);
+    double* b = (double*)calloc(n, sizeof(double));
+    orc_build_b_vector(b, f->p_rhs, f->solid, m);
+    double bn = orc_norm(b, n);
+    double tol = prm->tol_rel ? prm->tol * bn : prm->tol;
+    int its = orc_pcg(prm->A, b, f->p, m->nx, m->ny, m->nz, prm->precond, prm->L, prm->lmin,
+                      prm->lmax, prm->degree, prm->max_iter, tol, NULL);
+    /* final true residual, for reporting only */
+    double* q = (double*)calloc(n, sizeof(double));
+    orc_matvec(q, f->p, prm->A, m->nx, m->ny, m->nz);
+    orc_saxpy(q, 1.0, b, -1.0, q, n);
+    double res = orc_norm(q, n);
+    free(q);
+    free(b);
+
+    orc_correct(f->u, f->v, f->w, f->p, m, dt, prm->rho);
+    orc_velocity_bcs(f->u, f->v, f->w, f->solid, m);
+    if (st) { st->dt = dt; st->iters = its; st->bnorm = bn; st->res = res; }
+}

@@ -1,0 +1,436 @@
+"""ctypes bindings of the two CPU checkers used by the tests.
+
+* ``Oracle``  -- oracle/build/liblesoracle.so, the C restatement of the
+  reference hot path (oracle/les_oracle.c).
+* ``Reference`` -- oracle/_ref/libref.so, the unmodified reference solver
+  compiled from its own sources with oracle/ref_harness.cpp.
+
+Both expose the same Python methods so a test can run one against the other.
+Only tests, the smoke check and the benchmark's CPU-baseline leg use this.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "build", "liblesoracle.so")
+REF_SO = os.path.join(ROOT, "oracle", "_ref", "libref.so")
+
+DP = C.POINTER(C.c_double)
+U8P = C.POINTER(C.c_uint8)
+IP = C.POINTER(C.c_int)
+
+PC_NONE, PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV = 0, 1, 2, 3, 4
+
+
+def _p(a):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    if a.dtype == np.uint8:
+        return a.ctypes.data_as(U8P)
+    assert a.dtype == np.float64, a.dtype
+    return a.ctypes.data_as(DP)
+
+
+def _p7(arrs):
+    return (DP * 7)(*[_p(a) for a in arrs])
+
+
+def _p6(arrs):
+    return (DP * 6)(*[_p(a) for a in arrs])
+
+
+class OrcMesh(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+                ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double)]
+
+
+class OrcFields(C.Structure):
+    _fields_ = [(n, DP) for n in ("u", "v", "w", "u_new", "v_new", "w_new", "p", "p_rhs", "nu_t")] + [
+        ("solid", U8P)]
+
+
+class OrcStepParams(C.Structure):
+    _fields_ = [("nu", C.c_double), ("rho", C.c_double), ("Cs", C.c_double), ("step", C.c_int),
+                ("dt_fixed", C.c_double), ("precond", C.c_int), ("degree", C.c_int),
+                ("max_iter", C.c_int), ("lmin", C.c_double), ("lmax", C.c_double),
+                ("tol", C.c_double), ("tol_rel", C.c_int), ("L", C.POINTER(DP)),
+                ("A", C.POINTER(DP))]
+
+
+class OrcStepStats(C.Structure):
+    _fields_ = [("dt", C.c_double), ("iters", C.c_int), ("bnorm", C.c_double), ("res", C.c_double)]
+
+
+def _dims(m):
+    return m["nx"], m["ny"], m["nz"]
+
+
+def _ncell(m):
+    return m["nx"] * m["ny"] * m["nz"]
+
+
+STATE_KEYS = ("u", "v", "w", "u_new", "v_new", "w_new", "p", "p_rhs", "nu_t")
+
+
+class Oracle:
+    def __init__(self, path=ORACLE_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"oracle library not built: {path} (run make -C oracle oracle)")
+        L = self.lib = C.CDLL(path)
+        L.orc_dot.restype = C.c_double
+        L.orc_norm.restype = C.c_double
+        L.orc_estimate_lambda_max.restype = C.c_double
+        L.orc_adaptive_dt.restype = C.c_double
+        L.orc_cs_delta_sq.restype = C.c_double
+        for f in ("orc_dot", "orc_norm", "orc_axpy", "orc_saxpy", "orc_copy", "orc_jacobi_precond"):
+            pass
+        L.orc_dot.argtypes = [DP, DP, C.c_long]
+        L.orc_norm.argtypes = [DP, C.c_long]
+        L.orc_jacobi_precond.argtypes = [DP, DP, DP, C.c_long]
+        L.orc_cheb_apply.argtypes = [DP, DP, C.POINTER(DP), C.c_int, C.c_int, C.c_int,
+                                     C.c_double, C.c_double, C.c_int]
+        L.orc_pcg.argtypes = [C.POINTER(DP), DP, DP, C.c_int, C.c_int, C.c_int, C.c_int,
+                              C.POINTER(DP), C.c_double, C.c_double, C.c_int, C.c_int,
+                              C.c_double, DP]
+        L.orc_calculate_divergence.argtypes = [DP, DP, DP, DP, U8P, C.POINTER(OrcMesh),
+                                               C.c_double, C.c_double]
+        L.orc_solve_pressure_jacobi.argtypes = [DP, DP, DP, DP, U8P, C.POINTER(OrcMesh),
+                                                C.c_double, C.c_double]
+        L.orc_solve_pressure_gs.argtypes = [DP, DP, DP, DP, U8P, C.POINTER(OrcMesh), C.c_double,
+                                            C.c_double, C.c_int, C.c_double]
+        L.orc_solve_pressure_rbsor.argtypes = L.orc_solve_pressure_gs.argtypes
+        L.orc_adaptive_dt.argtypes = [DP, DP, DP, DP, C.POINTER(OrcMesh), C.c_double]
+        L.orc_cs_delta_sq.argtypes = [C.POINTER(OrcMesh), C.c_double]
+        L.orc_sgs.argtypes = [DP, C.POINTER(DP), DP, DP, DP, C.POINTER(OrcMesh), C.c_double]
+        L.orc_momentum.argtypes = [DP, DP, DP, DP, DP, DP, DP, C.POINTER(OrcMesh), C.c_double,
+                                   C.c_double]
+        L.orc_correct.argtypes = [DP, DP, DP, DP, C.POINTER(OrcMesh), C.c_double, C.c_double]
+        L.orc_estimate_lambda_max.argtypes = [C.POINTER(DP), C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_step.argtypes = [C.POINTER(OrcFields), C.POINTER(OrcMesh), C.POINTER(OrcStepParams),
+                               C.POINTER(OrcStepStats)]
+
+    name = "oracle"
+
+    def set_threads(self, n):
+        self.lib.orc_set_threads(int(n))
+
+    @staticmethod
+    def _m(m):
+        return C.byref(OrcMesh(m["nx"], m["ny"], m["nz"], m["dx"], m["dy"], m["dz"]))
+
+    def cs_delta_sq(self, m, Cs):
+        return self.lib.orc_cs_delta_sq(self._m(m), Cs)
+
+    def dot(self, a, b):
+        return self.lib.orc_dot(_p(a), _p(b), a.size)
+
+    def build_stencil(self, solid, m):
+        A = [np.zeros(_ncell(m)) for _ in range(7)]
+        self.lib.orc_build_stencil(_p7(A), _p(solid), self._m(m))
+        return A
+
+    def matvec(self, p, A, m):
+        q = np.zeros_like(p)
+        self.lib.orc_matvec(_p(q), _p(p), _p7(A), *_dims(m))
+        return q
+
+    def divergence(self, un, vn, wn, solid, m, dt, rho):
+        r = np.zeros(_ncell(m))
+        self.lib.orc_calculate_divergence(_p(r), _p(un), _p(vn), _p(wn), _p(solid), self._m(m),
+                                          dt, rho)
+        return r
+
+    def build_b(self, div, solid, m):
+        b = np.zeros(_ncell(m))
+        self.lib.orc_build_b_vector(_p(b), _p(div), _p(solid), self._m(m))
+        return b
+
+    def jacobi_precond(self, r, Ac):
+        z = np.zeros_like(r)
+        self.lib.orc_jacobi_precond(_p(z), _p(r), _p(Ac), r.size)
+        return z
+
+    def ic_build(self, A, m):
+        L = [np.zeros(_ncell(m)) for _ in range(7)]
+        self.lib.orc_ic_build(_p7(L), _p7(A), *_dims(m))
+        return L
+
+    def ic_apply(self, r, L, m):
+        z = np.zeros_like(r)
+        self.lib.orc_ic_apply(_p(z), _p(r), _p7(L), *_dims(m))
+        return z
+
+    def cheb_apply(self, r, A, m, lmin, lmax, degree):
+        z = np.zeros_like(r)
+        self.lib.orc_cheb_apply(_p(z), _p(r), _p7(A), *_dims(m), lmin, lmax, degree)
+        return z
+
+    def lambda_max(self, A, m, n_iter=20):
+        return self.lib.orc_estimate_lambda_max(_p7(A), *_dims(m), n_iter)
+
+    def pcg(self, A, b, x0, m, precond, L=None, lmin=0.0, lmax=0.0, degree=5, max_iter=1000,
+            tol=1e-6):
+        x = x0.copy()
+        tr = np.zeros(max_iter + 1)
+        Lp = _p7(L) if L is not None else _p7(A)
+        it = self.lib.orc_pcg(_p7(A), _p(b), _p(x), *_dims(m), precond, Lp, lmin, lmax, degree,
+                              max_iter, tol, _p(tr))
+        return x, it, tr[: it + 1]
+
+    def pressure_bcs(self, p, solid, m):
+        p = p.copy()
+        self.lib.orc_apply_pressure_bcs(_p(p), _p(solid), self._m(m))
+        return p
+
+    def jacobi_solve(self, p, u, v, w, solid, m, dt, rho):
+        p = p.copy()
+        s = self.lib.orc_solve_pressure_jacobi(_p(p), _p(u), _p(v), _p(w), _p(solid), self._m(m),
+                                               dt, rho)
+        return p, s
+
+    def gs_solve(self, p, u, v, w, solid, m, dt, rho, max_iter=200, omega=1.7):
+        p = p.copy()
+        s = self.lib.orc_solve_pressure_gs(_p(p), _p(u), _p(v), _p(w), _p(solid), self._m(m), dt,
+                                           rho, max_iter, omega)
+        return p, s
+
+    def rbsor_solve(self, p, u, v, w, solid, m, dt, rho, max_iter=200, omega=1.7):
+        p = p.copy()
+        s = self.lib.orc_solve_pressure_rbsor(_p(p), _p(u), _p(v), _p(w), _p(solid), self._m(m),
+                                              dt, rho, max_iter, omega)
+        return p, s
+
+    def adaptive_dt(self, u, v, w, nu_t, m, nu):
+        return self.lib.orc_adaptive_dt(_p(u), _p(v), _p(w), _p(nu_t), self._m(m), nu)
+
+    def sgs(self, u, v, w, m, Cs=0.18, nu_t=None, want_tau=False):
+        nt = np.zeros(_ncell(m)) if nu_t is None else nu_t.copy()
+        tau = [np.zeros(_ncell(m)) for _ in range(6)] if want_tau else None
+        self.lib.orc_sgs(_p(nt), _p6(tau) if tau else None, _p(u), _p(v), _p(w), self._m(m), Cs)
+        return (nt, tau) if want_tau else nt
+
+    def nut_bcs(self, nu_t, solid, m):
+        nt = nu_t.copy()
+        self.lib.orc_nut_bcs(_p(nt), _p(solid), self._m(m))
+        return nt
+
+    def momentum(self, u, v, w, un, vn, wn, nu_t, m, dt, nu):
+        un, vn, wn = un.copy(), vn.copy(), wn.copy()
+        self.lib.orc_momentum(_p(un), _p(vn), _p(wn), _p(u), _p(v), _p(w), _p(nu_t), self._m(m),
+                              dt, nu)
+        # the reference swaps: Field.u <- new buffer, Field.u_new <- old u
+        return un, vn, wn, u.copy(), v.copy(), w.copy()
+
+    def velocity_bcs(self, u, v, w, solid, m):
+        u, v, w = u.copy(), v.copy(), w.copy()
+        self.lib.orc_velocity_bcs(_p(u), _p(v), _p(w), _p(solid), self._m(m))
+        return u, v, w
+
+    def correct(self, u, v, w, p, m, dt, rho):
+        u, v, w = u.copy(), v.copy(), w.copy()
+        self.lib.orc_correct(_p(u), _p(v), _p(w), _p(p), self._m(m), dt, rho)
+        return u, v, w
+
+    def run_steps(self, state, solid, m, A, nsteps, step0=0, precond=PC_LEGACY_IC_A, nu=1e-3,
+                  rho=1.2, Cs=0.18, dt_fixed=0.0, lmin=0.0, lmax=0.0, degree=5, max_iter=1000,
+                  tol=1e-6, tol_rel=False, L=None):
+        st = {k: state[k].copy() for k in STATE_KEYS}
+        solid = np.ascontiguousarray(solid)
+        f = OrcFields(*[_p(st[k]) for k in STATE_KEYS], _p(solid))
+        A7 = _p7(A)
+        L7 = _p7(L) if L is not None else A7
+        iters, dts = [], []
+        for s in range(nsteps):
+            prm = OrcStepParams(nu, rho, Cs, step0 + s, dt_fixed, precond, degree, max_iter, lmin,
+                                lmax, tol, 1 if tol_rel else 0, L7, A7)
+            stats = OrcStepStats()
+            self.lib.orc_step(C.byref(f), self._m(m), C.byref(prm), C.byref(stats))
+            iters.append(stats.iters)
+            dts.append(stats.dt)
+        # orc_step swaps pointers; read back through the struct
+        n = _ncell(m)
+        out = {}
+        for k in STATE_KEYS:
+            ptr = getattr(f, k)
+            out[k] = np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+        return out, iters, dts
+
+
+class Reference:
+    """The unmodified reference (src/*.cpp) through oracle/ref_harness.cpp."""
+
+    name = "reference"
+
+    def __init__(self, path=REF_SO):
+        if not os.path.exists(path):
+            raise FileNotFoundError(path)
+        L = self.lib = C.CDLL(path)
+        L.ref_lambda_max.restype = C.c_double
+        L.ref_adaptive_dt.restype = C.c_double
+        L.ref_jacobi_precond.argtypes = [DP, DP, DP, C.c_long]
+        L.ref_cheb_apply.argtypes = [DP, DP, C.POINTER(DP), C.c_int, C.c_int, C.c_int, C.c_double,
+                                     C.c_double, C.c_int]
+        L.ref_lambda_max.argtypes = [C.POINTER(DP), C.c_int, C.c_int, C.c_int, C.c_int]
+        L.ref_pcg_legacy.argtypes = [C.POINTER(DP), DP, DP, C.c_int, C.c_int, C.c_int, C.c_int,
+                                     C.c_double]
+        L.ref_pcg_cheb.argtypes = [C.POINTER(DP), DP, DP, C.c_int, C.c_int, C.c_int, C.c_double,
+                                   C.c_double, C.c_int, C.c_int, C.c_double]
+        L.ref_pcg_harness.argtypes = [C.POINTER(DP), DP, DP, C.c_int, C.c_int, C.c_int, C.c_int,
+                                      C.POINTER(DP), C.c_double, C.c_double, C.c_int, C.c_int,
+                                      C.c_double, DP]
+        md = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double, C.c_double]
+        L.ref_build_stencil.argtypes = [C.POINTER(DP), U8P] + md
+        L.ref_divergence.argtypes = [DP, DP, DP, DP, U8P] + md + [C.c_double, C.c_double]
+        L.ref_jacobi_solve.argtypes = [DP, DP, DP, DP, U8P] + md + [C.c_double, C.c_double]
+        L.ref_gs_solve.argtypes = [DP, DP, DP, DP, U8P] + md + [C.c_double, C.c_double, C.c_int]
+        L.ref_adaptive_dt.argtypes = [DP, DP, DP, DP] + md + [C.c_double]
+        L.ref_sgs.argtypes = [DP, C.POINTER(DP), DP, DP, DP] + md
+        L.ref_momentum.argtypes = [DP] * 7 + md + [C.c_double, C.c_double]
+        L.ref_correct.argtypes = [DP, DP, DP, DP] + md + [C.c_double, C.c_double]
+        L.ref_run_steps.argtypes = [DP] * 9 + [U8P, C.POINTER(DP)] + md + [
+            C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
+            C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, IP, DP]
+
+    def set_threads(self, n):
+        self.lib.ref_set_threads(int(n))
+
+    @staticmethod
+    def _md(m):
+        return (m["nx"], m["ny"], m["nz"], m["dx"], m["dy"], m["dz"])
+
+    def build_stencil(self, solid, m):
+        A = [np.zeros(_ncell(m)) for _ in range(7)]
+        self.lib.ref_build_stencil(_p7(A), _p(solid), *self._md(m))
+        return A
+
+    def matvec(self, p, A, m):
+        q = np.zeros_like(p)
+        self.lib.ref_matvec(_p(q), _p(p), _p7(A), *_dims(m))
+        return q
+
+    def divergence(self, un, vn, wn, solid, m, dt, rho):
+        r = np.zeros(_ncell(m))
+        self.lib.ref_divergence(_p(r), _p(un), _p(vn), _p(wn), _p(solid), *self._md(m), dt, rho)
+        return r
+
+    def build_b(self, div, solid, m):
+        b = np.zeros(_ncell(m))
+        self.lib.ref_build_b(_p(b), _p(div), _p(solid), *_dims(m))
+        return b
+
+    def jacobi_precond(self, r, Ac):
+        z = np.zeros_like(r)
+        self.lib.ref_jacobi_precond(_p(z), _p(r), _p(Ac), r.size)
+        return z
+
+    def ic_build(self, A, m):
+        L = [np.zeros(_ncell(m)) for _ in range(7)]
+        self.lib.ref_ic_build(_p7(L), _p7(A), *_dims(m))
+        return L
+
+    def ic_apply(self, r, L, m):
+        z = np.zeros_like(r)
+        self.lib.ref_ic_apply(_p(z), _p(r), _p7(L), *_dims(m))
+        return z
+
+    def cheb_apply(self, r, A, m, lmin, lmax, degree):
+        z = np.zeros_like(r)
+        self.lib.ref_cheb_apply(_p(z), _p(r), _p7(A), *_dims(m), lmin, lmax, degree)
+        return z
+
+    def lambda_max(self, A, m, n_iter=20):
+        return self.lib.ref_lambda_max(_p7(A), *_dims(m), n_iter)
+
+    def pcg(self, A, b, x0, m, precond, L=None, lmin=0.0, lmax=0.0, degree=5, max_iter=1000,
+            tol=1e-6):
+        x = x0.copy()
+        tr = np.zeros(max_iter + 1)
+        Lp = _p7(L) if L is not None else _p7(A)
+        it = self.lib.ref_pcg_harness(_p7(A), _p(b), _p(x), *_dims(m), precond, Lp, lmin, lmax,
+                                      degree, max_iter, tol, _p(tr))
+        return x, it, tr[: it + 1]
+
+    def pcg_legacy(self, A, b, x0, m, max_iter=1000, tol=1e-6):
+        x = x0.copy()
+        it = self.lib.ref_pcg_legacy(_p7(A), _p(b), _p(x), *_dims(m), max_iter, tol)
+        return x, it
+
+    def pcg_cheb(self, A, b, x0, m, lmin, lmax, degree=5, max_iter=1000, tol=1e-6):
+        x = x0.copy()
+        it = self.lib.ref_pcg_cheb(_p7(A), _p(b), _p(x), *_dims(m), lmin, lmax, degree, max_iter,
+                                   tol)
+        return x, it
+
+    def pressure_bcs(self, p, solid, m):
+        p = p.copy()
+        self.lib.ref_pressure_bcs(_p(p), _p(solid), *_dims(m))
+        return p
+
+    def jacobi_solve(self, p, u, v, w, solid, m, dt, rho):
+        p = p.copy()
+        self.lib.ref_jacobi_solve(_p(p), _p(u), _p(v), _p(w), _p(solid), *self._md(m), dt, rho)
+        return p, None
+
+    def gs_solve(self, p, u, v, w, solid, m, dt, rho, max_iter=200, omega=1.7):
+        assert omega == 1.7
+        p = p.copy()
+        self.lib.ref_gs_solve(_p(p), _p(u), _p(v), _p(w), _p(solid), *self._md(m), dt, rho,
+                              max_iter)
+        return p, None
+
+    def adaptive_dt(self, u, v, w, nu_t, m, nu):
+        return self.lib.ref_adaptive_dt(_p(u), _p(v), _p(w), _p(nu_t), *self._md(m), nu)
+
+    def sgs(self, u, v, w, m, Cs=0.18, nu_t=None, want_tau=False):
+        assert Cs == 0.18, "the reference fixes Cs = 0.18 (main.cpp:425)"
+        nt = np.zeros(_ncell(m)) if nu_t is None else nu_t.copy()
+        tau = [np.zeros(_ncell(m)) for _ in range(6)] if want_tau else None
+        self.lib.ref_sgs(_p(nt), _p6(tau) if tau else None, _p(u), _p(v), _p(w), *self._md(m))
+        return (nt, tau) if want_tau else nt
+
+    def nut_bcs(self, nu_t, solid, m):
+        nt = nu_t.copy()
+        self.lib.ref_nut_bcs(_p(nt), _p(solid), *_dims(m))
+        return nt
+
+    def momentum(self, u, v, w, un, vn, wn, nu_t, m, dt, nu):
+        a = [x.copy() for x in (u, v, w, un, vn, wn)]
+        self.lib.ref_momentum(*[_p(x) for x in a], _p(nu_t), *self._md(m), dt, nu)
+        return tuple(a)
+
+    def velocity_bcs(self, u, v, w, solid, m):
+        u, v, w = u.copy(), v.copy(), w.copy()
+        self.lib.ref_velocity_bcs(_p(u), _p(v), _p(w), _p(solid), *_dims(m))
+        return u, v, w
+
+    def correct(self, u, v, w, p, m, dt, rho):
+        u, v, w = u.copy(), v.copy(), w.copy()
+        self.lib.ref_correct(_p(u), _p(v), _p(w), _p(p), *self._md(m), dt, rho)
+        return u, v, w
+
+    def run_steps(self, state, solid, m, A, nsteps, step0=0, solver=0, nu=1e-3, rho=1.2,
+                  dt_fixed=0.0, lmin=0.0, lmax=0.0, degree=5, max_iter=1000, tol=1e-6,
+                  tol_rel=False, gs_iters=200):
+        st = {k: state[k].copy() for k in STATE_KEYS}
+        iters = np.zeros(nsteps, dtype=np.int32)
+        dts = np.zeros(nsteps)
+        self.lib.ref_run_steps(*[_p(st[k]) for k in STATE_KEYS], _p(np.ascontiguousarray(solid)),
+                               _p7(A) if A is not None else None, *self._md(m), nu, rho, step0,
+                               nsteps, dt_fixed, solver, lmin, lmax, degree, max_iter, tol,
+                               1 if tol_rel else 0, gs_iters, iters.ctypes.data_as(IP), _p(dts))
+        return st, list(iters), list(dts)
+
+
+def initial_state(u, v, w, n, solid=None, m=None, checker=None):
+    """A Field as the reference driver starts it: zeros, then u,v,w set and
+    apply_velocity_BCs (main.cpp:102) applied when a checker is given."""
+    st = {k: np.zeros(n) for k in STATE_KEYS}
+    st["u"], st["v"], st["w"] = u.copy(), v.copy(), w.copy()
+    if checker is not None:
+        st["u"], st["v"], st["w"] = checker.velocity_bcs(st["u"], st["v"], st["w"], solid, m)
+    return st

@@ -1,0 +1,321 @@
+"""GPU parity: the HIP library (through its C ABI) against the oracle on the
+same seeded inputs.
+
+Bars: pointwise kernels bit for bit (the library is built without FP
+contraction and keeps the reference's operand order); max-reductions bit
+for bit (order-free); everything downstream of a dot product (PCG, power
+iteration) within the stated relative tolerance, because the GPU sums in a
+different (deterministic) order than the reference's sequential loop.
+"""
+import numpy as np
+import pytest
+
+import lesgpu
+from cases import random_case, box_case, bits_equal, rel_err
+from cpu_checkers import Oracle, initial_state, PC_NONE, PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, \
+    PC_CHEBYSHEV
+
+pytestmark = pytest.mark.gpu
+
+SHAPES = [(7, 6, 5, 1), (13, 9, 17, 2), (33, 20, 70, 3)]
+FP64_REL_TOL = 1e-10   # fields after PCG / whole steps vs the oracle
+
+
+@pytest.fixture(scope="module")
+def orc():
+    o = Oracle()
+    o.set_threads(1)
+    return o
+
+
+def ctx_for(m, solid=None):
+    g = lesgpu.LesGpu.from_mesh(m)
+    if solid is not None:
+        g.set_mask(solid)
+    return g
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_stencil_matvec_bitexact(orc, shape):
+    m, solid, f = random_case(*shape)
+    with ctx_for(m, solid) as g:
+        g.build_stencil()
+        A_g = g.get_stencil()
+        A_o = orc.build_stencil(solid, m)
+        for a, b in zip(A_g, A_o):
+            assert bits_equal(a, b)
+        assert bits_equal(g.matvec(f[0]), orc.matvec(f[0], A_o, m))
+        # a user-supplied (arbitrary) stencil through les_set_stencil
+        Ar = [f[q] for q in range(7)]
+        g.set_stencil(Ar)
+        assert bits_equal(g.matvec(f[7]), orc.matvec(f[7], Ar, m))
+        # the shipped driver's order: A built before the obstacle exists
+        g.build_stencil(ignore_mask=True)
+        A0 = orc.build_stencil(np.zeros_like(solid), m)
+        for a, b in zip(g.get_stencil(), A0):
+            assert bits_equal(a, b)
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_physics_bitexact(orc, shape):
+    m, solid, f = random_case(*shape)
+    u, v, w = f[0], f[1], f[2]
+    nt0 = np.abs(f[3]) * 1e-3
+    with ctx_for(m, solid) as g:
+        g.set_state({"u": u, "v": v, "w": w, "nu_t": nt0, "u_new": f[4], "v_new": f[5],
+                     "w_new": f[6]})
+        # adaptive dt (order-free max reductions)
+        assert g.adaptive_dt(1e-3) == orc.adaptive_dt(u, v, w, nt0, m, 1e-3)
+        # compute_sgs_stress alone, with the six stresses
+        tau = g.sgs(0.17, apply_bcs=False, want_tau=True)
+        nt_o, tau_o = orc.sgs(u, v, w, m, 0.17, nu_t=nt0, want_tau=True)
+        assert bits_equal(g.get_field("nu_t"), nt_o)
+        for a, b in zip(tau, tau_o):
+            assert bits_equal(a, b)
+        g.nut_bcs()
+        assert bits_equal(g.get_field("nu_t"), orc.nut_bcs(nt_o, solid, m))
+        # fused sgs + BCs
+        g.set_field("nu_t", nt0)
+        g.sgs(0.17, apply_bcs=True)
+        nt_fused = orc.nut_bcs(nt_o, solid, m)
+        assert bits_equal(g.get_field("nu_t"), nt_fused)
+        # solve_momentum alone (with its buffer swap), then BCs
+        g.momentum(1e-3, 1e-3, apply_bcs=False)
+        mo = orc.momentum(u, v, w, f[4], f[5], f[6], nt_fused, m, 1e-3, 1e-3)
+        for name, ref in zip(("u", "v", "w", "u_new", "v_new", "w_new"), mo):
+            assert bits_equal(g.get_field(name), ref), name
+        g.velocity_bcs()
+        ub = orc.velocity_bcs(mo[0], mo[1], mo[2], solid, m)
+        for name, ref in zip(("u", "v", "w"), ub):
+            assert bits_equal(g.get_field(name), ref), name
+        # divergence of u_new and b
+        g.divergence_b(1e-3, 1.2)
+        div = orc.divergence(mo[3], mo[4], mo[5], solid, m, 1e-3, 1.2)
+        assert bits_equal(g.get_field("p_rhs"), div)
+        assert bits_equal(g.get_field("b"), orc.build_b(div, solid, m))
+        # correct alone, then correct + BCs
+        g.set_field("p", f[7])
+        g.correct(1e-3, 1.2, apply_bcs=False)
+        co = orc.correct(ub[0], ub[1], ub[2], f[7], m, 1e-3, 1.2)
+        for name, ref in zip(("u", "v", "w"), co):
+            assert bits_equal(g.get_field(name), ref), name
+        g.set_state({"u": ub[0], "v": ub[1], "w": ub[2]})
+        g.correct(1e-3, 1.2, apply_bcs=True)
+        cb = orc.velocity_bcs(*co, solid, m)
+        for name, ref in zip(("u", "v", "w"), cb):
+            assert bits_equal(g.get_field(name), ref), name
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_momentum_fused_bcs_bitexact(orc, shape):
+    """The step's fused momentum+BC kernel equals solve_momentum followed by
+    apply_velocity_BCs, including inlet solid cells that keep stale values."""
+    m, solid, f = random_case(*shape)
+    solid = solid.copy()
+    nz = m["nz"]
+    solid[[nz + 1, 2 * nz + 2, 3 * nz + 1]] = 1  # inlet plane (i = 0)
+    with ctx_for(m, solid) as g:
+        g.set_state({"u": f[0], "v": f[1], "w": f[2], "nu_t": np.abs(f[3]) * 1e-3,
+                     "u_new": f[4], "v_new": f[5], "w_new": f[6]})
+        g.momentum(2e-3, 1e-3, apply_bcs=True)
+        mo = orc.momentum(f[0], f[1], f[2], f[4], f[5], f[6], np.abs(f[3]) * 1e-3, m, 2e-3, 1e-3)
+        ub = orc.velocity_bcs(mo[0], mo[1], mo[2], solid, m)
+        for name, ref in zip(("u", "v", "w"), ub):
+            assert bits_equal(g.get_field(name), ref), name
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_preconditioners_bitexact(orc, shape):
+    m, solid, f = random_case(*shape)
+    with ctx_for(m, solid) as g:
+        g.build_stencil()
+        A = orc.build_stencil(solid, m)
+        r = f[0]
+        assert bits_equal(g.precond_apply(lesgpu.PC_JACOBI, r), orc.jacobi_precond(r, A[0]))
+        assert bits_equal(g.precond_apply(lesgpu.PC_LEGACY_IC_A, r), orc.ic_apply(r, A, m))
+        g.ic0_factor()
+        L_o = orc.ic_build(A, m)
+        for a, b in zip(g.get_stencil(lesgpu.STENCIL_L), L_o):
+            assert bits_equal(a, b)
+        assert bits_equal(g.precond_apply(lesgpu.PC_IC0, r), orc.ic_apply(r, L_o, m))
+        lmax = orc.lambda_max(A, m, 20)
+        for deg in (1, 2, 4, 8):
+            zg = g.precond_apply(lesgpu.PC_CHEBYSHEV, r, 3 * np.pi ** 2, lmax, deg)
+            assert bits_equal(zg, orc.cheb_apply(r, A, m, 3 * np.pi ** 2, lmax, deg)), deg
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_power_iteration(orc, shape):
+    m, solid, _ = random_case(*shape)
+    with ctx_for(m, solid) as g:
+        g.build_stencil()
+        A = orc.build_stencil(solid, m)
+        lg = g.power_lambda_max(20)
+        lo = orc.lambda_max(A, m, 20)
+        assert abs(lg - lo) <= 1e-12 * abs(lo)
+
+
+@pytest.mark.parametrize("pc", [PC_NONE, PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV])
+@pytest.mark.parametrize("n,seed", [(12, 1), (24, 2)])
+def test_pcg(orc, pc, n, seed):
+    m, solid, f = box_case(n, seed)
+    A = orc.build_stencil(solid, m)
+    L = orc.ic_build(A, m)
+    lmax = orc.lambda_max(A, m, 20)
+    b = orc.build_b(orc.divergence(f[0], f[1], f[2], solid, m, 1e-3, 1.2), solid, m)
+    x0 = f[3] * 10.0
+    kw = dict(lmin=3 * np.pi ** 2, lmax=lmax, degree=4)
+    with ctx_for(m, solid) as g:
+        g.build_stencil()
+        if pc == PC_IC0:
+            g.ic0_factor()
+        g.set_field("b", b)
+        # fixed iteration count: x after k iterations
+        for k in (1, 5, 20):
+            xo, io, tr_o = orc.pcg(A, b, x0, m, pc, L=L, max_iter=k, tol=0.0, **kw)
+            g.set_field("p", x0)
+            res = g.pcg(precond=pc, max_iter=k, tol=0.0, **kw)
+            assert res["iters"] == io == k
+            xg = g.get_field("p")
+            assert rel_err(xg, xo) < FP64_REL_TOL, (k, rel_err(xg, xo))
+            tr_g = g.pcg_trace(k + 1)
+            assert np.allclose(tr_g, tr_o, rtol=1e-9, atol=0)
+        # to a relative tolerance of 1e-8
+        tol = 1e-8 * np.linalg.norm(b)
+        xo, io, _ = orc.pcg(A, b, x0, m, pc, L=L, max_iter=2000, tol=tol, **kw)
+        g.set_field("p", x0)
+        res = g.pcg(precond=pc, max_iter=2000, tol=1e-8, tol_rel=True, **kw)
+        assert res["converged"] == 1
+        assert abs(res["iters"] - io) <= 1
+        assert rel_err(g.get_field("p"), xo) < 1e-7
+
+
+@pytest.mark.parametrize("shape", SHAPES[:2])
+def test_stationary_bitexact(orc, shape):
+    m, solid, f = random_case(*shape, frac=0.05)
+    p0 = f[3] * 0.01
+    with ctx_for(m, solid) as g:
+        g.set_state({"u": f[0], "v": f[1], "w": f[2], "p": f[3]})
+        g.pressure_bcs()
+        assert bits_equal(g.get_field("p"), orc.pressure_bcs(f[3], solid, m))
+        g.set_field("p", p0)
+        s = g.jacobi_solve(1e-3, 1.2)
+        po, so = orc.jacobi_solve(p0, f[0], f[1], f[2], solid, m, 1e-3, 1.2)
+        assert s == so
+        assert bits_equal(g.get_field("p"), po)
+        for it in (1, 3, 40):
+            g.set_field("p", p0)
+            s = g.sor_solve(1e-3, 1.2, 1.7, it, lesgpu.SOLVER_SOR_LEX)
+            po, so = orc.gs_solve(p0, f[0], f[1], f[2], solid, m, 1e-3, 1.2, it)
+            assert s == so
+            assert bits_equal(g.get_field("p"), po), it
+            g.set_field("p", p0)
+            s = g.sor_solve(1e-3, 1.2, 1.7, it, lesgpu.SOLVER_SOR_RB)
+            po, so = orc.rbsor_solve(p0, f[0], f[1], f[2], solid, m, 1e-3, 1.2, it)
+            assert s == so
+            assert bits_equal(g.get_field("p"), po), it
+
+
+@pytest.mark.parametrize("pc", [PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV])
+def test_steps(orc, pc):
+    """Whole LES steps (main.cpp:115-144) from step 18 so the run crosses
+    into the adaptive time step, with inlet solid cells."""
+    n = 16
+    m, solid, f = box_case(n, 7, inlet_solids=True)
+    A = orc.build_stencil(solid, m)
+    L = orc.ic_build(A, m) if pc == PC_IC0 else None
+    lmax = orc.lambda_max(A, m, 20)
+    st0 = initial_state(f[0], f[1], f[2], n ** 3, solid, m, orc)
+    kw = dict(lmin=3 * np.pi ** 2, lmax=lmax, degree=4, max_iter=3000, tol=1e-10, tol_rel=True)
+    so, io, do = orc.run_steps(st0, solid, m, A, 5, step0=18, precond=pc, L=L, nu=1e-3, rho=1.2,
+                               Cs=0.17, **kw)
+    with ctx_for(m, solid) as g:
+        g.set_state(st0)
+        g.build_stencil()
+        if pc == PC_IC0:
+            g.ic0_factor()
+        its = []
+        for s in range(5):
+            r = g.step(step=18 + s, nu=1e-3, rho=1.2, Cs=0.17, precond=pc, **{
+                "lmin": kw["lmin"], "lmax": kw["lmax"], "degree": 4, "max_iter": 3000,
+                "tol": 1e-10, "tol_rel": True})
+            assert r["converged"] == 1
+            its.append(r["iters"])
+        out = g.get_state()
+    for a, b in zip(its, io):
+        assert abs(a - b) <= 1
+    for k in ("u", "v", "w", "p", "nu_t", "u_new", "v_new", "w_new", "p_rhs"):
+        assert rel_err(out[k], so[k]) < 1e-9, (k, rel_err(out[k], so[k]))
+
+
+def test_steps_sor(orc):
+    """The SOR alternative (main.cpp:133) over steps, bit for bit."""
+    n = 12
+    m, solid, f = box_case(n, 3)
+    st0 = initial_state(f[0], f[1], f[2], n ** 3, solid, m, orc)
+    with ctx_for(m, solid) as g:
+        g.set_state(st0)
+        for s in range(2):
+            g.step(step=s, nu=1e-3, rho=1.2, Cs=0.18, solver=lesgpu.SOLVER_SOR_LEX, omega=1.7,
+                   sor_sweeps=25)
+        out = g.get_state()
+    st = {k: st0[k].copy() for k in st0}
+    for _ in range(2):
+        dt, rho = 1e-3, 1.2
+        st["nu_t"] = orc.nut_bcs(orc.sgs(st["u"], st["v"], st["w"], m, 0.18, nu_t=st["nu_t"]),
+                                 solid, m)
+        st["u"], st["v"], st["w"], st["u_new"], st["v_new"], st["w_new"] = orc.momentum(
+            st["u"], st["v"], st["w"], st["u_new"], st["v_new"], st["w_new"], st["nu_t"], m, dt,
+            1e-3)
+        st["u"], st["v"], st["w"] = orc.velocity_bcs(st["u"], st["v"], st["w"], solid, m)
+        st["p"], _ = orc.gs_solve(st["p"], st["u"], st["v"], st["w"], solid, m, dt, rho, 25)
+        st["u"], st["v"], st["w"] = orc.correct(st["u"], st["v"], st["w"], st["p"], m, dt, rho)
+        st["u"], st["v"], st["w"] = orc.velocity_bcs(st["u"], st["v"], st["w"], solid, m)
+    for k in ("u", "v", "w", "u_new", "v_new", "w_new", "p", "nu_t"):
+        assert bits_equal(out[k], st[k]), k
+
+
+@pytest.mark.parametrize("pc", [PC_JACOBI, PC_CHEBYSHEV, PC_IC0])
+def test_mixed_precision_pcg(orc, pc):
+    """FP32 iteration + FP64 reliable replacement reaches 1e-8 relative in
+    FP64 and agrees with the FP64 oracle solution."""
+    n = 32
+    m, solid, f = box_case(n, 5)
+    A = orc.build_stencil(solid, m)
+    L = orc.ic_build(A, m)
+    lmax = orc.lambda_max(A, m, 20)
+    b = orc.build_b(orc.divergence(f[0], f[1], f[2], solid, m, 1e-3, 1.2), solid, m)
+    kw = dict(lmin=3 * np.pi ** 2, lmax=lmax, degree=8)
+    x_ref, _, _ = orc.pcg(A, b, np.zeros_like(b), m, pc, L=L, max_iter=5000,
+                          tol=1e-13 * np.linalg.norm(b), **kw)
+    with ctx_for(m, solid) as g:
+        g.build_stencil()
+        if pc == PC_IC0:
+            g.ic0_factor()
+        g.set_field("b", b)
+        g.set_field("p", np.zeros_like(b))
+        res = g.pcg(precond=pc, max_iter=5000, tol=1e-8, tol_rel=True,
+                    precision=lesgpu.PREC_MIXED, reliable_every=50, **kw)
+        x = g.get_field("p")
+    assert res["converged"] == 1
+    assert res["true_res_norm"] < 1e-8 * res["b_norm"]
+    true_res = np.linalg.norm(b - orc.matvec(x, A, m))
+    assert true_res < 1.01e-8 * np.linalg.norm(b)
+    assert rel_err(x, x_ref) < 1e-5
+
+
+def test_lanczos_closed_form():
+    """On an obstacle-free grid the largest eigenvalue of A is known in
+    closed form; 40 Lanczos steps approach it from below."""
+    n = 24
+    m = dict(nx=n, ny=n, nz=n, dx=1.0 / n, dy=1.0 / n, dz=1.0 / n)
+    lam_max = (4 * n * n * np.sin((2 * n - 3) * np.pi / (4 * (n - 1))) ** 2
+               + 4 * n * n + 4 * n * n)
+    with ctx_for(m, np.zeros(n ** 3, dtype=np.uint8)) as g:
+        g.build_stencil()
+        lo, hi = g.lanczos_bounds(40, seed=3)
+    lam_min_fluid = 4 * n * n * np.sin(np.pi / (4 * (n - 1))) ** 2
+    # Ritz values lie inside the fluid block's spectrum and approach its ends
+    assert hi <= lam_max * (1 + 1e-9)
+    assert hi > 0.97 * lam_max
+    assert lam_min_fluid * (1 - 1e-9) <= lo < hi
