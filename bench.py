@@ -4,7 +4,10 @@
 Workload (mixed-precision configuration of the reference's LES step):
   256^3 unit cube, spectral velocity field (E(k) ~ k^-5/3, rms 1, seed 0),
   5% of cells solid (Bernoulli, seed 0), p = 0, nu = 1e-3, rho = 1.2,
-  Cs = 0.17, dt = 1e-3 for steps <= 20 then the reference's adaptive rule,
+  Cs = 0.17, steps numbered from 21 so the reference's adaptive time step
+  (main.cpp:115-119: min(CFL 0.4, viscous 0.2, 1e-4)) sets dt -- at 256^3
+  the fixed dt = 1e-3 of steps 0-20 puts the explicit scheme at CFL ~1.3 and
+  the field diverges within 3 steps, in the reference's arithmetic as here,
   pressure by PCG-Chebyshev degree 8 on bounds from 20 Lanczos steps,
   stopping at ||r|| <= 1e-8 ||b||; mixed precision replaces the residual in
   FP64 every 50 iterations.
@@ -66,7 +69,7 @@ def make_workload(n, seed=0):
     return wl.mesh(n), (u, v, w), solid
 
 
-def run_mode(g, st0, precision, bounds, steps, warmup, kt_class, pcg_kw):
+def run_mode(g, st0, precision, bounds, steps, warmup, kt_class, pcg_kw, step0):
     import torch
     import lesgpu
     g.set_state(st0)
@@ -75,14 +78,14 @@ def run_mode(g, st0, precision, bounds, steps, warmup, kt_class, pcg_kw):
     kw = dict(nu=1e-3, rho=1.2, Cs=0.17, precond=lesgpu.PC_CHEBYSHEV, lmin=lmin, lmax=lmax,
               degree=8, max_iter=20000, tol=1e-8, tol_rel=True, precision=precision,
               reliable_every=50, **pcg_kw)
-    for s in range(warmup):
+    for s in range(step0, step0 + warmup):
         r = g.step(step=s, **kw)
         log(f"  warmup step {s}: iters {r['iters']} conv {r['converged']}")
     g.synchronize()
     torch.cuda.synchronize()
     g.kernel_timing(kt_class)
     t0 = time.perf_counter()
-    for s in range(warmup, warmup + steps):
+    for s in range(step0 + warmup, step0 + warmup + steps):
         r = g.step(step=s, **kw)
         its.append(r["iters"])
         info.append(r)
@@ -97,7 +100,7 @@ def run_mode(g, st0, precision, bounds, steps, warmup, kt_class, pcg_kw):
     return dict(seconds=t1 - t0, iters=its, last=info[-1], kt=kt, state=g.get_state())
 
 
-def cpu_baseline(m, st0, solid, bounds, threads):
+def cpu_baseline(m, st0, solid, bounds, threads, step0):
     """One LES step of the same workload by the reference solver built from
     its own sources (oracle/_ref), else by the oracle; wall time on the
     host cores."""
@@ -114,10 +117,10 @@ def cpu_baseline(m, st0, solid, bounds, threads):
     t0 = time.perf_counter()
     if kind == "reference":
         # reference Cs is fixed at 0.18 (main.cpp:425); the cost is the same
-        _, its, _ = chk.run_steps(st0, solid, m, None, 1, step0=0, solver=4, **kw)
+        _, its, _ = chk.run_steps(st0, solid, m, None, 1, step0=step0, solver=4, **kw)
     else:
-        _, its, _ = chk.run_steps(st0, solid, m, A, 1, step0=0, precond=PC_CHEBYSHEV, Cs=0.17,
-                                  **kw)
+        _, its, _ = chk.run_steps(st0, solid, m, A, 1, step0=step0, precond=PC_CHEBYSHEV,
+                                  Cs=0.17, **kw)
     t = time.perf_counter() - t0
     return dict(value=1.0 / t, unit="steps/s", cores=threads, kind=kind,
                 sample=f"1 step of the {m['nx']}^3 workload from the same initial state, "
@@ -135,6 +138,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--check-every", type=int, default=0)
+    ap.add_argument("--step0", type=int, default=21)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -169,7 +173,7 @@ def main():
         prec = lesgpu.PREC_MIXED if mode == "mixed" else lesgpu.PREC_FP64
         log(f"mode {mode}: {args.warmup} warmup + {args.steps} timed steps")
         res[mode] = run_mode(g, st0, prec, bounds, args.steps, args.warmup, lesgpu.KT_CHEB_STEP,
-                             pcg_kw)
+                             pcg_kw, args.step0)
         r = res[mode]
         log(f"  {mode}: {args.steps / r['seconds']:.3f} steps/s, iters/step "
             f"{np.mean(r['iters']):.1f}, cheb launches {r['kt'][0]} total {r['kt'][1]:.1f} ms")
@@ -197,6 +201,7 @@ def main():
                                f"bounds to 1e-8 rel, {head} precision",
                    "grid": [m["nx"], m["ny"], m["nz"]], "solid_fraction": 0.05,
                    "precond": "chebyshev", "degree": 8, "reliable_every": 50,
+                   "first_step": args.step0, "dt": "adaptive (main.cpp:208-250)",
                    "lanczos_bounds": list(bounds)},
         "pcg_iters_per_step": float(np.mean(R["iters"])),
         "pcg_iters_per_s": float(np.sum(R["iters"]) / R["seconds"]),
@@ -223,7 +228,8 @@ def main():
     if not args.no_cpu_baseline:
         log(f"CPU baseline: 1 reference step on {args.cpu_threads} threads")
         try:
-            out["cpu_baseline"] = cpu_baseline(m, st0, solid, bounds, args.cpu_threads)
+            out["cpu_baseline"] = cpu_baseline(m, st0, solid, bounds, args.cpu_threads,
+                                               args.step0)
         except Exception as e:  # reported, not hidden
             out["cpu_baseline"] = {"error": repr(e)}
     g.close()

@@ -112,8 +112,11 @@ int les_precond_apply(les_ctx* ctx, int precond, const double* r, double* z, int
 /* estimate_lambda_max (pressure.cpp:917): power iteration from the
  * mt19937(1337) U(-1,1) start vector drawn in sequence. */
 int les_power_lambda_max(les_ctx* ctx, int n_iter, double* out);
-/* Lanczos extreme Ritz values of A after `steps` steps from a seeded start
- * vector (no reference counterpart; bounds for the Chebyshev polynomial). */
+/* Lanczos extreme Ritz values after `steps` steps from a seeded start vector
+ * (no reference counterpart; bounds for the Chebyshev polynomial).  A is not
+ * symmetric (mirror-Neumann rows), so Lanczos runs on its similarity
+ * symmetrisation S_ij = -sqrt(A_ij A_ji), which has A's eigenvalues, and on
+ * the fluid subspace when A was built from the mask. */
 int les_lanczos_bounds(les_ctx* ctx, int steps, uint64_t seed, double* ritz_min,
                        double* ritz_max);
 

@@ -622,7 +622,7 @@ int les_lanczos_bounds(les_ctx* c, int steps, uint64_t seed, double* ritz_min, d
     std::vector<double> al, be;
     double beta_prev = 0.0;
     for (int j = 0; j < steps; ++j) {
-        HIPCHK(c, launch_matvec<double>(s, c->d, A, v, w));
+        HIPCHK(c, launch_matvec_sym(s, c->d, A, v, w));
         HIPCHK(c, launch_dot(s, c->d, w, v, c->red, c->dsc + 0));
         HIPCHK(c, hipMemcpyAsync(c->hsc, c->dsc, 8, hipMemcpyDeviceToHost, s));
         HIPCHK(c, hipStreamSynchronize(s));

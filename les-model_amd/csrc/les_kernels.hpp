@@ -151,6 +151,9 @@ hipError_t launch_pcg_update_mixed(hipStream_t s, const Dims& d, float* x, float
 // v = w / sqrt(tot) where tot = ||w||^2 is read from device memory
 hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double* v,
                                    const double* nrm2);
+// y = S x, S the geometric-mean symmetrisation of A (same spectrum)
+hipError_t launch_matvec_sym(hipStream_t s, const Dims& d, Stencil<double> A, const double* x,
+                             double* y);
 // w = w - a v - b vp
 hipError_t launch_lanczos_axpy(hipStream_t s, int n, double* w, const double* v, const double* vp,
                                double a, double b);

@@ -564,6 +564,35 @@ __global__ __launch_bounds__(kBlock) void k_ic_bwd_level(Dims d, int level, T* z
     z[id] = sum / Lc[id];
 }
 
+// y = S x with S the symmetrisation of A: S_ij = -sqrt(A_ij A_ji) where
+// both couplings are non-zero, else 0, and S_ii = A_ii.  When W A is
+// symmetric for a positive diagonal W (the mirror-Neumann rows of
+// build_stencil, pressure.cpp:662-696, give W = 1/2 per mirrored
+// direction), S = W^(1/2) A W^(-1/2) on the fluid block, so S and A share
+// their eigenvalues and Lanczos on S bounds the spectrum of A.
+__global__ __launch_bounds__(kBlock) void k_matvec_sym(Dims d, Chunking ch, Stencil<double> A,
+                                                       const double* x, double* y)
+{
+    const int lb = logical_block(ch.grid);
+    FOR_CELLS(d, ch, lb)
+    {
+        int i, j, k;
+        decode(d, id, i, j, k);
+        double val = A.a[SC][id] * x[id];
+        auto add = [&](const double* out, const double* back, int nb) {
+            const double pr = out[id] * back[nb];
+            if (pr > 0.0) val -= sqrt(pr) * x[nb];
+        };
+        if (i > 0) add(A.a[SW], A.a[SE], id - d.sij);
+        if (i < d.nx - 1) add(A.a[SE], A.a[SW], id + d.sij);
+        if (j > 0) add(A.a[SS], A.a[SN], id - d.nz);
+        if (j < d.ny - 1) add(A.a[SN], A.a[SS], id + d.nz);
+        if (k > 0) add(A.a[SB], A.a[ST], id - 1);
+        if (k < d.nz - 1) add(A.a[ST], A.a[SB], id + 1);
+        y[id] = val;
+    }
+}
+
 // w = w - a v - b vp  (Lanczos three-term step)
 __global__ __launch_bounds__(kBlock) void k_lanczos_axpy(int n, double* w, const double* v,
                                                          const double* vp, double a, double b)
@@ -752,6 +781,14 @@ hipError_t launch_pcg_update_mixed(hipStream_t s, const Dims& d, float* x, float
     const Chunking ch = make_chunking(d.N);
     hipLaunchKernelGGL(k_pcg_update_mixed, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, r, p, q, red,
                        sc);
+    return hipGetLastError();
+}
+
+hipError_t launch_matvec_sym(hipStream_t s, const Dims& d, Stencil<double> A, const double* x,
+                             double* y)
+{
+    const Chunking ch = make_chunking(d.N);
+    hipLaunchKernelGGL(k_matvec_sym, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, x, y);
     return hipGetLastError();
 }
 

@@ -1,0 +1,133 @@
+#!/usr/bin/env python3
+"""Writes the golden fixtures in this directory from the unmodified reference
+solver, compiled from its own sources by oracle/Makefile (oracle/_ref/).
+
+Run from the repository root where the reference build exists:
+    make -C oracle ref && python tests/golden/make_golden.py
+
+Everything runs on one OpenMP thread (the reference's reductions and its
+power-iteration start vector are deterministic only there).  The fixtures
+hold inputs and the reference's outputs:
+
+  kernels_10x9x8.npz   every hot-path function once, on a 10x9x8 grid with
+                       unequal spacings and a seeded solid mask that has
+                       1-cell gaps and solid cells on the inlet plane
+  steps_16.npz         3 steps of the reference driver sequence
+                       (main.cpp:115-144, solve_pressure_pcg as shipped) on
+                       16^3 with a box obstacle, from step 19 (crosses into
+                       the adaptive time step)
+  c1_digest.json       the CPU-runnable configuration (32^3, U(-1,1) seed 0,
+                       centred 8^3 obstacle, 10 steps, PCG-Jacobi to 1e-8
+                       relative): per-step PCG iterations and dt, and final
+                       field digests (sums, norms, 1024 seeded probes, sha256)
+"""
+import hashlib
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path[:0] = [os.path.join(ROOT, "tests"), os.path.join(ROOT, "les-model_amd")]
+
+import les_workloads as wl  # noqa: E402
+from cpu_checkers import Reference, initial_state, STATE_KEYS  # noqa: E402
+
+
+def kernels_fixture(ref):
+    nx, ny, nz = 10, 9, 8
+    m = dict(nx=nx, ny=ny, nz=nz, dx=1.0 / nx, dy=0.7 / ny, dz=1.3 / nz)
+    N = nx * ny * nz
+    solid = (wl.uniform(42, 9, N, 0, 1) < 0.08).astype(np.uint8)
+    solid[[ny * nz // 2 + 3, 2 * nz + 5]] = 1  # inlet-plane solids
+    f = [wl.uniform(42, s, N) for s in range(8)]
+    out = dict(nx=nx, ny=ny, nz=nz, dx=m["dx"], dy=m["dy"], dz=m["dz"], solid=solid)
+    for q in range(8):
+        out[f"in{q}"] = f[q]
+    A = ref.build_stencil(solid, m)
+    out["A"] = np.stack(A)
+    out["matvec"] = ref.matvec(f[0], A, m)
+    div = ref.divergence(f[0], f[1], f[2], solid, m, 1e-3, 1.2)
+    out["divergence"] = div
+    out["b"] = ref.build_b(div, solid, m)
+    out["jacobi"] = ref.jacobi_precond(f[3], A[0])
+    L = ref.ic_build(A, m)
+    out["L"] = np.stack(L)
+    out["ic_apply_L"] = ref.ic_apply(f[3], L, m)
+    out["ic_apply_A"] = ref.ic_apply(f[3], A, m)
+    lmax = ref.lambda_max(A, m, 20)
+    out["lambda_max"] = np.array([lmax])
+    out["cheb5"] = ref.cheb_apply(f[3], A, m, 3 * np.pi ** 2, lmax, 5)
+    bb = out["b"]
+    x, it = ref.pcg_legacy(A, bb, f[4], m, max_iter=30, tol=0.0)
+    out["pcg_legacy_30"] = x
+    x, it = ref.pcg_cheb(A, bb, f[4], m, 3 * np.pi ** 2, lmax, 5, 30, 0.0)
+    out["pcg_cheb5_30"] = x
+    out["pressure_bcs"] = ref.pressure_bcs(f[5], solid, m)
+    p0 = f[5] * 0.01
+    out["jacobi_solve"], _ = ref.jacobi_solve(p0, f[0], f[1], f[2], solid, m, 1e-3, 1.2)
+    out["gs_solve_10"], _ = ref.gs_solve(p0, f[0], f[1], f[2], solid, m, 1e-3, 1.2, 10)
+    nt0 = np.abs(f[6]) * 1e-3
+    out["adaptive_dt"] = np.array([ref.adaptive_dt(f[0], f[1], f[2], nt0, m, 1e-3)])
+    nt, tau = ref.sgs(f[0], f[1], f[2], m, 0.18, nu_t=nt0, want_tau=True)
+    out["sgs_nu_t"] = nt
+    out["sgs_tau"] = np.stack(tau)
+    out["nut_bcs"] = ref.nut_bcs(nt, solid, m)
+    mo = ref.momentum(f[0], f[1], f[2], f[5], f[6], f[7], out["nut_bcs"], m, 1e-3, 1e-3)
+    out["momentum"] = np.stack(mo)
+    out["velocity_bcs"] = np.stack(ref.velocity_bcs(mo[0], mo[1], mo[2], solid, m))
+    out["correct"] = np.stack(ref.correct(f[0], f[1], f[2], f[4], m, 1e-3, 1.2))
+    np.savez_compressed(os.path.join(HERE, "kernels_10x9x8.npz"), **out)
+
+
+def steps_fixture(ref):
+    n = 16
+    m = wl.mesh(n)
+    solid = wl.box_mask(n, n, n, [((5, 9), (6, 10), (4, 11))])
+    f = [wl.uniform(7, s, n ** 3) for s in range(3)]
+    st0 = initial_state(f[0], f[1], f[2], n ** 3, solid, m, ref)
+    st, its, dts = ref.run_steps(st0, solid, m, None, 3, step0=19, solver=0, nu=1e-3, rho=1.2,
+                                 max_iter=1000, tol=1e-6)
+    out = dict(n=n, solid=solid, step0=19, iters=np.array(its), dts=np.array(dts))
+    for k in STATE_KEYS:
+        out["in_" + k] = st0[k]
+        out["out_" + k] = st[k]
+    np.savez_compressed(os.path.join(HERE, "steps_16.npz"), **out)
+
+
+def digest(a):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    probes = wl.counter_bits(1234, 0, 1024) % np.uint64(a.size)
+    return dict(sum=float(a.sum()), l2=float(np.linalg.norm(a)), max=float(a.max()),
+                min=float(a.min()), sha256=hashlib.sha256(a.tobytes()).hexdigest(),
+                probe_idx=[int(i) for i in probes], probe=[float(a[int(i)]) for i in probes])
+
+
+def c1_fixture(ref):
+    m, (u, v, w), solid = wl.config_c1()
+    N = 32 ** 3
+    st0 = initial_state(u, v, w, N, solid, m, ref)
+    st, its, dts = ref.run_steps(st0, solid, m, None, 10, step0=0, solver=1, nu=1e-3, rho=1.2,
+                                 max_iter=20000, tol=1e-8, tol_rel=True)
+    out = dict(config="32^3 U(-1,1) seed 0, centred 8^3 box, nu=1e-3, Cs=0.18 (reference), "
+                      "dt=1e-3, 10 steps, PCG-Jacobi to 1e-8 relative",
+               iters=[int(i) for i in its], dts=[float(x) for x in dts],
+               fields={k: digest(st[k]) for k in STATE_KEYS})
+    with open(os.path.join(HERE, "c1_digest.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
+def main():
+    ref = Reference()
+    ref.set_threads(1)
+    kernels_fixture(ref)
+    steps_fixture(ref)
+    c1_fixture(ref)
+    for fn in sorted(os.listdir(HERE)):
+        print(fn, os.path.getsize(os.path.join(HERE, fn)))
+
+
+if __name__ == "__main__":
+    main()

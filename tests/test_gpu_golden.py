@@ -1,0 +1,110 @@
+"""GPU against the committed reference fixtures (tests/golden/): the same
+inputs the reference solver consumed, checked against its outputs."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+import les_workloads as wl
+import lesgpu
+from cases import bits_equal, rel_err
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def test_kernels_fixture():
+    K = dict(np.load(os.path.join(GOLD, "kernels_10x9x8.npz")))
+    m = dict(nx=int(K["nx"]), ny=int(K["ny"]), nz=int(K["nz"]), dx=float(K["dx"]),
+             dy=float(K["dy"]), dz=float(K["dz"]))
+    solid = K["solid"]
+    f = [K[f"in{q}"] for q in range(8)]
+    with lesgpu.LesGpu.from_mesh(m) as g:
+        g.set_mask(solid)
+        g.build_stencil()
+        assert bits_equal(np.stack(g.get_stencil()), K["A"])
+        assert bits_equal(g.matvec(f[0]), K["matvec"])
+        g.set_state({"u_new": f[0], "v_new": f[1], "w_new": f[2]})
+        g.divergence_b(1e-3, 1.2)
+        assert bits_equal(g.get_field("p_rhs"), K["divergence"])
+        assert bits_equal(g.get_field("b"), K["b"])
+        assert bits_equal(g.precond_apply(lesgpu.PC_JACOBI, f[3]), K["jacobi"])
+        g.ic0_factor()
+        assert bits_equal(np.stack(g.get_stencil(lesgpu.STENCIL_L)), K["L"])
+        assert bits_equal(g.precond_apply(lesgpu.PC_IC0, f[3]), K["ic_apply_L"])
+        assert bits_equal(g.precond_apply(lesgpu.PC_LEGACY_IC_A, f[3]), K["ic_apply_A"])
+        lmax = K["lambda_max"][0]
+        assert abs(g.power_lambda_max(20) - lmax) <= 1e-12 * lmax
+        assert bits_equal(g.precond_apply(lesgpu.PC_CHEBYSHEV, f[3], 3 * np.pi ** 2, lmax, 5),
+                          K["cheb5"])
+        g.set_field("b", K["b"])
+        g.set_field("p", f[4])
+        g.pcg(precond=lesgpu.PC_LEGACY_IC_A, max_iter=30, tol=0.0)
+        assert rel_err(g.get_field("p"), K["pcg_legacy_30"]) < 1e-10
+        g.set_field("p", f[4])
+        g.pcg(precond=lesgpu.PC_CHEBYSHEV, max_iter=30, tol=0.0, lmin=3 * np.pi ** 2, lmax=lmax,
+              degree=5)
+        assert rel_err(g.get_field("p"), K["pcg_cheb5_30"]) < 1e-10
+        g.set_field("p", f[5])
+        g.pressure_bcs()
+        assert bits_equal(g.get_field("p"), K["pressure_bcs"])
+        g.set_state({"u": f[0], "v": f[1], "w": f[2], "p": f[5] * 0.01})
+        g.jacobi_solve(1e-3, 1.2)
+        assert bits_equal(g.get_field("p"), K["jacobi_solve"])
+        g.set_field("p", f[5] * 0.01)
+        g.sor_solve(1e-3, 1.2, 1.7, 10, lesgpu.SOLVER_SOR_LEX)
+        assert bits_equal(g.get_field("p"), K["gs_solve_10"])
+        nt0 = np.abs(f[6]) * 1e-3
+        g.set_field("nu_t", nt0)
+        assert g.adaptive_dt(1e-3) == K["adaptive_dt"][0]
+        tau = g.sgs(0.18, apply_bcs=False, want_tau=True)
+        assert bits_equal(g.get_field("nu_t"), K["sgs_nu_t"])
+        assert bits_equal(np.stack(tau), K["sgs_tau"])
+        g.nut_bcs()
+        assert bits_equal(g.get_field("nu_t"), K["nut_bcs"])
+        g.set_state({"u_new": f[5], "v_new": f[6], "w_new": f[7]})
+        g.momentum(1e-3, 1e-3, apply_bcs=False)
+        got = np.stack([g.get_field(k) for k in ("u", "v", "w", "u_new", "v_new", "w_new")])
+        assert bits_equal(got, K["momentum"])
+        g.velocity_bcs()
+        assert bits_equal(np.stack([g.get_field(k) for k in ("u", "v", "w")]), K["velocity_bcs"])
+        g.set_state({"u": f[0], "v": f[1], "w": f[2], "p": f[4]})
+        g.correct(1e-3, 1.2, apply_bcs=False)
+        assert bits_equal(np.stack([g.get_field(k) for k in ("u", "v", "w")]), K["correct"])
+
+
+def test_c1_against_reference_digest():
+    """The CPU-runnable configuration (32^3, U(-1,1) seed 0, centred 8^3
+    obstacle, 10 steps, PCG-Jacobi to 1e-8 relative) on the GPU against the
+    reference build's per-step iteration counts and final-field digests.
+    Tolerance: the GPU sums dot products in another order, so iteration
+    counts may differ by one near the threshold and fields by roughly the
+    solver tolerance."""
+    with open(os.path.join(GOLD, "c1_digest.json")) as fh:
+        D = json.load(fh)
+    m, (u, v, w), solid = wl.config_c1()
+    N = 32 ** 3
+    with lesgpu.LesGpu.from_mesh(m) as g:
+        g.set_mask(solid)
+        st = {k: np.zeros(N) for k in lesgpu.FIELDS[:9]}
+        st["u"], st["v"], st["w"] = u, v, w
+        g.set_state(st)
+        g.velocity_bcs()
+        g.build_stencil()
+        its = []
+        for s in range(10):
+            r = g.step(step=s, nu=1e-3, rho=1.2, Cs=0.18, precond=lesgpu.PC_JACOBI,
+                       max_iter=20000, tol=1e-8, tol_rel=True)
+            assert r["converged"] == 1
+            its.append(r["iters"])
+        out = g.get_state()
+    for a, b in zip(its, D["iters"]):
+        assert abs(a - b) <= 1, (its, D["iters"])
+    for k, dg in D["fields"].items():
+        a = out[k]
+        scale = max(abs(dg["max"]), abs(dg["min"]), 1e-300)
+        probe = a[np.array(dg["probe_idx"])]
+        err = np.max(np.abs(probe - np.array(dg["probe"]))) / scale
+        assert err < 1e-7, (k, err)
+        assert abs(np.linalg.norm(a) - dg["l2"]) <= 1e-8 * dg["l2"], k
