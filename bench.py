@@ -139,6 +139,9 @@ def main():
     ap.add_argument("--modes", default="fp64,mixed")
     ap.add_argument("--check-every", type=int, default=0)
     ap.add_argument("--step0", type=int, default=21)
+    ap.add_argument("--copy-calibration", type=int, default=0,
+                    help="launch the plain FP64 copy kernel this many times first "
+                         "(reference size for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -160,6 +163,10 @@ def main():
     g.velocity_bcs()          # main.cpp:102
     st0 = g.get_state()
     g.build_stencil()
+    if args.copy_calibration:
+        g.diag_copy(args.copy_calibration)
+        log(f"copy calibration: {args.copy_calibration} x k_convert<double,double> of "
+            f"{8 * N} bytes read + {8 * N} written")
     t0 = time.perf_counter()
     rmin, rmax = g.lanczos_bounds(20, seed=0)
     bounds = (rmin, 1.05 * rmax)

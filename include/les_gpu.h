@@ -173,6 +173,8 @@ int les_velocity_bcs(les_ctx* ctx);                       /* apply_velocity_BCs 
 /* calculate_divergence (pressure.cpp:256) of u_new into p_rhs, then
  * build_b_vector (pressure.cpp:566) into b. */
 int les_divergence_b(les_ctx* ctx, double dt, double rho);
+/* build_b_vector (pressure.cpp:566) alone: b from the context's p_rhs. */
+int les_build_b(les_ctx* ctx);
 /* correct_velocity (main.cpp:671); apply_bcs = 1 also applies
  * apply_velocity_BCs. */
 int les_correct(les_ctx* ctx, double dt, double rho, int apply_bcs);
@@ -208,6 +210,10 @@ int les_kernel_timing(les_ctx* ctx, int kernel_class);
 /* launches recorded, their summed duration in ms, and the bytes per cell the
  * class is charged (element size of the run) are returned; synchronises. */
 int les_kernel_timing_read(les_ctx* ctx, int* launches, double* total_ms, int* elem_bytes);
+/* Copies one FP64 grid array to another `reps` times with the plain copy
+ * kernel (k_convert<double,double>): the known-size reference for HBM
+ * byte counters (8*N read and 8*N written per launch). */
+int les_diag_copy(les_ctx* ctx, int reps);
 
 #ifdef __cplusplus
 }

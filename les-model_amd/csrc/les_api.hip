@@ -1178,6 +1178,13 @@ int les_divergence_b(les_ctx* c, double dt, double rho)
     return LES_OK;
 }
 
+int les_build_b(les_ctx* c)
+{
+    if (!c) return LES_ERR_ARG;
+    HIPCHK(c, launch_build_b(c->stream, c->d, c->f[LES_F_P_RHS], c->solid, c->f[LES_F_B]));
+    return LES_OK;
+}
+
 int les_correct(les_ctx* c, double dt, double rho, int apply_bcs)
 {
     if (!c) return LES_ERR_ARG;
@@ -1284,6 +1291,18 @@ int les_kernel_timing_read(les_ctx* c, int* launches, double* total_ms, int* ele
     *launches = (int)(c->kt_used / 2);
     *total_ms = tot;
     if (elem_bytes) *elem_bytes = c->kt_elem;
+    return LES_OK;
+}
+
+
+int les_diag_copy(les_ctx* c, int reps)
+{
+    if (!c || reps < 0) return LES_ERR_ARG;
+    int rc;
+    if ((rc = ensure_krylov(c)) != LES_OK) return rc;
+    for (int r = 0; r < reps; ++r)
+        HIPCHK(c, (launch_convert<double, double>(c->stream, c->d.N, c->t1, c->t2)));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
     return LES_OK;
 }
 

@@ -301,6 +301,19 @@ __global__ __launch_bounds__(kBlock) void k_faces(Dims d, const uint8_t* solid, 
     }
 }
 
+// build_b_vector (pressure.cpp:566-599): b = -div on fluid, 0 on solid and
+// outlet cells
+__global__ __launch_bounds__(kBlock) void k_build_b(Dims d, Chunking ch, const double* div,
+                                                    const uint8_t* solid, double* b)
+{
+    const int lb = logical_block(ch.grid);
+    FOR_CELLS(d, ch, lb)
+    {
+        const int i = id / d.sij;
+        b[id] = (solid[id] || i == d.nx - 1) ? 0.0 : -div[id];
+    }
+}
+
 // max(u^2+v^2+w^2) and max nu_t over all cells (main.cpp:215-245)
 __global__ __launch_bounds__(kBlock) void k_dt_max(Dims d, Chunking ch, const double* u,
                                                    const double* v, const double* w,
@@ -553,6 +566,14 @@ hipError_t launch_nut_bcs(hipStream_t s, const Dims& d, const uint8_t* solid, do
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     hipLaunchKernelGGL(k_faces<false>, dim3(face_grid(d)), dim3(kBlock), 0, s, d, solid, nu_t,
                        (double*)nullptr, (double*)nullptr);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_b(hipStream_t s, const Dims& d, const double* div, const uint8_t* solid,
+                          double* b)
+{
+    const Chunking ch = make_chunking(d.N);
+    hipLaunchKernelGGL(k_build_b, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, div, solid, b);
     return hipGetLastError();
 }
 

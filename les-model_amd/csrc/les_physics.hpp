@@ -56,6 +56,8 @@ hipError_t launch_correct(hipStream_t s, const Dims& d, const PhysConsts& c, dou
 hipError_t launch_velocity_bcs(hipStream_t s, const Dims& d, const uint8_t* solid, double* u,
                                double* v, double* w);
 hipError_t launch_nut_bcs(hipStream_t s, const Dims& d, const uint8_t* solid, double* nu_t);
+hipError_t launch_build_b(hipStream_t s, const Dims& d, const double* div, const uint8_t* solid,
+                          double* b);
 hipError_t launch_dt_max(hipStream_t s, const Dims& d, const double* u, const double* v,
                          const double* w, const double* nu_t, RedScratch red, double* out2);
 // st may be nullptr (plain apply_pressure_BCs); otherwise the sweep check runs.

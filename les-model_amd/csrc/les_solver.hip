@@ -848,6 +848,7 @@ template hipError_t launch_pcg_update<float, float>(hipStream_t, const Dims&, fl
                                                     const float*, const float*, const float*,
                                                     RedScratch, SolveScalars*);
 template hipError_t launch_convert<float, double>(hipStream_t, int, const double*, float*);
+template hipError_t launch_convert<double, double>(hipStream_t, int, const double*, double*);
 template hipError_t launch_convert<double, float>(hipStream_t, int, const float*, double*);
 
 }  // namespace les

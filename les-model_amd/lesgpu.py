@@ -111,6 +111,7 @@ def load_library(path: str = LIB_PATH):
     L.les_step.argtypes = [vp, C.POINTER(StepParams), C.POINTER(StepStats)]
     L.les_kernel_timing.argtypes = [vp, C.c_int]
     L.les_kernel_timing_read.argtypes = [vp, C.POINTER(C.c_int), DP, C.POINTER(C.c_int)]
+    L.les_diag_copy.argtypes = [vp, C.c_int]
     _lib = L
     return L
 
@@ -336,3 +337,6 @@ class LesGpu:
         self._chk(self.lib.les_kernel_timing_read(self.h, C.byref(n), C.byref(ms), C.byref(eb)),
                   "les_kernel_timing_read")
         return n.value, ms.value, eb.value
+
+    def diag_copy(self, reps=1):
+        self._chk(self.lib.les_diag_copy(self.h, reps), "les_diag_copy")
