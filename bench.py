@@ -39,13 +39,14 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cheb_bytes_per_cell(degree, elem):
+def cheb_bytes_per_cell(degree, elem, coded=False):
     """Algorithmic bytes per cell per Chebyshev inner-step launch, averaged
     over the degree-1 launches of one application.  First step: r (stencil),
-    7 coefficients -> p, z.  Later steps: z (stencil), r, p, 7 coefficients
-    -> p, z."""
-    first = (1 + 7 + 2) * elem
-    later = (3 + 7 + 2) * elem
+    the operator -> p, z.  Later steps: z (stencil), r, p, the operator ->
+    p, z.  The operator is 7 coefficient arrays, or one 2-byte code."""
+    op = 2 if coded else 7 * elem
+    first = (1 + 2) * elem + op
+    later = (3 + 2) * elem + op
     m = degree - 1
     return (first + (m - 1) * later) / m
 
@@ -69,7 +70,7 @@ def make_workload(n, seed=0):
     return wl.mesh(n), (u, v, w), solid
 
 
-def run_mode(g, st0, precision, bounds, steps, warmup, kt_class, pcg_kw, step0):
+def run_mode(g, st0, precision, stencil, bounds, steps, warmup, kt_class, pcg_kw, step0):
     import torch
     import lesgpu
     g.set_state(st0)
@@ -77,7 +78,7 @@ def run_mode(g, st0, precision, bounds, steps, warmup, kt_class, pcg_kw, step0):
     lmin, lmax = bounds
     kw = dict(nu=1e-3, rho=1.2, Cs=0.17, precond=lesgpu.PC_CHEBYSHEV, lmin=lmin, lmax=lmax,
               degree=8, max_iter=20000, tol=1e-8, tol_rel=True, precision=precision,
-              reliable_every=50, **pcg_kw)
+              reliable_every=50, stencil_format=stencil, **pcg_kw)
     for s in range(step0, step0 + warmup):
         r = g.step(step=s, **kw)
         log(f"  warmup step {s}: iters {r['iters']} conv {r['converged']}")
@@ -136,7 +137,8 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
-    ap.add_argument("--modes", default="fp64,mixed")
+    ap.add_argument("--modes", default="fp64:arrays,mixed:arrays,fp64:coded,mixed:coded",
+                    help="precision:stencil pairs; precision fp64|mixed, stencil arrays|coded")
     ap.add_argument("--check-every", type=int, default=0)
     ap.add_argument("--step0", type=int, default=21)
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -177,19 +179,28 @@ def main():
     res = {}
     pcg_kw = dict(check_every=args.check_every)
     for mode in modes:
-        prec = lesgpu.PREC_MIXED if mode == "mixed" else lesgpu.PREC_FP64
+        prec_s, sten_s = (mode.split(":") + ["coded"])[:2]
+        prec = lesgpu.PREC_MIXED if prec_s == "mixed" else lesgpu.PREC_FP64
+        sten = lesgpu.STENCIL_CODED if sten_s == "coded" else lesgpu.STENCIL_ARRAYS
         log(f"mode {mode}: {args.warmup} warmup + {args.steps} timed steps")
-        res[mode] = run_mode(g, st0, prec, bounds, args.steps, args.warmup, lesgpu.KT_CHEB_STEP,
-                             pcg_kw, args.step0)
+        res[mode] = run_mode(g, st0, prec, sten, bounds, args.steps, args.warmup,
+                             lesgpu.KT_CHEB_STEP, pcg_kw, args.step0)
         r = res[mode]
+        r["elem"] = 4 if prec_s == "mixed" else 8
+        r["coded"] = sten_s == "coded"
+        n_l, ms, _ = r["kt"]
+        r["cheb_bpc"] = cheb_bytes_per_cell(8, r["elem"], r["coded"])
+        r["cheb_gbs"] = (r["cheb_bpc"] * N) / (ms / n_l * 1e-3) / 1e9 if n_l else 0.0
         log(f"  {mode}: {args.steps / r['seconds']:.3f} steps/s, iters/step "
-            f"{np.mean(r['iters']):.1f}, cheb launches {r['kt'][0]} total {r['kt'][1]:.1f} ms")
-    head = "mixed" if "mixed" in res else modes[0]
+            f"{np.mean(r['iters']):.1f}, cheb {n_l} launches avg {1e3 * ms / max(n_l, 1):.1f} us "
+            f"= {r['cheb_gbs']:.0f} GB/s algorithmic")
+    head = max((m for m in res if m.startswith("mixed")),
+               key=lambda m: args.steps / res[m]["seconds"], default=modes[0])
     R = res[head]
     steps_per_s = args.steps / R["seconds"]
     n_launch, kt_ms, elem = R["kt"]
-    bpc = cheb_bytes_per_cell(8, elem)
-    achieved = (bpc * N) / (kt_ms / n_launch * 1e-3) / 1e9 if n_launch else 0.0
+    bpc = R["cheb_bpc"]
+    achieved = R["cheb_gbs"]
 
     out = {
         "metric": "LES steps/s at 256^3, FP64 vs mixed PCG to 1e-8 rel.; PCG HBM GB/s vs peak",
@@ -202,34 +213,40 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32+f64" if head == "mixed" else "f64",
+        "dtype": "f32+f64" if head.startswith("mixed") else "f64",
         "data": "synthetic (seeded spectral velocity, seeded 5% solid mask)",
         "config": {"workload": f"LES step {m['nx']}^3, PCG-Chebyshev(8) on 20-step Lanczos "
-                               f"bounds to 1e-8 rel, {head} precision",
+                               f"bounds to 1e-8 rel, {head}",
                    "grid": [m["nx"], m["ny"], m["nz"]], "solid_fraction": 0.05,
                    "precond": "chebyshev", "degree": 8, "reliable_every": 50,
                    "first_step": args.step0, "dt": "adaptive (main.cpp:208-250)",
-                   "lanczos_bounds": list(bounds)},
+                   "lanczos_bounds": list(bounds), "mode": head},
         "pcg_iters_per_step": float(np.mean(R["iters"])),
         "pcg_iters_per_s": float(np.sum(R["iters"]) / R["seconds"]),
-        "roofline": {"bound": "hbm", "kernel": f"k_cheb_step<{'float' if elem == 4 else 'double'}>",
+        "roofline": {"bound": "hbm",
+                     "kernel": f"k_cheb_step<{'float' if elem == 4 else 'double'}>"
+                               f"{' coded' if R['coded'] else ''}",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_cell": bpc, "cells": N, "launches": n_launch,
                      "avg_us": 1e3 * kt_ms / n_launch if n_launch else None},
+        "modes": {mname: {"steps_per_s": args.steps / r["seconds"],
+                          "pcg_iters_per_step": float(np.mean(r["iters"])),
+                          "cheb_step_gbs": r["cheb_gbs"],
+                          "cheb_step_frac": r["cheb_gbs"] / HBM_PEAK_GBS,
+                          "cheb_bytes_per_cell": r["cheb_bpc"]} for mname, r in res.items()},
     }
-    if "fp64" in res:
-        F = res["fp64"]
+    fp64_plain = "fp64:arrays" if "fp64:arrays" in res else next(
+        (mm for mm in res if mm.startswith("fp64")), None)
+    if fp64_plain is not None:
+        F = res[fp64_plain]
+        out["fp64_plain_mode"] = fp64_plain
         out["fp64_steps_per_s"] = args.steps / F["seconds"]
-        out["fp64_pcg_iters_per_step"] = float(np.mean(F["iters"]))
-        n64, ms64, e64 = F["kt"]
-        a64 = (cheb_bytes_per_cell(8, e64) * N) / (ms64 / n64 * 1e-3) / 1e9 if n64 else 0.0
-        out["fp64_roofline_frac"] = a64 / HBM_PEAK_GBS
-        if "mixed" in res:
+        if head.startswith("mixed"):
             out["mixed_vs_fp64_speedup"] = steps_per_s / out["fp64_steps_per_s"]
             err = {}
             for k in ("u", "v", "w", "p"):
-                a, b = res["mixed"]["state"][k], F["state"][k]
+                a, b = R["state"][k], F["state"][k]
                 err[k] = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
             out["mixed_max_rel_err_vs_fp64"] = err
     if not args.no_cpu_baseline:

@@ -132,6 +132,8 @@ typedef struct les_pcg_params {
     int precision;        /* LES_PREC_FP64 or LES_PREC_MIXED */
     int reliable_every;   /* mixed: FP64 residual replacement period (0: 50) */
     int check_every;      /* host polls convergence every this many iterations (0: auto) */
+    int stencil_format;   /* 0 auto: coded when A came from les_build_stencil;
+                             1 the 7 coefficient arrays; 2 coded (16 bits per cell) */
 } les_pcg_params;
 
 typedef struct les_pcg_result {

@@ -35,6 +35,8 @@ struct les_ctx {
     double* A[7] = {};
     bool have_A = false;
     bool A_from_mask = false;  // A was built by les_build_stencil from the mask
+    uint16_t* code = nullptr;  // the same operator as one 16-bit code per cell
+    bool have_code = false;
     double* L[7] = {};
     bool have_L = false;
     float* A32[7] = {};
@@ -127,16 +129,40 @@ hipMemcpyKind kind_out(int ptr_kind) { return ptr_kind ? hipMemcpyDeviceToDevice
 
 Stencil<double> stencil_of(double* const a[7])
 {
-    Stencil<double> s;
+    Stencil<double> s{};
     for (int q = 0; q < 7; ++q) s.a[q] = a[q];
     return s;
 }
 Stencil<float> stencil_of(float* const a[7])
 {
-    Stencil<float> s;
+    Stencil<float> s{};
     for (int q = 0; q < 7; ++q) s.a[q] = a[q];
     return s;
 }
+
+// The coded form of the operator build_stencil makes (pressure.cpp:613-623):
+// the base coefficients formed with the reference's expressions, then
+// rounded to T.
+template <typename T>
+Stencil<T> coded_stencil(const les_ctx* c)
+{
+    Stencil<T> s{};
+    const double idx2 = 1.0 / (c->dx * c->dx), idy2 = 1.0 / (c->dy * c->dy),
+                 idz2 = 1.0 / (c->dz * c->dz);
+    s.code = c->code;
+    s.cfluid = (T)(2.0 * idx2 + 2.0 * idy2 + 2.0 * idz2);
+    s.ox = (T)(-idx2);
+    s.oy = (T)(-idy2);
+    s.oz = (T)(-idz2);
+    s.ox2 = (T)(-idx2 + -idx2);
+    s.oy2 = (T)(-idy2 + -idy2);
+    s.oz2 = (T)(-idz2 + -idz2);
+    return s;
+}
+
+// stencil_format: 0 auto (coded when A came from build_stencil), 1 the 7
+// arrays, 2 coded (error when A did not come from build_stencil)
+int pick_coded(les_ctx* c, int fmt, bool& coded);
 
 PhysConsts make_consts(const les_ctx* c, double dt, double nu, double rho, double Cs)
 {
@@ -215,10 +241,19 @@ struct KtScope {
 
 double tridiag_eig(const std::vector<double>& a, const std::vector<double>& b, int which);
 
+int pick_coded(les_ctx* c, int fmt, bool& coded)
+{
+    if (fmt == 2 && !c->have_code)
+        return fail(c, LES_ERR_STATE, "coded stencil requested but A was not built from the mask");
+    if (fmt < 0 || fmt > 2) return fail(c, LES_ERR_ARG, "unknown stencil format");
+    coded = (fmt == 2) || (fmt == 0 && c->have_code);
+    return LES_OK;
+}
+
 // z = M^-1 r for FP64 preconditioners.  When epi >= 0 the final kernel also
 // reduces r.z into the solve scalars with that epilogue.
 int precond_fp64(les_ctx* c, int pc, const double* r, double* zout, double lmin, double lmax,
-                 int degree, int epi, SolveScalars* sc)
+                 int degree, int epi, SolveScalars* sc, const Stencil<double>& Aop)
 {
     const Dims& d = c->d;
     hipStream_t s = c->stream;
@@ -244,7 +279,7 @@ int precond_fp64(les_ctx* c, int pc, const double* r, double* zout, double lmin,
         double a0;
         std::vector<double> be, al;
         cheb_coeffs(lmin, lmax, degree, a0, be, al);
-        Stencil<double> A = stencil_of(c->A);
+        const Stencil<double>& A = Aop;
         if (degree <= 1) {
             HIPCHK(c, launch_cheb_scale<double>(s, d, r, zout, a0, epi, c->red, sc));
             return LES_OK;
@@ -303,7 +338,7 @@ int les_ctx_create(int nx, int ny, int nz, double dx, double dy, double dz, les_
         if (alloc(c, &c->f[q], N) != LES_OK) { g_create_err = c->err; les_ctx_destroy(c); return LES_ERR_NOMEM; }
     if (alloc(c, &c->solid, N) != LES_OK) { g_create_err = c->err; les_ctx_destroy(c); return LES_ERR_NOMEM; }
     if (alloc(c, &c->red.partials, (size_t)kMaxGrid * kMaxRed) != LES_OK ||
-        alloc(c, &c->red.counter, 1) != LES_OK || alloc(c, &c->sc, 1) != LES_OK ||
+        alloc(c, &c->sc, 1) != LES_OK ||
         alloc(c, &c->st, 1) != LES_OK || alloc(c, &c->dsc, kScalarSlots) != LES_OK) {
         g_create_err = c->err;
         les_ctx_destroy(c);
@@ -332,6 +367,7 @@ int les_ctx_destroy(les_ctx* c)
     };
     for (auto* p : c->f) fr(p);
     fr(c->solid);
+    fr(c->code);
     for (int q = 0; q < 7; ++q) {
         fr(c->A[q]);
         fr(c->L[q]);
@@ -341,7 +377,6 @@ int les_ctx_destroy(les_ctx* c)
     for (double* p : {c->r, c->z, c->p, c->q, c->za, c->zb, c->pc, c->t1, c->t2}) fr(p);
     free_mixed(c->mw);
     fr(c->red.partials);
-    fr(c->red.counter);
     fr(c->sc);
     fr(c->st);
     fr(c->dsc);
@@ -429,11 +464,14 @@ int les_build_stencil(les_ctx* c, int ignore_mask)
     }
     HIPCHK(c, launch_build_stencil(c->stream, c->d, m, c->A, 1.0 / (c->dx * c->dx),
                                    1.0 / (c->dy * c->dy), 1.0 / (c->dz * c->dz)));
+    if ((rc = alloc(c, &c->code, c->d.N, false)) != LES_OK) return rc;
+    HIPCHK(c, launch_build_code(c->stream, c->d, m, c->code));
     if (zeros) {
         HIPCHK(c, hipStreamSynchronize(c->stream));
         hipFree(zeros);
     }
     c->have_A = true;
+    c->have_code = true;
     c->A_from_mask = !ignore_mask;
     c->have_A32 = false;
     c->have_L = false;
@@ -455,6 +493,7 @@ int les_set_stencil(les_ctx* c, int which, double* const coef[7], int ptr_kind)
     if (!ptr_kind) HIPCHK(c, hipStreamSynchronize(c->stream));
     if (which == LES_STENCIL_A) {
         c->have_A = true;
+        c->have_code = false;
         c->A_from_mask = false;
         c->have_A32 = false;
     } else {
@@ -551,7 +590,9 @@ int les_precond_apply(les_ctx* c, int pc, const double* r, double* z, int ptr_ki
         rd = c->t1;
         zd = c->t2;
     }
-    if ((rc = precond_fp64(c, pc, rd, zd, lmin, lmax, degree, -1, nullptr)) != LES_OK) return rc;
+    if ((rc = precond_fp64(c, pc, rd, zd, lmin, lmax, degree, -1, nullptr, stencil_of(c->A))) !=
+        LES_OK)
+        return rc;
     if (!ptr_kind) {
         HIPCHK(c, hipMemcpyAsync(z, zd, nbytes(c, 8), hipMemcpyDeviceToHost, c->stream));
         HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -670,7 +711,9 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     c->sc_host[0] = init;
     HIPCHK(c, hipMemcpyAsync(c->sc, &c->sc_host[0], sizeof(SolveScalars), hipMemcpyHostToDevice, s));
 
-    const Stencil<double> A = stencil_of(c->A);
+    bool coded = false;
+    if ((rc = pick_coded(c, prm->stencil_format, coded)) != LES_OK) return rc;
+    const Stencil<double> A = coded ? coded_stencil<double>(c) : stencil_of(c->A);
     double* x = c->f[LES_F_P];
     const double* b = c->f[LES_F_B];
     double *r = c->r, *p = c->p, *q = c->q, *z = c->z;
@@ -682,7 +725,7 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     if (pc == LES_PC_JACOBI)
         HIPCHK(c, launch_pcg_p_jacobi<double>(s, d, r, c->A[SC], p, 1, EPI_RHO, c->red, sc));
     else if ((rc = precond_fp64(c, pc, r, p, prm->lambda_min, prm->lambda_max, prm->degree, EPI_RHO,
-                                sc)) != LES_OK)
+                                sc, A)) != LES_OK)
         return rc;
 
     int check = prm->check_every > 0 ? prm->check_every : (d.N >= (1 << 22) ? 4 : 16);
@@ -701,7 +744,7 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
             } else {
                 HIPCHK(c, (launch_pcg_update<double, double>(s, d, x, r, p, q, nullptr, c->red, sc)));
                 if ((rc = precond_fp64(c, pc, r, z, prm->lambda_min, prm->lambda_max, prm->degree,
-                                       EPI_BETA, sc)) != LES_OK)
+                                       EPI_BETA, sc, A)) != LES_OK)
                     return rc;
                 HIPCHK(c, launch_pcg_p_update<double>(s, d, z, p, sc));
             }
@@ -738,10 +781,11 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
 }
 
 
-int ensure_fp32_ops(les_ctx* c, int pc)
+int ensure_fp32_ops(les_ctx* c, int pc, bool coded)
 {
     int rc;
-    if (!c->have_A32) {
+    const bool need_arrays = !coded || pc == LES_PC_LEGACY_IC_A || pc == LES_PC_JACOBI;
+    if (need_arrays && !c->have_A32) {
         for (int q = 0; q < 7; ++q) {
             if ((rc = alloc(c, &c->A32[q], c->d.N, false)) != LES_OK) return rc;
             HIPCHK(c, (launch_convert<float, double>(c->stream, c->d.N, c->A[q], c->A32[q])));
@@ -764,7 +808,7 @@ int ensure_fp32_ops(les_ctx* c, int pc)
 
 // z = M^-1 r in FP32 (operator and vectors FP32, r.z accumulated in FP64)
 int precond_fp32(les_ctx* c, int pc, const float* r, float* zout, double lmin, double lmax,
-                 int degree, int epi, SolveScalars* sc)
+                 int degree, int epi, SolveScalars* sc, const Stencil<float>& Aop)
 {
     const Dims& d = c->d;
     hipStream_t s = c->stream;
@@ -789,7 +833,7 @@ int precond_fp32(les_ctx* c, int pc, const float* r, float* zout, double lmin, d
         double a0;
         std::vector<double> be, al;
         cheb_coeffs(lmin, lmax, degree, a0, be, al);
-        Stencil<float> A = stencil_of(c->A32);
+        const Stencil<float>& A = Aop;
         if (degree <= 1) {
             HIPCHK(c, launch_cheb_scale<float>(s, d, r, zout, a0, epi, c->red, sc));
             return LES_OK;
@@ -826,7 +870,9 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     const int pc = prm->precond;
     int rc;
     if ((rc = ensure_krylov(c)) != LES_OK) return rc;
-    if ((rc = ensure_fp32_ops(c, pc)) != LES_OK) return rc;
+    bool coded = false;
+    if ((rc = pick_coded(c, prm->stencil_format, coded)) != LES_OK) return rc;
+    if ((rc = ensure_fp32_ops(c, pc, coded)) != LES_OK) return rc;
     if ((rc = ensure_trace(c, prm->max_iter + 1)) != LES_OK) return rc;
     MixedWork& w = c->mw;
 
@@ -840,8 +886,8 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     HIPCHK(c, hipMemcpyAsync(c->sc, &c->sc_host[0], sizeof(SolveScalars), hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemsetAsync(w.x, 0, nbytes(c, 4), s));
 
-    const Stencil<double> A64 = stencil_of(c->A);
-    const Stencil<float> A32 = stencil_of(c->A32);
+    const Stencil<double> A64 = coded ? coded_stencil<double>(c) : stencil_of(c->A);
+    const Stencil<float> A32 = coded ? coded_stencil<float>(c) : stencil_of(c->A32);
     double* x64 = c->f[LES_F_P];
     const double* b = c->f[LES_F_B];
     SolveScalars* sc = c->sc;
@@ -849,7 +895,7 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     // r = b - A x0 in FP64, rounded to FP32; ||r||, ||b||
     HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.r, 1, 0, c->red, sc, nullptr));
     if ((rc = precond_fp32(c, pc, w.r, w.p, prm->lambda_min, prm->lambda_max, prm->degree, EPI_RHO,
-                           sc)) != LES_OK)
+                           sc, A32)) != LES_OK)
         return rc;
 
     int check = prm->check_every > 0 ? prm->check_every : (d.N >= (1 << 22) ? 4 : 16);
@@ -869,7 +915,7 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
             HIPCHK(c, launch_mixed_fold(s, d, x64, w.x, 0, sc));
             HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.r, 0, 0, c->red, sc, nullptr));
             if ((rc = precond_fp32(c, pc, w.r, w.z, prm->lambda_min, prm->lambda_max, prm->degree,
-                                   EPI_BETA, sc)) != LES_OK)
+                                   EPI_BETA, sc, A32)) != LES_OK)
                 return rc;
             HIPCHK(c, launch_pcg_p_update<float>(s, d, w.z, w.p, sc));
         }

@@ -1,17 +1,21 @@
 // les_device.hpp -- device-side building blocks shared by the LES kernels.
 //
 // Grid layout (src/field.h:40): idx = i*ny*nz + j*nz + k, k contiguous.
-// Every kernel that touches whole grids walks the flat cell range in
-// per-workgroup chunks; the workgroup -> chunk map is XCD-aware (MI355X
-// dispatches workgroups round-robin over its 8 XCDs, so consecutive chunks
-// are given to workgroups that land on the same XCD and share its L2).
 //
-// Reductions are deterministic: each workgroup reduces its chunk in a fixed
-// order (lane-sequential, then a fixed xor-shuffle tree over the wave64, then
-// wave partials in order), writes one partial per workgroup, and the last
-// workgroup to finish sums the partials in a fixed order and runs the scalar
-// epilogue.  The grid size depends only on the problem size, so results do
-// not depend on the device, the run or the launch timing.
+// Work decomposition.  A "row" is one (i, j) line of nz contiguous cells.
+// Each workgroup owns a contiguous range of rows; threads run along k, so
+// the row's i and j are uniform across the wavefront (scalar ALU) and no
+// per-cell division is needed.  When nz < 256 one pass covers several rows.
+// The workgroup -> row-range map is XCD-aware: MI355X dispatches
+// workgroups round-robin over its 8 XCDs, so workgroups b, b+8, b+16, ...
+// (one XCD) receive consecutive row ranges and their i+-1 / j+-1 neighbours
+// are served by that XCD's L2.
+//
+// Reductions are deterministic and fence-free: every workgroup reduces its
+// rows in a fixed order (lane-sequential, a fixed xor-shuffle tree over the
+// wave64, waves in order) and writes one partial; a one-workgroup finalize
+// kernel then sums the partials in a fixed order and runs the scalar
+// epilogue on the device.  The grid size depends only on the problem size.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -31,37 +35,47 @@ struct Dims {
     int N;       // nx*ny*nz (< 2^31, src/field.h:40 uses int)
 };
 
+// Row ranges of a launch.
 struct Chunking {
-    int grid;    // workgroups
-    int chunk;   // cells per workgroup (multiple of kBlock)
+    int grid;    // workgroups (multiple of kXcd)
+    int rpb;     // rows per workgroup
+    int rpp;     // rows per pass (kBlock / nz when nz < kBlock, else 1)
 };
 
-inline Chunking make_chunking(int N)
+inline Chunking make_chunking(const Dims& d)
 {
-    int blocks = (N + kBlock - 1) / kBlock;
-    int grid = blocks < kMaxGrid ? blocks : kMaxGrid;
+    const int rows = d.nx * d.ny;
+    const int rpp = d.nz >= kBlock ? 1 : kBlock / d.nz;
+    // aim for about kBlock cells of work per thread-row pass, capped grid
+    int grid = (rows + rpp - 1) / rpp;
+    if (grid > kMaxGrid) grid = kMaxGrid;
     grid = ((grid + kXcd - 1) / kXcd) * kXcd;
-    int per = (blocks + grid - 1) / grid;
-    return Chunking{grid, per * kBlock};
+    int rpb = (rows + grid - 1) / grid;
+    rpb = ((rpb + rpp - 1) / rpp) * rpp;
+    return Chunking{grid, rpb, rpp};
 }
 
-// Logical chunk of this workgroup: workgroups b, b+8, b+16 ... run on one XCD
-// and receive consecutive chunks.
+// Logical range of this workgroup: workgroups b, b+8, b+16 ... run on one
+// XCD and receive consecutive ranges.
 __device__ __forceinline__ int logical_block(int grid)
 {
     const int b = blockIdx.x;
     return (b % kXcd) * (grid / kXcd) + b / kXcd;
 }
 
-__device__ __forceinline__ void decode(const Dims& d, int id, int& i, int& j, int& k)
-{
-    i = id / d.sij;
-    const int r = id - i * d.sij;
-    j = r / d.nz;
-    k = r - j * d.nz;
-}
+// FOR_CELLS(d, ch, lb) { ... } runs the body for every cell of the
+// workgroup's rows with i, j, k, id in scope.
+#define FOR_CELLS(d, ch, lb)                                                                      \
+    for (int sub_ = ((ch).rpp > 1 ? (int)threadIdx.x / (d).nz : 0),                              \
+             k0_ = (int)threadIdx.x - sub_ * (d).nz,                                              \
+             row_ = (lb) * (ch).rpb + sub_,                                                       \
+             rend_ = min((d).nx * (d).ny, ((lb) + 1) * (ch).rpb);                                 \
+         sub_ < (ch).rpp && row_ < rend_; row_ += (ch).rpp)                                       \
+        for (int i = row_ / (d).ny, j = row_ - i * (d).ny, k = k0_, id = row_ * (d).nz + k0_;     \
+             k < (d).nz; k += ((ch).rpp > 1 ? (d).nz : kBlock),                                   \
+                 id += ((ch).rpp > 1 ? (d).nz : kBlock))
 
-// ---- wave64 / workgroup sums in a fixed order ----
+// ---- wave64 / workgroup reductions in a fixed order ----
 __device__ __forceinline__ double wave_sum(double v)
 {
 #pragma unroll
@@ -79,108 +93,66 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-// Sums NR values over the workgroup; result valid in thread 0.
-template <int NR>
-__device__ __forceinline__ void block_sum(double (&v)[NR])
-{
-    __shared__ double sh[kWaves][kMaxRed];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) v[r] = wave_sum(v[r]);
-    if (lane == 0) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) sh[wave][r] = v[r];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            double s = sh[0][r];
-            for (int w = 1; w < kWaves; ++w) s += sh[w][r];
-            v[r] = s;
-        }
-    }
-    __syncthreads();
-}
-
-template <int NR>
-__device__ __forceinline__ void block_max(double (&v)[NR])
-{
-    __shared__ double sh[kWaves][kMaxRed];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-#pragma unroll
-    for (int r = 0; r < NR; ++r) v[r] = wave_max(v[r]);
-    if (lane == 0) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) sh[wave][r] = v[r];
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-#pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            double s = sh[0][r];
-            for (int w = 1; w < kWaves; ++w) s = s > sh[w][r] ? s : sh[w][r];
-            v[r] = s;
-        }
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ void store_agent(double* p, double v)
-{
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double load_agent(const double* p)
-{
-    return __hip_atomic_load(const_cast<double*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Grid-wide deterministic reduction tail.  Every workgroup calls it once with
-// its NR partial sums (valid in thread 0).  Returns true in exactly one
-// workgroup -- the last to arrive -- with the NR grid totals in `tot`
-// (valid in every thread of that workgroup).  `partials` holds grid*kMaxRed
-// doubles; `counter` is zero on entry and is left zero.
+// Reduces NR values over the workgroup; result valid in thread 0.
 template <int NR, bool MAX = false>
-__device__ __forceinline__ bool grid_reduce(double (&v)[NR], double* partials,
-                                            unsigned int* counter, int grid, int lb,
-                                            double (&tot)[NR])
+__device__ __forceinline__ void block_reduce(double (&v)[NR])
 {
-    __shared__ int am_last;
-    if (threadIdx.x == 0) {
+    __shared__ double sh[kWaves][kMaxRed];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
 #pragma unroll
-        for (int r = 0; r < NR; ++r) store_agent(&partials[lb * kMaxRed + r], v[r]);
-        __threadfence();
-        unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL,
-                                               __HIP_MEMORY_SCOPE_AGENT);
-        am_last = (prev == (unsigned)grid - 1u);
+    for (int r = 0; r < NR; ++r) v[r] = MAX ? wave_max(v[r]) : wave_sum(v[r]);
+    if (lane == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) sh[wave][r] = v[r];
     }
     __syncthreads();
-    if (!am_last) return false;
-    __threadfence();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) {
+            double s = sh[0][r];
+            for (int w = 1; w < kWaves; ++w) s = MAX ? (s > sh[w][r] ? s : sh[w][r]) : s + sh[w][r];
+            v[r] = s;
+        }
+    }
+    __syncthreads();
+}
+
+// Workgroup partial of NR sums (or maxima) into partials[lb*kMaxRed + r].
+// Every workgroup of the launch must call it.
+template <int NR, bool MAX = false>
+__device__ __forceinline__ void block_partial(double (&v)[NR], double* partials, int lb)
+{
+    block_reduce<NR, MAX>(v);
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int r = 0; r < NR; ++r) partials[lb * kMaxRed + r] = v[r];
+    }
+}
+
+// Fixed-order sum (or max) of the grid's partials by one workgroup; result
+// valid in every thread.
+template <int NR, bool MAX = false>
+__device__ __forceinline__ void reduce_partials(const double* partials, int grid, double (&tot)[NR])
+{
     double s[NR];
 #pragma unroll
-    for (int r = 0; r < NR; ++r) s[r] = MAX ? 0.0 : 0.0;
-    // fixed order: thread t takes partials t, t+kBlock, ...
+    for (int r = 0; r < NR; ++r) s[r] = 0.0;
     for (int b = threadIdx.x; b < grid; b += kBlock) {
 #pragma unroll
         for (int r = 0; r < NR; ++r) {
-            double x = load_agent(&partials[b * kMaxRed + r]);
-            if (MAX) s[r] = s[r] > x ? s[r] : x;
-            else s[r] += x;
+            const double x = partials[b * kMaxRed + r];
+            s[r] = MAX ? (s[r] > x ? s[r] : x) : s[r] + x;
         }
     }
-    if (MAX) block_max<NR>(s);
-    else block_sum<NR>(s);
+    block_reduce<NR, MAX>(s);
     __shared__ double out[kMaxRed];
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int r = 0; r < NR; ++r) out[r] = s[r];
-        *counter = 0u;
     }
     __syncthreads();
 #pragma unroll
     for (int r = 0; r < NR; ++r) tot[r] = out[r];
-    return true;
 }
 
 }  // namespace les

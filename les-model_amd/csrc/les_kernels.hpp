@@ -16,10 +16,24 @@ namespace les {
 // (src/pressure.h:22): c, n, s, e, w, t, b.
 enum { SC = 0, SN = 1, SS = 2, SE = 3, SW = 4, ST = 5, SB = 6 };
 
+// The operator, either as the 7 coefficient arrays or -- for the operator
+// build_stencil produces from a mask -- as one 16-bit code per cell.  Every
+// off-diagonal of that operator is 0, -1/d^2 or 2*(-1/d^2) (a mirrored
+// face adds the base coefficient to itself, pressure.cpp:662-696) and the
+// centre is 2/dx^2+2/dy^2+2/dz^2 or 1 (identity rows, pressure.cpp:634-642),
+// so the code reproduces the arrays bit for bit:
+//   bits 0-1 w, 2-3 e, 4-5 s, 6-7 n, 8-9 b, 10-11 t: multiplier 0, 1 or 2
+//   bit 12: identity row (centre 1)
 template <typename T>
 struct Stencil {
     const T* a[7];
+    const uint16_t* code;  // non-null: coefficients come from the code
+    T cfluid;              // centre of a fluid row
+    T ox, oy, oz;          // -1/dx^2, -1/dy^2, -1/dz^2
+    T ox2, oy2, oz2;       // 2*(-1/dx^2) ...
 };
+
+enum { CODE_IDENTITY = 1 << 12 };
 
 // Scalars of one Krylov solve, resident in device memory so that the
 // iteration never needs the host.
@@ -48,10 +62,17 @@ struct SolveScalars {
     double* trace;       // optional: ||r_k|| for k = 0..max_iter
 };
 
-// Scratch used by every fused reduction kernel.
+// Per-workgroup partials of a reduction (kMaxGrid * kMaxRed doubles); a
+// null pointer means the kernel reduces nothing.
 struct RedScratch {
-    double* partials;      // kMaxGrid * kMaxRed
-    unsigned int* counter; // one counter (zero between kernels)
+    double* partials;
+};
+
+// When a gated kernel (and its finalize) runs.
+enum Gate {
+    GATE_NONE = 0,      // always
+    GATE_DONE = 1,      // unless the solve is done
+    GATE_RELIABLE = 2,  // only when a reliable replacement is pending
 };
 
 // ---- scalar epilogues run by the last workgroup of a reduction ----
@@ -67,9 +88,16 @@ enum Epilogue {
                          // err < tol, iter % reliable_every == 0 or iter == max_iter
     EPI_RELIABLE = 8,    // tot = {rr64}: true residual; done if below tol
     EPI_INIT_MIXED = 9,  // tot = {rr64, bb}: as EPI_INIT, also true_err
+    EPI_STORE2 = 10,     // out[0..1] = tot[0..1]
+    EPI_STORE_SQRT = 11, // out[0] = sqrt(tot[0])
 };
 
+// The one-workgroup tail of a reduction over `grid` workgroup partials.
+hipError_t launch_finalize(hipStream_t s, int grid, int nr, bool max, int epi, int gate,
+                           RedScratch red, SolveScalars* sc, double* out);
+
 // ---- operator ----
+hipError_t launch_build_code(hipStream_t s, const Dims& d, const uint8_t* solid, uint16_t* code);
 hipError_t launch_build_stencil(hipStream_t s, const Dims& d, const uint8_t* solid,
                                 double* const A[7], double idx2, double idy2, double idz2);
 template <typename T>

@@ -16,11 +16,6 @@ namespace les {
 
 namespace {
 
-#define FOR_CELLS(d, ch, lb)                                                         \
-    for (int id = (lb) * (ch).chunk + (int)threadIdx.x,                              \
-             id_end_ = min((d).N, ((lb) + 1) * (ch).chunk);                          \
-         id < id_end_; id += kBlock)
-
 __device__ __forceinline__ bool on_wall(const Dims& d, int j, int k)
 {
     return j == 0 || j == d.ny - 1 || k == 0 || k == d.nz - 1;
@@ -59,14 +54,12 @@ __device__ __forceinline__ double sgs_at(const Dims& d, const PhysConsts& c, con
 __global__ __launch_bounds__(kBlock) void k_sgs(Dims d, Chunking ch, PhysConsts c, const double* u,
                                                 const double* v, const double* w,
                                                 const uint8_t* solid, double* nu_t, Tau tau,
-                                                int apply_bcs, RedScratch red, double* nut_max)
+                                                int apply_bcs, RedScratch red)
 {
     const int lb = logical_block(ch.grid);
     double mx[1] = {0.0};
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
         double out;
         if (!apply_bcs) {
             // compute_sgs_stress alone: interior cells, raw values
@@ -96,13 +89,7 @@ __global__ __launch_bounds__(kBlock) void k_sgs(Dims d, Chunking ch, PhysConsts 
         nu_t[id] = out;
         mx[0] = mx[0] > out ? mx[0] : out;
     }
-    if (nut_max) {
-        block_max<1>(mx);
-        double tot[1];
-        if (grid_reduce<1, true>(mx, red.partials, red.counter, ch.grid, lb, tot) &&
-            threadIdx.x == 0)
-            nut_max[0] = tot[0];
-    }
+    if (red.partials) block_partial<1, true>(mx, red.partials, lb);
 }
 
 // one component of solve_momentum at an interior cell (main.cpp:584-600)
@@ -141,8 +128,6 @@ __global__ __launch_bounds__(kBlock) void k_momentum(Dims d, Chunking ch, PhysCo
     const int lb = logical_block(ch.grid);
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
         const bool sol = solid[id];
         // ---- provisional velocity ----
         if (mode == MOM_RAW) {
@@ -202,8 +187,6 @@ __global__ __launch_bounds__(kBlock) void k_correct(Dims d, Chunking ch, PhysCon
     const int lb = logical_block(ch.grid);
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
         if (i == 0 || i == d.nx - 1 || on_wall(d, j, k)) continue;
         if (apply_bcs && solid[id] && i <= d.nx - 3) {
             u[id] = 0.0; v[id] = 0.0; w[id] = 0.0;
@@ -223,8 +206,6 @@ __global__ __launch_bounds__(kBlock) void k_zero_solids(Dims d, Chunking ch, con
     FOR_CELLS(d, ch, lb)
     {
         if (!solid[id]) continue;
-        int i, j, k;
-        decode(d, id, i, j, k);
         if (i == 0 || i > d.nx - 3 || on_wall(d, j, k)) continue;
         a0[id] = 0.0;
         if (a1) a1[id] = 0.0;
@@ -307,18 +288,13 @@ __global__ __launch_bounds__(kBlock) void k_build_b(Dims d, Chunking ch, const d
                                                     const uint8_t* solid, double* b)
 {
     const int lb = logical_block(ch.grid);
-    FOR_CELLS(d, ch, lb)
-    {
-        const int i = id / d.sij;
-        b[id] = (solid[id] || i == d.nx - 1) ? 0.0 : -div[id];
-    }
+    FOR_CELLS(d, ch, lb) b[id] = (solid[id] || i == d.nx - 1) ? 0.0 : -div[id];
 }
 
 // max(u^2+v^2+w^2) and max nu_t over all cells (main.cpp:215-245)
 __global__ __launch_bounds__(kBlock) void k_dt_max(Dims d, Chunking ch, const double* u,
                                                    const double* v, const double* w,
-                                                   const double* nu_t, RedScratch red,
-                                                   double* out)
+                                                   const double* nu_t, RedScratch red)
 {
     const int lb = logical_block(ch.grid);
     double mx[2] = {0.0, 0.0};
@@ -328,12 +304,7 @@ __global__ __launch_bounds__(kBlock) void k_dt_max(Dims d, Chunking ch, const do
         mx[0] = mx[0] > s ? mx[0] : s;
         mx[1] = mx[1] > nu_t[id] ? mx[1] : nu_t[id];
     }
-    block_max<2>(mx);
-    double tot[2];
-    if (grid_reduce<2, true>(mx, red.partials, red.counter, ch.grid, lb, tot) && threadIdx.x == 0) {
-        out[0] = tot[0];
-        out[1] = tot[1];
-    }
+    block_partial<2, true>(mx, red.partials, lb);
 }
 
 // ------------------------------------------------------------------
@@ -368,8 +339,6 @@ __global__ __launch_bounds__(kBlock) void k_pbc_solids(Dims d, Chunking ch, doub
     FOR_CELLS(d, ch, lb)
     {
         if (!solid[id]) continue;
-        int i, j, k;
-        decode(d, id, i, j, k);
         if (i == 0 || i == d.nx - 1 || on_wall(d, j, k)) continue;
         if (!solid[id + d.sij]) p[id] = p[id + d.sij];
         else if (!solid[id - d.sij]) p[id] = p[id - d.sij];
@@ -400,8 +369,6 @@ __global__ __launch_bounds__(kBlock) void k_rhs_central(Dims d, Chunking ch, con
     const int lb = logical_block(ch.grid);
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
         if (i == 0 || i == d.nx - 1 || on_wall(d, j, k)) {
             rhs[id] = 0.0;
             continue;
@@ -432,8 +399,6 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_sweep(Dims d, Chunking ch, co
     double mx[1] = {0.0};
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
         if (i == 0 || i == d.nx - 1 || on_wall(d, j, k)) continue;
         const double pold = pc[id];
         const double val = ((pc[id + d.sij] + pc[id - d.sij]) / dx2 +
@@ -444,7 +409,7 @@ __global__ __launch_bounds__(kBlock) void k_jacobi_sweep(Dims d, Chunking ch, co
         const double r = fabs(val - pold);
         mx[0] = mx[0] > r ? mx[0] : r;
     }
-    block_max<1>(mx);
+    block_reduce<1, true>(mx);
     if (threadIdx.x == 0 && mx[0] > 0.0) atomic_max_nonneg(&st->maxr, mx[0]);
 }
 
@@ -476,7 +441,7 @@ __global__ __launch_bounds__(kBlock) void k_sor_level(Dims d, int level, double*
         }
     }
     double mx[1] = {r};
-    block_max<1>(mx);
+    block_reduce<1, true>(mx);
     if (threadIdx.x == 0 && mx[0] > 0.0) atomic_max_nonneg(&st->maxr, mx[0]);
 }
 
@@ -492,8 +457,6 @@ __global__ __launch_bounds__(kBlock) void k_sor_colour(Dims d, Chunking ch, doub
     double mx[1] = {0.0};
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
         if (i == 0 || i == d.nx - 1 || on_wall(d, j, k) || ((i + j + k) & 1) != colour) continue;
         const double pold = p[id];
         const double val = one_m_omega * p[id] +
@@ -504,7 +467,7 @@ __global__ __launch_bounds__(kBlock) void k_sor_colour(Dims d, Chunking ch, doub
         const double r = fabs(val - pold);
         mx[0] = mx[0] > r ? mx[0] : r;
     }
-    block_max<1>(mx);
+    block_reduce<1, true>(mx);
     if (threadIdx.x == 0 && mx[0] > 0.0) atomic_max_nonneg(&st->maxr, mx[0]);
 }
 
@@ -519,10 +482,15 @@ hipError_t launch_sgs(hipStream_t s, const Dims& d, const PhysConsts& c, const d
                       const double* v, const double* w, const uint8_t* solid, double* nu_t,
                       Tau tau, int apply_bcs, RedScratch red, double* nut_max)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
+    RedScratch rd = red;
+    if (!(apply_bcs && nut_max)) rd.partials = nullptr;
     hipLaunchKernelGGL(k_sgs, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, c, u, v, w, solid, nu_t,
-                       tau, apply_bcs, red, apply_bcs ? nut_max : nullptr);
-    return hipGetLastError();
+                       tau, apply_bcs, rd);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    if (rd.partials)
+        return launch_finalize(s, ch.grid, 1, true, EPI_STORE, GATE_NONE, red, nullptr, nut_max);
+    return hipSuccess;
 }
 
 hipError_t launch_momentum(hipStream_t s, const Dims& d, const PhysConsts& c, const double* u,
@@ -530,7 +498,7 @@ hipError_t launch_momentum(hipStream_t s, const Dims& d, const PhysConsts& c, co
                            const uint8_t* solid, double* un, double* vn, double* wn,
                            double* p_rhs, double* b, int mode)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_momentum, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, c, u, v, w, nu_t,
                        solid, un, vn, wn, p_rhs, b, mode);
     return hipGetLastError();
@@ -539,7 +507,7 @@ hipError_t launch_momentum(hipStream_t s, const Dims& d, const PhysConsts& c, co
 hipError_t launch_correct(hipStream_t s, const Dims& d, const PhysConsts& c, double* u, double* v,
                           double* w, const double* p, const uint8_t* solid, int apply_bcs)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_correct, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, c, u, v, w, p, solid,
                        apply_bcs);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
@@ -551,7 +519,7 @@ hipError_t launch_correct(hipStream_t s, const Dims& d, const PhysConsts& c, dou
 hipError_t launch_velocity_bcs(hipStream_t s, const Dims& d, const uint8_t* solid, double* u,
                                double* v, double* w)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_zero_solids, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, solid, u, v, w);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     hipLaunchKernelGGL(k_faces<true>, dim3(face_grid(d)), dim3(kBlock), 0, s, d, solid, u, v, w);
@@ -560,7 +528,7 @@ hipError_t launch_velocity_bcs(hipStream_t s, const Dims& d, const uint8_t* soli
 
 hipError_t launch_nut_bcs(hipStream_t s, const Dims& d, const uint8_t* solid, double* nu_t)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_zero_solids, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, solid, nu_t,
                        (double*)nullptr, (double*)nullptr);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
@@ -572,7 +540,7 @@ hipError_t launch_nut_bcs(hipStream_t s, const Dims& d, const uint8_t* solid, do
 hipError_t launch_build_b(hipStream_t s, const Dims& d, const double* div, const uint8_t* solid,
                           double* b)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_build_b, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, div, solid, b);
     return hipGetLastError();
 }
@@ -580,16 +548,16 @@ hipError_t launch_build_b(hipStream_t s, const Dims& d, const double* div, const
 hipError_t launch_dt_max(hipStream_t s, const Dims& d, const double* u, const double* v,
                          const double* w, const double* nu_t, RedScratch red, double* out2)
 {
-    const Chunking ch = make_chunking(d.N);
-    hipLaunchKernelGGL(k_dt_max, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, u, v, w, nu_t, red,
-                       out2);
-    return hipGetLastError();
+    const Chunking ch = make_chunking(d);
+    hipLaunchKernelGGL(k_dt_max, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, u, v, w, nu_t, red);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    return launch_finalize(s, ch.grid, 2, true, EPI_STORE2, GATE_NONE, red, nullptr, out2);
 }
 
 hipError_t launch_pressure_bcs(hipStream_t s, const Dims& d, double* p, const uint8_t* solid,
                                StatScalars* st, int sweep)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_pbc_faces, dim3(face_grid(d)), dim3(kBlock), 0, s, d, p,
                        (const StatScalars*)st);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
@@ -604,7 +572,7 @@ hipError_t launch_rhs_central(hipStream_t s, const Dims& d, const double* u, con
                               const double* w, double* rhs, double coef, double two_dx,
                               double two_dy, double two_dz)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_rhs_central, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, u, v, w, rhs, coef,
                        two_dx, two_dy, two_dz);
     return hipGetLastError();
@@ -614,7 +582,7 @@ hipError_t launch_jacobi_sweep(hipStream_t s, const Dims& d, const double* pc, d
                                const double* rhs, double dx2, double dy2, double dz2,
                                double denom, StatScalars* st)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_jacobi_sweep, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, pc, pn, rhs, dx2,
                        dy2, dz2, denom, st);
     return hipGetLastError();
@@ -634,7 +602,7 @@ hipError_t launch_sor_colour(hipStream_t s, const Dims& d, double* p, const doub
                              int colour, double one_m_omega, double omega_coef, double dx2,
                              double dy2, double dz2, StatScalars* st)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_sor_colour, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, p, rhs, colour,
                        one_m_omega, omega_coef, dx2, dy2, dz2, st);
     return hipGetLastError();

@@ -12,27 +12,57 @@ namespace les {
 
 namespace {
 
-#define FOR_CELLS(d, ch, lb)                                                         \
-    for (int id = (lb) * (ch).chunk + (int)threadIdx.x,                              \
-             id_end_ = min((d).N, ((lb) + 1) * (ch).chunk);                          \
-         id < id_end_; id += kBlock)
+// ------------------------------------------------------------------
+// operator application
+// ------------------------------------------------------------------
+template <typename T>
+__device__ __forceinline__ T mult(unsigned m, T one, T two)
+{
+    return m == 0u ? T(0) : (m == 1u ? one : two);
+}
+
+// The seven coefficients of a cell.  CODED selects the storage at compile
+// time (1 coded, 0 arrays; the run-time A.code must agree), -1 at run time.
+template <int CODED, typename T>
+__device__ __forceinline__ void coefs7(const Stencil<T>& A, int id, T& c, T& w, T& e, T& s,
+                                       T& n, T& b, T& t)
+{
+    if (CODED == 1 || (CODED == -1 && A.code)) {
+        const unsigned cd = A.code[id];
+        c = (cd & CODE_IDENTITY) ? T(1) : A.cfluid;
+        w = mult<T>(cd & 3u, A.ox, A.ox2);
+        e = mult<T>((cd >> 2) & 3u, A.ox, A.ox2);
+        s = mult<T>((cd >> 4) & 3u, A.oy, A.oy2);
+        n = mult<T>((cd >> 6) & 3u, A.oy, A.oy2);
+        b = mult<T>((cd >> 8) & 3u, A.oz, A.oz2);
+        t = mult<T>((cd >> 10) & 3u, A.oz, A.oz2);
+    } else {
+        c = A.a[SC][id]; w = A.a[SW][id]; e = A.a[SE][id]; s = A.a[SS][id];
+        n = A.a[SN][id]; b = A.a[SB][id]; t = A.a[ST][id];
+    }
+}
 
 // q = A p at one cell, neighbour order c,w,e,s,n,b,t with the reference's
 // bounds guards (pressure.cpp:726-744).  Load(n) returns p at flat index n.
-template <typename T, typename Load>
+template <int CODED, typename T, typename Load>
 __device__ __forceinline__ T apply7(const Stencil<T>& A, const Dims& d, int id, int i, int j, int k,
                                     Load ld)
 {
-    T val = A.a[SC][id] * ld(id);
-    if (i > 0) val += A.a[SW][id] * ld(id - d.sij);
-    if (i < d.nx - 1) val += A.a[SE][id] * ld(id + d.sij);
-    if (j > 0) val += A.a[SS][id] * ld(id - d.nz);
-    if (j < d.ny - 1) val += A.a[SN][id] * ld(id + d.nz);
-    if (k > 0) val += A.a[SB][id] * ld(id - 1);
-    if (k < d.nz - 1) val += A.a[ST][id] * ld(id + 1);
+    T c, w, e, s, n, b, t;
+    coefs7<CODED>(A, id, c, w, e, s, n, b, t);
+    T val = c * ld(id);
+    if (i > 0) val += w * ld(id - d.sij);
+    if (i < d.nx - 1) val += e * ld(id + d.sij);
+    if (j > 0) val += s * ld(id - d.nz);
+    if (j < d.ny - 1) val += n * ld(id + d.nz);
+    if (k > 0) val += b * ld(id - 1);
+    if (k < d.nz - 1) val += t * ld(id + 1);
     return val;
 }
 
+// ------------------------------------------------------------------
+// device-resident scalar epilogues
+// ------------------------------------------------------------------
 __device__ __forceinline__ void epi_init(const double* tot, SolveScalars* sc)
 {
     sc->rr = tot[0];
@@ -58,8 +88,19 @@ __device__ void epilogue(int epi, const double* tot, SolveScalars* sc, double* o
     case EPI_STORE:
         out[0] = tot[0];
         break;
+    case EPI_STORE2:
+        out[0] = tot[0];
+        out[1] = tot[1];
+        break;
+    case EPI_STORE_SQRT:
+        out[0] = sqrt(tot[0]);
+        break;
     case EPI_INIT:
         epi_init(tot, sc);
+        break;
+    case EPI_INIT_MIXED:
+        epi_init(tot, sc);
+        sc->true_err = sc->err0;
         break;
     case EPI_RHO:
         sc->rho = tot[0];
@@ -132,16 +173,35 @@ __device__ void epilogue(int epi, const double* tot, SolveScalars* sc, double* o
         }
         break;
     }
-    case EPI_INIT_MIXED:
-        epi_init(tot, sc);
-        sc->true_err = sc->err0;
-        break;
     default:
         break;
     }
 }
 
 __device__ __forceinline__ bool is_done(const SolveScalars* sc) { return sc && sc->done; }
+
+// Whether a kernel gated by `gate` runs.  Producer and finalize apply the
+// same test, and neither changes the scalars it reads.
+__device__ __forceinline__ bool gate_open(int gate, const SolveScalars* sc)
+{
+    switch (gate) {
+    case GATE_DONE: return !sc->done;
+    case GATE_RELIABLE: return !sc->done && sc->reliable_now;
+    default: return true;
+    }
+}
+
+// The one-workgroup tail of every reduction: fixed-order sum (or max) of the
+// producer's per-workgroup partials, then the scalar epilogue.
+template <int NR, bool MAX>
+__global__ __launch_bounds__(kBlock) void k_finalize(int grid, const double* partials, int epi,
+                                                     int gate, SolveScalars* sc, double* out)
+{
+    if (!gate_open(gate, sc)) return;
+    double tot[NR];
+    reduce_partials<NR, MAX>(partials, grid, tot);
+    if (threadIdx.x == 0) epilogue(epi, tot, sc, out);
+}
 
 // ------------------------------------------------------------------
 // build_stencil (pressure.cpp:608-699)
@@ -157,8 +217,6 @@ __global__ __launch_bounds__(kBlock) void k_build_stencil(Dims d, Chunking ch, c
     const double c_b = (2.0 * idx2 + 2.0 * idy2 + 2.0 * idz2);
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
         double c = 1.0, e = 0.0, w = 0.0, n = 0.0, s = 0.0, t = 0.0, b = 0.0;
         if (!solid[id] && i != d.nx - 1) {
             c = c_b; e = e_b; w = w_b; n = n_b; s = s_b; t = t_b; b = b_b;
@@ -179,17 +237,41 @@ __global__ __launch_bounds__(kBlock) void k_build_stencil(Dims d, Chunking ch, c
     }
 }
 
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_matvec(Dims d, Chunking ch, Stencil<T> A, const T* x,
-                                                   T* y)
+// The same case analysis as build_stencil, recording multipliers of the
+// base coefficients instead of their values.
+__global__ __launch_bounds__(kBlock) void k_build_code(Dims d, Chunking ch, const uint8_t* solid,
+                                                       uint16_t* code)
 {
     const int lb = logical_block(ch.grid);
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
-        y[id] = apply7(A, d, id, i, j, k, [&](int n) { return x[n]; });
+        unsigned cd = CODE_IDENTITY;
+        if (!solid[id] && i != d.nx - 1) {
+            unsigned e = 1, w = 1, n = 1, s = 1, t = 1, b = 1;
+            if (i == 0) { e += 1; w = 0; }
+            else if (solid[id - d.sij]) { e += 1; w = 0; }
+            if (i == d.nx - 2) { e = 0; }
+            else if (solid[id + d.sij]) { w += 1; e = 0; }
+            if (j == 0) { n += 1; s = 0; }
+            else if (solid[id - d.nz]) { n += 1; s = 0; }
+            if (j == d.ny - 1) { s += 1; n = 0; }
+            else if (solid[id + d.nz]) { s += 1; n = 0; }
+            if (k == 0) { t += 1; b = 0; }
+            else if (solid[id - 1]) { t += 1; b = 0; }
+            if (k == d.nz - 1) { b += 1; t = 0; }
+            else if (solid[id + 1]) { b += 1; t = 0; }
+            cd = w | (e << 2) | (s << 4) | (n << 6) | (b << 8) | (t << 10);
+        }
+        code[id] = (uint16_t)cd;
     }
+}
+
+template <typename T, int CODED>
+__global__ __launch_bounds__(kBlock) void k_matvec(Dims d, Chunking ch, Stencil<T> A,
+                                                   const T* __restrict__ x, T* __restrict__ y)
+{
+    const int lb = logical_block(ch.grid);
+    FOR_CELLS(d, ch, lb) y[id] = apply7<CODED>(A, d, id, i, j, k, [&](int n) { return x[n]; });
 }
 
 template <typename TO, typename TI>
@@ -200,70 +282,62 @@ __global__ __launch_bounds__(kBlock) void k_convert(int n, const TI* in, TO* out
 }
 
 __global__ __launch_bounds__(kBlock) void k_dot(Dims d, Chunking ch, const double* a,
-                                                const double* b, RedScratch red, double* out)
+                                                const double* b, RedScratch red)
 {
     const int lb = logical_block(ch.grid);
     double acc[1] = {0.0};
     FOR_CELLS(d, ch, lb) acc[0] += a[id] * b[id];
-    block_sum<1>(acc);
-    double tot[1];
-    if (grid_reduce<1>(acc, red.partials, red.counter, ch.grid, lb, tot) && threadIdx.x == 0)
-        out[0] = tot[0];
+    block_partial<1>(acc, red.partials, lb);
 }
 
 // ------------------------------------------------------------------
 // PCG (pressure.cpp:342-414)
 // ------------------------------------------------------------------
-template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_init(Dims d, Chunking ch, Stencil<T> A, const T* b,
-                                                     const T* x, T* r, RedScratch red,
-                                                     SolveScalars* sc)
+template <typename T, int CODED>
+__global__ __launch_bounds__(kBlock) void k_pcg_init(Dims d, Chunking ch, Stencil<T> A,
+                                                     const T* __restrict__ b,
+                                                     const T* __restrict__ x, T* __restrict__ r,
+                                                     RedScratch red)
 {
     const int lb = logical_block(ch.grid);
     double acc[2] = {0.0, 0.0};
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
-        const T q = apply7(A, d, id, i, j, k, [&](int n) { return x[n]; });
+        const T q = apply7<CODED>(A, d, id, i, j, k, [&](int n) { return x[n]; });
         const T bv = b[id];
         const T rv = bv - q;
         r[id] = rv;
         acc[0] += (double)rv * (double)rv;
         acc[1] += (double)bv * (double)bv;
     }
-    block_sum<2>(acc);
-    double tot[2];
-    if (grid_reduce<2>(acc, red.partials, red.counter, ch.grid, lb, tot) && threadIdx.x == 0)
-        epilogue(EPI_INIT, tot, sc, nullptr);
+    block_partial<2>(acc, red.partials, lb);
 }
 
-template <typename T>
+template <typename T, int CODED>
 __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Stencil<T> A,
-                                                       const T* p, T* q, RedScratch red,
-                                                       SolveScalars* sc)
+                                                       const T* __restrict__ p,
+                                                       T* __restrict__ q, RedScratch red,
+                                                       const SolveScalars* sc)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
     double acc[1] = {0.0};
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
-        const T qv = apply7(A, d, id, i, j, k, [&](int n) { return p[n]; });
+        const T qv = apply7<CODED>(A, d, id, i, j, k, [&](int n) { return p[n]; });
         q[id] = qv;
         acc[0] += (double)p[id] * (double)qv;
     }
-    block_sum<1>(acc);
-    double tot[1];
-    if (grid_reduce<1>(acc, red.partials, red.counter, ch.grid, lb, tot) && threadIdx.x == 0)
-        epilogue(EPI_ALPHA, tot, sc, nullptr);
+    block_partial<1>(acc, red.partials, lb);
 }
 
 template <typename T, typename TX>
-__global__ __launch_bounds__(kBlock) void k_pcg_update(Dims d, Chunking ch, TX* x, T* r,
-                                                       const T* p, const T* q, const T* diag,
-                                                       RedScratch red, SolveScalars* sc)
+__global__ __launch_bounds__(kBlock) void k_pcg_update(Dims d, Chunking ch, TX* __restrict__ x,
+                                                       T* __restrict__ r,
+                                                       const T* __restrict__ p,
+                                                       const T* __restrict__ q,
+                                                       const T* __restrict__ diag,
+                                                       RedScratch red, const SolveScalars* sc)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
@@ -279,28 +353,20 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Dims d, Chunking ch, TX* 
         acc[0] += (double)rv * (double)rv;
         if (diag) acc[1] += (double)rv * (double)(rv / diag[id]);
     }
-    double tot[2];
-    if (diag) {
-        block_sum<2>(acc);
-        if (grid_reduce<2>(acc, red.partials, red.counter, ch.grid, lb, tot) && threadIdx.x == 0)
-            epilogue(EPI_CONV_RZ, tot, sc, nullptr);
-    } else {
-        double a1[1] = {acc[0]}, t1[1];
-        block_sum<1>(a1);
-        if (grid_reduce<1>(a1, red.partials, red.counter, ch.grid, lb, t1) && threadIdx.x == 0)
-            epilogue(EPI_CONV, t1, sc, nullptr);
-    }
+    block_partial<2>(acc, red.partials, lb);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_p_jacobi(Dims d, Chunking ch, const T* r,
-                                                         const T* diag, T* p, int init, int epi,
-                                                         RedScratch red, SolveScalars* sc)
+__global__ __launch_bounds__(kBlock) void k_pcg_p_jacobi(Dims d, Chunking ch,
+                                                         const T* __restrict__ r,
+                                                         const T* __restrict__ diag,
+                                                         T* __restrict__ p, int init,
+                                                         RedScratch red, const SolveScalars* sc)
 {
     if (is_done(sc)) return;
     const int lb = logical_block(ch.grid);
     if (init) {
-        // z = r / A_c into p (pressure.cpp:420-428), r.z with the epilogue
+        // z = r / A_c into p (pressure.cpp:420-428), r.z partials
         double acc[1] = {0.0};
         FOR_CELLS(d, ch, lb)
         {
@@ -309,13 +375,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p_jacobi(Dims d, Chunking ch, co
             p[id] = z;
             acc[0] += (double)rv * (double)z;
         }
-        if (epi >= 0) {
-            block_sum<1>(acc);
-            double tot[1];
-            if (grid_reduce<1>(acc, red.partials, red.counter, ch.grid, lb, tot) &&
-                threadIdx.x == 0)
-                epilogue(epi, tot, sc, nullptr);
-        }
+        if (red.partials) block_partial<1>(acc, red.partials, lb);
     } else {
         const T beta = (T)sc->beta;
         FOR_CELLS(d, ch, lb) p[id] = r[id] / diag[id] + beta * p[id];
@@ -323,8 +383,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p_jacobi(Dims d, Chunking ch, co
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_p_update(Dims d, Chunking ch, const T* z, T* p,
-                                                         SolveScalars* sc)
+__global__ __launch_bounds__(kBlock) void k_pcg_p_update(Dims d, Chunking ch,
+                                                         const T* __restrict__ z,
+                                                         T* __restrict__ p, const SolveScalars* sc)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
@@ -333,28 +394,29 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p_update(Dims d, Chunking ch, co
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_pcg_rz(Dims d, Chunking ch, const T* r, const T* z,
-                                                   int epi, RedScratch red, SolveScalars* sc)
+__global__ __launch_bounds__(kBlock) void k_pcg_rz(Dims d, Chunking ch, const T* __restrict__ r,
+                                                   const T* __restrict__ z, RedScratch red,
+                                                   const SolveScalars* sc)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
     double acc[1] = {0.0};
     FOR_CELLS(d, ch, lb) acc[0] += (double)r[id] * (double)z[id];
-    block_sum<1>(acc);
-    double tot[1];
-    if (grid_reduce<1>(acc, red.partials, red.counter, ch.grid, lb, tot) && threadIdx.x == 0)
-        epilogue(epi, tot, sc, nullptr);
+    block_partial<1>(acc, red.partials, lb);
 }
 
 // ------------------------------------------------------------------
-// Chebyshev (pressure.cpp:851-906)
+// Chebyshev (pressure.cpp:851-906): one inner step of the recurrence
+//   res = r - A z_k ; p = res + beta p ; z_{k+1} = z_k + alpha p
+// FIRST forms z_1 = 0 + alpha0 r and p_0 = r on the fly from r.
 // ------------------------------------------------------------------
-template <typename T>
+template <typename T, bool FIRST, int CODED, bool REDUCE>
 __global__ __launch_bounds__(kBlock) void k_cheb_step(Dims d, Chunking ch, Stencil<T> A,
-                                                      const T* r, const T* z_in, T* z_out, T* p,
+                                                      const T* __restrict__ r,
+                                                      const T* __restrict__ z_in,
+                                                      T* __restrict__ z_out, T* __restrict__ p,
                                                       double alpha0, double beta, double alpha,
-                                                      int first, int epi, RedScratch red,
-                                                      SolveScalars* sc)
+                                                      RedScratch red, const SolveScalars* sc)
 {
     if (is_done(sc)) return;
     const int lb = logical_block(ch.grid);
@@ -362,16 +424,14 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(Dims d, Chunking ch, Stenc
     double acc[1] = {0.0};
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
         T az, zc, pp;
         const T rv = r[id];
-        if (first) {
-            az = apply7(A, d, id, i, j, k, [&](int n) { return T(0) + a0 * r[n]; });
+        if (FIRST) {
+            az = apply7<CODED>(A, d, id, i, j, k, [&](int n) { return T(0) + a0 * r[n]; });
             zc = T(0) + a0 * rv;
             pp = rv;
         } else {
-            az = apply7(A, d, id, i, j, k, [&](int n) { return z_in[n]; });
+            az = apply7<CODED>(A, d, id, i, j, k, [&](int n) { return z_in[n]; });
             zc = z_in[id];
             pp = p[id];
         }
@@ -380,20 +440,16 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(Dims d, Chunking ch, Stenc
         p[id] = pn;
         const T zn = zc + at * pn;
         z_out[id] = zn;
-        if (epi >= 0) acc[0] += (double)rv * (double)zn;
+        if (REDUCE) acc[0] += (double)rv * (double)zn;
     }
-    if (epi >= 0) {
-        block_sum<1>(acc);
-        double tot[1];
-        if (grid_reduce<1>(acc, red.partials, red.counter, ch.grid, lb, tot) && threadIdx.x == 0)
-            epilogue(epi, tot, sc, nullptr);
-    }
+    if (REDUCE) block_partial<1>(acc, red.partials, lb);
 }
 
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_cheb_scale(Dims d, Chunking ch, const T* r, T* z,
-                                                       double alpha0, int epi, RedScratch red,
-                                                       SolveScalars* sc)
+__global__ __launch_bounds__(kBlock) void k_cheb_scale(Dims d, Chunking ch,
+                                                       const T* __restrict__ r,
+                                                       T* __restrict__ z, double alpha0,
+                                                       RedScratch red, const SolveScalars* sc)
 {
     if (is_done(sc)) return;
     const int lb = logical_block(ch.grid);
@@ -404,23 +460,20 @@ __global__ __launch_bounds__(kBlock) void k_cheb_scale(Dims d, Chunking ch, cons
         const T rv = r[id];
         const T zn = T(0) + a0 * rv;
         z[id] = zn;
-        if (epi >= 0) acc[0] += (double)rv * (double)zn;
+        acc[0] += (double)rv * (double)zn;
     }
-    if (epi >= 0) {
-        block_sum<1>(acc);
-        double tot[1];
-        if (grid_reduce<1>(acc, red.partials, red.counter, ch.grid, lb, tot) && threadIdx.x == 0)
-            epilogue(epi, tot, sc, nullptr);
-    }
+    if (red.partials) block_partial<1>(acc, red.partials, lb);
 }
 
 // ------------------------------------------------------------------
 // mixed precision: FP32 iteration with FP64 reliable residual replacement
 // ------------------------------------------------------------------
-__global__ __launch_bounds__(kBlock) void k_mixed_fold(Dims d, Chunking ch, double* x64,
-                                                       float* x32, int force, SolveScalars* sc)
+__global__ __launch_bounds__(kBlock) void k_mixed_fold(Dims d, Chunking ch,
+                                                       double* __restrict__ x64,
+                                                       float* __restrict__ x32, int gate,
+                                                       const SolveScalars* sc)
 {
-    if (!force && (sc->done || !sc->reliable_now)) return;
+    if (!gate_open(gate, sc)) return;
     const int lb = logical_block(ch.grid);
     FOR_CELLS(d, ch, lb)
     {
@@ -429,39 +482,35 @@ __global__ __launch_bounds__(kBlock) void k_mixed_fold(Dims d, Chunking ch, doub
     }
 }
 
+template <int CODED>
 __global__ __launch_bounds__(kBlock) void k_mixed_residual(Dims d, Chunking ch, Stencil<double> A,
-                                                           const double* b, const double* x,
-                                                           float* r32, int init, int force,
-                                                           RedScratch red, SolveScalars* sc,
-                                                           double* out)
+                                                           const double* __restrict__ b,
+                                                           const double* __restrict__ x,
+                                                           float* __restrict__ r32, int gate,
+                                                           RedScratch red, const SolveScalars* sc)
 {
-    if (!force && !init && (sc->done || !sc->reliable_now)) return;
+    if (!gate_open(gate, sc)) return;
     const int lb = logical_block(ch.grid);
     double acc[2] = {0.0, 0.0};
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
-        const double q = apply7(A, d, id, i, j, k, [&](int n) { return x[n]; });
+        const double q = apply7<CODED>(A, d, id, i, j, k, [&](int n) { return x[n]; });
         const double bv = b[id];
         const double rv = bv - q;
         r32[id] = (float)rv;
         acc[0] += rv * rv;
         acc[1] += bv * bv;
     }
-    block_sum<2>(acc);
-    double tot[2];
-    if (grid_reduce<2>(acc, red.partials, red.counter, ch.grid, lb, tot) && threadIdx.x == 0) {
-        if (init) epilogue(EPI_INIT_MIXED, tot, sc, nullptr);
-        else if (force) out[0] = sqrt(tot[0]);
-        else epilogue(EPI_RELIABLE, tot, sc, nullptr);
-    }
+    block_partial<2>(acc, red.partials, lb);
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_update_mixed(Dims d, Chunking ch, float* x,
-                                                             float* r, const float* p,
-                                                             const float* q, RedScratch red,
-                                                             SolveScalars* sc)
+__global__ __launch_bounds__(kBlock) void k_pcg_update_mixed(Dims d, Chunking ch,
+                                                             float* __restrict__ x,
+                                                             float* __restrict__ r,
+                                                             const float* __restrict__ p,
+                                                             const float* __restrict__ q,
+                                                             RedScratch red,
+                                                             const SolveScalars* sc)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
@@ -475,10 +524,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_mixed(Dims d, Chunking ch
         r[id] = rv;
         acc[0] += (double)rv * (double)rv;
     }
-    block_sum<1>(acc);
-    double tot[1];
-    if (grid_reduce<1>(acc, red.partials, red.counter, ch.grid, lb, tot) && threadIdx.x == 0)
-        epilogue(EPI_CONV_MIXED, tot, sc, nullptr);
+    block_partial<1>(acc, red.partials, lb);
 }
 
 // ------------------------------------------------------------------
@@ -576,8 +622,6 @@ __global__ __launch_bounds__(kBlock) void k_matvec_sym(Dims d, Chunking ch, Sten
     const int lb = logical_block(ch.grid);
     FOR_CELLS(d, ch, lb)
     {
-        int i, j, k;
-        decode(d, id, i, j, k);
         double val = A.a[SC][id] * x[id];
         auto add = [&](const double* out, const double* back, int nb) {
             const double pr = out[id] * back[nb];
@@ -611,90 +655,140 @@ __global__ __launch_bounds__(kBlock) void k_scale_by_invnorm(int n, const double
 
 int level_grid(const Dims& d) { return (d.nx * d.ny + kBlock - 1) / kBlock; }
 
+int flat_grid(int n)
+{
+    int g = (n + kBlock - 1) / kBlock;
+    if (g > 8192) g = 8192;
+    return g < 1 ? 1 : g;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------
+hipError_t launch_finalize(hipStream_t s, int grid, int nr, bool max, int epi, int gate,
+                           RedScratch red, SolveScalars* sc, double* out)
+{
+#define LES_FIN(NR, MX)                                                                         \
+    hipLaunchKernelGGL((k_finalize<NR, MX>), dim3(1), dim3(kBlock), 0, s, grid, red.partials,  \
+                       epi, gate, sc, out)
+    if (max) {
+        if (nr == 1) LES_FIN(1, true);
+        else LES_FIN(2, true);
+    } else {
+        if (nr == 1) LES_FIN(1, false);
+        else LES_FIN(2, false);
+    }
+#undef LES_FIN
+    return hipGetLastError();
+}
+
 hipError_t launch_build_stencil(hipStream_t s, const Dims& d, const uint8_t* solid,
                                 double* const A[7], double idx2, double idy2, double idz2)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_build_stencil, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, solid, A[SC],
                        A[SN], A[SS], A[SE], A[SW], A[ST], A[SB], idx2, idy2, idz2);
+    return hipGetLastError();
+}
+
+hipError_t launch_build_code(hipStream_t s, const Dims& d, const uint8_t* solid, uint16_t* code)
+{
+    const Chunking ch = make_chunking(d);
+    hipLaunchKernelGGL(k_build_code, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, solid, code);
     return hipGetLastError();
 }
 
 template <typename T>
 hipError_t launch_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T* x, T* y)
 {
-    const Chunking ch = make_chunking(d.N);
-    hipLaunchKernelGGL(k_matvec<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, x, y);
+    const Chunking ch = make_chunking(d);
+    if (A.code)
+        hipLaunchKernelGGL((k_matvec<T, 1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, x, y);
+    else
+        hipLaunchKernelGGL((k_matvec<T, 0>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, x, y);
     return hipGetLastError();
 }
 
 template <typename TO, typename TI>
 hipError_t launch_convert(hipStream_t s, int n, const TI* in, TO* out)
 {
-    int g = (n + kBlock - 1) / kBlock;
-    if (g > 8192) g = 8192;
-    if (g < 1) g = 1;
-    hipLaunchKernelGGL((k_convert<TO, TI>), dim3(g), dim3(kBlock), 0, s, n, in, out);
+    hipLaunchKernelGGL((k_convert<TO, TI>), dim3(flat_grid(n)), dim3(kBlock), 0, s, n, in, out);
     return hipGetLastError();
 }
 
 hipError_t launch_dot(hipStream_t s, const Dims& d, const double* a, const double* b,
                       RedScratch red, double* out)
 {
-    const Chunking ch = make_chunking(d.N);
-    hipLaunchKernelGGL(k_dot, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, a, b, red, out);
-    return hipGetLastError();
+    const Chunking ch = make_chunking(d);
+    hipLaunchKernelGGL(k_dot, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, a, b, red);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    return launch_finalize(s, ch.grid, 1, false, EPI_STORE, GATE_NONE, red, nullptr, out);
 }
 
 template <typename T>
 hipError_t launch_pcg_init(hipStream_t s, const Dims& d, Stencil<T> A, const T* b, const T* x,
                            T* r, RedScratch red, SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d.N);
-    hipLaunchKernelGGL(k_pcg_init<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, b, x, r, red,
-                       sc);
-    return hipGetLastError();
+    const Chunking ch = make_chunking(d);
+    if (A.code)
+        hipLaunchKernelGGL((k_pcg_init<T, 1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, b, x,
+                           r, red);
+    else
+        hipLaunchKernelGGL((k_pcg_init<T, 0>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, b, x,
+                           r, red);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    return launch_finalize(s, ch.grid, 2, false, EPI_INIT, GATE_NONE, red, sc, nullptr);
 }
 
 template <typename T>
 hipError_t launch_pcg_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T* p, T* q,
                              RedScratch red, SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d.N);
-    hipLaunchKernelGGL(k_pcg_matvec<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, p, q, red,
-                       sc);
-    return hipGetLastError();
+    const Chunking ch = make_chunking(d);
+    if (A.code)
+        hipLaunchKernelGGL((k_pcg_matvec<T, 1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, p,
+                           q, red, sc);
+    else
+        hipLaunchKernelGGL((k_pcg_matvec<T, 0>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, p,
+                           q, red, sc);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    return launch_finalize(s, ch.grid, 1, false, EPI_ALPHA, GATE_DONE, red, sc, nullptr);
 }
 
 template <typename T, typename TX>
 hipError_t launch_pcg_update(hipStream_t s, const Dims& d, TX* x, T* r, const T* p, const T* q,
                              const T* jacobi_diag, RedScratch red, SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL((k_pcg_update<T, TX>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, r, p,
                        q, jacobi_diag, red, sc);
-    return hipGetLastError();
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    return launch_finalize(s, ch.grid, 2, false, jacobi_diag ? EPI_CONV_RZ : EPI_CONV, GATE_DONE,
+                           red, sc, nullptr);
 }
 
 template <typename T>
 hipError_t launch_pcg_p_jacobi(hipStream_t s, const Dims& d, const T* r, const T* diag, T* p,
                                int init, int epi, RedScratch red, SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
+    RedScratch rd = red;
+    if (!(init && epi >= 0)) rd.partials = nullptr;
     hipLaunchKernelGGL(k_pcg_p_jacobi<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r, diag, p,
-                       init, epi, red, sc);
-    return hipGetLastError();
+                       init, rd, sc);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    if (init && epi >= 0)
+        return launch_finalize(s, ch.grid, 1, false, epi, sc ? GATE_DONE : GATE_NONE, red, sc,
+                               nullptr);
+    return hipSuccess;
 }
 
 template <typename T>
 hipError_t launch_pcg_p_update(hipStream_t s, const Dims& d, const T* z, T* p, SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_pcg_p_update<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, z, p, sc);
     return hipGetLastError();
 }
@@ -703,10 +797,10 @@ template <typename T>
 hipError_t launch_pcg_rz(hipStream_t s, const Dims& d, const T* r, const T* z, int epilogue,
                          RedScratch red, SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d.N);
-    hipLaunchKernelGGL(k_pcg_rz<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r, z, epilogue, red,
-                       sc);
-    return hipGetLastError();
+    const Chunking ch = make_chunking(d);
+    hipLaunchKernelGGL(k_pcg_rz<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r, z, red, sc);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    return launch_finalize(s, ch.grid, 1, false, epilogue, GATE_DONE, red, sc, nullptr);
 }
 
 template <typename T>
@@ -714,20 +808,45 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
                             const T* z_in, T* z_out, T* p, double alpha0, double beta,
                             double alpha, int first, int epi, RedScratch red, SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d.N);
-    hipLaunchKernelGGL(k_cheb_step<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, r, z_in,
-                       z_out, p, alpha0, beta, alpha, first, epi, red, sc);
-    return hipGetLastError();
+    const Chunking ch = make_chunking(d);
+#define LES_CHEB(F, CD, RD)                                                                     \
+    hipLaunchKernelGGL((k_cheb_step<T, F, CD, RD>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, \
+                       r, z_in, z_out, p, alpha0, beta, alpha, red, sc)
+#define LES_CHEB_R(F, CD)                   \
+    do {                                    \
+        if (epi >= 0) LES_CHEB(F, CD, true);  \
+        else LES_CHEB(F, CD, false);          \
+    } while (0)
+    if (first) {
+        if (A.code) LES_CHEB_R(true, 1);
+        else LES_CHEB_R(true, 0);
+    } else {
+        if (A.code) LES_CHEB_R(false, 1);
+        else LES_CHEB_R(false, 0);
+    }
+#undef LES_CHEB_R
+#undef LES_CHEB
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    if (epi >= 0)
+        return launch_finalize(s, ch.grid, 1, false, epi, sc ? GATE_DONE : GATE_NONE, red, sc,
+                               nullptr);
+    return hipSuccess;
 }
 
 template <typename T>
 hipError_t launch_cheb_scale(hipStream_t s, const Dims& d, const T* r, T* z, double alpha0,
                              int epi, RedScratch red, SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
+    RedScratch rd = red;
+    if (epi < 0) rd.partials = nullptr;
     hipLaunchKernelGGL(k_cheb_scale<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r, z, alpha0,
-                       epi, red, sc);
-    return hipGetLastError();
+                       rd, sc);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    if (epi >= 0)
+        return launch_finalize(s, ch.grid, 1, false, epi, sc ? GATE_DONE : GATE_NONE, red, sc,
+                               nullptr);
+    return hipSuccess;
 }
 
 hipError_t launch_ic_factor_level(hipStream_t s, const Dims& d, int level, Stencil<double> A,
@@ -760,8 +879,9 @@ hipError_t launch_ic_bwd_level(hipStream_t s, const Dims& d, int level, T* z, co
 hipError_t launch_mixed_fold(hipStream_t s, const Dims& d, double* x64, float* x32, int force,
                              SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d.N);
-    hipLaunchKernelGGL(k_mixed_fold, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x64, x32, force, sc);
+    const Chunking ch = make_chunking(d);
+    hipLaunchKernelGGL(k_mixed_fold, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x64, x32,
+                       force ? (int)GATE_NONE : (int)GATE_RELIABLE, sc);
     return hipGetLastError();
 }
 
@@ -769,25 +889,33 @@ hipError_t launch_mixed_residual(hipStream_t s, const Dims& d, Stencil<double> A
                                  const double* x64, float* r32, int init, int force,
                                  RedScratch red, SolveScalars* sc, double* out)
 {
-    const Chunking ch = make_chunking(d.N);
-    hipLaunchKernelGGL(k_mixed_residual, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, b, x64, r32,
-                       init, force, red, sc, out);
-    return hipGetLastError();
+    const Chunking ch = make_chunking(d);
+    const int gate = (init || force) ? (int)GATE_NONE : (int)GATE_RELIABLE;
+    if (A.code)
+        hipLaunchKernelGGL((k_mixed_residual<1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, b,
+                           x64, r32, gate, red, sc);
+    else
+        hipLaunchKernelGGL((k_mixed_residual<0>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, b,
+                           x64, r32, gate, red, sc);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    const int epi = init ? EPI_INIT_MIXED : (force ? EPI_STORE_SQRT : EPI_RELIABLE);
+    return launch_finalize(s, ch.grid, 2, false, epi, gate, red, sc, out);
 }
 
 hipError_t launch_pcg_update_mixed(hipStream_t s, const Dims& d, float* x, float* r, const float* p,
                                    const float* q, RedScratch red, SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d.N);
-    hipLaunchKernelGGL(k_pcg_update_mixed, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, r, p, q, red,
-                       sc);
-    return hipGetLastError();
+    const Chunking ch = make_chunking(d);
+    hipLaunchKernelGGL(k_pcg_update_mixed, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, r, p, q,
+                       red, sc);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    return launch_finalize(s, ch.grid, 1, false, EPI_CONV_MIXED, GATE_DONE, red, sc, nullptr);
 }
 
 hipError_t launch_matvec_sym(hipStream_t s, const Dims& d, Stencil<double> A, const double* x,
                              double* y)
 {
-    const Chunking ch = make_chunking(d.N);
+    const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_matvec_sym, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, x, y);
     return hipGetLastError();
 }
@@ -795,20 +923,14 @@ hipError_t launch_matvec_sym(hipStream_t s, const Dims& d, Stencil<double> A, co
 hipError_t launch_lanczos_axpy(hipStream_t s, int n, double* w, const double* v, const double* vp,
                                double a, double b)
 {
-    int g = (n + kBlock - 1) / kBlock;
-    if (g > 8192) g = 8192;
-    if (g < 1) g = 1;
-    hipLaunchKernelGGL(k_lanczos_axpy, dim3(g), dim3(kBlock), 0, s, n, w, v, vp, a, b);
+    hipLaunchKernelGGL(k_lanczos_axpy, dim3(flat_grid(n)), dim3(kBlock), 0, s, n, w, v, vp, a, b);
     return hipGetLastError();
 }
 
 hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double* v,
                                    const double* nrm2)
 {
-    int g = (n + kBlock - 1) / kBlock;
-    if (g > 8192) g = 8192;
-    if (g < 1) g = 1;
-    hipLaunchKernelGGL(k_scale_by_invnorm, dim3(g), dim3(kBlock), 0, s, n, w, v, nrm2);
+    hipLaunchKernelGGL(k_scale_by_invnorm, dim3(flat_grid(n)), dim3(kBlock), 0, s, n, w, v, nrm2);
     return hipGetLastError();
 }
 
@@ -848,7 +970,7 @@ template hipError_t launch_pcg_update<float, float>(hipStream_t, const Dims&, fl
                                                     const float*, const float*, const float*,
                                                     RedScratch, SolveScalars*);
 template hipError_t launch_convert<float, double>(hipStream_t, int, const double*, float*);
-template hipError_t launch_convert<double, double>(hipStream_t, int, const double*, double*);
 template hipError_t launch_convert<double, float>(hipStream_t, int, const float*, double*);
+template hipError_t launch_convert<double, double>(hipStream_t, int, const double*, double*);
 
 }  // namespace les

@@ -23,6 +23,7 @@ LES_PTR_HOST, LES_PTR_DEVICE = 0, 1
 FIELDS = ("u", "v", "w", "u_new", "v_new", "w_new", "p", "p_rhs", "nu_t", "b")
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 STENCIL_A, STENCIL_L = 0, 1
+STENCIL_AUTO, STENCIL_ARRAYS, STENCIL_CODED = 0, 1, 2
 PC_NONE, PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
 SOLVER_PCG, SOLVER_SOR_LEX, SOLVER_SOR_RB, SOLVER_JACOBI = 0, 1, 2, 3
@@ -35,7 +36,8 @@ class PcgParams(C.Structure):
     _fields_ = [("precond", C.c_int), ("max_iter", C.c_int), ("tol", C.c_double),
                 ("tol_is_relative", C.c_int), ("lambda_min", C.c_double),
                 ("lambda_max", C.c_double), ("degree", C.c_int), ("precision", C.c_int),
-                ("reliable_every", C.c_int), ("check_every", C.c_int)]
+                ("reliable_every", C.c_int), ("check_every", C.c_int),
+                ("stencil_format", C.c_int)]
 
 
 class PcgResult(C.Structure):
@@ -263,9 +265,10 @@ class LesGpu:
     # ---- PCG ----
     @staticmethod
     def pcg_params(precond=PC_LEGACY_IC_A, max_iter=1000, tol=1e-6, tol_rel=False, lmin=0.0,
-                   lmax=0.0, degree=5, precision=PREC_FP64, reliable_every=50, check_every=0):
+                   lmax=0.0, degree=5, precision=PREC_FP64, reliable_every=50, check_every=0,
+                   stencil_format=STENCIL_AUTO):
         return PcgParams(precond, max_iter, tol, 1 if tol_rel else 0, lmin, lmax, degree,
-                         precision, reliable_every, check_every)
+                         precision, reliable_every, check_every, stencil_format)
 
     def pcg(self, **kw):
         prm = self.pcg_params(**kw)

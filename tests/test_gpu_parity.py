@@ -338,3 +338,40 @@ def test_lanczos_closed_form():
     assert hi <= lam_max * (1 + 1e-9)
     assert hi > 0.97 * lam_max
     assert lam_min_fluid * (1 - 1e-9) <= lo < hi
+
+
+@pytest.mark.parametrize("precision", [lesgpu.PREC_FP64, lesgpu.PREC_MIXED])
+@pytest.mark.parametrize("pc", [PC_NONE, PC_JACOBI, PC_CHEBYSHEV])
+@pytest.mark.parametrize("case", ["box", "random"])
+def test_coded_stencil_identical(orc, pc, precision, case):
+    """The 16-bit coded operator reproduces the 7 coefficient arrays bit for
+    bit, so whole solves and steps are identical with either storage."""
+    if case == "box":
+        m, solid, f = box_case(20, 4, inlet_solids=True)
+    else:
+        m, solid, f = random_case(17, 12, 23, 5, frac=0.1)
+    N = solid.size
+    A = orc.build_stencil(solid, m)
+    lmax = orc.lambda_max(A, m, 20)
+    kw = dict(precond=pc, lmin=3 * np.pi ** 2, lmax=lmax, degree=5, precision=precision,
+              reliable_every=7)
+    st0 = initial_state(f[0], f[1], f[2], N, solid, m, orc)
+    outs = []
+    for fmt in (lesgpu.STENCIL_ARRAYS, lesgpu.STENCIL_CODED):
+        with ctx_for(m, solid) as g:
+            g.build_stencil()
+            b = orc.build_b(f[3], solid, m)
+            g.set_field("b", b)
+            g.set_field("p", f[4])
+            r1 = g.pcg(max_iter=15, tol=0.0, stencil_format=fmt, **kw)
+            x = g.get_field("p")
+            tr = g.pcg_trace(16)
+            g.set_state(st0)
+            r2 = [g.step(step=19 + s, max_iter=40, tol=1e-9, tol_rel=True, stencil_format=fmt,
+                         **kw) for s in range(2)]
+            outs.append((r1, x, tr, r2, g.get_state()))
+    (a1, ax, atr, a2, ast), (b1, bx, btr, b2, bst) = outs
+    assert a1 == b1 and a2 == b2
+    assert bits_equal(ax, bx) and bits_equal(atr, btr)
+    for k in ast:
+        assert bits_equal(ast[k], bst[k]), k
