@@ -364,9 +364,28 @@ static void orc_precond(int pc, double* z, const double* r, double* const A[7],
     }
 }
 
+static double wdot(const double* a, const double* b, const double* W, long n)
+{
+    if (!W) return orc_dot(a, b, n);
+    double sum = 0.0;
+    for (long i = 0; i < n; ++i) sum += W[i] * a[i] * b[i];
+    return sum;
+}
+
 int orc_pcg(double* const A[7], const double* b, double* x, int nx, int ny, int nz,
             int precond, double* const L[7], double lmin, double lmax, int degree,
             int max_iter, double tol, double* res_trace)
+{
+    return orc_pcg_w(A, b, x, nx, ny, nz, precond, L, lmin, lmax, degree, max_iter, tol,
+                     res_trace, NULL);
+}
+
+/* PCG (pressure.cpp:342-414) with the inner products r.z and p.q taken in
+ * the W-weighted inner product when W != NULL (no reference counterpart;
+ * the stopping test stays the Euclidean ||r||). */
+int orc_pcg_w(double* const A[7], const double* b, double* x, int nx, int ny, int nz,
+              int precond, double* const L[7], double lmin, double lmax, int degree,
+              int max_iter, double tol, double* res_trace, const double* W)
 {
     const long n = (long)nx * ny * nz;
     double* r = (double*)calloc(n, sizeof(double));
@@ -380,17 +399,17 @@ int orc_pcg(double* const A[7], const double* b, double* x, int nx, int ny, int 
     if (res_trace) res_trace[0] = err0;
     if (err0 < tol) { ret = 0; goto out; }
     orc_precond(precond, p, r, A, L, nx, ny, nz, lmin, lmax, degree);
-    double rho = orc_dot(r, p, n);
+    double rho = wdot(r, p, W, n);
     for (int k = 1; k <= max_iter; ++k) {
         orc_matvec(q, p, A, nx, ny, nz);
-        double alpha = rho / orc_dot(p, q, n);
+        double alpha = rho / wdot(p, q, W, n);
         orc_axpy(x, alpha, p, n);
         orc_axpy(r, -alpha, q, n);
         double err = orc_norm(r, n);
         if (res_trace) res_trace[k] = err;
         if (err < tol) { ret = k; goto out; }
         orc_precond(precond, z, r, A, L, nx, ny, nz, lmin, lmax, degree);
-        double rho_new = orc_dot(r, z, n);
+        double rho_new = wdot(r, z, W, n);
         double beta = rho_new / rho;
         orc_saxpy(p, 1.0, z, beta, p, n);
         rho = rho_new;
@@ -790,8 +809,8 @@ void orc_step(orc_fields* f, const orc_mesh* m, const orc_step_params* prm, orc_
     orc_build_b_vector(b, f->p_rhs, f->solid, m);
     double bn = orc_norm(b, n);
     double tol = prm->tol_rel ? prm->tol * bn : prm->tol;
-    int its = orc_pcg(prm->A, b, f->p, m->nx, m->ny, m->nz, prm->precond, prm->L, prm->lmin,
-                      prm->lmax, prm->degree, prm->max_iter, tol, NULL);
+    int its = orc_pcg_w(prm->A, b, f->p, m->nx, m->ny, m->nz, prm->precond, prm->L, prm->lmin,
+                        prm->lmax, prm->degree, prm->max_iter, tol, NULL, prm->W);
     /* final true residual, for reporting only */
     double* q = (double*)calloc(n, sizeof(double));
     orc_matvec(q, f->p, prm->A, m->nx, m->ny, m->nz);
@@ -803,4 +822,405 @@ void orc_step(orc_fields* f, const orc_mesh* m, const orc_step_params* prm, orc_
     orc_correct(f->u, f->v, f->w, f->p, m, dt, prm->rho);
     orc_velocity_bcs(f->u, f->v, f->w, f->solid, m);
     if (st) { st->dt = dt; st->iters = its; st->bnorm = bn; st->res = res; }
+}
+
+/* ------------------------------------------------------------------ */
+/* differentiation: the transpose of each forward piece                */
+/* (no reference counterpart; pinned by central finite differences of  */
+/* orc_rollout in tests/test_oracle_adjoint.py)                         */
+/* ------------------------------------------------------------------ */
+static int interior(int i, int j, int k, int nx, int ny, int nz)
+{
+    return i >= 1 && i <= nx - 2 && j >= 1 && j <= ny - 2 && k >= 1 && k <= nz - 2;
+}
+
+/* (A^T)_w[m] = A_e[m-sx], (A^T)_e[m] = A_w[m+sx], ... (bounds guarded) */
+void orc_transpose_stencil(double* const AT[7], double* const A[7], int nx, int ny, int nz)
+{
+    const int sij = ny * nz;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                AT[ORC_C][id] = A[ORC_C][id];
+                AT[ORC_W][id] = i > 0 ? A[ORC_E][id - sij] : 0.0;
+                AT[ORC_E][id] = i < nx - 1 ? A[ORC_W][id + sij] : 0.0;
+                AT[ORC_S][id] = j > 0 ? A[ORC_N][id - nz] : 0.0;
+                AT[ORC_N][id] = j < ny - 1 ? A[ORC_S][id + nz] : 0.0;
+                AT[ORC_B][id] = k > 0 ? A[ORC_T][id - 1] : 0.0;
+                AT[ORC_T][id] = k < nz - 1 ? A[ORC_B][id + 1] : 0.0;
+            }
+}
+
+/* W: 1/2 per direction in which a fluid row of build_stencil's operator is
+ * mirrored (pressure.cpp:662-696: one coefficient doubled, the opposite one
+ * 0), 1 otherwise.  On grids without obstacles W A is symmetric, so CG in
+ * the W inner product is exact CG for A. */
+void orc_mirror_weights(double* W, double* const A[7], const orc_mesh* m)
+{
+    const long n = (long)m->nx * m->ny * m->nz;
+    const double ib[3] = {m->dx * m->dx, m->dy * m->dy, m->dz * m->dz};
+    const int pa[3] = {ORC_E, ORC_N, ORC_T}, pb[3] = {ORC_W, ORC_S, ORC_B};
+    for (long id = 0; id < n; ++id) {
+        double wt = 1.0;
+        for (int d = 0; d < 3; ++d) {
+            const double ma = nearbyint(-A[pa[d]][id] * ib[d]);
+            const double mb = nearbyint(-A[pb[d]][id] * ib[d]);
+            if ((ma == 2.0 && mb == 0.0) || (mb == 2.0 && ma == 0.0)) wt *= 0.5;
+        }
+        W[id] = wt;
+    }
+}
+
+void orc_rollout(double* u, double* v, double* w, double* p, const uint8_t* solid,
+                 const orc_mesh* m, double* const A[7], const orc_adj_params* prm, double* traj)
+{
+    const long n = (long)m->nx * m->ny * m->nz;
+    double* un = (double*)calloc(n, sizeof(double));
+    double* vn = (double*)calloc(n, sizeof(double));
+    double* wn = (double*)calloc(n, sizeof(double));
+    double* pr = (double*)calloc(n, sizeof(double));
+    double* nt = (double*)calloc(n, sizeof(double));
+    orc_fields f = {u, v, w, un, vn, wn, p, pr, nt, (uint8_t*)solid};
+    orc_step_params sp = {prm->nu, prm->rho, prm->Cs, 0, prm->dt, prm->precond, prm->degree,
+                          prm->max_iter, prm->lmin, prm->lmax, prm->tol_rel, 1, NULL, A, NULL};
+    double* W = NULL;
+    if (prm->weighted) {
+        W = (double*)malloc(n * sizeof(double));
+        orc_mirror_weights(W, A, m);
+        sp.W = W;
+    }
+    for (int s = 0; s < prm->nsteps; ++s) {
+        if (traj) {
+            memcpy(traj + (size_t)s * 3 * n, f.u, n * sizeof(double));
+            memcpy(traj + (size_t)s * 3 * n + n, f.v, n * sizeof(double));
+            memcpy(traj + (size_t)s * 3 * n + 2 * n, f.w, n * sizeof(double));
+        }
+        sp.step = s;
+        /* each Poisson problem from zero: its exact solution does not depend
+         * on the start, and a warm start leaves ||r0|| >> ||b||, which puts a
+         * tight relative tolerance out of FP64 reach */
+        memset(f.p, 0, n * sizeof(double));
+        orc_step(&f, m, &sp, NULL);
+    }
+    /* orc_step swaps pointers: copy the result into the caller's arrays */
+    if (f.u != u) { memcpy(u, f.u, n * sizeof(double)); memcpy(v, f.v, n * sizeof(double));
+                    memcpy(w, f.w, n * sizeof(double)); }
+    /* free the three buffers that are not the caller's */
+    if (f.u == u) { free(f.u_new); free(f.v_new); free(f.w_new); }
+    else { free(f.u); free(f.v); free(f.w); }
+    free(pr);
+    free(nt);
+    free(W);
+}
+
+/* velocity BCs (main.cpp:323-410) as a map y -> BC(y); transpose in place */
+void orc_adj_velocity_bcs(double* bu, double* bv, double* bw, const uint8_t* solid,
+                          const orc_mesh* m)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const long n = (long)nx * sij;
+    double* in[3] = {bu, bv, bw};
+    double* out = (double*)calloc(n, sizeof(double));
+    for (int q = 0; q < 3; ++q) {
+        double* b = in[q];
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                for (int k = 0; k < nz; ++k) {
+                    const int id = IDX(i, j, k);
+                    const int wall = (j == 0 || j == ny - 1 || k == 0 || k == nz - 1);
+                    double g = 0.0;
+                    if (!wall) {
+                        if (i == 0) g = solid[id] ? b[id] : 0.0;   /* solid inlet keeps y */
+                        else if (i == nx - 1) g = 0.0;              /* overwritten by the copy */
+                        else {
+                            g = solid[id] ? 0.0 : b[id];
+                            if (i == nx - 2) g += b[id + sij];      /* outlet copies raw y */
+                        }
+                    }
+                    out[id] = g;
+                }
+        memcpy(b, out, n * sizeof(double));
+    }
+    free(out);
+}
+
+/* correct_velocity (main.cpp:671-714): pbar = (d y / d p)^T ybar */
+void orc_adj_correct(double* pbar, const double* yu, const double* yv, const double* yw,
+                     const orc_mesh* m, double dt, double rho)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const double dor = dt / rho;
+    const double i2x = 1.0 / (2.0 * m->dx), i2y = 1.0 / (2.0 * m->dy), i2z = 1.0 / (2.0 * m->dz);
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                double g = 0.0;
+                if (interior(i + 1, j, k, nx, ny, nz)) g += dor * i2x * yu[id + sij];
+                if (interior(i - 1, j, k, nx, ny, nz)) g -= dor * i2x * yu[id - sij];
+                if (interior(i, j + 1, k, nx, ny, nz)) g += dor * i2y * yv[id + nz];
+                if (interior(i, j - 1, k, nx, ny, nz)) g -= dor * i2y * yv[id - nz];
+                if (interior(i, j, k + 1, nx, ny, nz)) g += dor * i2z * yw[id + 1];
+                if (interior(i, j, k - 1, nx, ny, nz)) g -= dor * i2z * yw[id - 1];
+                pbar[id] = g;
+            }
+}
+
+/* calculate_divergence + build_b_vector (pressure.cpp:256-329, 566-599):
+ * (bu,bv,bw) += B^T bbar */
+void orc_adj_divergence(double* bu, double* bv, double* bw, const double* bbar,
+                        const uint8_t* solid, const orc_mesh* m, double dt, double rho)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const long n = (long)nx * sij;
+    const double scale = rho / dt;
+    const double ix = 1.0 / (2.0 * m->dx), iy = 1.0 / (2.0 * m->dy), iz = 1.0 / (2.0 * m->dz);
+    const double ix1 = 1.0 / m->dx, iy1 = 1.0 / m->dy, iz1 = 1.0 / m->dz;
+    double* beta = (double*)calloc(n, sizeof(double));
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                beta[id] = (solid[id] || i == nx - 1) ? 0.0 : -scale * bbar[id];
+            }
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                double gu = 0.0, gv = 0.0, gw = 0.0;
+                /* x: forward one-sided at i = 0, central elsewhere */
+                if (i == 0) gu -= ix1 * beta[id];
+                if (i >= 1) gu += (i - 1 == 0 ? ix1 : ix) * beta[id - sij];
+                if (i + 1 <= nx - 1) gu -= ix * beta[id + sij];
+                /* y: one-sided at j = 0 and j = ny-1 */
+                if (j == 0) gv -= iy1 * beta[id];
+                if (j == ny - 1) gv += iy1 * beta[id];
+                if (j >= 1) gv += (j - 1 == 0 ? iy1 : iy) * beta[id - nz];
+                if (j + 1 <= ny - 1) gv -= (j + 1 == ny - 1 ? iy1 : iy) * beta[id + nz];
+                /* z: one-sided at k = 0 and k = nz-1 */
+                if (k == 0) gw -= iz1 * beta[id];
+                if (k == nz - 1) gw += iz1 * beta[id];
+                if (k >= 1) gw += (k - 1 == 0 ? iz1 : iz) * beta[id - 1];
+                if (k + 1 <= nz - 1) gw -= (k + 1 == nz - 1 ? iz1 : iz) * beta[id + 1];
+                bu[id] += gu;
+                bv[id] += gv;
+                bw[id] += gw;
+            }
+    free(beta);
+}
+
+/* solve_momentum (main.cpp:563-659) linearised at (u,v,w,nu_t):
+ * (bu,bv,bw) += (dM/du)^T mbar, bnut = (dM/dnu_t)^T mbar */
+void orc_adj_momentum(double* bu, double* bv, double* bw, double* bnut, const double* mu,
+                      const double* mv, const double* mw, const double* u, const double* v,
+                      const double* w, const double* nu_t, const orc_mesh* m, double dt,
+                      double nu)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const double bx = 1.0 / (m->dx * m->dx), by = 1.0 / (m->dy * m->dy), bz = 1.0 / (m->dz * m->dz);
+    const double ax = 1.0 / (2.0 * m->dx), ay = 1.0 / (2.0 * m->dy), az = 1.0 / (2.0 * m->dz);
+    const double* mb[3] = {mu, mv, mw};
+    const double* fld[3] = {u, v, w};
+    double* out[3] = {bu, bv, bw};
+    const int str[3] = {sij, nz, 1};
+    const double a[3] = {ax, ay, az}, b[3] = {bx, by, bz};
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                const int ijk[3] = {i, j, k};
+                const int in_m = interior(i, j, k, nx, ny, nz);
+                /* gradient with respect to f through its own stencil */
+                for (int q = 0; q < 3; ++q) {
+                    double g = 0.0;
+                    if (in_m) g += mb[q][id] * (1.0 + dt * (nu + nu_t[id]) * (-2.0 * (bx + by + bz)));
+                    for (int dd = 0; dd < 3; ++dd) {
+                        int c;
+                        int cc[3] = {ijk[0], ijk[1], ijk[2]};
+                        /* c = m - s: m is its +s neighbour */
+                        cc[dd] = ijk[dd] - 1;
+                        if (interior(cc[0], cc[1], cc[2], nx, ny, nz)) {
+                            c = id - str[dd];
+                            g += mb[q][c] * dt * (-fld[dd][c] * a[dd] + (nu + nu_t[c]) * b[dd]);
+                        }
+                        cc[dd] = ijk[dd] + 1;
+                        if (interior(cc[0], cc[1], cc[2], nx, ny, nz)) {
+                            c = id + str[dd];
+                            g += mb[q][c] * dt * (fld[dd][c] * a[dd] + (nu + nu_t[c]) * b[dd]);
+                        }
+                    }
+                    out[q][id] += g;
+                }
+                if (!in_m) { bnut[id] = 0.0; continue; }
+                /* gradient with respect to the advecting velocity and nu_t */
+                double gnut = 0.0;
+                for (int dd = 0; dd < 3; ++dd) {
+                    double gv = 0.0;
+                    for (int q = 0; q < 3; ++q) {
+                        const double* f = fld[q];
+                        const double D = (f[id + str[dd]] - f[id - str[dd]]) * a[dd];
+                        gv += mb[q][id] * (-dt * D);
+                    }
+                    out[dd][id] += gv;
+                }
+                for (int q = 0; q < 3; ++q) {
+                    const double* f = fld[q];
+                    double lap = 0.0;
+                    for (int dd = 0; dd < 3; ++dd)
+                        lap += (f[id + str[dd]] - 2 * f[id] + f[id - str[dd]]) * b[dd];
+                    gnut += mb[q][id] * dt * lap;
+                }
+                bnut[id] = gnut;
+            }
+}
+
+/* apply_nut_BCs (main.cpp:483-547) as a map from the raw interior SGS
+ * values: braw = BC^T bnut */
+void orc_adj_nut_bcs(double* braw, const double* bnut, const uint8_t* solid, const orc_mesh* m)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                double g = 0.0;
+                if (interior(i, j, k, nx, ny, nz)) {
+                    g = solid[id] ? 0.0 : bnut[id];
+                    if (i == nx - 2) g += bnut[id + sij];
+                }
+                braw[id] = g;
+            }
+}
+
+/* compute_sgs_stress (main.cpp:423-461): (bu,bv,bw) += (d nu_t/d u)^T braw */
+void orc_adj_sgs(double* bu, double* bv, double* bw, const double* braw, const double* u,
+                 const double* v, const double* w, const orc_mesh* m, double Cs)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz, sij = ny * nz;
+    const long n = (long)nx * sij;
+    const double csd2 = orc_cs_delta_sq(m, Cs);
+    const double t2[3] = {2 * m->dx, 2 * m->dy, 2 * m->dz};
+    /* G[a][b] = dL/d(d_b f_a) at each interior cell, f = (u,v,w), d = (x,y,z) */
+    double* G = (double*)calloc(9 * n, sizeof(double));
+    const double* F[3] = {u, v, w};
+    const int str[3] = {sij, nz, 1};
+    for (int i = 1; i < nx - 1; ++i)
+        for (int j = 1; j < ny - 1; ++j)
+            for (int k = 1; k < nz - 1; ++k) {
+                const int id = IDX(i, j, k);
+                double D[3][3];
+                for (int a = 0; a < 3; ++a)
+                    for (int bb = 0; bb < 3; ++bb)
+                        D[a][bb] = (F[a][id + str[bb]] - F[a][id - str[bb]]) / t2[bb];
+                const double Sxx = D[0][0], Syy = D[1][1], Szz = D[2][2];
+                const double Sxy = 0.5 * (D[0][1] + D[1][0]);
+                const double Sxz = 0.5 * (D[0][2] + D[2][0]);
+                const double Syz = 0.5 * (D[1][2] + D[2][1]);
+                const double Smag = sqrt(2.0 * (Sxx * Sxx + Syy * Syy + Szz * Szz) +
+                                         4.0 * (Sxy * Sxy + Sxz * Sxz + Syz * Syz));
+                if (!(Smag > 0.0)) continue;
+                const double g = braw[id] * csd2 / Smag;
+                double* Gc = G + (size_t)9 * id;
+                Gc[0] = 2 * g * Sxx; Gc[4] = 2 * g * Syy; Gc[8] = 2 * g * Szz;
+                Gc[1] = Gc[3] = 2 * g * Sxy;   /* du/dy, dv/dx */
+                Gc[2] = Gc[6] = 2 * g * Sxz;   /* du/dz, dw/dx */
+                Gc[5] = Gc[7] = 2 * g * Syz;   /* dv/dz, dw/dy */
+            }
+    double* out[3] = {bu, bv, bw};
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                const int ijk[3] = {i, j, k};
+                for (int a = 0; a < 3; ++a) {
+                    double g = 0.0;
+                    for (int bb = 0; bb < 3; ++bb) {
+                        int cc[3] = {ijk[0], ijk[1], ijk[2]};
+                        cc[bb] = ijk[bb] - 1;   /* c = m - s: m is c's + neighbour */
+                        if (interior(cc[0], cc[1], cc[2], nx, ny, nz))
+                            g += G[(size_t)9 * (id - str[bb]) + 3 * a + bb] / t2[bb];
+                        cc[bb] = ijk[bb] + 1;
+                        if (interior(cc[0], cc[1], cc[2], nx, ny, nz))
+                            g -= G[(size_t)9 * (id + str[bb]) + 3 * a + bb] / t2[bb];
+                    }
+                    out[a][id] += g;
+                }
+            }
+    free(G);
+}
+
+void orc_adjoint(double* ubar, double* vbar, double* wbar, const double* traj,
+                 const uint8_t* solid, const orc_mesh* m, double* const A[7],
+                 const orc_adj_params* prm)
+{
+    const int nx = m->nx, ny = m->ny, nz = m->nz;
+    const long n = (long)nx * ny * nz;
+    double* AT[7];
+    for (int q = 0; q < 7; ++q) AT[q] = (double*)calloc(n, sizeof(double));
+    orc_transpose_stencil(AT, A, nx, ny, nz);
+    double* W = NULL;
+    if (prm->weighted) {
+        W = (double*)malloc(n * sizeof(double));
+        orc_mirror_weights(W, A, m);
+    }
+    double* nut = (double*)calloc(n, sizeof(double));
+    double* mu = (double*)calloc(n, sizeof(double));
+    double* mv = (double*)calloc(n, sizeof(double));
+    double* mw = (double*)calloc(n, sizeof(double));
+    double* pbar = (double*)calloc(n, sizeof(double));
+    double* bbar = (double*)calloc(n, sizeof(double));
+    double* bnut = (double*)calloc(n, sizeof(double));
+    double* braw = (double*)calloc(n, sizeof(double));
+    double* su = (double*)calloc(n, sizeof(double));
+    double* sv = (double*)calloc(n, sizeof(double));
+    double* sw = (double*)calloc(n, sizeof(double));
+    for (int s = prm->nsteps - 1; s >= 0; --s) {
+        const double* u = traj + (size_t)s * 3 * n;
+        const double* v = u + n;
+        const double* w = v + n;
+        /* recompute nu_t of the step (sgs + BCs) */
+        memset(nut, 0, n * sizeof(double));
+        orc_sgs(nut, NULL, u, v, w, m, prm->Cs);
+        orc_nut_bcs(nut, solid, m);
+        /* 7': y-bar from u^{n+1}-bar */
+        orc_adj_velocity_bcs(ubar, vbar, wbar, solid, m);
+        /* 6': u*-bar = y-bar; p-bar */
+        orc_adj_correct(pbar, ubar, vbar, wbar, m, prm->dt, prm->rho);
+        /* 5': b-bar = A^-T p-bar; with W A symmetric, A^-T = W A^-1 W^-1 */
+        memset(bbar, 0, n * sizeof(double));
+        if (W) {
+            for (long q = 0; q < n; ++q) pbar[q] /= W[q];
+            const double pn = orc_norm(pbar, n);
+            if (pn > 0.0)
+                orc_pcg_w(A, pbar, bbar, nx, ny, nz, prm->precond, NULL, prm->lmin, prm->lmax,
+                          prm->degree, prm->max_iter, prm->tol_rel * pn, NULL, W);
+            for (long q = 0; q < n; ++q) bbar[q] *= W[q];
+        } else {
+            const double pn = orc_norm(pbar, n);
+            if (pn > 0.0)
+                orc_pcg(AT, pbar, bbar, nx, ny, nz, prm->precond, NULL, prm->lmin, prm->lmax,
+                        prm->degree, prm->max_iter, prm->tol_rel * pn, NULL);
+        }
+        /* 3': m-bar = BC^T u*-bar */
+        memcpy(mu, ubar, n * sizeof(double));
+        memcpy(mv, vbar, n * sizeof(double));
+        memcpy(mw, wbar, n * sizeof(double));
+        orc_adj_velocity_bcs(mu, mv, mw, solid, m);
+        /* u^n-bar accumulates the momentum, divergence and SGS paths */
+        memset(su, 0, n * sizeof(double));
+        memset(sv, 0, n * sizeof(double));
+        memset(sw, 0, n * sizeof(double));
+        orc_adj_momentum(su, sv, sw, bnut, mu, mv, mw, u, v, w, nut, m, prm->dt, prm->nu);
+        orc_adj_divergence(su, sv, sw, bbar, solid, m, prm->dt, prm->rho);
+        orc_adj_nut_bcs(braw, bnut, solid, m);
+        orc_adj_sgs(su, sv, sw, braw, u, v, w, m, prm->Cs);
+        memcpy(ubar, su, n * sizeof(double));
+        memcpy(vbar, sv, n * sizeof(double));
+        memcpy(wbar, sw, n * sizeof(double));
+    }
+    for (int q = 0; q < 7; ++q) free(AT[q]);
+    free(nut); free(mu); free(mv); free(mw); free(pbar); free(bbar); free(bnut); free(braw);
+    free(su); free(sv); free(sw);
+    free(W);
 }

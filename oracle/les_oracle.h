@@ -71,6 +71,10 @@ double orc_estimate_lambda_max(double* const A[7], int nx, int ny, int nz, int n
 int orc_pcg(double* const A[7], const double* b, double* x, int nx, int ny, int nz,
             int precond, double* const L[7], double lmin, double lmax, int degree,
             int max_iter, double tol, double* res_trace);
+/* the same with r.z and p.q in the W-weighted inner product (W may be NULL) */
+int orc_pcg_w(double* const A[7], const double* b, double* x, int nx, int ny, int nz,
+              int precond, double* const L[7], double lmin, double lmax, int degree,
+              int max_iter, double tol, double* res_trace, const double* W);
 
 /* ---- stationary solvers (pressure.cpp:14-247) ---- */
 void orc_apply_pressure_bcs(double* p, const uint8_t* solid, const orc_mesh* m);
@@ -116,11 +120,58 @@ typedef struct {
     int tol_rel;
     double* const* L;    /* IC(0) factor for ORC_PC_IC0 */
     double* const* A;    /* the stencil */
+    const double* W;     /* NULL: the reference's PCG; else W-weighted inner products */
 } orc_step_params;
 
 typedef struct { double dt; int iters; double bnorm; double res; } orc_step_stats;
 
 void orc_step(orc_fields* f, const orc_mesh* m, const orc_step_params* prm, orc_step_stats* st);
+
+/* ---- differentiation (no reference counterpart) ----
+ * The step map F: u^n -> u^{n+1} of orc_step with a fixed dt, the pressure
+ * solved to a tight relative tolerance (treated as exact), and no solid
+ * cells on the inlet plane (their stale ping-pong values are not a function
+ * of u^n).  orc_rollout runs nsteps steps from (u,v,w) and records u^n for
+ * n = 0..nsteps-1 in traj (3*N doubles per step, may be NULL).
+ * Each step's pressure solve starts from p = 0 (same exact map as the warm
+ * start of main.cpp:137, but a tight relative tolerance stays reachable).
+ * orc_adjoint takes (ubar, vbar, wbar) = dL/du^{nsteps} and returns
+ * dL/du^0 in place, re-using traj. */
+typedef struct {
+    double nu, rho, Cs, dt;
+    int nsteps;
+    int precond;         /* ORC_PC_JACOBI or ORC_PC_CHEBYSHEV */
+    double lmin, lmax;
+    int degree, max_iter;
+    double tol_rel;      /* relative tolerance of the forward and adjoint solves */
+    int weighted;        /* CG in the mirror-weight inner product (obstacle-free grids) */
+} orc_adj_params;
+
+/* W = 1/2 per mirrored direction of each row; W A is symmetric on grids
+ * without obstacles. */
+void orc_mirror_weights(double* W, double* const A[7], const orc_mesh* m);
+
+void orc_transpose_stencil(double* const AT[7], double* const A[7], int nx, int ny, int nz);
+void orc_rollout(double* u, double* v, double* w, double* p, const uint8_t* solid,
+                 const orc_mesh* m, double* const A[7], const orc_adj_params* prm, double* traj);
+void orc_adjoint(double* ubar, double* vbar, double* wbar, const double* traj,
+                 const uint8_t* solid, const orc_mesh* m, double* const A[7],
+                 const orc_adj_params* prm);
+
+/* the pieces of orc_adjoint, each the transpose of one forward piece */
+void orc_adj_velocity_bcs(double* bu, double* bv, double* bw, const uint8_t* solid,
+                          const orc_mesh* m);
+void orc_adj_correct(double* pbar, const double* yu, const double* yv, const double* yw,
+                     const orc_mesh* m, double dt, double rho);
+void orc_adj_divergence(double* bu, double* bv, double* bw, const double* bbar,
+                        const uint8_t* solid, const orc_mesh* m, double dt, double rho);
+void orc_adj_momentum(double* bu, double* bv, double* bw, double* bnut, const double* mu,
+                      const double* mv, const double* mw, const double* u, const double* v,
+                      const double* w, const double* nu_t, const orc_mesh* m, double dt,
+                      double nu);
+void orc_adj_nut_bcs(double* braw, const double* bnut, const uint8_t* solid, const orc_mesh* m);
+void orc_adj_sgs(double* bu, double* bv, double* bw, const double* braw, const double* u,
+                 const double* v, const double* w, const orc_mesh* m, double Cs);
 
 #ifdef __cplusplus
 }

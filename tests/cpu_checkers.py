@@ -59,7 +59,7 @@ class OrcStepParams(C.Structure):
                 ("dt_fixed", C.c_double), ("precond", C.c_int), ("degree", C.c_int),
                 ("max_iter", C.c_int), ("lmin", C.c_double), ("lmax", C.c_double),
                 ("tol", C.c_double), ("tol_rel", C.c_int), ("L", C.POINTER(DP)),
-                ("A", C.POINTER(DP))]
+                ("A", C.POINTER(DP)), ("W", DP)]
 
 
 class OrcStepStats(C.Structure):
@@ -247,7 +247,7 @@ class Oracle:
         iters, dts = [], []
         for s in range(nsteps):
             prm = OrcStepParams(nu, rho, Cs, step0 + s, dt_fixed, precond, degree, max_iter, lmin,
-                                lmax, tol, 1 if tol_rel else 0, L7, A7)
+                                lmax, tol, 1 if tol_rel else 0, L7, A7, None)
             stats = OrcStepStats()
             self.lib.orc_step(C.byref(f), self._m(m), C.byref(prm), C.byref(stats))
             iters.append(stats.iters)
@@ -259,6 +259,49 @@ class Oracle:
             ptr = getattr(f, k)
             out[k] = np.ctypeslib.as_array(ptr, shape=(n,)).copy()
         return out, iters, dts
+
+
+class OrcAdjParams(C.Structure):
+    _fields_ = [("nu", C.c_double), ("rho", C.c_double), ("Cs", C.c_double), ("dt", C.c_double),
+                ("nsteps", C.c_int), ("precond", C.c_int), ("lmin", C.c_double),
+                ("lmax", C.c_double), ("degree", C.c_int), ("max_iter", C.c_int),
+                ("tol_rel", C.c_double), ("weighted", C.c_int)]
+
+
+def _oracle_adjoint_bindings(L):
+    L.orc_rollout.argtypes = [DP, DP, DP, DP, U8P, C.POINTER(OrcMesh), C.POINTER(DP),
+                              C.POINTER(OrcAdjParams), DP]
+    L.orc_adjoint.argtypes = [DP, DP, DP, DP, U8P, C.POINTER(OrcMesh), C.POINTER(DP),
+                              C.POINTER(OrcAdjParams)]
+
+
+def oracle_rollout(orc, u, v, w, solid, m, A, nsteps, dt=1e-3, nu=1e-3, rho=1.2, Cs=0.17,
+                   precond=PC_JACOBI, lmin=0.0, lmax=0.0, degree=5, max_iter=20000,
+                   tol_rel=1e-13, p=None, record=False, weighted=True):
+    """nsteps LES steps with a fixed dt (orc_rollout); returns the final
+    (u, v, w, p) and, with record, the trajectory u^n, n < nsteps."""
+    _oracle_adjoint_bindings(orc.lib)
+    N = u.size
+    u, v, w = u.copy(), v.copy(), w.copy()
+    p = np.zeros(N) if p is None else p.copy()
+    prm = OrcAdjParams(nu, rho, Cs, dt, nsteps, precond, lmin, lmax, degree, max_iter, tol_rel,
+                       1 if weighted else 0)
+    traj = np.zeros(3 * N * nsteps) if record else None
+    orc.lib.orc_rollout(_p(u), _p(v), _p(w), _p(p), _p(solid), orc._m(m), _p7(A), C.byref(prm),
+                        _p(traj) if record else None)
+    return u, v, w, p, traj
+
+
+def oracle_adjoint(orc, bu, bv, bw, traj, solid, m, A, nsteps, dt=1e-3, nu=1e-3, rho=1.2,
+                   Cs=0.17, precond=PC_JACOBI, lmin=0.0, lmax=0.0, degree=5, max_iter=20000,
+                   tol_rel=1e-13, weighted=True):
+    _oracle_adjoint_bindings(orc.lib)
+    bu, bv, bw = bu.copy(), bv.copy(), bw.copy()
+    prm = OrcAdjParams(nu, rho, Cs, dt, nsteps, precond, lmin, lmax, degree, max_iter, tol_rel,
+                       1 if weighted else 0)
+    orc.lib.orc_adjoint(_p(bu), _p(bv), _p(bw), _p(traj), _p(solid), orc._m(m), _p7(A),
+                        C.byref(prm))
+    return bu, bv, bw
 
 
 class Reference:
