@@ -134,7 +134,17 @@ typedef struct les_pcg_params {
     int check_every;      /* host polls convergence every this many iterations (0: auto) */
     int stencil_format;   /* 0 auto: coded when A came from les_build_stencil;
                              1 the 7 coefficient arrays; 2 coded (16 bits per cell) */
+    int inner_product;    /* LES_IP_EUCLID: r.z and p.q as the reference forms them;
+                             LES_IP_MIRROR: weighted by the mirror weights W (FP64;
+                             preconditioner NONE, JACOBI or CHEBYSHEV) */
 } les_pcg_params;
+
+/* Inner product of the CG scalars.  W = 1/2 per direction in which a row of
+ * build_stencil's operator is mirrored (pressure.cpp:662-696: one
+ * neighbour coefficient doubled, the opposite one 0).  W A is symmetric on
+ * masks without interior obstacles, so CG in the W inner product is exact
+ * CG there; the stopping test stays the Euclidean ||r||. */
+enum { LES_IP_EUCLID = 0, LES_IP_MIRROR = 1 };
 
 typedef struct les_pcg_result {
     int iters;            /* reference convention: 0 if converged at start, max_iter if not */
@@ -201,6 +211,43 @@ typedef struct les_step_stats {
 } les_step_stats;
 
 int les_step(les_ctx* ctx, const les_step_params* prm, les_step_stats* stats);
+
+/* ---- differentiation (no reference counterpart) ----
+ * The step map F: u^n -> u^{n+1} of les_step at a fixed dt with each
+ * pressure solve started from p = 0 and run to a relative tolerance (its
+ * solution treated as exact).  Inlet-plane solid cells are rejected: the
+ * step leaves their ping-pong values stale (main.cpp:340), which makes the
+ * map depend on more than u^n.
+ *
+ * les_rollout advances the context's u, v, w by nsteps such steps and keeps
+ * the velocity at steps 0, k, 2k, ... (k = checkpoint_every) on the device.
+ * les_adjoint takes dL/du^nsteps (ubar, vbar, wbar; overwritten with
+ * dL/du^0) and runs the transposed steps backwards, recomputing each
+ * segment of k steps from its checkpoint.  Every pressure solve (forward
+ * and adjoint) is CG in the mirror-weighted inner product
+ * (pcg.inner_product = LES_IP_MIRROR), and the adjoint one uses
+ * A^-T = W A^-1 W^-1.  That CG is exact CG only when A has no rows
+ * mirrored around interior obstacles, so A must come from les_build_stencil
+ * with ignore_mask = 1 (the shipped driver's order) or an obstacle-free
+ * mask; the mask itself may hold obstacles away from the inlet plane.  The
+ * context's fields are left as les_rollout left them. */
+typedef struct les_adj_params {
+    double nu, rho, Cs, dt;
+    int nsteps;
+    int checkpoint_every;   /* k >= 1 */
+    les_pcg_params pcg;     /* FP64; tol_is_relative must be 1 */
+} les_adj_params;
+
+typedef struct les_adj_stats {
+    int solves;             /* pressure solves run (forward, recompute and adjoint) */
+    int iters;              /* their summed iteration counts */
+    int unconverged;        /* solves that stopped at max_iter or on a fault */
+    double max_rel_res;     /* max over solves of ||r|| / ||b|| at exit */
+} les_adj_stats;
+
+int les_rollout(les_ctx* ctx, const les_adj_params* prm, les_adj_stats* stats);
+int les_adjoint(les_ctx* ctx, const les_adj_params* prm, double* ubar, double* vbar,
+                double* wbar, int ptr_kind, les_adj_stats* stats);
 
 /* ---- diagnostics (no reference counterpart) ----
  * Brackets every launch of one kernel class with HIP events on the

@@ -19,8 +19,9 @@
 #include <vector>
 
 #include "../../include/les_gpu.h"
-#include "les_physics.hpp"
+#include "les_adjoint.hpp"
 #include "les_mixed.hpp"
+#include "les_physics.hpp"
 
 using namespace les;
 
@@ -35,6 +36,7 @@ struct les_ctx {
     double* A[7] = {};
     bool have_A = false;
     bool A_from_mask = false;  // A was built by les_build_stencil from the mask
+    bool A_has_obstacles = false;  // ... and that mask had solid cells
     uint16_t* code = nullptr;  // the same operator as one 16-bit code per cell
     bool have_code = false;
     double* L[7] = {};
@@ -58,6 +60,18 @@ struct les_ctx {
     int trace_cap = 0;
     int last_trace_n = 0;
     hipEvent_t ev[2] = {};
+    // differentiation
+    double* W = nullptr;              // mirror weights (LES_IP_MIRROR)
+    bool have_W = false;
+    double* ckpt = nullptr;           // checkpoints, 3N doubles each
+    int ckpt_cap = 0;
+    double* seg = nullptr;            // states of the segment being recomputed
+    int seg_cap = 0;
+    double* adjb[16] = {};            // adjoint work arrays
+    double* shadow[LES_F_COUNT] = {}; // field set used while recomputing
+    bool have_roll = false;
+    int roll_nsteps = 0, roll_every = 0;
+    double roll_dt = 0, roll_nu = 0, roll_rho = 0, roll_Cs = 0;
     // kernel timing diagnostics
     int kt_class = 0;
     int kt_elem = 0;
@@ -253,7 +267,8 @@ int pick_coded(les_ctx* c, int fmt, bool& coded)
 // z = M^-1 r for FP64 preconditioners.  When epi >= 0 the final kernel also
 // reduces r.z into the solve scalars with that epilogue.
 int precond_fp64(les_ctx* c, int pc, const double* r, double* zout, double lmin, double lmax,
-                 int degree, int epi, SolveScalars* sc, const Stencil<double>& Aop)
+                 int degree, int epi, SolveScalars* sc, const Stencil<double>& Aop,
+                 const double* wgt = nullptr)
 {
     const Dims& d = c->d;
     hipStream_t s = c->stream;
@@ -262,7 +277,7 @@ int precond_fp64(les_ctx* c, int pc, const double* r, double* zout, double lmin,
         HIPCHK(c, hipMemcpyAsync(zout, r, nbytes(c, 8), hipMemcpyDeviceToDevice, s));
         break;
     case LES_PC_JACOBI:
-        HIPCHK(c, launch_pcg_p_jacobi<double>(s, d, r, c->A[SC], zout, 1, epi, c->red, sc));
+        HIPCHK(c, launch_pcg_p_jacobi<double>(s, d, r, c->A[SC], zout, 1, epi, c->red, sc, wgt));
         return LES_OK;
     case LES_PC_IC0:
     case LES_PC_LEGACY_IC_A: {
@@ -281,7 +296,7 @@ int precond_fp64(les_ctx* c, int pc, const double* r, double* zout, double lmin,
         cheb_coeffs(lmin, lmax, degree, a0, be, al);
         const Stencil<double>& A = Aop;
         if (degree <= 1) {
-            HIPCHK(c, launch_cheb_scale<double>(s, d, r, zout, a0, epi, c->red, sc));
+            HIPCHK(c, launch_cheb_scale<double>(s, d, r, zout, a0, epi, c->red, sc, wgt));
             return LES_OK;
         }
         double* buf[2] = {c->za, c->zb};
@@ -291,14 +306,15 @@ int precond_fp64(les_ctx* c, int pc, const double* r, double* zout, double lmin,
             double* zo = last ? zout : buf[(k + 1) & 1];
             KtScope kt(c, LES_KT_CHEB_STEP, 8);
             HIPCHK(c, launch_cheb_step<double>(s, d, A, r, zin, zo, c->pc, a0, be[k - 1],
-                                               al[k - 1], k == 1, last ? epi : -1, c->red, sc));
+                                               al[k - 1], k == 1, last ? epi : -1, c->red, sc,
+                                               wgt));
         }
         return LES_OK;
     }
     default:
         return fail(c, LES_ERR_ARG, "unknown preconditioner");
     }
-    if (epi >= 0) HIPCHK(c, launch_pcg_rz<double>(s, d, r, zout, epi, c->red, sc));
+    if (epi >= 0) HIPCHK(c, launch_pcg_rz<double>(s, d, r, zout, epi, c->red, sc, wgt));
     return LES_OK;
 }
 
@@ -375,6 +391,11 @@ int les_ctx_destroy(les_ctx* c)
         fr(c->L32[q]);
     }
     for (double* p : {c->r, c->z, c->p, c->q, c->za, c->zb, c->pc, c->t1, c->t2}) fr(p);
+    fr(c->W);
+    fr(c->ckpt);
+    fr(c->seg);
+    for (auto* p : c->adjb) fr(p);
+    for (auto* p : c->shadow) fr(p);
     free_mixed(c->mw);
     fr(c->red.partials);
     fr(c->sc);
@@ -438,6 +459,7 @@ int les_set_mask(les_ctx* c, const uint8_t* solid, int ptr_kind)
     if (!c || !solid) return fail(c, LES_ERR_ARG, "mask is NULL");
     HIPCHK(c, hipMemcpyAsync(c->solid, solid, nbytes(c, 1), kind_in(ptr_kind), c->stream));
     if (!ptr_kind) HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_roll = false;
     return LES_OK;
 }
 
@@ -476,6 +498,17 @@ int les_build_stencil(les_ctx* c, int ignore_mask)
     c->have_A32 = false;
     c->have_L = false;
     c->have_L32 = false;
+    c->have_W = false;
+    c->have_roll = false;
+    c->A_has_obstacles = false;
+    if (!ignore_mask) {
+        // whether A has mirror rows around interior obstacles (then no
+        // diagonal W symmetrises it)
+        std::vector<uint8_t> h((size_t)c->d.N);
+        HIPCHK(c, hipMemcpyAsync(h.data(), m, h.size(), hipMemcpyDeviceToHost, c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        c->A_has_obstacles = std::any_of(h.begin(), h.end(), [](uint8_t v) { return v != 0; });
+    }
     return LES_OK;
 }
 
@@ -496,6 +529,8 @@ int les_set_stencil(les_ctx* c, int which, double* const coef[7], int ptr_kind)
         c->have_code = false;
         c->A_from_mask = false;
         c->have_A32 = false;
+        c->have_W = false;
+        c->have_roll = false;
     } else {
         c->have_L = true;
         c->have_L32 = false;
@@ -693,7 +728,10 @@ int les_lanczos_bounds(les_ctx* c, int steps, uint64_t seed, double* ritz_min, d
 
 namespace {
 
-int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
+// op (may be nullptr: the context's A) is the operator solved with; wgt
+// (may be nullptr) weights the r.z and p.q inner products.
+int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res,
+             const Stencil<double>* op = nullptr, const double* wgt = nullptr)
 {
     const Dims& d = c->d;
     hipStream_t s = c->stream;
@@ -713,7 +751,7 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
 
     bool coded = false;
     if ((rc = pick_coded(c, prm->stencil_format, coded)) != LES_OK) return rc;
-    const Stencil<double> A = coded ? coded_stencil<double>(c) : stencil_of(c->A);
+    const Stencil<double> A = op ? *op : (coded ? coded_stencil<double>(c) : stencil_of(c->A));
     double* x = c->f[LES_F_P];
     const double* b = c->f[LES_F_B];
     double *r = c->r, *p = c->p, *q = c->q, *z = c->z;
@@ -723,9 +761,9 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     HIPCHK(c, launch_pcg_init<double>(s, d, A, b, x, r, c->red, sc));
     // p = M^-1 r, rho = r.p (pressure.cpp:363-369)
     if (pc == LES_PC_JACOBI)
-        HIPCHK(c, launch_pcg_p_jacobi<double>(s, d, r, c->A[SC], p, 1, EPI_RHO, c->red, sc));
+        HIPCHK(c, launch_pcg_p_jacobi<double>(s, d, r, c->A[SC], p, 1, EPI_RHO, c->red, sc, wgt));
     else if ((rc = precond_fp64(c, pc, r, p, prm->lambda_min, prm->lambda_max, prm->degree, EPI_RHO,
-                                sc, A)) != LES_OK)
+                                sc, A, wgt)) != LES_OK)
         return rc;
 
     int check = prm->check_every > 0 ? prm->check_every : (d.N >= (1 << 22) ? 4 : 16);
@@ -736,15 +774,16 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
         for (int it = 0; it < n; ++it) {
             {
                 KtScope kt(c, LES_KT_PCG_MATVEC, 8);
-                HIPCHK(c, launch_pcg_matvec<double>(s, d, A, p, q, c->red, sc));
+                HIPCHK(c, launch_pcg_matvec<double>(s, d, A, p, q, c->red, sc, wgt));
             }
             if (pc == LES_PC_JACOBI) {
-                HIPCHK(c, (launch_pcg_update<double, double>(s, d, x, r, p, q, c->A[SC], c->red, sc)));
+                HIPCHK(c, (launch_pcg_update<double, double>(s, d, x, r, p, q, c->A[SC], c->red, sc,
+                                                             wgt)));
                 HIPCHK(c, launch_pcg_p_jacobi<double>(s, d, r, c->A[SC], p, 0, -1, c->red, sc));
             } else {
                 HIPCHK(c, (launch_pcg_update<double, double>(s, d, x, r, p, q, nullptr, c->red, sc)));
                 if ((rc = precond_fp64(c, pc, r, z, prm->lambda_min, prm->lambda_max, prm->degree,
-                                       EPI_BETA, sc, A)) != LES_OK)
+                                       EPI_BETA, sc, A, wgt)) != LES_OK)
                     return rc;
                 HIPCHK(c, launch_pcg_p_update<double>(s, d, z, p, sc));
             }
@@ -780,6 +819,18 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     return LES_OK;
 }
 
+
+int ensure_mirror_weights(les_ctx* c)
+{
+    if (c->have_W) return LES_OK;
+    if (!c->have_code)
+        return fail(c, LES_ERR_STATE, "mirror weights need A built from the mask (les_build_stencil)");
+    int rc;
+    if ((rc = alloc(c, &c->W, c->d.N, false)) != LES_OK) return rc;
+    HIPCHK(c, launch_mirror_weights(c->stream, c->d, c->code, c->W));
+    c->have_W = true;
+    return LES_OK;
+}
 
 int ensure_fp32_ops(les_ctx* c, int pc, bool coded)
 {
@@ -994,6 +1045,17 @@ int les_pcg(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
         return fail(c, LES_ERR_ARG, "unknown preconditioner");
     if (prm->max_iter < 0) return fail(c, LES_ERR_ARG, "max_iter < 0");
     std::memset(res, 0, sizeof(*res));
+    if (prm->inner_product == LES_IP_MIRROR) {
+        int rc;
+        if (prm->precision != LES_PREC_FP64)
+            return fail(c, LES_ERR_ARG, "the mirror-weighted inner product is FP64 only");
+        if (prm->precond == LES_PC_IC0 || prm->precond == LES_PC_LEGACY_IC_A)
+            return fail(c, LES_ERR_ARG, "the mirror-weighted inner product needs a preconditioner "
+                                        "self-adjoint in it (NONE, JACOBI or CHEBYSHEV)");
+        if ((rc = ensure_mirror_weights(c)) != LES_OK) return rc;
+        return pcg_fp64(c, prm, res, nullptr, c->W);
+    }
+    if (prm->inner_product != LES_IP_EUCLID) return fail(c, LES_ERR_ARG, "unknown inner product");
     if (prm->precision == LES_PREC_MIXED) return pcg_mixed(c, prm, res);
     if (prm->precision != LES_PREC_FP64) return fail(c, LES_ERR_ARG, "unknown precision");
     return pcg_fp64(c, prm, res);
@@ -1283,12 +1345,14 @@ int les_step(les_ctx* c, const les_step_params* prm, les_step_stats* stats)
     const bool pcg = prm->solver == LES_SOLVER_PCG;
     // solve_momentum + apply_velocity_BCs, and (PCG path) calculate_divergence
     // + build_b_vector of the step's starting velocity
-    KtScope* ktm = new KtScope(c, LES_KT_MOMENTUM, 8);
-    HIPCHK(c, launch_momentum(s, c->d, k, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W],
-                              c->f[LES_F_NU_T], c->solid, c->f[LES_F_U_NEW], c->f[LES_F_V_NEW],
-                              c->f[LES_F_W_NEW], pcg ? c->f[LES_F_P_RHS] : nullptr,
-                              pcg ? c->f[LES_F_B] : nullptr, MOM_STEP));
-    delete ktm;
+    {
+        KtScope ktm(c, LES_KT_MOMENTUM, 8);
+        HIPCHK(c, launch_momentum(s, c->d, k, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W],
+                                  c->f[LES_F_NU_T], c->solid, c->f[LES_F_U_NEW],
+                                  c->f[LES_F_V_NEW], c->f[LES_F_W_NEW],
+                                  pcg ? c->f[LES_F_P_RHS] : nullptr,
+                                  pcg ? c->f[LES_F_B] : nullptr, MOM_STEP));
+    }
     swap_velocity(c);
     switch (prm->solver) {
     case LES_SOLVER_PCG:
@@ -1313,6 +1377,232 @@ int les_step(les_ctx* c, const les_step_params* prm, les_step_stats* stats)
     return LES_OK;
 }
 
+}  // extern "C"
+
+// ---------------- differentiation ----------------
+namespace {
+
+constexpr int kAdjBufs = 16;  // g(3) y(3) m(3) G(6) nu_t(1)
+
+int adj_check(les_ctx* c, const les_adj_params* prm)
+{
+    if (!prm) return fail(c, LES_ERR_ARG, "NULL params");
+    if (!c->have_A) return fail(c, LES_ERR_STATE, "stencil A not built");
+    if (!(prm->dt > 0)) return fail(c, LES_ERR_ARG, "dt must be > 0");
+    if (prm->nsteps < 1 || prm->checkpoint_every < 1)
+        return fail(c, LES_ERR_ARG, "nsteps and checkpoint_every must be >= 1");
+    const les_pcg_params& q = prm->pcg;
+    if (q.precision != LES_PREC_FP64) return fail(c, LES_ERR_ARG, "the adjoint path is FP64");
+    if (!q.tol_is_relative || !(q.tol > 0))
+        return fail(c, LES_ERR_ARG, "the adjoint path needs a relative pressure tolerance > 0");
+    if (q.precond != LES_PC_NONE && q.precond != LES_PC_JACOBI && q.precond != LES_PC_CHEBYSHEV)
+        return fail(c, LES_ERR_ARG, "the adjoint path takes preconditioner NONE, JACOBI or CHEBYSHEV");
+    if (q.inner_product != LES_IP_MIRROR)
+        return fail(c, LES_ERR_ARG, "the adjoint path solves in the mirror-weighted inner product "
+                                    "(pcg.inner_product = LES_IP_MIRROR)");
+    if (!c->have_code)
+        return fail(c, LES_ERR_STATE, "the adjoint path needs A built by les_build_stencil");
+    if (c->A_has_obstacles)
+        return fail(c, LES_ERR_STATE, "A has mirror rows around interior obstacles, which no "
+                                      "diagonal weight symmetrises, so CG would not give the exact "
+                                      "map; build A with ignore_mask = 1 (as the shipped driver "
+                                      "does) or from an obstacle-free mask");
+    return LES_OK;
+}
+
+int ensure_states(les_ctx* c, double** buf, int* cap, int count)
+{
+    if (*cap >= count) return LES_OK;
+    if (*buf) hipFree(*buf);
+    *buf = nullptr;
+    *cap = 0;
+    int rc = alloc(c, buf, (size_t)count * 3 * c->d.N, false);
+    if (rc == LES_OK) *cap = count;
+    return rc;
+}
+
+void copy_state(les_ctx* c, double* dst, const double* u, const double* v, const double* w,
+                hipError_t& e)
+{
+    const size_t b = nbytes(c, 8), n = (size_t)c->d.N;
+    if (e == hipSuccess) e = hipMemcpyAsync(dst, u, b, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst + n, v, b, hipMemcpyDeviceToDevice, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(dst + 2 * n, w, b, hipMemcpyDeviceToDevice, c->stream);
+}
+
+void adj_account(les_adj_stats* st, const les_pcg_result& r)
+{
+    if (!st) return;
+    st->solves += 1;
+    st->iters += r.iters;
+    if (!r.converged) st->unconverged += 1;
+    const double rel = r.b_norm > 0 ? r.res_norm / r.b_norm : 0.0;
+    st->max_rel_res = std::max(st->max_rel_res, rel);
+}
+
+// One step of the differentiated map: p from zero, fixed dt.
+int roll_step(les_ctx* c, const les_adj_params* prm, les_adj_stats* st)
+{
+    HIPCHK(c, hipMemsetAsync(c->f[LES_F_P], 0, nbytes(c, 8), c->stream));
+    les_step_params sp{};
+    sp.dt_fixed = prm->dt;
+    sp.nu = prm->nu;
+    sp.rho = prm->rho;
+    sp.Cs = prm->Cs;
+    sp.solver = LES_SOLVER_PCG;
+    sp.pcg = prm->pcg;
+    les_step_stats ss{};
+    const int rc = les_step(c, &sp, &ss);
+    if (rc != LES_OK) return rc;
+    adj_account(st, ss.pcg);
+    return LES_OK;
+}
+
+// The transposed step at state (u, v, w): g <- (dF/du)^T g.
+int adj_step(les_ctx* c, const les_adj_params* prm, const double* u, const double* v,
+             const double* w, les_adj_stats* st)
+{
+    const Dims& d = c->d;
+    hipStream_t s = c->stream;
+    const PhysConsts k = make_consts(c, prm->dt, prm->nu, prm->rho, prm->Cs);
+    double** b = c->adjb;
+    double *gu = b[0], *gv = b[1], *gw = b[2], *yu = b[3], *yv = b[4], *yw = b[5];
+    double *mu = b[6], *mv = b[7], *mw = b[8], *nut = b[15];
+    const StrainSens G{{b[9], b[10], b[11], b[12], b[13], b[14]}};
+    const double* W = c->W;
+    // nu_t of the step (compute_sgs_stress + apply_nut_BCs)
+    HIPCHK(c, launch_sgs(s, d, k, u, v, w, c->solid, nut, Tau{}, 1, c->red, nullptr));
+    // u^{n+1} = BC(u* - (dt/rho) grad p): y = BC^T g, rhs = W^-1 grad^T y
+    HIPCHK(c, launch_adj_velocity_bcs(s, d, c->solid, gu, gv, gw, yu, yv, yw));
+    HIPCHK(c, launch_adj_correct(s, d, k, yu, yv, yw, W, c->f[LES_F_B]));
+    // x = A^-1 rhs by CG in the W inner product (A^-T = W A^-1 W^-1)
+    HIPCHK(c, hipMemsetAsync(c->f[LES_F_P], 0, nbytes(c, 8), s));
+    les_pcg_result r{};
+    const int rc = pcg_fp64(c, &prm->pcg, &r, nullptr, c->W);
+    if (rc != LES_OK) return rc;
+    adj_account(st, r);
+    // u* = BC(M(u^n, nu_t)): m = BC^T y, then the momentum, lagged
+    // divergence and nu_t paths back to u^n
+    HIPCHK(c, launch_adj_velocity_bcs(s, d, c->solid, yu, yv, yw, mu, mv, mw));
+    HIPCHK(c, launch_adj_momentum(s, d, k, mu, mv, mw, u, v, w, nut, c->f[LES_F_P], W, c->solid,
+                                  gu, gv, gw, G));
+    HIPCHK(c, launch_adj_sgs_gather(s, d, k, G, gu, gv, gw));
+    return LES_OK;
+}
+
+// Swaps the context's field set with the shadow set for its lifetime.
+struct ShadowFields {
+    les_ctx* c;
+    explicit ShadowFields(les_ctx* ctx) : c(ctx) { std::swap(c->f, c->shadow); }
+    ~ShadowFields() { std::swap(c->f, c->shadow); }
+};
+
+}  // namespace
+
+extern "C" {
+
+int les_rollout(les_ctx* c, const les_adj_params* prm, les_adj_stats* stats)
+{
+    if (!c) return LES_ERR_ARG;
+    int rc;
+    if ((rc = adj_check(c, prm)) != LES_OK) return rc;
+    {
+        std::vector<uint8_t> inlet((size_t)c->d.sij);
+        HIPCHK(c, hipMemcpyAsync(inlet.data(), c->solid, inlet.size(), hipMemcpyDeviceToHost,
+                                 c->stream));
+        HIPCHK(c, hipStreamSynchronize(c->stream));
+        if (std::any_of(inlet.begin(), inlet.end(), [](uint8_t v) { return v != 0; }))
+            return fail(c, LES_ERR_STATE, "solid cells on the inlet plane keep stale ping-pong "
+                                          "values (main.cpp:340); the step map is not a function "
+                                          "of the velocity alone");
+    }
+    if ((rc = ensure_mirror_weights(c)) != LES_OK) return rc;
+    const int every = prm->checkpoint_every;
+    const int nck = (prm->nsteps - 1) / every + 1;
+    if ((rc = ensure_states(c, &c->ckpt, &c->ckpt_cap, nck)) != LES_OK) return rc;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    c->have_roll = false;
+    for (int t = 0; t < prm->nsteps; ++t) {
+        if (t % every == 0) {
+            hipError_t e = hipSuccess;
+            copy_state(c, c->ckpt + (size_t)(t / every) * 3 * c->d.N, c->f[LES_F_U],
+                       c->f[LES_F_V], c->f[LES_F_W], e);
+            HIPCHK(c, e);
+        }
+        if ((rc = roll_step(c, prm, stats)) != LES_OK) return rc;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    c->have_roll = true;
+    c->roll_nsteps = prm->nsteps;
+    c->roll_every = every;
+    c->roll_dt = prm->dt;
+    c->roll_nu = prm->nu;
+    c->roll_rho = prm->rho;
+    c->roll_Cs = prm->Cs;
+    return LES_OK;
+}
+
+int les_adjoint(les_ctx* c, const les_adj_params* prm, double* ubar, double* vbar, double* wbar,
+                int ptr_kind, les_adj_stats* stats)
+{
+    if (!c) return LES_ERR_ARG;
+    int rc;
+    if ((rc = adj_check(c, prm)) != LES_OK) return rc;
+    if (!ubar || !vbar || !wbar) return fail(c, LES_ERR_ARG, "NULL gradient array");
+    if (!c->have_roll || c->roll_nsteps != prm->nsteps || c->roll_every != prm->checkpoint_every ||
+        c->roll_dt != prm->dt || c->roll_nu != prm->nu || c->roll_rho != prm->rho ||
+        c->roll_Cs != prm->Cs)
+        return fail(c, LES_ERR_STATE, "no les_rollout with these parameters since the operator, "
+                                      "mask or parameters last changed");
+    if ((rc = ensure_mirror_weights(c)) != LES_OK) return rc;
+    for (int q = 0; q < kAdjBufs; ++q)
+        if ((rc = alloc(c, &c->adjb[q], c->d.N, false)) != LES_OK) return rc;
+    for (int q = 0; q < LES_F_COUNT; ++q)
+        if ((rc = alloc(c, &c->shadow[q], c->d.N)) != LES_OK) return rc;
+    const int every = prm->checkpoint_every;
+    if ((rc = ensure_states(c, &c->seg, &c->seg_cap, every)) != LES_OK) return rc;
+    if ((rc = ensure_krylov(c)) != LES_OK) return rc;
+    if (stats) std::memset(stats, 0, sizeof(*stats));
+    const size_t n = (size_t)c->d.N;
+    double* gin[3] = {ubar, vbar, wbar};
+    for (int q = 0; q < 3; ++q)
+        HIPCHK(c, hipMemcpyAsync(c->adjb[q], gin[q], nbytes(c, 8), kind_in(ptr_kind), c->stream));
+    {
+        ShadowFields sh(c);
+        const int nck = (prm->nsteps - 1) / every + 1;
+        for (int sg = nck - 1; sg >= 0; --sg) {
+            const int len = std::min(every, prm->nsteps - sg * every);
+            const double* ck = c->ckpt + (size_t)sg * 3 * n;
+            HIPCHK(c, hipMemcpyAsync(c->seg, ck, 3 * nbytes(c, 8), hipMemcpyDeviceToDevice,
+                                     c->stream));
+            // recompute the states before steps 1..len-1 of the segment
+            if (len > 1) {
+                HIPCHK(c, hipMemcpyAsync(c->f[LES_F_U], ck, nbytes(c, 8), hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(c->f[LES_F_V], ck + n, nbytes(c, 8), hipMemcpyDeviceToDevice, c->stream));
+                HIPCHK(c, hipMemcpyAsync(c->f[LES_F_W], ck + 2 * n, nbytes(c, 8), hipMemcpyDeviceToDevice, c->stream));
+            }
+            for (int t = 1; t < len; ++t) {
+                if ((rc = roll_step(c, prm, stats)) != LES_OK) return rc;
+                hipError_t e = hipSuccess;
+                copy_state(c, c->seg + (size_t)t * 3 * n, c->f[LES_F_U], c->f[LES_F_V],
+                           c->f[LES_F_W], e);
+                HIPCHK(c, e);
+            }
+            for (int t = len - 1; t >= 0; --t) {
+                const double* st = c->seg + (size_t)t * 3 * n;
+                if ((rc = adj_step(c, prm, st, st + n, st + 2 * n, stats)) != LES_OK) return rc;
+            }
+        }
+    }
+    for (int q = 0; q < 3; ++q)
+        HIPCHK(c, hipMemcpyAsync(gin[q], c->adjb[q], nbytes(c, 8), kind_out(ptr_kind), c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+}  // extern "C"
+
+extern "C" {
 
 int les_kernel_timing(les_ctx* c, int kernel_class)
 {

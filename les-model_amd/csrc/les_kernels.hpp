@@ -98,6 +98,8 @@ hipError_t launch_finalize(hipStream_t s, int grid, int nr, bool max, int epi, i
 
 // ---- operator ----
 hipError_t launch_build_code(hipStream_t s, const Dims& d, const uint8_t* solid, uint16_t* code);
+// W = 1/2 per mirrored direction of each coded row (W A symmetric without obstacles)
+hipError_t launch_mirror_weights(hipStream_t s, const Dims& d, const uint16_t* code, double* W);
 hipError_t launch_build_stencil(hipStream_t s, const Dims& d, const uint8_t* solid,
                                 double* const A[7], double idx2, double idy2, double idz2);
 template <typename T>
@@ -113,28 +115,32 @@ template <typename T>
 hipError_t launch_pcg_init(hipStream_t s, const Dims& d, Stencil<T> A, const T* b, const T* x,
                            T* r, RedScratch red, SolveScalars* sc);
 // q = A p; alpha = rho / (p.q)  (pressure.cpp:375-378)
+// wgt (may be nullptr): per-cell weights of the inner products r.z and p.q
+// (mirror weights W; the stopping test stays the Euclidean ||r||).
 template <typename T>
 hipError_t launch_pcg_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T* p, T* q,
-                             RedScratch red, SolveScalars* sc);
+                             RedScratch red, SolveScalars* sc, const double* wgt = nullptr);
 // x += alpha p; r -= alpha q; ||r|| test (pressure.cpp:381-392).  With
 // jacobi_diag != nullptr also r.(r/A_c) and beta (pressure.cpp:396-402).
 template <typename T, typename TX>
 hipError_t launch_pcg_update(hipStream_t s, const Dims& d, TX* x, T* r, const T* p, const T* q,
-                             const T* jacobi_diag, RedScratch red, SolveScalars* sc);
+                             const T* jacobi_diag, RedScratch red, SolveScalars* sc,
+                             const double* wgt = nullptr);
 // p = M^-1 r + beta p for Jacobi (beta ignored and p overwritten when init);
 // init also reduces rho = r.p (pressure.cpp:363-369, 395-405).
 // When init, z = r / A_c is written into p and r.z reduced with epilogue
 // epi (< 0: no reduction; sc may be nullptr then).
 template <typename T>
 hipError_t launch_pcg_p_jacobi(hipStream_t s, const Dims& d, const T* r, const T* diag, T* p,
-                               int init, int epi, RedScratch red, SolveScalars* sc);
+                               int init, int epi, RedScratch red, SolveScalars* sc,
+                               const double* wgt = nullptr);
 // p = z + beta p (pressure.cpp:405)
 template <typename T>
 hipError_t launch_pcg_p_update(hipStream_t s, const Dims& d, const T* z, T* p, SolveScalars* sc);
 // rho / beta from r.z (pressure.cpp:369, 399-402)
 template <typename T>
 hipError_t launch_pcg_rz(hipStream_t s, const Dims& d, const T* r, const T* z, int epilogue,
-                         RedScratch red, SolveScalars* sc);
+                         RedScratch red, SolveScalars* sc, const double* wgt = nullptr);
 
 // ---- Chebyshev preconditioner (pressure.cpp:851-906) ----
 // One inner step k >= 1 of the recurrence:
@@ -145,10 +151,12 @@ hipError_t launch_pcg_rz(hipStream_t s, const Dims& d, const T* r, const T* z, i
 template <typename T>
 hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
                             const T* z_in, T* z_out, T* p, double alpha0, double beta,
-                            double alpha, int first, int epi, RedScratch red, SolveScalars* sc);
+                            double alpha, int first, int epi, RedScratch red, SolveScalars* sc,
+                            const double* wgt = nullptr);
 template <typename T>
 hipError_t launch_cheb_scale(hipStream_t s, const Dims& d, const T* r, T* z, double alpha0,
-                             int epi, RedScratch red, SolveScalars* sc);
+                             int epi, RedScratch red, SolveScalars* sc,
+                             const double* wgt = nullptr);
 
 // ---- IC(0) (pressure.cpp:442-556), i+j+k hyperplane order ----
 hipError_t launch_ic_factor_level(hipStream_t s, const Dims& d, int level, Stencil<double> A,

@@ -73,7 +73,9 @@ __device__ __forceinline__ void epi_init(const double* tot, SolveScalars* sc)
     if (sc->tol_rel > 0.0) sc->tol = sc->tol_rel * sqrt(tot[1]);
     if (sc->trace) sc->trace[0] = e;
     sc->iter = 0;
-    if (e < sc->tol) {
+    // a relative tolerance of a zero right-hand side is 0: an exactly zero
+    // residual then counts as converged rather than dividing 0 by 0
+    if (e < sc->tol || (sc->tol_rel > 0.0 && e == 0.0)) {
         sc->done = 1;
         sc->converged = 1;
     } else if (!(e == e) || sc->max_iter <= 0) {
@@ -266,6 +268,25 @@ __global__ __launch_bounds__(kBlock) void k_build_code(Dims d, Chunking ch, cons
     }
 }
 
+// W = 1/2 per direction in which a row of the coded operator is mirrored
+// (one multiplier 2, the opposite 0); W A is symmetric on grids without
+// obstacles.
+__global__ __launch_bounds__(kBlock) void k_mirror_weights(Dims d, Chunking ch,
+                                                           const uint16_t* code, double* W)
+{
+    const int lb = logical_block(ch.grid);
+    FOR_CELLS(d, ch, lb)
+    {
+        const unsigned cd = code[id];
+        double wt = 1.0;
+        for (int ax = 0; ax < 3; ++ax) {
+            const unsigned lo = (cd >> (4 * ax)) & 3u, hi = (cd >> (4 * ax + 2)) & 3u;
+            if ((lo == 2u && hi == 0u) || (hi == 2u && lo == 0u)) wt *= 0.5;
+        }
+        W[id] = wt;
+    }
+}
+
 template <typename T, int CODED>
 __global__ __launch_bounds__(kBlock) void k_matvec(Dims d, Chunking ch, Stencil<T> A,
                                                    const T* __restrict__ x, T* __restrict__ y)
@@ -317,7 +338,8 @@ template <typename T, int CODED>
 __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Stencil<T> A,
                                                        const T* __restrict__ p,
                                                        T* __restrict__ q, RedScratch red,
-                                                       const SolveScalars* sc)
+                                                       const SolveScalars* sc,
+                                                       const double* __restrict__ wgt)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
@@ -326,7 +348,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Sten
     {
         const T qv = apply7<CODED>(A, d, id, i, j, k, [&](int n) { return p[n]; });
         q[id] = qv;
-        acc[0] += (double)p[id] * (double)qv;
+        const double pq = (double)p[id] * (double)qv;
+        acc[0] += wgt ? wgt[id] * pq : pq;
     }
     block_partial<1>(acc, red.partials, lb);
 }
@@ -337,7 +360,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Dims d, Chunking ch, TX* 
                                                        const T* __restrict__ p,
                                                        const T* __restrict__ q,
                                                        const T* __restrict__ diag,
-                                                       RedScratch red, const SolveScalars* sc)
+                                                       RedScratch red, const SolveScalars* sc,
+                                                       const double* __restrict__ wgt)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
@@ -351,7 +375,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Dims d, Chunking ch, TX* 
         const T rv = r[id] + ar * q[id];
         r[id] = rv;
         acc[0] += (double)rv * (double)rv;
-        if (diag) acc[1] += (double)rv * (double)(rv / diag[id]);
+        if (diag) {
+            const double rz = (double)rv * (double)(rv / diag[id]);
+            acc[1] += wgt ? wgt[id] * rz : rz;
+        }
     }
     block_partial<2>(acc, red.partials, lb);
 }
@@ -361,7 +388,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p_jacobi(Dims d, Chunking ch,
                                                          const T* __restrict__ r,
                                                          const T* __restrict__ diag,
                                                          T* __restrict__ p, int init,
-                                                         RedScratch red, const SolveScalars* sc)
+                                                         RedScratch red, const SolveScalars* sc,
+                                                         const double* __restrict__ wgt)
 {
     if (is_done(sc)) return;
     const int lb = logical_block(ch.grid);
@@ -373,7 +401,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p_jacobi(Dims d, Chunking ch,
             const T rv = r[id];
             const T z = rv / diag[id];
             p[id] = z;
-            acc[0] += (double)rv * (double)z;
+            const double rz = (double)rv * (double)z;
+            acc[0] += wgt ? wgt[id] * rz : rz;
         }
         if (red.partials) block_partial<1>(acc, red.partials, lb);
     } else {
@@ -396,12 +425,17 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p_update(Dims d, Chunking ch,
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_rz(Dims d, Chunking ch, const T* __restrict__ r,
                                                    const T* __restrict__ z, RedScratch red,
-                                                   const SolveScalars* sc)
+                                                   const SolveScalars* sc,
+                                                   const double* __restrict__ wgt)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
     double acc[1] = {0.0};
-    FOR_CELLS(d, ch, lb) acc[0] += (double)r[id] * (double)z[id];
+    FOR_CELLS(d, ch, lb)
+    {
+        const double rz = (double)r[id] * (double)z[id];
+        acc[0] += wgt ? wgt[id] * rz : rz;
+    }
     block_partial<1>(acc, red.partials, lb);
 }
 
@@ -416,7 +450,8 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(Dims d, Chunking ch, Stenc
                                                       const T* __restrict__ z_in,
                                                       T* __restrict__ z_out, T* __restrict__ p,
                                                       double alpha0, double beta, double alpha,
-                                                      RedScratch red, const SolveScalars* sc)
+                                                      RedScratch red, const SolveScalars* sc,
+                                                      const double* __restrict__ wgt)
 {
     if (is_done(sc)) return;
     const int lb = logical_block(ch.grid);
@@ -440,7 +475,10 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(Dims d, Chunking ch, Stenc
         p[id] = pn;
         const T zn = zc + at * pn;
         z_out[id] = zn;
-        if (REDUCE) acc[0] += (double)rv * (double)zn;
+        if (REDUCE) {
+            const double rz = (double)rv * (double)zn;
+            acc[0] += wgt ? wgt[id] * rz : rz;
+        }
     }
     if (REDUCE) block_partial<1>(acc, red.partials, lb);
 }
@@ -449,7 +487,8 @@ template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cheb_scale(Dims d, Chunking ch,
                                                        const T* __restrict__ r,
                                                        T* __restrict__ z, double alpha0,
-                                                       RedScratch red, const SolveScalars* sc)
+                                                       RedScratch red, const SolveScalars* sc,
+                                                       const double* __restrict__ wgt)
 {
     if (is_done(sc)) return;
     const int lb = logical_block(ch.grid);
@@ -460,7 +499,8 @@ __global__ __launch_bounds__(kBlock) void k_cheb_scale(Dims d, Chunking ch,
         const T rv = r[id];
         const T zn = T(0) + a0 * rv;
         z[id] = zn;
-        acc[0] += (double)rv * (double)zn;
+        const double rz = (double)rv * (double)zn;
+        acc[0] += wgt ? wgt[id] * rz : rz;
     }
     if (red.partials) block_partial<1>(acc, red.partials, lb);
 }
@@ -742,28 +782,36 @@ hipError_t launch_pcg_init(hipStream_t s, const Dims& d, Stencil<T> A, const T* 
     return launch_finalize(s, ch.grid, 2, false, EPI_INIT, GATE_NONE, red, sc, nullptr);
 }
 
+hipError_t launch_mirror_weights(hipStream_t s, const Dims& d, const uint16_t* code, double* W)
+{
+    const Chunking ch = make_chunking(d);
+    hipLaunchKernelGGL(k_mirror_weights, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, code, W);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_pcg_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T* p, T* q,
-                             RedScratch red, SolveScalars* sc)
+                             RedScratch red, SolveScalars* sc, const double* wgt)
 {
     const Chunking ch = make_chunking(d);
     if (A.code)
         hipLaunchKernelGGL((k_pcg_matvec<T, 1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, p,
-                           q, red, sc);
+                           q, red, sc, wgt);
     else
         hipLaunchKernelGGL((k_pcg_matvec<T, 0>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, p,
-                           q, red, sc);
+                           q, red, sc, wgt);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     return launch_finalize(s, ch.grid, 1, false, EPI_ALPHA, GATE_DONE, red, sc, nullptr);
 }
 
 template <typename T, typename TX>
 hipError_t launch_pcg_update(hipStream_t s, const Dims& d, TX* x, T* r, const T* p, const T* q,
-                             const T* jacobi_diag, RedScratch red, SolveScalars* sc)
+                             const T* jacobi_diag, RedScratch red, SolveScalars* sc,
+                             const double* wgt)
 {
     const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL((k_pcg_update<T, TX>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, r, p,
-                       q, jacobi_diag, red, sc);
+                       q, jacobi_diag, red, sc, wgt);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     return launch_finalize(s, ch.grid, 2, false, jacobi_diag ? EPI_CONV_RZ : EPI_CONV, GATE_DONE,
                            red, sc, nullptr);
@@ -771,13 +819,14 @@ hipError_t launch_pcg_update(hipStream_t s, const Dims& d, TX* x, T* r, const T*
 
 template <typename T>
 hipError_t launch_pcg_p_jacobi(hipStream_t s, const Dims& d, const T* r, const T* diag, T* p,
-                               int init, int epi, RedScratch red, SolveScalars* sc)
+                               int init, int epi, RedScratch red, SolveScalars* sc,
+                               const double* wgt)
 {
     const Chunking ch = make_chunking(d);
     RedScratch rd = red;
     if (!(init && epi >= 0)) rd.partials = nullptr;
     hipLaunchKernelGGL(k_pcg_p_jacobi<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r, diag, p,
-                       init, rd, sc);
+                       init, rd, sc, wgt);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     if (init && epi >= 0)
         return launch_finalize(s, ch.grid, 1, false, epi, sc ? GATE_DONE : GATE_NONE, red, sc,
@@ -795,10 +844,11 @@ hipError_t launch_pcg_p_update(hipStream_t s, const Dims& d, const T* z, T* p, S
 
 template <typename T>
 hipError_t launch_pcg_rz(hipStream_t s, const Dims& d, const T* r, const T* z, int epilogue,
-                         RedScratch red, SolveScalars* sc)
+                         RedScratch red, SolveScalars* sc, const double* wgt)
 {
     const Chunking ch = make_chunking(d);
-    hipLaunchKernelGGL(k_pcg_rz<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r, z, red, sc);
+    hipLaunchKernelGGL(k_pcg_rz<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r, z, red, sc,
+                       wgt);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     return launch_finalize(s, ch.grid, 1, false, epilogue, GATE_DONE, red, sc, nullptr);
 }
@@ -806,12 +856,13 @@ hipError_t launch_pcg_rz(hipStream_t s, const Dims& d, const T* r, const T* z, i
 template <typename T>
 hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
                             const T* z_in, T* z_out, T* p, double alpha0, double beta,
-                            double alpha, int first, int epi, RedScratch red, SolveScalars* sc)
+                            double alpha, int first, int epi, RedScratch red, SolveScalars* sc,
+                            const double* wgt)
 {
     const Chunking ch = make_chunking(d);
 #define LES_CHEB(F, CD, RD)                                                                     \
     hipLaunchKernelGGL((k_cheb_step<T, F, CD, RD>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, \
-                       r, z_in, z_out, p, alpha0, beta, alpha, red, sc)
+                       r, z_in, z_out, p, alpha0, beta, alpha, red, sc, wgt)
 #define LES_CHEB_R(F, CD)                   \
     do {                                    \
         if (epi >= 0) LES_CHEB(F, CD, true);  \
@@ -835,13 +886,13 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
 
 template <typename T>
 hipError_t launch_cheb_scale(hipStream_t s, const Dims& d, const T* r, T* z, double alpha0,
-                             int epi, RedScratch red, SolveScalars* sc)
+                             int epi, RedScratch red, SolveScalars* sc, const double* wgt)
 {
     const Chunking ch = make_chunking(d);
     RedScratch rd = red;
     if (epi < 0) rd.partials = nullptr;
     hipLaunchKernelGGL(k_cheb_scale<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r, z, alpha0,
-                       rd, sc);
+                       rd, sc, wgt);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     if (epi >= 0)
         return launch_finalize(s, ch.grid, 1, false, epi, sc ? GATE_DONE : GATE_NONE, red, sc,
@@ -940,18 +991,19 @@ hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double
     template hipError_t launch_pcg_init<T>(hipStream_t, const Dims&, Stencil<T>, const T*,      \
                                            const T*, T*, RedScratch, SolveScalars*);            \
     template hipError_t launch_pcg_matvec<T>(hipStream_t, const Dims&, Stencil<T>, const T*,    \
-                                             T*, RedScratch, SolveScalars*);                    \
+                                             T*, RedScratch, SolveScalars*, const double*);     \
     template hipError_t launch_pcg_p_jacobi<T>(hipStream_t, const Dims&, const T*, const T*,    \
-                                               T*, int, int, RedScratch, SolveScalars*);        \
+                                               T*, int, int, RedScratch, SolveScalars*,         \
+                                               const double*);                                  \
     template hipError_t launch_pcg_p_update<T>(hipStream_t, const Dims&, const T*, T*,          \
                                                SolveScalars*);                                  \
     template hipError_t launch_pcg_rz<T>(hipStream_t, const Dims&, const T*, const T*, int,     \
-                                         RedScratch, SolveScalars*);                            \
+                                         RedScratch, SolveScalars*, const double*);             \
     template hipError_t launch_cheb_step<T>(hipStream_t, const Dims&, Stencil<T>, const T*,     \
                                             const T*, T*, T*, double, double, double, int, int, \
-                                            RedScratch, SolveScalars*);                         \
+                                            RedScratch, SolveScalars*, const double*);          \
     template hipError_t launch_cheb_scale<T>(hipStream_t, const Dims&, const T*, T*, double,    \
-                                             int, RedScratch, SolveScalars*);                   \
+                                             int, RedScratch, SolveScalars*, const double*);    \
     template hipError_t launch_ic_fwd_level<T>(hipStream_t, const Dims&, int, const T*, T*,     \
                                                const T*, const T*, const T*, const T*,          \
                                                const SolveScalars*);                            \
@@ -965,10 +1017,11 @@ LES_INST(float)
 
 template hipError_t launch_pcg_update<double, double>(hipStream_t, const Dims&, double*, double*,
                                                       const double*, const double*,
-                                                      const double*, RedScratch, SolveScalars*);
+                                                      const double*, RedScratch, SolveScalars*,
+                                                      const double*);
 template hipError_t launch_pcg_update<float, float>(hipStream_t, const Dims&, float*, float*,
                                                     const float*, const float*, const float*,
-                                                    RedScratch, SolveScalars*);
+                                                    RedScratch, SolveScalars*, const double*);
 template hipError_t launch_convert<float, double>(hipStream_t, int, const double*, float*);
 template hipError_t launch_convert<double, float>(hipStream_t, int, const float*, double*);
 template hipError_t launch_convert<double, double>(hipStream_t, int, const double*, double*);

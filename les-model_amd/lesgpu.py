@@ -28,6 +28,7 @@ PC_NONE, PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
 SOLVER_PCG, SOLVER_SOR_LEX, SOLVER_SOR_RB, SOLVER_JACOBI = 0, 1, 2, 3
 KT_NONE, KT_CHEB_STEP, KT_PCG_MATVEC, KT_PCG_UPDATE, KT_MOMENTUM, KT_IC_SOLVE = 0, 1, 2, 3, 4, 5
+IP_EUCLID, IP_MIRROR = 0, 1
 
 DP = C.POINTER(C.c_double)
 
@@ -37,7 +38,7 @@ class PcgParams(C.Structure):
                 ("tol_is_relative", C.c_int), ("lambda_min", C.c_double),
                 ("lambda_max", C.c_double), ("degree", C.c_int), ("precision", C.c_int),
                 ("reliable_every", C.c_int), ("check_every", C.c_int),
-                ("stencil_format", C.c_int)]
+                ("stencil_format", C.c_int), ("inner_product", C.c_int)]
 
 
 class PcgResult(C.Structure):
@@ -57,6 +58,19 @@ class StepParams(C.Structure):
 
 class StepStats(C.Structure):
     _fields_ = [("dt", C.c_double), ("pcg", PcgResult), ("sweeps", C.c_int)]
+
+
+class AdjParams(C.Structure):
+    _fields_ = [("nu", C.c_double), ("rho", C.c_double), ("Cs", C.c_double), ("dt", C.c_double),
+                ("nsteps", C.c_int), ("checkpoint_every", C.c_int), ("pcg", PcgParams)]
+
+
+class AdjStats(C.Structure):
+    _fields_ = [("solves", C.c_int), ("iters", C.c_int), ("unconverged", C.c_int),
+                ("max_rel_res", C.c_double)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class LesError(RuntimeError):
@@ -111,6 +125,8 @@ def load_library(path: str = LIB_PATH):
     L.les_correct.argtypes = [vp, C.c_double, C.c_double, C.c_int]
     L.les_adaptive_dt.argtypes = [vp, C.c_double, DP]
     L.les_step.argtypes = [vp, C.POINTER(StepParams), C.POINTER(StepStats)]
+    L.les_rollout.argtypes = [vp, C.POINTER(AdjParams), C.POINTER(AdjStats)]
+    L.les_adjoint.argtypes = [vp, C.POINTER(AdjParams), vp, vp, vp, C.c_int, C.POINTER(AdjStats)]
     L.les_kernel_timing.argtypes = [vp, C.c_int]
     L.les_kernel_timing_read.argtypes = [vp, C.POINTER(C.c_int), DP, C.POINTER(C.c_int)]
     L.les_diag_copy.argtypes = [vp, C.c_int]
@@ -266,9 +282,9 @@ class LesGpu:
     @staticmethod
     def pcg_params(precond=PC_LEGACY_IC_A, max_iter=1000, tol=1e-6, tol_rel=False, lmin=0.0,
                    lmax=0.0, degree=5, precision=PREC_FP64, reliable_every=50, check_every=0,
-                   stencil_format=STENCIL_AUTO):
+                   stencil_format=STENCIL_AUTO, inner_product=IP_EUCLID):
         return PcgParams(precond, max_iter, tol, 1 if tol_rel else 0, lmin, lmax, degree,
-                         precision, reliable_every, check_every, stencil_format)
+                         precision, reliable_every, check_every, stencil_format, inner_product)
 
     def pcg(self, **kw):
         prm = self.pcg_params(**kw)
@@ -330,6 +346,34 @@ class LesGpu:
         st = StepStats()
         self._chk(self.lib.les_step(self.h, C.byref(prm), C.byref(st)), "les_step")
         return {"dt": st.dt, "sweeps": st.sweeps, **st.pcg.as_dict()}
+
+    # ---- differentiation (no reference counterpart) ----
+    @classmethod
+    def adj_params(cls, dt=1e-3, nsteps=20, checkpoint_every=5, nu=1e-3, rho=1.2, Cs=0.17,
+                   **pcg_kw):
+        """Parameters of les_rollout / les_adjoint.  The pressure solves default
+        to Jacobi-PCG in the mirror-weighted inner product to 1e-13 relative."""
+        kw = dict(precond=PC_JACOBI, max_iter=20000, tol=1e-13, tol_rel=True,
+                  inner_product=IP_MIRROR)
+        kw.update(pcg_kw)
+        return AdjParams(nu, rho, Cs, dt, nsteps, checkpoint_every, cls.pcg_params(**kw))
+
+    def rollout(self, prm):
+        """Advances u, v, w by prm.nsteps steps of the fixed-dt step map and
+        keeps checkpoints for adjoint()."""
+        st = AdjStats()
+        self._chk(self.lib.les_rollout(self.h, C.byref(prm), C.byref(st)), "les_rollout")
+        return st.as_dict()
+
+    def adjoint(self, prm, ubar, vbar, wbar):
+        """dL/du^0 from dL/du^nsteps, in place (numpy or CUDA tensors)."""
+        pu, k = _ptr(ubar)
+        pv, _ = _ptr(vbar)
+        pw, _ = _ptr(wbar)
+        st = AdjStats()
+        self._chk(self.lib.les_adjoint(self.h, C.byref(prm), pu, pv, pw, k, C.byref(st)),
+                  "les_adjoint")
+        return st.as_dict()
 
     # ---- diagnostics ----
     def kernel_timing(self, kernel_class):
