@@ -51,6 +51,24 @@ def cheb_bytes_per_cell(degree, elem, coded=False):
     return (first + (m - 1) * later) / m
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_cheb_step_256_traffic.json")
+
+
+def measured_traffic(mode, n):
+    """HBM bytes per launch of the Chebyshev step kernel in this mode, from
+    the committed rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE, each
+    scaled by what the counter reads for the plain copy kernel), or None."""
+    try:
+        with open(TRAFFIC_FILE) as fh:
+            rec = json.load(fh)
+    except (OSError, ValueError):
+        return None
+    if rec.get("n") != n:
+        return None
+    m = rec.get("modes", {}).get(mode)
+    return None if m is None else m.get("traffic_bytes_per_launch")
+
+
 def cpu_model():
     try:
         out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
@@ -132,8 +150,8 @@ def cpu_baseline(m, st0, solid, bounds, threads, step0):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
@@ -227,7 +245,9 @@ def main():
                      "kernel": f"k_cheb_step<{'float' if elem == 4 else 'double'}>"
                                f"{' coded' if R['coded'] else ''}",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(head, m["nx"]),
+                     "algorithmic_bytes_per_launch": bpc * N,
                      "bytes_per_cell": bpc, "cells": N, "launches": n_launch,
                      "avg_us": 1e3 * kt_ms / n_launch if n_launch else None},
         "modes": {mname: {"steps_per_s": args.steps / r["seconds"],

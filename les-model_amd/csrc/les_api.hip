@@ -620,17 +620,20 @@ int les_precond_apply(les_ctx* c, int pc, const double* r, double* z, int ptr_ki
     if ((rc = ensure_krylov(c)) != LES_OK) return rc;
     const double* rd = r;
     double* zd = z;
-    if (!ptr_kind) {
-        HIPCHK(c, hipMemcpyAsync(c->t1, r, nbytes(c, 8), hipMemcpyHostToDevice, c->stream));
+    // host vectors, and device vectors the 16-byte vector loads cannot
+    // address, go through the context's own buffers
+    const bool stage = !ptr_kind || (((uintptr_t)r | (uintptr_t)z) % 16) != 0;
+    if (stage) {
+        HIPCHK(c, hipMemcpyAsync(c->t1, r, nbytes(c, 8), kind_in(ptr_kind), c->stream));
         rd = c->t1;
         zd = c->t2;
     }
     if ((rc = precond_fp64(c, pc, rd, zd, lmin, lmax, degree, -1, nullptr, stencil_of(c->A))) !=
         LES_OK)
         return rc;
-    if (!ptr_kind) {
-        HIPCHK(c, hipMemcpyAsync(z, zd, nbytes(c, 8), hipMemcpyDeviceToHost, c->stream));
-        HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (stage) {
+        HIPCHK(c, hipMemcpyAsync(z, zd, nbytes(c, 8), kind_out(ptr_kind), c->stream));
+        if (!ptr_kind) HIPCHK(c, hipStreamSynchronize(c->stream));
     }
     return LES_OK;
 }

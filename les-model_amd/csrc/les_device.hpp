@@ -42,10 +42,13 @@ struct Chunking {
     int rpp;     // rows per pass (kBlock / nz when nz < kBlock, else 1)
 };
 
-inline Chunking make_chunking(const Dims& d)
+// V > 1: each thread owns V consecutive cells of a row (vector loads);
+// requires nz % V == 0.
+inline Chunking make_chunking(const Dims& d, int V = 1)
 {
     const int rows = d.nx * d.ny;
-    const int rpp = d.nz >= kBlock ? 1 : kBlock / d.nz;
+    const int tpr = d.nz / V;  // threads per row
+    const int rpp = tpr >= kBlock ? 1 : kBlock / tpr;
     // aim for about kBlock cells of work per thread-row pass, capped grid
     int grid = (rows + rpp - 1) / rpp;
     if (grid > kMaxGrid) grid = kMaxGrid;
@@ -74,6 +77,19 @@ __device__ __forceinline__ int logical_block(int grid)
         for (int i = row_ / (d).ny, j = row_ - i * (d).ny, k = k0_, id = row_ * (d).nz + k0_;     \
              k < (d).nz; k += ((ch).rpp > 1 ? (d).nz : kBlock),                                   \
                  id += ((ch).rpp > 1 ? (d).nz : kBlock))
+
+// FOR_VCELLS(d, ch, lb, V) { ... }: as FOR_CELLS, with each thread owning
+// the V cells k .. k+V-1 (k a multiple of V) of its row; ch from
+// make_chunking(d, V).
+#define FOR_VCELLS(d, ch, lb, V)                                                                  \
+    for (int tpr_ = (d).nz / (V), sub_ = ((ch).rpp > 1 ? (int)threadIdx.x / tpr_ : 0),           \
+             k0_ = ((int)threadIdx.x - sub_ * tpr_) * (V),                                        \
+             row_ = (lb) * (ch).rpb + sub_,                                                       \
+             rend_ = min((d).nx * (d).ny, ((lb) + 1) * (ch).rpb);                                 \
+         sub_ < (ch).rpp && row_ < rend_; row_ += (ch).rpp)                                       \
+        for (int i = row_ / (d).ny, j = row_ - i * (d).ny, k = k0_, id = row_ * (d).nz + k0_;     \
+             k < (d).nz; k += ((ch).rpp > 1 ? (d).nz : kBlock * (V)),                             \
+                 id += ((ch).rpp > 1 ? (d).nz : kBlock * (V)))
 
 // ---- wave64 / workgroup reductions in a fixed order ----
 __device__ __forceinline__ double wave_sum(double v)

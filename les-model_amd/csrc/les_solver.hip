@@ -60,6 +60,119 @@ __device__ __forceinline__ T apply7(const Stencil<T>& A, const Dims& d, int id, 
     return val;
 }
 
+// ---- vector forms: V consecutive cells of one row per thread ----
+template <typename T, int V>
+using vec_t = T __attribute__((ext_vector_type(V)));
+
+template <typename T, int V>
+__device__ __forceinline__ void vload(const T* __restrict__ p, T (&out)[V])
+{
+    if constexpr (V == 1) {
+        out[0] = *p;
+    } else {
+        const vec_t<T, V> x = *reinterpret_cast<const vec_t<T, V>*>(p);
+#pragma unroll
+        for (int v = 0; v < V; ++v) out[v] = x[v];
+    }
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void vstore(T* __restrict__ p, const T (&in)[V])
+{
+    if constexpr (V == 1) {
+        *p = in[0];
+    } else {
+        vec_t<T, V> x;
+#pragma unroll
+        for (int v = 0; v < V; ++v) x[v] = in[v];
+        *reinterpret_cast<vec_t<T, V>*>(p) = x;
+    }
+}
+
+// Cells per thread of the streaming kernels: 16-byte vectors when the row
+// length allows them, else 1.
+template <typename T>
+constexpr int vec_width() { return 16 / (int)sizeof(T); }
+
+template <typename T>
+inline int pick_width(const Dims& d) { return d.nz % vec_width<T>() == 0 ? vec_width<T>() : 1; }
+
+// The seven-point neighbourhood of cells id .. id+V-1 of one row, each
+// loaded value passed through f (the first Chebyshev step scales r).
+template <typename T, int V>
+struct Nbr {
+    T c[V], w[V], e[V], s[V], n[V];
+    T km, kp;  // k-1 of the first cell, k+1 of the last
+};
+
+template <typename T, int V, typename F>
+__device__ __forceinline__ void load_nbr(const T* __restrict__ x, const Dims& d, int id, int i,
+                                         int j, int k, F f, Nbr<T, V>& nb)
+{
+    vload<T, V>(x + id, nb.c);
+#pragma unroll
+    for (int v = 0; v < V; ++v) nb.w[v] = nb.e[v] = nb.s[v] = nb.n[v] = T(0);
+    if (i > 0) vload<T, V>(x + id - d.sij, nb.w);
+    if (i < d.nx - 1) vload<T, V>(x + id + d.sij, nb.e);
+    if (j > 0) vload<T, V>(x + id - d.nz, nb.s);
+    if (j < d.ny - 1) vload<T, V>(x + id + d.nz, nb.n);
+    nb.km = k > 0 ? x[id - 1] : T(0);
+    nb.kp = k + V < d.nz ? x[id + V] : T(0);
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        nb.c[v] = f(nb.c[v]);
+        nb.w[v] = f(nb.w[v]);
+        nb.e[v] = f(nb.e[v]);
+        nb.s[v] = f(nb.s[v]);
+        nb.n[v] = f(nb.n[v]);
+    }
+    nb.km = f(nb.km);
+    nb.kp = f(nb.kp);
+}
+
+// q = A x at cells id .. id+V-1: per cell exactly apply7's operand order
+// and guards.
+template <int CODED, typename T, int V>
+__device__ __forceinline__ void apply7v(const Stencil<T>& A, const Dims& d, int id, int i, int j,
+                                        int k, const Nbr<T, V>& nb, T (&q)[V])
+{
+    T cc[V], cw[V], ce[V], cs[V], cn[V], cb[V], ct[V];
+    if (CODED == 1) {
+        uint16_t cd[V];
+        vload<uint16_t, V>(A.code + id, cd);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const unsigned c = cd[v];
+            cc[v] = (c & CODE_IDENTITY) ? T(1) : A.cfluid;
+            cw[v] = mult<T>(c & 3u, A.ox, A.ox2);
+            ce[v] = mult<T>((c >> 2) & 3u, A.ox, A.ox2);
+            cs[v] = mult<T>((c >> 4) & 3u, A.oy, A.oy2);
+            cn[v] = mult<T>((c >> 6) & 3u, A.oy, A.oy2);
+            cb[v] = mult<T>((c >> 8) & 3u, A.oz, A.oz2);
+            ct[v] = mult<T>((c >> 10) & 3u, A.oz, A.oz2);
+        }
+    } else {
+        vload<T, V>(A.a[SC] + id, cc);
+        vload<T, V>(A.a[SW] + id, cw);
+        vload<T, V>(A.a[SE] + id, ce);
+        vload<T, V>(A.a[SS] + id, cs);
+        vload<T, V>(A.a[SN] + id, cn);
+        vload<T, V>(A.a[SB] + id, cb);
+        vload<T, V>(A.a[ST] + id, ct);
+    }
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        T val = cc[v] * nb.c[v];
+        if (i > 0) val += cw[v] * nb.w[v];
+        if (i < d.nx - 1) val += ce[v] * nb.e[v];
+        if (j > 0) val += cs[v] * nb.s[v];
+        if (j < d.ny - 1) val += cn[v] * nb.n[v];
+        if (k + v > 0) val += cb[v] * (v == 0 ? nb.km : nb.c[v - 1]);
+        if (k + v < d.nz - 1) val += ct[v] * (v == V - 1 ? nb.kp : nb.c[v + 1]);
+        q[v] = val;
+    }
+}
+
 // ------------------------------------------------------------------
 // device-resident scalar epilogues
 // ------------------------------------------------------------------
@@ -334,7 +447,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(Dims d, Chunking ch, Stenci
     block_partial<2>(acc, red.partials, lb);
 }
 
-template <typename T, int CODED>
+template <typename T, int CODED, int V>
 __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Stencil<T> A,
                                                        const T* __restrict__ p,
                                                        T* __restrict__ q, RedScratch red,
@@ -344,17 +457,23 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Sten
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
     double acc[1] = {0.0};
-    FOR_CELLS(d, ch, lb)
+    FOR_VCELLS(d, ch, lb, V)
     {
-        const T qv = apply7<CODED>(A, d, id, i, j, k, [&](int n) { return p[n]; });
-        q[id] = qv;
-        const double pq = (double)p[id] * (double)qv;
-        acc[0] += wgt ? wgt[id] * pq : pq;
+        Nbr<T, V> nb;
+        load_nbr<T, V>(p, d, id, i, j, k, [](T x) { return x; }, nb);
+        T qv[V];
+        apply7v<CODED, T, V>(A, d, id, i, j, k, nb, qv);
+        vstore<T, V>(q + id, qv);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const double pq = (double)nb.c[v] * (double)qv[v];
+            acc[0] += wgt ? wgt[id + v] * pq : pq;
+        }
     }
     block_partial<1>(acc, red.partials, lb);
 }
 
-template <typename T, typename TX>
+template <typename T, typename TX, int V>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(Dims d, Chunking ch, TX* __restrict__ x,
                                                        T* __restrict__ r,
                                                        const T* __restrict__ p,
@@ -369,21 +488,36 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Dims d, Chunking ch, TX* 
     const TX ax = (TX)alpha;
     const T ar = (T)(-alpha);
     double acc[2] = {0.0, 0.0};
-    FOR_CELLS(d, ch, lb)
+    FOR_VCELLS(d, ch, lb, V)
     {
-        x[id] = x[id] + ax * (TX)p[id];
-        const T rv = r[id] + ar * q[id];
-        r[id] = rv;
-        acc[0] += (double)rv * (double)rv;
-        if (diag) {
-            const double rz = (double)rv * (double)(rv / diag[id]);
-            acc[1] += wgt ? wgt[id] * rz : rz;
+        TX xv[V];
+        T rv[V], pv[V], qv[V];
+        vload<TX, V>(x + id, xv);
+        vload<T, V>(r + id, rv);
+        vload<T, V>(p + id, pv);
+        vload<T, V>(q + id, qv);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            xv[v] = xv[v] + ax * (TX)pv[v];
+            rv[v] = rv[v] + ar * qv[v];
+        }
+        vstore<TX, V>(x + id, xv);
+        vstore<T, V>(r + id, rv);
+        T dv[V];
+        if (diag) vload<T, V>(diag + id, dv);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            acc[0] += (double)rv[v] * (double)rv[v];
+            if (diag) {
+                const double rz = (double)rv[v] * (double)(rv[v] / dv[v]);
+                acc[1] += wgt ? wgt[id + v] * rz : rz;
+            }
         }
     }
     block_partial<2>(acc, red.partials, lb);
 }
 
-template <typename T>
+template <typename T, int V>
 __global__ __launch_bounds__(kBlock) void k_pcg_p_jacobi(Dims d, Chunking ch,
                                                          const T* __restrict__ r,
                                                          const T* __restrict__ diag,
@@ -396,22 +530,37 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p_jacobi(Dims d, Chunking ch,
     if (init) {
         // z = r / A_c into p (pressure.cpp:420-428), r.z partials
         double acc[1] = {0.0};
-        FOR_CELLS(d, ch, lb)
+        FOR_VCELLS(d, ch, lb, V)
         {
-            const T rv = r[id];
-            const T z = rv / diag[id];
-            p[id] = z;
-            const double rz = (double)rv * (double)z;
-            acc[0] += wgt ? wgt[id] * rz : rz;
+            T rv[V], dv[V], z[V];
+            vload<T, V>(r + id, rv);
+            vload<T, V>(diag + id, dv);
+#pragma unroll
+            for (int v = 0; v < V; ++v) z[v] = rv[v] / dv[v];
+            vstore<T, V>(p + id, z);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const double rz = (double)rv[v] * (double)z[v];
+                acc[0] += wgt ? wgt[id + v] * rz : rz;
+            }
         }
         if (red.partials) block_partial<1>(acc, red.partials, lb);
     } else {
         const T beta = (T)sc->beta;
-        FOR_CELLS(d, ch, lb) p[id] = r[id] / diag[id] + beta * p[id];
+        FOR_VCELLS(d, ch, lb, V)
+        {
+            T rv[V], dv[V], pv[V];
+            vload<T, V>(r + id, rv);
+            vload<T, V>(diag + id, dv);
+            vload<T, V>(p + id, pv);
+#pragma unroll
+            for (int v = 0; v < V; ++v) pv[v] = rv[v] / dv[v] + beta * pv[v];
+            vstore<T, V>(p + id, pv);
+        }
     }
 }
 
-template <typename T>
+template <typename T, int V>
 __global__ __launch_bounds__(kBlock) void k_pcg_p_update(Dims d, Chunking ch,
                                                          const T* __restrict__ z,
                                                          T* __restrict__ p, const SolveScalars* sc)
@@ -419,10 +568,18 @@ __global__ __launch_bounds__(kBlock) void k_pcg_p_update(Dims d, Chunking ch,
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
     const T beta = (T)sc->beta;
-    FOR_CELLS(d, ch, lb) p[id] = z[id] + beta * p[id];
+    FOR_VCELLS(d, ch, lb, V)
+    {
+        T zv[V], pv[V];
+        vload<T, V>(z + id, zv);
+        vload<T, V>(p + id, pv);
+#pragma unroll
+        for (int v = 0; v < V; ++v) pv[v] = zv[v] + beta * pv[v];
+        vstore<T, V>(p + id, pv);
+    }
 }
 
-template <typename T>
+template <typename T, int V>
 __global__ __launch_bounds__(kBlock) void k_pcg_rz(Dims d, Chunking ch, const T* __restrict__ r,
                                                    const T* __restrict__ z, RedScratch red,
                                                    const SolveScalars* sc,
@@ -431,10 +588,16 @@ __global__ __launch_bounds__(kBlock) void k_pcg_rz(Dims d, Chunking ch, const T*
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
     double acc[1] = {0.0};
-    FOR_CELLS(d, ch, lb)
+    FOR_VCELLS(d, ch, lb, V)
     {
-        const double rz = (double)r[id] * (double)z[id];
-        acc[0] += wgt ? wgt[id] * rz : rz;
+        T rv[V], zv[V];
+        vload<T, V>(r + id, rv);
+        vload<T, V>(z + id, zv);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const double rz = (double)rv[v] * (double)zv[v];
+            acc[0] += wgt ? wgt[id + v] * rz : rz;
+        }
     }
     block_partial<1>(acc, red.partials, lb);
 }
@@ -444,40 +607,53 @@ __global__ __launch_bounds__(kBlock) void k_pcg_rz(Dims d, Chunking ch, const T*
 //   res = r - A z_k ; p = res + beta p ; z_{k+1} = z_k + alpha p
 // FIRST forms z_1 = 0 + alpha0 r and p_0 = r on the fly from r.
 // ------------------------------------------------------------------
-template <typename T, bool FIRST, int CODED, bool REDUCE>
+template <typename T, bool FIRST, int CODED, bool REDUCE, int V>
 __global__ __launch_bounds__(kBlock) void k_cheb_step(Dims d, Chunking ch, Stencil<T> A,
-                                                      const T* __restrict__ r,
-                                                      const T* __restrict__ z_in,
-                                                      T* __restrict__ z_out, T* __restrict__ p,
-                                                      double alpha0, double beta, double alpha,
-                                                      RedScratch red, const SolveScalars* sc,
-                                                      const double* __restrict__ wgt)
+                                                        const T* __restrict__ r,
+                                                        const T* __restrict__ z_in,
+                                                        T* __restrict__ z_out, T* __restrict__ p,
+                                                        double alpha0, double beta, double alpha,
+                                                        RedScratch red, const SolveScalars* sc,
+                                                        const double* __restrict__ wgt)
 {
     if (is_done(sc)) return;
     const int lb = logical_block(ch.grid);
     const T a0 = (T)alpha0, bt = (T)beta, at = (T)alpha;
     double acc[1] = {0.0};
-    FOR_CELLS(d, ch, lb)
+    FOR_VCELLS(d, ch, lb, V)
     {
-        T az, zc, pp;
-        const T rv = r[id];
+        Nbr<T, V> nb;
+        T rv[V], az[V], pp[V], zc[V];
+        vload<T, V>(r + id, rv);
         if (FIRST) {
-            az = apply7<CODED>(A, d, id, i, j, k, [&](int n) { return T(0) + a0 * r[n]; });
-            zc = T(0) + a0 * rv;
-            pp = rv;
+            load_nbr<T, V>(r, d, id, i, j, k, [&](T x) { return T(0) + a0 * x; }, nb);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                zc[v] = nb.c[v];
+                pp[v] = rv[v];
+            }
         } else {
-            az = apply7<CODED>(A, d, id, i, j, k, [&](int n) { return z_in[n]; });
-            zc = z_in[id];
-            pp = p[id];
+            load_nbr<T, V>(z_in, d, id, i, j, k, [](T x) { return x; }, nb);
+            vload<T, V>(p + id, pp);
+#pragma unroll
+            for (int v = 0; v < V; ++v) zc[v] = nb.c[v];
         }
-        const T res = rv - az;
-        const T pn = res + bt * pp;
-        p[id] = pn;
-        const T zn = zc + at * pn;
-        z_out[id] = zn;
+        apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az);
+        T pn[V], zn[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const T res = rv[v] - az[v];
+            pn[v] = res + bt * pp[v];
+            zn[v] = zc[v] + at * pn[v];
+        }
+        vstore<T, V>(p + id, pn);
+        vstore<T, V>(z_out + id, zn);
         if (REDUCE) {
-            const double rz = (double)rv * (double)zn;
-            acc[0] += wgt ? wgt[id] * rz : rz;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const double rz = (double)rv[v] * (double)zn[v];
+                acc[0] += wgt ? wgt[id + v] * rz : rz;
+            }
         }
     }
     if (REDUCE) block_partial<1>(acc, red.partials, lb);
@@ -544,6 +720,7 @@ __global__ __launch_bounds__(kBlock) void k_mixed_residual(Dims d, Chunking ch, 
     block_partial<2>(acc, red.partials, lb);
 }
 
+template <int V>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_mixed(Dims d, Chunking ch,
                                                              float* __restrict__ x,
                                                              float* __restrict__ r,
@@ -557,12 +734,21 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_mixed(Dims d, Chunking ch
     const float ax = (float)sc->alpha;
     const float ar = (float)(-sc->alpha);
     double acc[1] = {0.0};
-    FOR_CELLS(d, ch, lb)
+    FOR_VCELLS(d, ch, lb, V)
     {
-        x[id] = x[id] + ax * p[id];
-        const float rv = r[id] + ar * q[id];
-        r[id] = rv;
-        acc[0] += (double)rv * (double)rv;
+        float xv[V], rv[V], pv[V], qv[V];
+        vload<float, V>(x + id, xv);
+        vload<float, V>(r + id, rv);
+        vload<float, V>(p + id, pv);
+        vload<float, V>(q + id, qv);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            xv[v] = xv[v] + ax * pv[v];
+            rv[v] = rv[v] + ar * qv[v];
+            acc[0] += (double)rv[v] * (double)rv[v];
+        }
+        vstore<float, V>(x + id, xv);
+        vstore<float, V>(r + id, rv);
     }
     block_partial<1>(acc, red.partials, lb);
 }
@@ -793,13 +979,20 @@ template <typename T>
 hipError_t launch_pcg_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T* p, T* q,
                              RedScratch red, SolveScalars* sc, const double* wgt)
 {
-    const Chunking ch = make_chunking(d);
-    if (A.code)
-        hipLaunchKernelGGL((k_pcg_matvec<T, 1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, p,
-                           q, red, sc, wgt);
-    else
-        hipLaunchKernelGGL((k_pcg_matvec<T, 0>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, p,
-                           q, red, sc, wgt);
+    constexpr int W = vec_width<T>();
+    const int V = pick_width<T>(d);
+    const Chunking ch = make_chunking(d, V);
+#define LES_MV(CD, VV)                                                                           \
+    hipLaunchKernelGGL((k_pcg_matvec<T, CD, VV>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, p, \
+                       q, red, sc, wgt)
+    if (A.code) {
+        if (V == W) LES_MV(1, W);
+        else LES_MV(1, 1);
+    } else {
+        if (V == W) LES_MV(0, W);
+        else LES_MV(0, 1);
+    }
+#undef LES_MV
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     return launch_finalize(s, ch.grid, 1, false, EPI_ALPHA, GATE_DONE, red, sc, nullptr);
 }
@@ -809,9 +1002,16 @@ hipError_t launch_pcg_update(hipStream_t s, const Dims& d, TX* x, T* r, const T*
                              const T* jacobi_diag, RedScratch red, SolveScalars* sc,
                              const double* wgt)
 {
-    const Chunking ch = make_chunking(d);
-    hipLaunchKernelGGL((k_pcg_update<T, TX>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, r, p,
-                       q, jacobi_diag, red, sc, wgt);
+    static_assert(sizeof(T) == sizeof(TX), "x and r share the vector width");
+    constexpr int W = vec_width<T>();
+    const int V = pick_width<T>(d);
+    const Chunking ch = make_chunking(d, V);
+    if (V == W)
+        hipLaunchKernelGGL((k_pcg_update<T, TX, W>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x,
+                           r, p, q, jacobi_diag, red, sc, wgt);
+    else
+        hipLaunchKernelGGL((k_pcg_update<T, TX, 1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x,
+                           r, p, q, jacobi_diag, red, sc, wgt);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     return launch_finalize(s, ch.grid, 2, false, jacobi_diag ? EPI_CONV_RZ : EPI_CONV, GATE_DONE,
                            red, sc, nullptr);
@@ -822,11 +1022,17 @@ hipError_t launch_pcg_p_jacobi(hipStream_t s, const Dims& d, const T* r, const T
                                int init, int epi, RedScratch red, SolveScalars* sc,
                                const double* wgt)
 {
-    const Chunking ch = make_chunking(d);
+    constexpr int W = vec_width<T>();
+    const int V = pick_width<T>(d);
+    const Chunking ch = make_chunking(d, V);
     RedScratch rd = red;
     if (!(init && epi >= 0)) rd.partials = nullptr;
-    hipLaunchKernelGGL(k_pcg_p_jacobi<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r, diag, p,
-                       init, rd, sc, wgt);
+    if (V == W)
+        hipLaunchKernelGGL((k_pcg_p_jacobi<T, W>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r,
+                           diag, p, init, rd, sc, wgt);
+    else
+        hipLaunchKernelGGL((k_pcg_p_jacobi<T, 1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r,
+                           diag, p, init, rd, sc, wgt);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     if (init && epi >= 0)
         return launch_finalize(s, ch.grid, 1, false, epi, sc ? GATE_DONE : GATE_NONE, red, sc,
@@ -837,8 +1043,15 @@ hipError_t launch_pcg_p_jacobi(hipStream_t s, const Dims& d, const T* r, const T
 template <typename T>
 hipError_t launch_pcg_p_update(hipStream_t s, const Dims& d, const T* z, T* p, SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d);
-    hipLaunchKernelGGL(k_pcg_p_update<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, z, p, sc);
+    constexpr int W = vec_width<T>();
+    const int V = pick_width<T>(d);
+    const Chunking ch = make_chunking(d, V);
+    if (V == W)
+        hipLaunchKernelGGL((k_pcg_p_update<T, W>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, z, p,
+                           sc);
+    else
+        hipLaunchKernelGGL((k_pcg_p_update<T, 1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, z, p,
+                           sc);
     return hipGetLastError();
 }
 
@@ -846,9 +1059,15 @@ template <typename T>
 hipError_t launch_pcg_rz(hipStream_t s, const Dims& d, const T* r, const T* z, int epilogue,
                          RedScratch red, SolveScalars* sc, const double* wgt)
 {
-    const Chunking ch = make_chunking(d);
-    hipLaunchKernelGGL(k_pcg_rz<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r, z, red, sc,
-                       wgt);
+    constexpr int W = vec_width<T>();
+    const int V = pick_width<T>(d);
+    const Chunking ch = make_chunking(d, V);
+    if (V == W)
+        hipLaunchKernelGGL((k_pcg_rz<T, W>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r, z, red,
+                           sc, wgt);
+    else
+        hipLaunchKernelGGL((k_pcg_rz<T, 1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, r, z, red,
+                           sc, wgt);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     return launch_finalize(s, ch.grid, 1, false, epilogue, GATE_DONE, red, sc, nullptr);
 }
@@ -859,10 +1078,19 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
                             double alpha, int first, int epi, RedScratch red, SolveScalars* sc,
                             const double* wgt)
 {
-    const Chunking ch = make_chunking(d);
+    constexpr int VW = vec_width<T>();
+    const bool vec = pick_width<T>(d) == VW;
+    const Chunking ch = make_chunking(d, vec ? VW : 1);
 #define LES_CHEB(F, CD, RD)                                                                     \
-    hipLaunchKernelGGL((k_cheb_step<T, F, CD, RD>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, \
-                       r, z_in, z_out, p, alpha0, beta, alpha, red, sc, wgt)
+    do {                                                                                        \
+        if (vec)                                                                                \
+            hipLaunchKernelGGL((k_cheb_step<T, F, CD, RD, VW>), dim3(ch.grid), dim3(kBlock), 0,   \
+                               s, d, ch, A, r, z_in, z_out, p, alpha0, beta, alpha, red, sc,    \
+                               wgt);                                                            \
+        else                                                                                    \
+            hipLaunchKernelGGL((k_cheb_step<T, F, CD, RD, 1>), dim3(ch.grid), dim3(kBlock), 0, s, \
+                               d, ch, A, r, z_in, z_out, p, alpha0, beta, alpha, red, sc, wgt); \
+    } while (0)
 #define LES_CHEB_R(F, CD)                   \
     do {                                    \
         if (epi >= 0) LES_CHEB(F, CD, true);  \
@@ -956,9 +1184,15 @@ hipError_t launch_mixed_residual(hipStream_t s, const Dims& d, Stencil<double> A
 hipError_t launch_pcg_update_mixed(hipStream_t s, const Dims& d, float* x, float* r, const float* p,
                                    const float* q, RedScratch red, SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d);
-    hipLaunchKernelGGL(k_pcg_update_mixed, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, r, p, q,
-                       red, sc);
+    constexpr int W = vec_width<float>();
+    const int V = pick_width<float>(d);
+    const Chunking ch = make_chunking(d, V);
+    if (V == W)
+        hipLaunchKernelGGL((k_pcg_update_mixed<W>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, r,
+                           p, q, red, sc);
+    else
+        hipLaunchKernelGGL((k_pcg_update_mixed<1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, r,
+                           p, q, red, sc);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     return launch_finalize(s, ch.grid, 1, false, EPI_CONV_MIXED, GATE_DONE, red, sc, nullptr);
 }

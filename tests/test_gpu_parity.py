@@ -17,7 +17,9 @@ from cpu_checkers import Oracle, initial_state, PC_NONE, PC_JACOBI, PC_IC0, PC_L
 
 pytestmark = pytest.mark.gpu
 
-SHAPES = [(7, 6, 5, 1), (13, 9, 17, 2), (33, 20, 70, 3)]
+# nz = 24 and 1032 take the vector-load kernels (rows of V = 16 bytes per
+# thread), 1032 with more threads per row than a workgroup has.
+SHAPES = [(7, 6, 5, 1), (13, 9, 17, 2), (33, 20, 70, 3), (9, 7, 24, 4), (4, 3, 1032, 5)]
 FP64_REL_TOL = 1e-10   # fields after PCG / whole steps vs the oracle
 
 
@@ -321,6 +323,30 @@ def test_mixed_precision_pcg(orc, pc):
     true_res = np.linalg.norm(b - orc.matvec(x, A, m))
     assert true_res < 1.01e-8 * np.linalg.norm(b)
     assert rel_err(x, x_ref) < 1e-5
+
+
+@pytest.mark.parametrize("shape", [(5, 4, 1032, 6), (10, 9, 36, 7)])
+def test_mixed_precision_vector_rows(orc, shape):
+    """Mixed-precision Chebyshev PCG on rows that take the 4-wide FP32
+    vector kernels (nz % 4 == 0), one with rows longer than a workgroup."""
+    m, solid, f = random_case(*shape, frac=0.05)
+    A = orc.build_stencil(solid, m)
+    lmax = orc.lambda_max(A, m, 20)
+    b = orc.build_b(orc.divergence(f[0], f[1], f[2], solid, m, 1e-3, 1.2), solid, m)
+    kw = dict(precond=PC_CHEBYSHEV, lmin=3 * np.pi ** 2, lmax=lmax, degree=8)
+    x_ref, it_ref, _ = orc.pcg(A, b, np.zeros_like(b), m, PC_CHEBYSHEV, max_iter=5000,
+                               tol=1e-13 * np.linalg.norm(b), lmin=kw["lmin"], lmax=lmax, degree=8)
+    for fmt in (lesgpu.STENCIL_ARRAYS, lesgpu.STENCIL_CODED):
+        with ctx_for(m, solid) as g:
+            g.build_stencil()
+            g.set_field("b", b)
+            g.set_field("p", np.zeros_like(b))
+            res = g.pcg(max_iter=5000, tol=1e-8, tol_rel=True, precision=lesgpu.PREC_MIXED,
+                        reliable_every=50, stencil_format=fmt, **kw)
+            x = g.get_field("p")
+        assert res["converged"] == 1, res
+        assert np.linalg.norm(b - orc.matvec(x, A, m)) < 1.01e-8 * np.linalg.norm(b)
+        assert rel_err(x, x_ref) < 1e-5
 
 
 def test_lanczos_closed_form():
