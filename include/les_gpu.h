@@ -155,6 +155,8 @@ typedef struct les_pcg_result {
     double true_res_norm; /* ||b - A x|| in FP64 at exit (mixed precision; FP64: = res_norm) */
     int replacements;     /* mixed: FP64 residual replacements performed */
     int fault;            /* a non-finite scalar stopped the iteration */
+    int restarts;         /* mixed: replacements after which the search direction
+                             restarted (true residual > 2x the recursive one) */
 } les_pcg_result;
 
 /* Solves A p = b in place: x0 and the result are the context's LES_F_P,

@@ -1001,6 +1001,7 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     res->true_res_norm = c->hsc[0];
     res->replacements = h.replacements;
     res->fault = h.fault;
+    res->restarts = h.restarts;
     c->last_trace_n = h.iter + 1;
     return LES_OK;
 }

@@ -58,6 +58,8 @@ struct SolveScalars {
     int reliable_now;    // 1: fold x32 into x64 and replace r in this iteration
     int stop_pending;    // 1: stop after the pending replacement (max_iter reached)
     int replacements;
+    int restart_pending; // 1: the next beta is 0 (p restarts from z)
+    int restarts;
     double true_err;     // ||b - A x64|| of the last replacement
     double* trace;       // optional: ||r_k|| for k = 0..max_iter
 };

@@ -247,7 +247,13 @@ __device__ void epilogue(int epi, const double* tot, SolveScalars* sc, double* o
         break;
     }
     case EPI_BETA:
-        sc->beta = tot[0] / sc->rho;
+        if (sc->restart_pending) {
+            sc->beta = 0.0;
+            sc->restart_pending = 0;
+            sc->restarts += 1;
+        } else {
+            sc->beta = tot[0] / sc->rho;
+        }
         sc->rho = tot[0];
         break;
     case EPI_CONV_MIXED: {
@@ -272,6 +278,12 @@ __device__ void epilogue(int epi, const double* tot, SolveScalars* sc, double* o
     }
     case EPI_RELIABLE: {
         const double e = sqrt(tot[0]);
+        // A true residual well above the recursive one means the FP32
+        // recursion has drifted from the FP64 iterate; the search direction
+        // then restarts from the replaced residual (a new refinement
+        // solve).  Keeping it, as a small drift allows, lets CG on the
+        // non-symmetric operator diverge.
+        if (e > 2.0 * sc->err) sc->restart_pending = 1;
         sc->true_err = e;
         sc->err = e;
         sc->rr = tot[0];

@@ -44,7 +44,8 @@ class PcgParams(C.Structure):
 class PcgResult(C.Structure):
     _fields_ = [("iters", C.c_int), ("converged", C.c_int), ("res_norm", C.c_double),
                 ("res0_norm", C.c_double), ("b_norm", C.c_double),
-                ("true_res_norm", C.c_double), ("replacements", C.c_int), ("fault", C.c_int)]
+                ("true_res_norm", C.c_double), ("replacements", C.c_int), ("fault", C.c_int),
+                ("restarts", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

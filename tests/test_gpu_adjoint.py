@@ -13,9 +13,8 @@ les_adjoint and the torch.autograd.Function over them).
   checkpoints every 5 steps, loss = kinetic energy over a seeded 16^3
   sub-box at step 20, gradient with respect to the post-BC initial
   velocity through torch autograd, checked against central differences
-  (eps = 1e-6) of the GPU forward on 64 seeded unit directions to 1e-4
-  relative, the perturbed runs replaying the unperturbed run's pressure
-  iteration counts.
+  (eps = 1e-6) of the GPU forward on 64 seeded directions of unit RMS per
+  cell to 1e-4 relative.
 """
 import numpy as np
 import pytest
@@ -153,54 +152,22 @@ def test_autograd_gradient_vs_finite_differences_96():
     grad = [x.grad.detach() for x in X]
     assert all(torch.isfinite(t).all() for t in grad)
 
-    # Central differences of the same forward map with each step's pressure
-    # iteration count held at the unperturbed run's (a tolerance-terminated
-    # solve stopping one iteration earlier or later in the perturbed runs
-    # would add noise of order tol/eps to the differences).
-    zeros = np.zeros(N)
-    pcg = dict(precond=lesgpu.PC_JACOBI, inner_product=lesgpu.IP_MIRROR)
-
-    def forward(U, iters=None):
-        torch.cuda.current_stream().synchronize()   # U is handed to the library's stream
-        for name, t in zip("uvw", U):
-            g.set_field(name, t.contiguous())
-        counts = []
-        for s in range(nsteps):
-            g.set_field("p", zeros)
-            if iters is None:
-                r = g.step(step=s, dt_fixed=1e-3, nu=1e-3, rho=1.2, Cs=0.17, max_iter=20000,
-                           tol=1e-13, tol_rel=True, **pcg)
-                assert r["converged"] == 1, r
-            else:
-                r = g.step(step=s, dt_fixed=1e-3, nu=1e-3, rho=1.2, Cs=0.17, max_iter=iters[s],
-                           tol=0.0, tol_rel=False, **pcg)
-            counts.append(r["iters"])
-        out = [torch.empty_like(U[0]) for _ in range(3)]
-        for name, t in zip("uvw", out):
-            g.get_field(name, t)
-        g.synchronize()
-        return out, counts
-
+    # Central differences of the GPU forward (the same rollout, pressure
+    # solved to 1e-13 relative) along seeded directions of unit RMS per cell:
+    # each velocity value moves by about eps, so the differences resolve the
+    # map far above the solves' tolerance.
+    eps = 1e-6
+    drng = np.random.default_rng(64)
+    errs = []
     with torch.no_grad():
-        torch.cuda.synchronize()
-        base, iters = forward(U0)
-        # the step-by-step forward is the rollout the gradient was taken of
-        ref = rollout(*U0, g, prm)
-        assert all(torch.equal(a, b) for a, b in zip(base, ref))
-        eps = 1e-6
-        drng = np.random.default_rng(64)
-        errs = []
         for _ in range(64):
             d = [torch.from_numpy(drng.standard_normal(N)).to(dev) for _ in range(3)]
-            nrm = torch.sqrt(sum(torch.sum(t * t) for t in d))
-            d = [t / nrm for t in d]
-            lp = loss_of(*forward([a + eps * b for a, b in zip(U0, d)], iters)[0]).item()
-            lm = loss_of(*forward([a - eps * b for a, b in zip(U0, d)], iters)[0]).item()
+            lp = loss_of(*rollout(*[a + eps * b for a, b in zip(U0, d)], g, prm)).item()
+            lm = loss_of(*rollout(*[a - eps * b for a, b in zip(U0, d)], g, prm)).item()
             fd = (lp - lm) / (2 * eps)
             ad = sum(torch.sum(a * b) for a, b in zip(grad, d)).item()
             errs.append(abs(fd - ad) / abs(fd))
-    print(f"[adjoint 96^3] loss {L.item():.6e}, PCG iterations/step {iters}; FD vs adjoint "
-          f"rel err over 64 directions: max {max(errs):.2e}, median {float(np.median(errs)):.2e}",
-          flush=True)
+    print(f"[adjoint 96^3] loss {L.item():.6e}; FD vs adjoint rel err over 64 directions: "
+          f"max {max(errs):.2e}, median {float(np.median(errs)):.2e}", flush=True)
     g.close()
     assert max(errs) <= 1e-4, errs
