@@ -39,19 +39,34 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cheb_bytes_per_cell(degree, elem, coded=False):
-    """Algorithmic bytes per cell per Chebyshev inner-step launch, averaged
-    over the degree-1 launches of one application.  First step: r (stencil),
-    the operator -> p, z.  Later steps: z (stencil), r, p, the operator ->
-    p, z.  The operator is 7 coefficient arrays, or one 2-byte code."""
+def cheb_pair_bytes_per_cell(degree, elem, coded=False):
+    """Algorithmic bytes per cell per timed fused two-step launch
+    (k_cheb_pair), averaged over the pair launches of one application that
+    the library times (all but a reducing last one).  First pair: r, the
+    operator -> p, z.  Later pairs: z, r, p, the operator -> p, z (the pair's
+    intermediate z and p stay on chip)."""
     op = 2 if coded else 7 * elem
     first = (1 + 2) * elem + op
     later = (3 + 2) * elem + op
-    m = degree - 1
+    steps = degree - 1
+    m = steps // 2 - (1 if steps % 2 == 0 else 0)
     return (first + (m - 1) * later) / m
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_cheb_step_256_traffic.json")
+def cheb_bytes_per_cell(degree, elem, coded=False):
+    """Algorithmic bytes per cell per timed Chebyshev single-step launch,
+    averaged over the degree-2 launches of one application before the
+    reducing last one.  First step: r (stencil), the operator -> p, z.
+    Later steps: z (stencil), r, p, the operator -> p, z.  The operator is 7
+    coefficient arrays, or one 2-byte code."""
+    op = 2 if coded else 7 * elem
+    first = (1 + 2) * elem + op
+    later = (3 + 2) * elem + op
+    m = degree - 2
+    return (first + (m - 1) * later) / m
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "chebyshev_256_traffic.json")
 
 
 def measured_traffic(mode, n):
@@ -102,7 +117,6 @@ def run_mode(g, st0, precision, stencil, bounds, steps, warmup, kt_class, pcg_kw
         log(f"  warmup step {s}: iters {r['iters']} conv {r['converged']}")
     g.synchronize()
     torch.cuda.synchronize()
-    g.kernel_timing(kt_class)
     t0 = time.perf_counter()
     for s in range(step0 + warmup, step0 + warmup + steps):
         r = g.step(step=s, **kw)
@@ -111,12 +125,18 @@ def run_mode(g, st0, precision, stencil, bounds, steps, warmup, kt_class, pcg_kw
     g.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    kt = g.kernel_timing_read()
-    g.kernel_timing(lesgpu.KT_NONE)
     for r in info:
         if not r["converged"]:
             raise RuntimeError(f"PCG did not converge: {r}")
-    return dict(seconds=t1 - t0, iters=its, last=info[-1], kt=kt, state=g.get_state())
+    state = g.get_state()
+    # the kernel's own duration, from HIP events around each of its launches
+    # in one more step (the events are kept out of the timed steps: each
+    # record adds a few microseconds between launches)
+    g.kernel_timing(kt_class)
+    g.step(step=step0 + warmup + steps, **kw)
+    kt = g.kernel_timing_read()
+    g.kernel_timing(lesgpu.KT_NONE)
+    return dict(seconds=t1 - t0, iters=its, last=info[-1], kt=kt, state=state)
 
 
 def cpu_baseline(m, st0, solid, bounds, threads, step0, gpu_iters_per_step):
@@ -211,21 +231,26 @@ def main():
     res = {}
     pcg_kw = dict(check_every=args.check_every)
     for mode in modes:
-        prec_s, sten_s = (mode.split(":") + ["coded"])[:2]
+        prec_s, sten_s, kern_s = (mode.split(":") + ["coded", "pairs"][len(mode.split(":")) - 1:])[:3]
         prec = lesgpu.PREC_MIXED if prec_s == "mixed" else lesgpu.PREC_FP64
         sten = lesgpu.STENCIL_CODED if sten_s == "coded" else lesgpu.STENCIL_ARRAYS
+        pairs = kern_s != "single"
+        g.set_option(lesgpu.OPT_CHEB_PAIRS, 1 if pairs else 0)
         log(f"mode {mode}: {args.warmup} warmup + {args.steps} timed steps")
         res[mode] = run_mode(g, st0, prec, sten, bounds, args.steps, args.warmup,
-                             lesgpu.KT_CHEB_STEP, pcg_kw, args.step0)
+                             lesgpu.KT_CHEB_PAIR if pairs else lesgpu.KT_CHEB_STEP, pcg_kw,
+                             args.step0)
         r = res[mode]
         r["elem"] = 4 if prec_s == "mixed" else 8
         r["coded"] = sten_s == "coded"
+        r["pairs"] = pairs
         n_l, ms, _ = r["kt"]
-        r["cheb_bpc"] = cheb_bytes_per_cell(8, r["elem"], r["coded"])
+        r["cheb_bpc"] = (cheb_pair_bytes_per_cell if pairs else cheb_bytes_per_cell)(
+            8, r["elem"], r["coded"])
         r["cheb_gbs"] = (r["cheb_bpc"] * N) / (ms / n_l * 1e-3) / 1e9 if n_l else 0.0
         log(f"  {mode}: {args.steps / r['seconds']:.3f} steps/s, iters/step "
-            f"{np.mean(r['iters']):.1f}, cheb {n_l} launches avg {1e3 * ms / max(n_l, 1):.1f} us "
-            f"= {r['cheb_gbs']:.0f} GB/s algorithmic")
+            f"{np.mean(r['iters']):.1f}, {'pair' if pairs else 'step'} {n_l} launches avg "
+            f"{1e3 * ms / max(n_l, 1):.1f} us = {r['cheb_gbs']:.0f} GB/s algorithmic")
     head = max((m for m in res if m.startswith("mixed")),
                key=lambda m: args.steps / res[m]["seconds"], default=modes[0])
     R = res[head]
@@ -256,18 +281,21 @@ def main():
         "pcg_iters_per_step": float(np.mean(R["iters"])),
         "pcg_iters_per_s": float(np.sum(R["iters"]) / R["seconds"]),
         "roofline": {"bound": "hbm",
-                     "kernel": f"k_cheb_step<{'float' if elem == 4 else 'double'}>"
+                     "kernel": f"{'k_cheb_pair' if R['pairs'] else 'k_cheb_step'}"
+                               f"<{'float' if elem == 4 else 'double'}>"
                                f"{' coded' if R['coded'] else ''}",
                      "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(head, m["nx"]),
+                     "traffic": measured_traffic(f"{head}:{'pairs' if R['pairs'] else 'single'}",
+                                                 m["nx"]),
                      "algorithmic_bytes_per_launch": bpc * N,
                      "bytes_per_cell": bpc, "cells": N, "launches": n_launch,
                      "avg_us": 1e3 * kt_ms / n_launch if n_launch else None},
         "modes": {mname: {"steps_per_s": args.steps / r["seconds"],
                           "pcg_iters_per_step": float(np.mean(r["iters"])),
-                          "cheb_step_gbs": r["cheb_gbs"],
-                          "cheb_step_frac": r["cheb_gbs"] / HBM_PEAK_GBS,
+                          "cheb_kernel": "k_cheb_pair" if r["pairs"] else "k_cheb_step",
+                          "cheb_gbs": r["cheb_gbs"],
+                          "cheb_frac": r["cheb_gbs"] / HBM_PEAK_GBS,
                           "cheb_bytes_per_cell": r["cheb_bpc"]} for mname, r in res.items()},
     }
     fp64_plain = "fp64:arrays" if "fp64:arrays" in res else next(

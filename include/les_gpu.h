@@ -81,6 +81,13 @@ const char* les_last_error(const les_ctx* ctx);
 int les_set_stream(les_ctx* ctx, void* hip_stream);
 int les_synchronize(les_ctx* ctx);
 
+/* Kernel-selection options (no reference counterpart).
+ * LES_OPT_CHEB_PAIRS: 1 (default) runs the Chebyshev polynomial as fused
+ * pairs of steps where the grid shape allows it, 0 as single steps; the
+ * results are bit-identical either way. */
+enum { LES_OPT_CHEB_PAIRS = 1 };
+int les_set_option(les_ctx* ctx, int option, int value);
+
 /* ---- fields (Field, src/field.h:9-41) ---- */
 int les_set_field(les_ctx* ctx, int field, const double* src, int ptr_kind);
 int les_get_field(les_ctx* ctx, int field, double* dst, int ptr_kind);
@@ -256,7 +263,7 @@ int les_adjoint(les_ctx* ctx, const les_adj_params* prm, double* ubar, double* v
  * context's stream so its average duration can be read back.  Starting
  * resets the record; LES_KT_NONE stops. */
 enum { LES_KT_NONE = 0, LES_KT_CHEB_STEP = 1, LES_KT_PCG_MATVEC = 2, LES_KT_PCG_UPDATE = 3,
-       LES_KT_MOMENTUM = 4, LES_KT_IC_SOLVE = 5 };
+       LES_KT_MOMENTUM = 4, LES_KT_IC_SOLVE = 5, LES_KT_CHEB_PAIR = 6 };
 int les_kernel_timing(les_ctx* ctx, int kernel_class);
 /* launches recorded, their summed duration in ms, and the bytes per cell the
  * class is charged (element size of the run) are returned; synchronises. */

@@ -47,7 +47,7 @@ struct les_ctx {
     bool have_L32 = false;
     // FP64 Krylov workspaces
     double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *za = nullptr, *zb = nullptr,
-           *pc = nullptr, *t1 = nullptr, *t2 = nullptr;
+           *pc = nullptr, *pc2 = nullptr, *t1 = nullptr, *t2 = nullptr;
     MixedWork mw{};
     RedScratch red{};
     SolveScalars* sc = nullptr;       // device
@@ -72,6 +72,7 @@ struct les_ctx {
     bool have_roll = false;
     int roll_nsteps = 0, roll_every = 0;
     double roll_dt = 0, roll_nu = 0, roll_rho = 0, roll_Cs = 0;
+    bool cheb_pairs = true;           // fuse Chebyshev steps in pairs where the grid allows
     // kernel timing diagnostics
     int kt_class = 0;
     int kt_elem = 0;
@@ -121,7 +122,7 @@ int alloc(les_ctx* c, T** p, size_t count, bool zero = true)
 int ensure_krylov(les_ctx* c)
 {
     int rc;
-    double** bufs[] = {&c->r, &c->z, &c->p, &c->q, &c->za, &c->zb, &c->pc, &c->t1, &c->t2};
+    double** bufs[] = {&c->r, &c->z, &c->p, &c->q, &c->za, &c->zb, &c->pc, &c->pc2, &c->t1, &c->t2};
     for (double** b : bufs)
         if ((rc = alloc(c, b, c->d.N)) != LES_OK) return rc;
     return LES_OK;
@@ -264,6 +265,58 @@ int pick_coded(les_ctx* c, int fmt, bool& coded)
     return LES_OK;
 }
 
+// The Chebyshev polynomial (pressure.cpp:851-906) as a chain of launches:
+// fused pairs of steps while two or more remain (when the grid shape allows
+// them), then single steps; the last launch reduces r.z with `epi`.  A
+// reducing last step always runs as a single step, so the reduction sums in
+// the same order with or without pairs and results are bit-identical.  z
+// ping-pongs between za and zb, the recurrence's p between pa and pb.
+template <typename T>
+int cheb_chain(les_ctx* c, const Stencil<T>& A, const T* r, T* zout, T* za, T* zb, T* pa, T* pb,
+               double lmin, double lmax, int degree, int epi, SolveScalars* sc, const double* wgt)
+{
+    const Dims& d = c->d;
+    hipStream_t s = c->stream;
+    double a0;
+    std::vector<double> be, al;
+    cheb_coeffs(lmin, lmax, degree, a0, be, al);
+    const int elem = (int)sizeof(T);
+    if (degree <= 1) {
+        HIPCHK(c, launch_cheb_scale<T>(s, d, r, zout, a0, epi, c->red, sc, wgt));
+        return LES_OK;
+    }
+    const bool pairs = c->cheb_pairs && cheb_pair_supported<T>(d);
+    const T* zin = nullptr;
+    T* zbuf[2] = {za, zb};
+    int nz = 0;
+    T* pin = pa;
+    T* pout = pb;
+    for (int k = 1; k < degree;) {
+        const bool pair = pairs && degree - k >= 2 && !(k + 1 == degree - 1 && epi >= 0);
+        const int klast = pair ? k + 1 : k;
+        const bool last = klast == degree - 1;
+        T* zo = last ? zout : zbuf[nz];
+        nz ^= 1;
+        // the timing brackets leave out the reducing launch (it is followed
+        // by its finalize kernel)
+        const bool timed = !(last && epi >= 0);
+        if (pair) {
+            KtScope kt(c, timed ? LES_KT_CHEB_PAIR : -1, elem);
+            HIPCHK(c, launch_cheb_pair<T>(s, d, A, r, zin, zo, pin, pout, a0, be[k - 1],
+                                          al[k - 1], be[k], al[k], k == 1, last ? epi : -1,
+                                          c->red, sc, wgt));
+            std::swap(pin, pout);
+        } else {
+            KtScope kt(c, timed ? LES_KT_CHEB_STEP : -1, elem);
+            HIPCHK(c, launch_cheb_step<T>(s, d, A, r, zin, zo, pin, a0, be[k - 1], al[k - 1],
+                                          k == 1, last ? epi : -1, c->red, sc, wgt));
+        }
+        zin = zo;
+        k = klast + 1;
+    }
+    return LES_OK;
+}
+
 // z = M^-1 r for FP64 preconditioners.  When epi >= 0 the final kernel also
 // reduces r.z into the solve scalars with that epilogue.
 int precond_fp64(les_ctx* c, int pc, const double* r, double* zout, double lmin, double lmax,
@@ -290,27 +343,9 @@ int precond_fp64(les_ctx* c, int pc, const double* r, double* zout, double lmin,
             HIPCHK(c, launch_ic_bwd_level<double>(s, d, l, zout, L[SC], L[SW], L[SS], L[SB], sc));
         break;
     }
-    case LES_PC_CHEBYSHEV: {
-        double a0;
-        std::vector<double> be, al;
-        cheb_coeffs(lmin, lmax, degree, a0, be, al);
-        const Stencil<double>& A = Aop;
-        if (degree <= 1) {
-            HIPCHK(c, launch_cheb_scale<double>(s, d, r, zout, a0, epi, c->red, sc, wgt));
-            return LES_OK;
-        }
-        double* buf[2] = {c->za, c->zb};
-        for (int k = 1; k < degree; ++k) {
-            const bool last = (k == degree - 1);
-            const double* zin = (k == 1) ? nullptr : buf[k & 1];
-            double* zo = last ? zout : buf[(k + 1) & 1];
-            KtScope kt(c, LES_KT_CHEB_STEP, 8);
-            HIPCHK(c, launch_cheb_step<double>(s, d, A, r, zin, zo, c->pc, a0, be[k - 1],
-                                               al[k - 1], k == 1, last ? epi : -1, c->red, sc,
-                                               wgt));
-        }
-        return LES_OK;
-    }
+    case LES_PC_CHEBYSHEV:
+        return cheb_chain<double>(c, Aop, r, zout, c->za, c->zb, c->pc, c->pc2, lmin, lmax,
+                                  degree, epi, sc, wgt);
     default:
         return fail(c, LES_ERR_ARG, "unknown preconditioner");
     }
@@ -390,7 +425,7 @@ int les_ctx_destroy(les_ctx* c)
         fr(c->A32[q]);
         fr(c->L32[q]);
     }
-    for (double* p : {c->r, c->z, c->p, c->q, c->za, c->zb, c->pc, c->t1, c->t2}) fr(p);
+    for (double* p : {c->r, c->z, c->p, c->q, c->za, c->zb, c->pc, c->pc2, c->t1, c->t2}) fr(p);
     fr(c->W);
     fr(c->ckpt);
     fr(c->seg);
@@ -421,6 +456,18 @@ int les_set_stream(les_ctx* c, void* stream)
     HIPCHK(c, hipStreamSynchronize(c->stream));
     c->stream = stream ? (hipStream_t)stream : c->own;
     return LES_OK;
+}
+
+int les_set_option(les_ctx* c, int option, int value)
+{
+    if (!c) return LES_ERR_ARG;
+    switch (option) {
+    case LES_OPT_CHEB_PAIRS:
+        c->cheb_pairs = value != 0;
+        return LES_OK;
+    default:
+        return fail(c, LES_ERR_ARG, "unknown option");
+    }
 }
 
 int les_synchronize(les_ctx* c)
@@ -855,7 +902,7 @@ int ensure_fp32_ops(les_ctx* c, int pc, bool coded)
         c->have_L32 = true;
     }
     MixedWork& w = c->mw;
-    for (float** b : {&w.r, &w.p, &w.q, &w.z, &w.za, &w.zb, &w.pc, &w.x})
+    for (float** b : {&w.r, &w.p, &w.q, &w.z, &w.za, &w.zb, &w.pc, &w.pc2, &w.x})
         if ((rc = alloc(c, b, c->d.N)) != LES_OK) return rc;
     return LES_OK;
 }
@@ -883,26 +930,9 @@ int precond_fp32(les_ctx* c, int pc, const float* r, float* zout, double lmin, d
             HIPCHK(c, launch_ic_bwd_level<float>(s, d, l, zout, L[SC], L[SW], L[SS], L[SB], sc));
         break;
     }
-    case LES_PC_CHEBYSHEV: {
-        double a0;
-        std::vector<double> be, al;
-        cheb_coeffs(lmin, lmax, degree, a0, be, al);
-        const Stencil<float>& A = Aop;
-        if (degree <= 1) {
-            HIPCHK(c, launch_cheb_scale<float>(s, d, r, zout, a0, epi, c->red, sc));
-            return LES_OK;
-        }
-        float* buf[2] = {c->mw.za, c->mw.zb};
-        for (int k = 1; k < degree; ++k) {
-            const bool last = (k == degree - 1);
-            const float* zin = (k == 1) ? nullptr : buf[k & 1];
-            float* zo = last ? zout : buf[(k + 1) & 1];
-            KtScope kt(c, LES_KT_CHEB_STEP, 4);
-            HIPCHK(c, launch_cheb_step<float>(s, d, A, r, zin, zo, c->mw.pc, a0, be[k - 1],
-                                              al[k - 1], k == 1, last ? epi : -1, c->red, sc));
-        }
-        return LES_OK;
-    }
+    case LES_PC_CHEBYSHEV:
+        return cheb_chain<float>(c, Aop, r, zout, c->mw.za, c->mw.zb, c->mw.pc, c->mw.pc2, lmin,
+                                 lmax, degree, epi, sc, nullptr);
     default:
         return fail(c, LES_ERR_ARG, "unknown preconditioner");
     }

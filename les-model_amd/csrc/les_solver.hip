@@ -8,6 +8,8 @@
 
 #include <math.h>
 
+#include <algorithm>
+
 namespace les {
 
 namespace {
@@ -671,6 +673,148 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(Dims d, Chunking ch, Stenc
     if (REDUCE) block_partial<1>(acc, red.partials, lb);
 }
 
+// ------------------------------------------------------------------
+// Two Chebyshev steps in one pass over HBM (steps k and k+1):
+//   A: res = r - A z_k ; p_k = res + beta_k p_{k-1} ; z_{k+1} = z_k + alpha_k p_k
+//   B: res = r - A z_{k+1} ; p_{k+1} = res + beta_{k+1} p_k ;
+//      z_{k+2} = z_{k+1} + alpha_{k+1} p_{k+1}
+// A workgroup owns J consecutive j-rows (whole rows, all k) of a range of
+// i-planes and marches along i.  For plane t it runs A on its rows plus one
+// halo row on each side into a 3-plane LDS ring of z_{k+1}, then B on plane
+// t-1 from that ring; p_k of its own rows stays in registers from one plane
+// to the next.  z_{k+1} and p_k never reach HBM.  Each cell's arithmetic is
+// exactly k_cheb_step's, so results are bit-identical to two single steps.
+// p_in / p_out and z_in / z_out must be distinct buffers: halo rows and
+// planes read inputs that neighbouring workgroups own.
+// Thread layout: tpr = nz / V threads per row, rpp = kBlock / tpr rows per
+// pass, P passes; row slot q = pass * rpp + sub holds core row q (q < J) or
+// one of the two halo rows (q = J: j0 - 1, q = J + 1: j0 + J), J = P*rpp - 2.
+// ------------------------------------------------------------------
+struct PairGeom {
+    int tpr, rpp, J;   // threads per row, rows per pass, core rows
+    int njb, irange;   // j-blocks, i-planes per workgroup
+    int grid;          // workgroups (multiple of kXcd)
+};
+
+template <typename T, bool FIRST, int CODED, bool REDUCE, int V, int P>
+__global__ __launch_bounds__(kBlock) void k_cheb_pair(Dims d, PairGeom g, Stencil<T> A,
+                                                      const T* __restrict__ r,
+                                                      const T* __restrict__ z_in,
+                                                      T* __restrict__ z_out,
+                                                      const T* __restrict__ p_in,
+                                                      T* __restrict__ p_out, double alpha0,
+                                                      double beta_a, double alpha_a,
+                                                      double beta_b, double alpha_b,
+                                                      RedScratch red, const SolveScalars* sc,
+                                                      const double* __restrict__ wgt)
+{
+    if (is_done(sc)) return;
+    extern __shared__ unsigned char les_smem[];
+    T* zs = reinterpret_cast<T*>(les_smem);  // [3][P*rpp][nz]
+    const int lb = logical_block(g.grid);
+    const int jb = lb % g.njb, ic = lb / g.njb;
+    const int j0 = jb * g.J;
+    const int ia = ic * g.irange, ib = min(ia + g.irange, d.nx);
+    const int sub = (int)threadIdx.x / g.tpr;
+    const int kk = ((int)threadIdx.x - sub * g.tpr) * V;
+    const int slots = P * g.rpp;
+    const int plane_sz = slots * d.nz;
+    const T a0 = (T)alpha0, bta = (T)beta_a, ata = (T)alpha_a, btb = (T)beta_b, atb = (T)alpha_b;
+    T pkeep[P][V], pnew[P][V];
+#pragma unroll
+    for (int pp = 0; pp < P; ++pp)
+#pragma unroll
+        for (int v = 0; v < V; ++v) pkeep[pp][v] = pnew[pp][v] = T(0);
+    double acc[1] = {0.0};
+    for (int t = ia - 1; t <= ib && ia < d.nx; ++t) {
+        // ---- A at plane t: z_{k+1} into the ring, p_k of core rows ----
+        if (t >= 0 && t < d.nx) {
+#pragma unroll
+            for (int pp = 0; pp < P; ++pp) {
+                const int q = pp * g.rpp + sub;
+                const int a = q < g.J ? q + 1 : (q == g.J ? 0 : g.J + 1);
+                const int j = j0 - 1 + a;
+                if (j < 0 || j >= d.ny) continue;
+                const int i = t, k = kk, id = (i * d.ny + j) * d.nz + kk;
+                Nbr<T, V> nb;
+                T rv[V], az[V], pp_[V], zc[V];
+                vload<T, V>(r + id, rv);
+                if (FIRST) {
+                    load_nbr<T, V>(r, d, id, i, j, k, [&](T x) { return T(0) + a0 * x; }, nb);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) pp_[v] = rv[v];
+                } else {
+                    load_nbr<T, V>(z_in, d, id, i, j, k, [](T x) { return x; }, nb);
+                    vload<T, V>(p_in + id, pp_);
+                }
+#pragma unroll
+                for (int v = 0; v < V; ++v) zc[v] = nb.c[v];
+                apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az);
+                T zn[V];
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const T res = rv[v] - az[v];
+                    const T pn = res + bta * pp_[v];
+                    zn[v] = zc[v] + ata * pn;
+                    pnew[pp][v] = pn;
+                }
+                vstore<T, V>(zs + (t + 3) % 3 * plane_sz + a * d.nz + kk, zn);
+            }
+        }
+        __syncthreads();
+        // ---- B at plane t-1 from the ring ----
+        const int iB = t - 1;
+        if (iB >= ia && iB < ib) {
+            const T* zc_pl = zs + (iB + 3) % 3 * plane_sz;
+            const T* zm_pl = zs + (iB + 2) % 3 * plane_sz;
+            const T* zp_pl = zs + (iB + 4) % 3 * plane_sz;
+#pragma unroll
+            for (int pp = 0; pp < P; ++pp) {
+                const int q = pp * g.rpp + sub;
+                if (q >= g.J) continue;
+                const int a = q + 1, j = j0 + q;
+                if (j >= d.ny) continue;
+                const int i = iB, k = kk, id = (i * d.ny + j) * d.nz + kk;
+                Nbr<T, V> nb;
+                vload<T, V>(zc_pl + a * d.nz + kk, nb.c);
+#pragma unroll
+                for (int v = 0; v < V; ++v) nb.w[v] = nb.e[v] = nb.s[v] = nb.n[v] = T(0);
+                if (i > 0) vload<T, V>(zm_pl + a * d.nz + kk, nb.w);
+                if (i < d.nx - 1) vload<T, V>(zp_pl + a * d.nz + kk, nb.e);
+                if (j > 0) vload<T, V>(zc_pl + (a - 1) * d.nz + kk, nb.s);
+                if (j < d.ny - 1) vload<T, V>(zc_pl + (a + 1) * d.nz + kk, nb.n);
+                nb.km = k > 0 ? zc_pl[a * d.nz + kk - 1] : T(0);
+                nb.kp = k + V < d.nz ? zc_pl[a * d.nz + kk + V] : T(0);
+                T rv[V], az[V];
+                vload<T, V>(r + id, rv);
+                apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az);
+                T pn[V], zn[V];
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const T res = rv[v] - az[v];
+                    pn[v] = res + btb * pkeep[pp][v];
+                    zn[v] = nb.c[v] + atb * pn[v];
+                }
+                vstore<T, V>(p_out + id, pn);
+                vstore<T, V>(z_out + id, zn);
+                if (REDUCE) {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        const double rz = (double)rv[v] * (double)zn[v];
+                        acc[0] += wgt ? wgt[id + v] * rz : rz;
+                    }
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp)
+#pragma unroll
+            for (int v = 0; v < V; ++v) pkeep[pp][v] = pnew[pp][v];
+    }
+    if (REDUCE) block_partial<1>(acc, red.partials, lb);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cheb_scale(Dims d, Chunking ch,
                                                        const T* __restrict__ r,
@@ -1125,6 +1269,71 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
 }
 
 template <typename T>
+constexpr int pair_passes() { return sizeof(T) == 4 ? 2 : 3; }
+
+template <typename T>
+bool cheb_pair_geom(const Dims& d, PairGeom& g)
+{
+    constexpr int V = vec_width<T>(), P = pair_passes<T>();
+    if (d.nz % V) return false;
+    g.tpr = d.nz / V;
+    if (g.tpr > kBlock || kBlock % g.tpr) return false;
+    g.rpp = kBlock / g.tpr;
+    g.J = P * g.rpp - 2;
+    if (g.J < 2) return false;
+    if ((size_t)3 * P * g.rpp * d.nz * sizeof(T) > kPairMaxLds) return false;
+    g.njb = (d.ny + g.J - 1) / g.J;
+    int nxc = std::max(1, std::min(d.nx, kPairTargetGrid / g.njb));
+    g.irange = (d.nx + nxc - 1) / nxc;
+    nxc = (d.nx + g.irange - 1) / g.irange;
+    g.grid = ((g.njb * nxc + kXcd - 1) / kXcd) * kXcd;
+    return g.grid <= kMaxGrid;  // one reduction partial per workgroup
+}
+
+template <typename T>
+bool cheb_pair_supported(const Dims& d)
+{
+    PairGeom g;
+    return cheb_pair_geom<T>(d, g);
+}
+
+template <typename T>
+hipError_t launch_cheb_pair(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
+                            const T* z_in, T* z_out, const T* p_in, T* p_out, double alpha0,
+                            double beta_a, double alpha_a, double beta_b, double alpha_b,
+                            int first, int epi, RedScratch red, SolveScalars* sc,
+                            const double* wgt)
+{
+    constexpr int V = vec_width<T>(), P = pair_passes<T>();
+    PairGeom g;
+    if (!cheb_pair_geom<T>(d, g)) return hipErrorNotSupported;
+    const size_t lds = (size_t)3 * P * g.rpp * d.nz * sizeof(T);
+#define LES_PAIR(F, CD, RD)                                                                      \
+    hipLaunchKernelGGL((k_cheb_pair<T, F, CD, RD, V, P>), dim3(g.grid), dim3(kBlock), lds, s, d, \
+                       g, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a, beta_b,       \
+                       alpha_b, red, sc, wgt)
+#define LES_PAIR_R(F, CD)                    \
+    do {                                     \
+        if (epi >= 0) LES_PAIR(F, CD, true);  \
+        else LES_PAIR(F, CD, false);          \
+    } while (0)
+    if (first) {
+        if (A.code) LES_PAIR_R(true, 1);
+        else LES_PAIR_R(true, 0);
+    } else {
+        if (A.code) LES_PAIR_R(false, 1);
+        else LES_PAIR_R(false, 0);
+    }
+#undef LES_PAIR_R
+#undef LES_PAIR
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    if (epi >= 0)
+        return launch_finalize(s, g.grid, 1, false, epi, sc ? GATE_DONE : GATE_NONE, red, sc,
+                               nullptr);
+    return hipSuccess;
+}
+
+template <typename T>
 hipError_t launch_cheb_scale(hipStream_t s, const Dims& d, const T* r, T* z, double alpha0,
                              int epi, RedScratch red, SolveScalars* sc, const double* wgt)
 {
@@ -1250,6 +1459,11 @@ hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double
                                             RedScratch, SolveScalars*, const double*);          \
     template hipError_t launch_cheb_scale<T>(hipStream_t, const Dims&, const T*, T*, double,    \
                                              int, RedScratch, SolveScalars*, const double*);    \
+    template bool cheb_pair_supported<T>(const Dims&);                                          \
+    template hipError_t launch_cheb_pair<T>(hipStream_t, const Dims&, Stencil<T>, const T*,     \
+                                            const T*, T*, const T*, T*, double, double, double, \
+                                            double, double, int, int, RedScratch,               \
+                                            SolveScalars*, const double*);                      \
     template hipError_t launch_ic_fwd_level<T>(hipStream_t, const Dims&, int, const T*, T*,     \
                                                const T*, const T*, const T*, const T*,          \
                                                const SolveScalars*);                            \

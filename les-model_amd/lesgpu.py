@@ -28,7 +28,9 @@ PC_NONE, PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
 SOLVER_PCG, SOLVER_SOR_LEX, SOLVER_SOR_RB, SOLVER_JACOBI = 0, 1, 2, 3
 KT_NONE, KT_CHEB_STEP, KT_PCG_MATVEC, KT_PCG_UPDATE, KT_MOMENTUM, KT_IC_SOLVE = 0, 1, 2, 3, 4, 5
+KT_CHEB_PAIR = 6
 IP_EUCLID, IP_MIRROR = 0, 1
+OPT_CHEB_PAIRS = 1
 
 DP = C.POINTER(C.c_double)
 
@@ -97,6 +99,7 @@ def load_library(path: str = LIB_PATH):
     L.les_last_error.restype = C.c_char_p
     L.les_set_stream.argtypes = [vp, vp]
     L.les_synchronize.argtypes = [vp]
+    L.les_set_option.argtypes = [vp, C.c_int, C.c_int]
     L.les_set_field.argtypes = [vp, C.c_int, vp, C.c_int]
     L.les_get_field.argtypes = [vp, C.c_int, vp, C.c_int]
     L.les_field_ptr.argtypes = [vp, C.c_int, C.POINTER(vp)]
@@ -220,6 +223,9 @@ class LesGpu:
 
     def get_state(self, names=FIELDS[:9]):
         return {n: self.get_field(n) for n in names}
+
+    def set_option(self, option, value):
+        self._chk(self.lib.les_set_option(self.h, option, int(value)), "les_set_option")
 
     def synchronize(self):
         self._chk(self.lib.les_synchronize(self.h), "les_synchronize")

@@ -18,8 +18,11 @@ from cpu_checkers import Oracle, initial_state, PC_NONE, PC_JACOBI, PC_IC0, PC_L
 pytestmark = pytest.mark.gpu
 
 # nz = 24 and 1032 take the vector-load kernels (rows of V = 16 bytes per
-# thread), 1032 with more threads per row than a workgroup has.
-SHAPES = [(7, 6, 5, 1), (13, 9, 17, 2), (33, 20, 70, 3), (9, 7, 24, 4), (4, 3, 1032, 5)]
+# thread), 1032 with more threads per row than a workgroup has; nz = 256 and
+# 64 also take the fused Chebyshev pairs (several j-blocks and one-plane
+# i-chunks at (6, 10, 256)).
+SHAPES = [(7, 6, 5, 1), (13, 9, 17, 2), (33, 20, 70, 3), (9, 7, 24, 4), (4, 3, 1032, 5),
+          (6, 10, 256, 6), (9, 13, 64, 7)]
 FP64_REL_TOL = 1e-10   # fields after PCG / whole steps vs the oracle
 
 
@@ -347,6 +350,36 @@ def test_mixed_precision_vector_rows(orc, shape):
         assert res["converged"] == 1, res
         assert np.linalg.norm(b - orc.matvec(x, A, m)) < 1.01e-8 * np.linalg.norm(b)
         assert rel_err(x, x_ref) < 1e-5
+
+
+@pytest.mark.parametrize("precision", [lesgpu.PREC_FP64, lesgpu.PREC_MIXED])
+@pytest.mark.parametrize("fmt", [lesgpu.STENCIL_ARRAYS, lesgpu.STENCIL_CODED])
+@pytest.mark.parametrize("shape,degree", [((6, 10, 256, 8), 8), ((7, 23, 32, 9), 5),
+                                          ((5, 14, 256, 10), 3), ((64, 120, 256, 11), 8)])
+def test_cheb_pairs_identical(orc, precision, fmt, shape, degree):
+    """Fused pairs of Chebyshev steps reproduce single steps bit for bit:
+    fixed-iteration solves and their residual histories are identical.  The
+    last shape gives each workgroup a run of several i-planes."""
+    m, solid, f = random_case(*shape, frac=0.05)
+    A = orc.build_stencil(solid, m)
+    lmax = orc.lambda_max(A, m, 20)
+    b = orc.build_b(orc.divergence(f[0], f[1], f[2], solid, m, 1e-3, 1.2), solid, m)
+    outs = []
+    for pairs in (0, 1):
+        with ctx_for(m, solid) as g:
+            g.set_option(lesgpu.OPT_CHEB_PAIRS, pairs)
+            g.build_stencil()
+            g.set_field("b", b)
+            g.set_field("p", f[4])
+            r = g.pcg(precond=PC_CHEBYSHEV, lmin=3 * np.pi ** 2, lmax=lmax, degree=degree,
+                      max_iter=12, tol=0.0, precision=precision, reliable_every=5,
+                      stencil_format=fmt)
+            outs.append((r, g.get_field("p"), g.pcg_trace(13)))
+    (ra, xa, ta), (rb, xb, tb) = outs
+    assert ra == rb
+    assert bits_equal(xa, xb)
+    if precision == lesgpu.PREC_FP64:
+        assert bits_equal(ta, tb)
 
 
 def test_lanczos_closed_form():
