@@ -84,8 +84,11 @@ int les_synchronize(les_ctx* ctx);
 /* Kernel-selection options (no reference counterpart).
  * LES_OPT_CHEB_PAIRS: 1 (default) runs the Chebyshev polynomial as fused
  * pairs of steps where the grid shape allows it, 0 as single steps; the
- * results are bit-identical either way. */
-enum { LES_OPT_CHEB_PAIRS = 1 };
+ * results are bit-identical either way.
+ * LES_OPT_PAIR_GRID: about how many workgroups a fused pair launch uses
+ * (0: the built-in default; at most 4096).  A tuning knob: the grid shape
+ * changes only where the reduction-free pair launches split their work. */
+enum { LES_OPT_CHEB_PAIRS = 1, LES_OPT_PAIR_GRID = 2 };
 int les_set_option(les_ctx* ctx, int option, int value);
 
 /* ---- fields (Field, src/field.h:9-41) ---- */

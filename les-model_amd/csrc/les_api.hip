@@ -46,8 +46,8 @@ struct les_ctx {
     float* L32[7] = {};
     bool have_L32 = false;
     // FP64 Krylov workspaces
-    double *r = nullptr, *z = nullptr, *p = nullptr, *q = nullptr, *za = nullptr, *zb = nullptr,
-           *pc = nullptr, *pc2 = nullptr, *t1 = nullptr, *t2 = nullptr;
+    double *r = nullptr, *z = nullptr, *p = nullptr, *p2 = nullptr, *q = nullptr, *za = nullptr,
+           *zb = nullptr, *pc = nullptr, *pc2 = nullptr, *t1 = nullptr, *t2 = nullptr;
     MixedWork mw{};
     RedScratch red{};
     SolveScalars* sc = nullptr;       // device
@@ -73,6 +73,7 @@ struct les_ctx {
     int roll_nsteps = 0, roll_every = 0;
     double roll_dt = 0, roll_nu = 0, roll_rho = 0, roll_Cs = 0;
     bool cheb_pairs = true;           // fuse Chebyshev steps in pairs where the grid allows
+    int pair_grid = 0;                // workgroups of a pair launch (0: built-in default)
     // kernel timing diagnostics
     int kt_class = 0;
     int kt_elem = 0;
@@ -122,7 +123,8 @@ int alloc(les_ctx* c, T** p, size_t count, bool zero = true)
 int ensure_krylov(les_ctx* c)
 {
     int rc;
-    double** bufs[] = {&c->r, &c->z, &c->p, &c->q, &c->za, &c->zb, &c->pc, &c->pc2, &c->t1, &c->t2};
+    double** bufs[] = {&c->r,  &c->z,  &c->p,  &c->p2,  &c->q, &c->za,
+                       &c->zb, &c->pc, &c->pc2, &c->t1, &c->t2};
     for (double** b : bufs)
         if ((rc = alloc(c, b, c->d.N)) != LES_OK) return rc;
     return LES_OK;
@@ -285,7 +287,7 @@ int cheb_chain(les_ctx* c, const Stencil<T>& A, const T* r, T* zout, T* za, T* z
         HIPCHK(c, launch_cheb_scale<T>(s, d, r, zout, a0, epi, c->red, sc, wgt));
         return LES_OK;
     }
-    const bool pairs = c->cheb_pairs && cheb_pair_supported<T>(d);
+    const bool pairs = c->cheb_pairs && cheb_pair_supported<T>(d, c->pair_grid);
     const T* zin = nullptr;
     T* zbuf[2] = {za, zb};
     int nz = 0;
@@ -304,7 +306,7 @@ int cheb_chain(les_ctx* c, const Stencil<T>& A, const T* r, T* zout, T* za, T* z
             KtScope kt(c, timed ? LES_KT_CHEB_PAIR : -1, elem);
             HIPCHK(c, launch_cheb_pair<T>(s, d, A, r, zin, zo, pin, pout, a0, be[k - 1],
                                           al[k - 1], be[k], al[k], k == 1, last ? epi : -1,
-                                          c->red, sc, wgt));
+                                          c->red, sc, wgt, c->pair_grid));
             std::swap(pin, pout);
         } else {
             KtScope kt(c, timed ? LES_KT_CHEB_STEP : -1, elem);
@@ -425,7 +427,8 @@ int les_ctx_destroy(les_ctx* c)
         fr(c->A32[q]);
         fr(c->L32[q]);
     }
-    for (double* p : {c->r, c->z, c->p, c->q, c->za, c->zb, c->pc, c->pc2, c->t1, c->t2}) fr(p);
+    for (double* p : {c->r, c->z, c->p, c->p2, c->q, c->za, c->zb, c->pc, c->pc2, c->t1, c->t2})
+        fr(p);
     fr(c->W);
     fr(c->ckpt);
     fr(c->seg);
@@ -464,6 +467,10 @@ int les_set_option(les_ctx* c, int option, int value)
     switch (option) {
     case LES_OPT_CHEB_PAIRS:
         c->cheb_pairs = value != 0;
+        return LES_OK;
+    case LES_OPT_PAIR_GRID:
+        if (value < 0 || value > kMaxGrid) return fail(c, LES_ERR_ARG, "pair grid out of range");
+        c->pair_grid = value;
         return LES_OK;
     default:
         return fail(c, LES_ERR_ARG, "unknown option");
@@ -804,7 +811,12 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res,
     const Stencil<double> A = op ? *op : (coded ? coded_stencil<double>(c) : stencil_of(c->A));
     double* x = c->f[LES_F_P];
     const double* b = c->f[LES_F_B];
-    double *r = c->r, *p = c->p, *q = c->q, *z = c->z;
+    double *r = c->r, *q = c->q, *z = c->z;
+    // the search direction ping-pongs: each iteration's matvec forms the new
+    // p from the previous one (the direction update fused into it)
+    double* pb[2] = {c->p, c->p2};
+    int cur = 0;
+    double* p = pb[0];
     SolveScalars* sc = c->sc;
 
     // r = b - A x, ||r||, stop if converged (pressure.cpp:355-359)
@@ -824,18 +836,26 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res,
         for (int it = 0; it < n; ++it) {
             {
                 KtScope kt(c, LES_KT_PCG_MATVEC, 8);
-                HIPCHK(c, launch_pcg_matvec<double>(s, d, A, p, q, c->red, sc, wgt));
+                if (issued + it == 0) {
+                    HIPCHK(c, launch_pcg_matvec<double>(s, d, A, p, q, c->red, sc, wgt));
+                } else {
+                    // p = M^-1 r + beta p_old, fused into q = A p
+                    const bool jac = pc == LES_PC_JACOBI;
+                    HIPCHK(c, launch_pcg_matvec_pu<double>(s, d, A, pb[cur], jac ? r : z,
+                                                           jac ? c->A[SC] : nullptr, pb[cur ^ 1],
+                                                           q, c->red, sc, wgt));
+                    cur ^= 1;
+                    p = pb[cur];
+                }
             }
             if (pc == LES_PC_JACOBI) {
                 HIPCHK(c, (launch_pcg_update<double, double>(s, d, x, r, p, q, c->A[SC], c->red, sc,
                                                              wgt)));
-                HIPCHK(c, launch_pcg_p_jacobi<double>(s, d, r, c->A[SC], p, 0, -1, c->red, sc));
             } else {
                 HIPCHK(c, (launch_pcg_update<double, double>(s, d, x, r, p, q, nullptr, c->red, sc)));
                 if ((rc = precond_fp64(c, pc, r, z, prm->lambda_min, prm->lambda_max, prm->degree,
                                        EPI_BETA, sc, A, wgt)) != LES_OK)
                     return rc;
-                HIPCHK(c, launch_pcg_p_update<double>(s, d, z, p, sc));
             }
         }
         issued += n;
@@ -902,7 +922,7 @@ int ensure_fp32_ops(les_ctx* c, int pc, bool coded)
         c->have_L32 = true;
     }
     MixedWork& w = c->mw;
-    for (float** b : {&w.r, &w.p, &w.q, &w.z, &w.za, &w.zb, &w.pc, &w.pc2, &w.x})
+    for (float** b : {&w.r, &w.p, &w.p2, &w.q, &w.z, &w.za, &w.zb, &w.pc, &w.pc2, &w.x})
         if ((rc = alloc(c, b, c->d.N)) != LES_OK) return rc;
     return LES_OK;
 }
@@ -975,6 +995,8 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     double* x64 = c->f[LES_F_P];
     const double* b = c->f[LES_F_B];
     SolveScalars* sc = c->sc;
+    float* pb[2] = {w.p, w.p2};
+    int cur = 0;
 
     // r = b - A x0 in FP64, rounded to FP32; ||r||, ||b||
     HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.r, 1, 0, c->red, sc, nullptr));
@@ -990,18 +1012,24 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
         for (int it = 0; it < n; ++it) {
             {
                 KtScope kt(c, LES_KT_PCG_MATVEC, 4);
-                HIPCHK(c, launch_pcg_matvec<float>(s, d, A32, w.p, w.q, c->red, sc));
+                if (issued + it == 0) {
+                    HIPCHK(c, launch_pcg_matvec<float>(s, d, A32, pb[cur], w.q, c->red, sc));
+                } else {
+                    // p = z + beta p_old fused into q = A p
+                    HIPCHK(c, launch_pcg_matvec_pu<float>(s, d, A32, pb[cur], w.z, nullptr,
+                                                          pb[cur ^ 1], w.q, c->red, sc));
+                    cur ^= 1;
+                }
             }
             {
                 KtScope kt(c, LES_KT_PCG_UPDATE, 4);
-                HIPCHK(c, launch_pcg_update_mixed(s, d, w.x, w.r, w.p, w.q, c->red, sc));
+                HIPCHK(c, launch_pcg_update_mixed(s, d, w.x, w.r, pb[cur], w.q, c->red, sc));
             }
             HIPCHK(c, launch_mixed_fold(s, d, x64, w.x, 0, sc));
             HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.r, 0, 0, c->red, sc, nullptr));
             if ((rc = precond_fp32(c, pc, w.r, w.z, prm->lambda_min, prm->lambda_max, prm->degree,
                                    EPI_BETA, sc, A32)) != LES_OK)
                 return rc;
-            HIPCHK(c, launch_pcg_p_update<float>(s, d, w.z, w.p, sc));
         }
         issued += n;
         const int slot = batch & 1;

@@ -122,6 +122,14 @@ hipError_t launch_pcg_init(hipStream_t s, const Dims& d, Stencil<T> A, const T* 
 template <typename T>
 hipError_t launch_pcg_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T* p, T* q,
                              RedScratch red, SolveScalars* sc, const double* wgt = nullptr);
+// The same with the previous iteration's direction update fused in:
+// p = z + beta p_old (diag == nullptr) or p = r/diag + beta p_old (z = r,
+// Jacobi), formed at every stencil point and written to p (distinct from
+// p_old); q = A p.
+template <typename T>
+hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, const T* p_old,
+                                const T* z, const T* diag, T* p, T* q, RedScratch red,
+                                SolveScalars* sc, const double* wgt = nullptr);
 // x += alpha p; r -= alpha q; ||r|| test (pressure.cpp:381-392).  With
 // jacobi_diag != nullptr also r.(r/A_c) and beta (pressure.cpp:396-402).
 template <typename T, typename TX>
@@ -159,14 +167,16 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
 // (beta_b, alpha_b); p_in/p_out and z_in/z_out distinct.  Not every grid
 // shape is supported (cheb_pair_supported); the caller then runs two
 // launch_cheb_step.
+// target_grid: about how many workgroups to spread the grid over (0: the
+// built-in default).
 template <typename T>
-bool cheb_pair_supported(const Dims& d);
+bool cheb_pair_supported(const Dims& d, int target_grid);
 template <typename T>
 hipError_t launch_cheb_pair(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
                             const T* z_in, T* z_out, const T* p_in, T* p_out, double alpha0,
                             double beta_a, double alpha_a, double beta_b, double alpha_b,
                             int first, int epi, RedScratch red, SolveScalars* sc,
-                            const double* wgt = nullptr);
+                            const double* wgt, int target_grid);
 template <typename T>
 hipError_t launch_cheb_scale(hipStream_t s, const Dims& d, const T* r, T* z, double alpha0,
                              int epi, RedScratch red, SolveScalars* sc,

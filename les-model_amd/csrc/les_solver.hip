@@ -461,20 +461,49 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(Dims d, Chunking ch, Stenci
     block_partial<2>(acc, red.partials, lb);
 }
 
-template <typename T, int CODED, int V>
+// q = A p and p.q.  PU (the previous iteration's direction update fused
+// in): 0 p given; 1 p = z + beta p_old; 2 p = r / diag + beta p_old
+// (Jacobi), formed at every stencil point from z (or r, diag) and p_old,
+// written to p at the cell itself; p_old and p are distinct buffers.
+template <typename T, int CODED, int V, int PU>
 __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Stencil<T> A,
-                                                       const T* __restrict__ p,
+                                                       const T* __restrict__ p_old,
+                                                       const T* __restrict__ z,
+                                                       const T* __restrict__ diag,
+                                                       T* __restrict__ p,
                                                        T* __restrict__ q, RedScratch red,
                                                        const SolveScalars* sc,
                                                        const double* __restrict__ wgt)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
+    const T beta = (T)sc->beta;
     double acc[1] = {0.0};
     FOR_VCELLS(d, ch, lb, V)
     {
         Nbr<T, V> nb;
-        load_nbr<T, V>(p, d, id, i, j, k, [](T x) { return x; }, nb);
+        if (PU == 0) {
+            load_nbr<T, V>(p, d, id, i, j, k, [](T x) { return x; }, nb);
+        } else {
+            Nbr<T, V> nz_, np_;
+            load_nbr<T, V>(z, d, id, i, j, k, [](T x) { return x; }, nz_);
+            load_nbr<T, V>(p_old, d, id, i, j, k, [](T x) { return x; }, np_);
+            auto comb = [&](T zz, T pp, int n) {
+                if (PU == 1) return zz + beta * pp;
+                return zz / diag[n] + beta * pp;
+            };
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                nb.c[v] = comb(nz_.c[v], np_.c[v], id + v);
+                nb.w[v] = i > 0 ? comb(nz_.w[v], np_.w[v], id - d.sij + v) : T(0);
+                nb.e[v] = i < d.nx - 1 ? comb(nz_.e[v], np_.e[v], id + d.sij + v) : T(0);
+                nb.s[v] = j > 0 ? comb(nz_.s[v], np_.s[v], id - d.nz + v) : T(0);
+                nb.n[v] = j < d.ny - 1 ? comb(nz_.n[v], np_.n[v], id + d.nz + v) : T(0);
+            }
+            nb.km = k > 0 ? comb(nz_.km, np_.km, id - 1) : T(0);
+            nb.kp = k + V < d.nz ? comb(nz_.kp, np_.kp, id + V) : T(0);
+            vstore<T, V>(p + id, nb.c);
+        }
         T qv[V];
         apply7v<CODED, T, V>(A, d, id, i, j, k, nb, qv);
         vstore<T, V>(q + id, qv);
@@ -1135,19 +1164,37 @@ template <typename T>
 hipError_t launch_pcg_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T* p, T* q,
                              RedScratch red, SolveScalars* sc, const double* wgt)
 {
+    return launch_pcg_matvec_pu<T>(s, d, A, nullptr, nullptr, nullptr, const_cast<T*>(p), q, red,
+                                   sc, wgt);
+}
+
+template <typename T>
+hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, const T* p_old,
+                                const T* z, const T* diag, T* p, T* q, RedScratch red,
+                                SolveScalars* sc, const double* wgt)
+{
     constexpr int W = vec_width<T>();
     const int V = pick_width<T>(d);
     const Chunking ch = make_chunking(d, V);
-#define LES_MV(CD, VV)                                                                           \
-    hipLaunchKernelGGL((k_pcg_matvec<T, CD, VV>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, p, \
-                       q, red, sc, wgt)
-    if (A.code) {
-        if (V == W) LES_MV(1, W);
-        else LES_MV(1, 1);
-    } else {
-        if (V == W) LES_MV(0, W);
-        else LES_MV(0, 1);
-    }
+    const int pu = p_old == nullptr ? 0 : (diag ? 2 : 1);
+#define LES_MV(CD, VV, PU)                                                                      \
+    hipLaunchKernelGGL((k_pcg_matvec<T, CD, VV, PU>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, \
+                       A, p_old, z, diag, p, q, red, sc, wgt)
+#define LES_MV_V(CD, PU)                 \
+    do {                                 \
+        if (V == W) LES_MV(CD, W, PU);   \
+        else LES_MV(CD, 1, PU);          \
+    } while (0)
+#define LES_MV_P(CD)                     \
+    do {                                 \
+        if (pu == 0) LES_MV_V(CD, 0);    \
+        else if (pu == 1) LES_MV_V(CD, 1); \
+        else LES_MV_V(CD, 2);            \
+    } while (0)
+    if (A.code) LES_MV_P(1);
+    else LES_MV_P(0);
+#undef LES_MV_P
+#undef LES_MV_V
 #undef LES_MV
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     return launch_finalize(s, ch.grid, 1, false, EPI_ALPHA, GATE_DONE, red, sc, nullptr);
@@ -1272,7 +1319,7 @@ template <typename T>
 constexpr int pair_passes() { return sizeof(T) == 4 ? 2 : 3; }
 
 template <typename T>
-bool cheb_pair_geom(const Dims& d, PairGeom& g)
+bool cheb_pair_geom(const Dims& d, PairGeom& g, int target_grid)
 {
     constexpr int V = vec_width<T>(), P = pair_passes<T>();
     if (d.nz % V) return false;
@@ -1283,7 +1330,8 @@ bool cheb_pair_geom(const Dims& d, PairGeom& g)
     if (g.J < 2) return false;
     if ((size_t)3 * P * g.rpp * d.nz * sizeof(T) > kPairMaxLds) return false;
     g.njb = (d.ny + g.J - 1) / g.J;
-    int nxc = std::max(1, std::min(d.nx, kPairTargetGrid / g.njb));
+    if (target_grid <= 0) target_grid = kPairTargetGrid;
+    int nxc = std::max(1, std::min(d.nx, target_grid / g.njb));
     g.irange = (d.nx + nxc - 1) / nxc;
     nxc = (d.nx + g.irange - 1) / g.irange;
     g.grid = ((g.njb * nxc + kXcd - 1) / kXcd) * kXcd;
@@ -1291,10 +1339,10 @@ bool cheb_pair_geom(const Dims& d, PairGeom& g)
 }
 
 template <typename T>
-bool cheb_pair_supported(const Dims& d)
+bool cheb_pair_supported(const Dims& d, int target_grid)
 {
     PairGeom g;
-    return cheb_pair_geom<T>(d, g);
+    return cheb_pair_geom<T>(d, g, target_grid);
 }
 
 template <typename T>
@@ -1302,11 +1350,11 @@ hipError_t launch_cheb_pair(hipStream_t s, const Dims& d, Stencil<T> A, const T*
                             const T* z_in, T* z_out, const T* p_in, T* p_out, double alpha0,
                             double beta_a, double alpha_a, double beta_b, double alpha_b,
                             int first, int epi, RedScratch red, SolveScalars* sc,
-                            const double* wgt)
+                            const double* wgt, int target_grid)
 {
     constexpr int V = vec_width<T>(), P = pair_passes<T>();
     PairGeom g;
-    if (!cheb_pair_geom<T>(d, g)) return hipErrorNotSupported;
+    if (!cheb_pair_geom<T>(d, g, target_grid)) return hipErrorNotSupported;
     const size_t lds = (size_t)3 * P * g.rpp * d.nz * sizeof(T);
 #define LES_PAIR(F, CD, RD)                                                                      \
     hipLaunchKernelGGL((k_cheb_pair<T, F, CD, RD, V, P>), dim3(g.grid), dim3(kBlock), lds, s, d, \
@@ -1447,6 +1495,9 @@ hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double
                                            const T*, T*, RedScratch, SolveScalars*);            \
     template hipError_t launch_pcg_matvec<T>(hipStream_t, const Dims&, Stencil<T>, const T*,    \
                                              T*, RedScratch, SolveScalars*, const double*);     \
+    template hipError_t launch_pcg_matvec_pu<T>(hipStream_t, const Dims&, Stencil<T>, const T*, \
+                                                const T*, const T*, T*, T*, RedScratch,         \
+                                                SolveScalars*, const double*);                  \
     template hipError_t launch_pcg_p_jacobi<T>(hipStream_t, const Dims&, const T*, const T*,    \
                                                T*, int, int, RedScratch, SolveScalars*,         \
                                                const double*);                                  \
@@ -1459,11 +1510,11 @@ hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double
                                             RedScratch, SolveScalars*, const double*);          \
     template hipError_t launch_cheb_scale<T>(hipStream_t, const Dims&, const T*, T*, double,    \
                                              int, RedScratch, SolveScalars*, const double*);    \
-    template bool cheb_pair_supported<T>(const Dims&);                                          \
+    template bool cheb_pair_supported<T>(const Dims&, int);                                     \
     template hipError_t launch_cheb_pair<T>(hipStream_t, const Dims&, Stencil<T>, const T*,     \
                                             const T*, T*, const T*, T*, double, double, double, \
                                             double, double, int, int, RedScratch,               \
-                                            SolveScalars*, const double*);                      \
+                                            SolveScalars*, const double*, int);                 \
     template hipError_t launch_ic_fwd_level<T>(hipStream_t, const Dims&, int, const T*, T*,     \
                                                const T*, const T*, const T*, const T*,          \
                                                const SolveScalars*);                            \

@@ -30,7 +30,7 @@ SOLVER_PCG, SOLVER_SOR_LEX, SOLVER_SOR_RB, SOLVER_JACOBI = 0, 1, 2, 3
 KT_NONE, KT_CHEB_STEP, KT_PCG_MATVEC, KT_PCG_UPDATE, KT_MOMENTUM, KT_IC_SOLVE = 0, 1, 2, 3, 4, 5
 KT_CHEB_PAIR = 6
 IP_EUCLID, IP_MIRROR = 0, 1
-OPT_CHEB_PAIRS = 1
+OPT_CHEB_PAIRS, OPT_PAIR_GRID = 1, 2
 
 DP = C.POINTER(C.c_double)
 
