@@ -136,12 +136,18 @@ __device__ __forceinline__ void load_nbr(const T* __restrict__ x, const Dims& d,
 // and guards.
 template <int CODED, typename T, int V>
 __device__ __forceinline__ void apply7v(const Stencil<T>& A, const Dims& d, int id, int i, int j,
-                                        int k, const Nbr<T, V>& nb, T (&q)[V])
+                                        int k, const Nbr<T, V>& nb, T (&q)[V],
+                                        const uint16_t* codes = nullptr)
 {
     T cc[V], cw[V], ce[V], cs[V], cn[V], cb[V], ct[V];
     if (CODED == 1) {
         uint16_t cd[V];
-        vload<uint16_t, V>(A.code + id, cd);
+        if (codes) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) cd[v] = codes[v];
+        } else {
+            vload<uint16_t, V>(A.code + id, cd);
+        }
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             const unsigned c = cd[v];
@@ -749,11 +755,17 @@ __global__ __launch_bounds__(kBlock) void k_cheb_pair(Dims d, PairGeom g, Stenci
     const int slots = P * g.rpp;
     const int plane_sz = slots * d.nz;
     const T a0 = (T)alpha0, bta = (T)beta_a, ata = (T)alpha_a, btb = (T)beta_b, atb = (T)alpha_b;
-    T pkeep[P][V], pnew[P][V];
+    // p_k, r and the operator code of the core rows, carried from step A
+    // of a plane to step B of the same plane one iteration later
+    T pkeep[P][V], pnew[P][V], rkeep[P][V], rnew[P][V];
+    uint16_t ckeep[P][V], cnew[P][V];
 #pragma unroll
     for (int pp = 0; pp < P; ++pp)
 #pragma unroll
-        for (int v = 0; v < V; ++v) pkeep[pp][v] = pnew[pp][v] = T(0);
+        for (int v = 0; v < V; ++v) {
+            pkeep[pp][v] = pnew[pp][v] = rkeep[pp][v] = rnew[pp][v] = T(0);
+            ckeep[pp][v] = cnew[pp][v] = 0;
+        }
     double acc[1] = {0.0};
     for (int t = ia - 1; t <= ib && ia < d.nx; ++t) {
         // ---- A at plane t: z_{k+1} into the ring, p_k of core rows ----
@@ -778,7 +790,9 @@ __global__ __launch_bounds__(kBlock) void k_cheb_pair(Dims d, PairGeom g, Stenci
                 }
 #pragma unroll
                 for (int v = 0; v < V; ++v) zc[v] = nb.c[v];
-                apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az);
+                uint16_t cd[V];
+                if (CODED == 1) vload<uint16_t, V>(A.code + id, cd);
+                apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az, CODED == 1 ? cd : nullptr);
                 T zn[V];
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
@@ -786,6 +800,8 @@ __global__ __launch_bounds__(kBlock) void k_cheb_pair(Dims d, PairGeom g, Stenci
                     const T pn = res + bta * pp_[v];
                     zn[v] = zc[v] + ata * pn;
                     pnew[pp][v] = pn;
+                    rnew[pp][v] = rv[v];
+                    if (CODED == 1) cnew[pp][v] = cd[v];
                 }
                 vstore<T, V>(zs + (t + 3) % 3 * plane_sz + a * d.nz + kk, zn);
             }
@@ -814,10 +830,11 @@ __global__ __launch_bounds__(kBlock) void k_cheb_pair(Dims d, PairGeom g, Stenci
                 if (j < d.ny - 1) vload<T, V>(zc_pl + (a + 1) * d.nz + kk, nb.n);
                 nb.km = k > 0 ? zc_pl[a * d.nz + kk - 1] : T(0);
                 nb.kp = k + V < d.nz ? zc_pl[a * d.nz + kk + V] : T(0);
-                T rv[V], az[V];
-                vload<T, V>(r + id, rv);
-                apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az);
+                T az[V];
+                apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az,
+                                     CODED == 1 ? ckeep[pp] : nullptr);
                 T pn[V], zn[V];
+                const T* rv = rkeep[pp];
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
                     const T res = rv[v] - az[v];
@@ -839,7 +856,11 @@ __global__ __launch_bounds__(kBlock) void k_cheb_pair(Dims d, PairGeom g, Stenci
 #pragma unroll
         for (int pp = 0; pp < P; ++pp)
 #pragma unroll
-            for (int v = 0; v < V; ++v) pkeep[pp][v] = pnew[pp][v];
+            for (int v = 0; v < V; ++v) {
+                pkeep[pp][v] = pnew[pp][v];
+                rkeep[pp][v] = rnew[pp][v];
+                ckeep[pp][v] = cnew[pp][v];
+            }
     }
     if (REDUCE) block_partial<1>(acc, red.partials, lb);
 }
@@ -1330,7 +1351,7 @@ bool cheb_pair_geom(const Dims& d, PairGeom& g, int target_grid)
     if (g.J < 2) return false;
     if ((size_t)3 * P * g.rpp * d.nz * sizeof(T) > kPairMaxLds) return false;
     g.njb = (d.ny + g.J - 1) / g.J;
-    if (target_grid <= 0) target_grid = kPairTargetGrid;
+    if (target_grid <= 0) target_grid = sizeof(T) == 4 ? kPairTargetGridF32 : kPairTargetGridF64;
     int nxc = std::max(1, std::min(d.nx, target_grid / g.njb));
     g.irange = (d.nx + nxc - 1) / nxc;
     nxc = (d.nx + g.irange - 1) / g.irange;
