@@ -731,8 +731,12 @@ struct PairGeom {
     int grid;          // workgroups (multiple of kXcd)
 };
 
+#ifndef LES_PAIR_WAVES_PER_EU
+#define LES_PAIR_WAVES_PER_EU 1
+#endif
 template <typename T, bool FIRST, int CODED, bool REDUCE, int V, int P>
-__global__ __launch_bounds__(kBlock) void k_cheb_pair(Dims d, PairGeom g, Stencil<T> A,
+__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(LES_PAIR_WAVES_PER_EU)))
+void k_cheb_pair(Dims d, PairGeom g, Stencil<T> A,
                                                       const T* __restrict__ r,
                                                       const T* __restrict__ z_in,
                                                       T* __restrict__ z_out,
@@ -1336,8 +1340,16 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
     return hipSuccess;
 }
 
+#ifndef LES_PAIR_PASSES_F32
+#define LES_PAIR_PASSES_F32 2
+#endif
+#ifndef LES_PAIR_PASSES_F64
+#define LES_PAIR_PASSES_F64 3
+#endif
+// Row passes per thread of the pair kernel (J = passes * rows-per-pass - 2
+// core rows per workgroup).
 template <typename T>
-constexpr int pair_passes() { return sizeof(T) == 4 ? 2 : 3; }
+constexpr int pair_passes() { return sizeof(T) == 4 ? LES_PAIR_PASSES_F32 : LES_PAIR_PASSES_F64; }
 
 template <typename T>
 bool cheb_pair_geom(const Dims& d, PairGeom& g, int target_grid)

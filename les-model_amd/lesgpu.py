@@ -91,6 +91,14 @@ def load_library(path: str = LIB_PATH):
     if not os.path.exists(path):
         raise LesError(f"{path} is not built: run __graft_entry__.build() "
                        "(make -C les-model_amd); there is no CPU fallback")
+    # PyTorch-ROCm carries its own HIP runtime.  When this library (linked
+    # against the system one) loads first and torch initialises its GPU
+    # afterwards, this library's runtime no longer finds the device; with
+    # torch's runtime loaded first, both share it.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(path)
     vp = C.c_void_p
     L.les_ctx_create.argtypes = [C.c_int] * 3 + [C.c_double] * 3 + [C.POINTER(vp)]
