@@ -294,6 +294,7 @@ int cheb_chain(les_ctx* c, const Stencil<T>& A, const T* r, T* zout, T* za, T* z
     T* pin = pa;
     T* pout = pb;
     for (int k = 1; k < degree;) {
+        // a pair never runs the reducing last step
         const bool pair = pairs && degree - k >= 2 && !(k + 1 == degree - 1 && epi >= 0);
         const int klast = pair ? k + 1 : k;
         const bool last = klast == degree - 1;
@@ -305,8 +306,7 @@ int cheb_chain(les_ctx* c, const Stencil<T>& A, const T* r, T* zout, T* za, T* z
         if (pair) {
             KtScope kt(c, timed ? LES_KT_CHEB_PAIR : -1, elem);
             HIPCHK(c, launch_cheb_pair<T>(s, d, A, r, zin, zo, pin, pout, a0, be[k - 1],
-                                          al[k - 1], be[k], al[k], k == 1, last ? epi : -1,
-                                          c->red, sc, wgt, c->pair_grid));
+                                          al[k - 1], be[k], al[k], k == 1, sc, c->pair_grid));
             std::swap(pin, pout);
         } else {
             KtScope kt(c, timed ? LES_KT_CHEB_STEP : -1, elem);
@@ -1007,6 +1007,8 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     int check = prm->check_every > 0 ? prm->check_every : (d.N >= (1 << 22) ? 4 : 16);
     int issued = 0, batch = 0;
     bool finished = prm->max_iter <= 0;
+    bool near = false;  // replacement kernels on every iteration from then on
+    double last_err = 0.0;
     while (!finished) {
         const int n = std::min(check, prm->max_iter - issued);
         for (int it = 0; it < n; ++it) {
@@ -1025,8 +1027,16 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
                 KtScope kt(c, LES_KT_PCG_UPDATE, 4);
                 HIPCHK(c, launch_pcg_update_mixed(s, d, w.x, w.r, pb[cur], w.q, c->red, sc));
             }
-            HIPCHK(c, launch_mixed_fold(s, d, x64, w.x, 0, sc));
-            HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.r, 0, 0, c->red, sc, nullptr));
+            // The replacement kernels run gated on the device's request; they
+            // are enqueued on the iterations where a request can stand: the
+            // periodic ones, max_iter, and every iteration once the recursive
+            // residual is near the tolerance or a request was seen pending.
+            const int itn = issued + it + 1;
+            if (near || itn % init.reliable_every == 0 || itn >= prm->max_iter) {
+                HIPCHK(c, launch_mixed_fold(s, d, x64, w.x, 0, sc));
+                HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.r, 0, 0, c->red, sc,
+                                                nullptr));
+            }
             if ((rc = precond_fp32(c, pc, w.r, w.z, prm->lambda_min, prm->lambda_max, prm->degree,
                                    EPI_BETA, sc, A32)) != LES_OK)
                 return rc;
@@ -1040,7 +1050,15 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
             finished = true;
         } else if (batch > 0) {
             HIPCHK(c, hipEventSynchronize(c->ev[slot ^ 1]));
-            if (c->sc_host[slot ^ 1].done) finished = true;
+            const SolveScalars& h = c->sc_host[slot ^ 1];
+            if (h.done) finished = true;
+            // near: the residual would reach the tolerance within about two
+            // more polls at the rate of the last one (the host sees a request
+            // up to two batches late)
+            const double drop = last_err > 0.0 && h.err > 0.0 ? last_err / h.err : 1.0;
+            const double reach = std::max(4.0, drop * drop * drop);
+            if (h.reliable_now || h.err < reach * h.tol) near = true;
+            last_err = h.err;
         }
         ++batch;
     }

@@ -164,19 +164,17 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
                             double alpha, int first, int epi, RedScratch red, SolveScalars* sc,
                             const double* wgt = nullptr);
 // Two Chebyshev steps in one pass (k_cheb_pair): steps (beta_a, alpha_a) and
-// (beta_b, alpha_b); p_in/p_out and z_in/z_out distinct.  Not every grid
-// shape is supported (cheb_pair_supported); the caller then runs two
-// launch_cheb_step.
-// target_grid: about how many workgroups to spread the grid over (0: the
-// built-in default).
+// (beta_b, alpha_b); p_in/p_out and z_in/z_out distinct; no reduction.  Not
+// every grid shape is supported (cheb_pair_supported); the caller then runs
+// two launch_cheb_step.  target_grid: about how many workgroups to spread
+// the grid over (0: the built-in default).
 template <typename T>
 bool cheb_pair_supported(const Dims& d, int target_grid);
 template <typename T>
 hipError_t launch_cheb_pair(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
                             const T* z_in, T* z_out, const T* p_in, T* p_out, double alpha0,
                             double beta_a, double alpha_a, double beta_b, double alpha_b,
-                            int first, int epi, RedScratch red, SolveScalars* sc,
-                            const double* wgt, int target_grid);
+                            int first, SolveScalars* sc, int target_grid);
 template <typename T>
 hipError_t launch_cheb_scale(hipStream_t s, const Dims& d, const T* r, T* z, double alpha0,
                              int epi, RedScratch red, SolveScalars* sc,

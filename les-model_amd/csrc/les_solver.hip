@@ -721,7 +721,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(Dims d, Chunking ch, Stenc
 // exactly k_cheb_step's, so results are bit-identical to two single steps.
 // p_in / p_out and z_in / z_out must be distinct buffers: halo rows and
 // planes read inputs that neighbouring workgroups own.
-// Thread layout: tpr = nz / V threads per row, rpp = kBlock / tpr rows per
+// Thread layout: tpr = nz / V threads per row, rpp = TB / tpr rows per
 // pass, P passes; row slot q = pass * rpp + sub holds core row q (q < J) or
 // one of the two halo rows (q = J: j0 - 1, q = J + 1: j0 + J), J = P*rpp - 2.
 // ------------------------------------------------------------------
@@ -731,21 +731,17 @@ struct PairGeom {
     int grid;          // workgroups (multiple of kXcd)
 };
 
-#ifndef LES_PAIR_WAVES_PER_EU
-#define LES_PAIR_WAVES_PER_EU 1
-#endif
-template <typename T, bool FIRST, int CODED, bool REDUCE, int V, int P>
-__global__ __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(LES_PAIR_WAVES_PER_EU)))
-void k_cheb_pair(Dims d, PairGeom g, Stencil<T> A,
-                                                      const T* __restrict__ r,
-                                                      const T* __restrict__ z_in,
-                                                      T* __restrict__ z_out,
-                                                      const T* __restrict__ p_in,
-                                                      T* __restrict__ p_out, double alpha0,
-                                                      double beta_a, double alpha_a,
-                                                      double beta_b, double alpha_b,
-                                                      RedScratch red, const SolveScalars* sc,
-                                                      const double* __restrict__ wgt)
+// TB threads per workgroup.  The pair never closes an application (a
+// reducing last step runs single), so it reduces nothing.
+template <typename T, bool FIRST, int CODED, int V, int P, int TB>
+__global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T> A,
+                                                  const T* __restrict__ r,
+                                                  const T* __restrict__ z_in,
+                                                  T* __restrict__ z_out,
+                                                  const T* __restrict__ p_in,
+                                                  T* __restrict__ p_out, double alpha0,
+                                                  double beta_a, double alpha_a, double beta_b,
+                                                  double alpha_b, const SolveScalars* sc)
 {
     if (is_done(sc)) return;
     extern __shared__ unsigned char les_smem[];
@@ -770,7 +766,6 @@ void k_cheb_pair(Dims d, PairGeom g, Stencil<T> A,
             pkeep[pp][v] = pnew[pp][v] = rkeep[pp][v] = rnew[pp][v] = T(0);
             ckeep[pp][v] = cnew[pp][v] = 0;
         }
-    double acc[1] = {0.0};
     for (int t = ia - 1; t <= ib && ia < d.nx; ++t) {
         // ---- A at plane t: z_{k+1} into the ring, p_k of core rows ----
         if (t >= 0 && t < d.nx) {
@@ -847,13 +842,6 @@ void k_cheb_pair(Dims d, PairGeom g, Stencil<T> A,
                 }
                 vstore<T, V>(p_out + id, pn);
                 vstore<T, V>(z_out + id, zn);
-                if (REDUCE) {
-#pragma unroll
-                    for (int v = 0; v < V; ++v) {
-                        const double rz = (double)rv[v] * (double)zn[v];
-                        acc[0] += wgt ? wgt[id + v] * rz : rz;
-                    }
-                }
             }
         }
         __syncthreads();
@@ -866,7 +854,6 @@ void k_cheb_pair(Dims d, PairGeom g, Stencil<T> A,
                 ckeep[pp][v] = cnew[pp][v];
             }
     }
-    if (REDUCE) block_partial<1>(acc, red.partials, lb);
 }
 
 template <typename T>
@@ -1340,25 +1327,33 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
     return hipSuccess;
 }
 
+// Workgroup size and row passes per thread of the pair kernel (J = passes
+// * rows-per-pass - 2 core rows per workgroup), per element type.
+#ifndef LES_PAIR_BLOCK_F32
+#define LES_PAIR_BLOCK_F32 512
+#endif
 #ifndef LES_PAIR_PASSES_F32
-#define LES_PAIR_PASSES_F32 2
+#define LES_PAIR_PASSES_F32 1
+#endif
+#ifndef LES_PAIR_BLOCK_F64
+#define LES_PAIR_BLOCK_F64 256
 #endif
 #ifndef LES_PAIR_PASSES_F64
 #define LES_PAIR_PASSES_F64 3
 #endif
-// Row passes per thread of the pair kernel (J = passes * rows-per-pass - 2
-// core rows per workgroup).
 template <typename T>
 constexpr int pair_passes() { return sizeof(T) == 4 ? LES_PAIR_PASSES_F32 : LES_PAIR_PASSES_F64; }
+template <typename T>
+constexpr int pair_block() { return sizeof(T) == 4 ? LES_PAIR_BLOCK_F32 : LES_PAIR_BLOCK_F64; }
 
 template <typename T>
 bool cheb_pair_geom(const Dims& d, PairGeom& g, int target_grid)
 {
-    constexpr int V = vec_width<T>(), P = pair_passes<T>();
+    constexpr int V = vec_width<T>(), P = pair_passes<T>(), TB = pair_block<T>();
     if (d.nz % V) return false;
     g.tpr = d.nz / V;
-    if (g.tpr > kBlock || kBlock % g.tpr) return false;
-    g.rpp = kBlock / g.tpr;
+    if (g.tpr > TB || TB % g.tpr) return false;
+    g.rpp = TB / g.tpr;
     g.J = P * g.rpp - 2;
     if (g.J < 2) return false;
     if ((size_t)3 * P * g.rpp * d.nz * sizeof(T) > kPairMaxLds) return false;
@@ -1368,7 +1363,7 @@ bool cheb_pair_geom(const Dims& d, PairGeom& g, int target_grid)
     g.irange = (d.nx + nxc - 1) / nxc;
     nxc = (d.nx + g.irange - 1) / g.irange;
     g.grid = ((g.njb * nxc + kXcd - 1) / kXcd) * kXcd;
-    return g.grid <= kMaxGrid;  // one reduction partial per workgroup
+    return true;
 }
 
 template <typename T>
@@ -1382,36 +1377,25 @@ template <typename T>
 hipError_t launch_cheb_pair(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
                             const T* z_in, T* z_out, const T* p_in, T* p_out, double alpha0,
                             double beta_a, double alpha_a, double beta_b, double alpha_b,
-                            int first, int epi, RedScratch red, SolveScalars* sc,
-                            const double* wgt, int target_grid)
+                            int first, SolveScalars* sc, int target_grid)
 {
-    constexpr int V = vec_width<T>(), P = pair_passes<T>();
+    constexpr int V = vec_width<T>(), P = pair_passes<T>(), TB = pair_block<T>();
     PairGeom g;
     if (!cheb_pair_geom<T>(d, g, target_grid)) return hipErrorNotSupported;
     const size_t lds = (size_t)3 * P * g.rpp * d.nz * sizeof(T);
-#define LES_PAIR(F, CD, RD)                                                                      \
-    hipLaunchKernelGGL((k_cheb_pair<T, F, CD, RD, V, P>), dim3(g.grid), dim3(kBlock), lds, s, d, \
-                       g, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a, beta_b,       \
-                       alpha_b, red, sc, wgt)
-#define LES_PAIR_R(F, CD)                    \
-    do {                                     \
-        if (epi >= 0) LES_PAIR(F, CD, true);  \
-        else LES_PAIR(F, CD, false);          \
-    } while (0)
+#define LES_PAIR(F, CD)                                                                         \
+    hipLaunchKernelGGL((k_cheb_pair<T, F, CD, V, P, TB>), dim3(g.grid), dim3(TB), lds, s, d, g, \
+                       A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a, beta_b,         \
+                       alpha_b, sc)
     if (first) {
-        if (A.code) LES_PAIR_R(true, 1);
-        else LES_PAIR_R(true, 0);
+        if (A.code) LES_PAIR(true, 1);
+        else LES_PAIR(true, 0);
     } else {
-        if (A.code) LES_PAIR_R(false, 1);
-        else LES_PAIR_R(false, 0);
+        if (A.code) LES_PAIR(false, 1);
+        else LES_PAIR(false, 0);
     }
-#undef LES_PAIR_R
 #undef LES_PAIR
-    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
-    if (epi >= 0)
-        return launch_finalize(s, g.grid, 1, false, epi, sc ? GATE_DONE : GATE_NONE, red, sc,
-                               nullptr);
-    return hipSuccess;
+    return hipGetLastError();
 }
 
 template <typename T>
@@ -1546,8 +1530,7 @@ hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double
     template bool cheb_pair_supported<T>(const Dims&, int);                                     \
     template hipError_t launch_cheb_pair<T>(hipStream_t, const Dims&, Stencil<T>, const T*,     \
                                             const T*, T*, const T*, T*, double, double, double, \
-                                            double, double, int, int, RedScratch,               \
-                                            SolveScalars*, const double*, int);                 \
+                                            double, double, int, SolveScalars*, int);           \
     template hipError_t launch_ic_fwd_level<T>(hipStream_t, const Dims&, int, const T*, T*,     \
                                                const T*, const T*, const T*, const T*,          \
                                                const SolveScalars*);                            \
