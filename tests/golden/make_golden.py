@@ -20,6 +20,11 @@ hold inputs and the reference's outputs:
                        centred 8^3 obstacle, 10 steps, PCG-Jacobi to 1e-8
                        relative): per-step PCG iterations and dt, and final
                        field digests (sums, norms, 1024 seeded probes, sha256)
+  c2_128_digest.json   the plain GPU configuration's inputs at 128^3, two
+                       steps with each of its pressure solvers (PCG-Jacobi,
+                       PCG-IC(0), PCG-Chebyshev 4, GS/SOR 200 sweeps): per-step
+                       iterations (sweeps for GS) and dt, final field digests
+                       (`python tests/golden/make_golden.py c2` remakes only it)
 """
 import hashlib
 import json
@@ -119,12 +124,46 @@ def c1_fixture(ref):
         json.dump(out, fh, indent=1)
 
 
+def c2_fixture(ref):
+    """The plain GPU configuration's inputs at 128^3 (spectral velocity seed 0,
+    two random 10-20 cell boxes, seed 1), two driver steps from step 0 with
+    each pressure solver the configuration names: PCG-Jacobi, PCG-IC(0) (20
+    iterations: it does not converge on this operator), PCG-Chebyshev degree
+    4 on lambda from estimate_lambda_max and 3 pi^2, and GS/SOR 200 sweeps."""
+    n = 128
+    m = wl.mesh(n)
+    N = n ** 3
+    solid = wl.random_boxes_mask(n, 2, 1)
+    u, v, w = wl.spectral_velocity(n, 0)
+    st0 = initial_state(u, v, w, N, solid, m, ref)
+    A = ref.build_stencil(solid, m)
+    lmax = ref.lambda_max(A, m, 20)
+    out = dict(config="128^3 spectral seed 0, 2 random boxes (seed 1), nu=1e-3, Cs=0.18 "
+                      "(reference), dt=1e-3, 2 steps per solver", lambda_max=lmax, runs={})
+    runs = {"pcg_jacobi": dict(solver=1, max_iter=20000, tol=1e-8, tol_rel=True),
+            "pcg_ic0_20": dict(solver=2, max_iter=20, tol=0.0, tol_rel=False),
+            "pcg_cheb4": dict(solver=4, max_iter=20000, tol=1e-8, tol_rel=True,
+                              lmin=3 * np.pi ** 2, lmax=lmax, degree=4),
+            "gs_200": dict(solver=5, gs_iters=200)}
+    for name, kw in runs.items():
+        st, its, dts = ref.run_steps(st0, solid, m, None, 2, step0=0, nu=1e-3, rho=1.2, **kw)
+        out["runs"][name] = dict(iters=[int(i) for i in its], dts=[float(x) for x in dts],
+                                 fields={k: digest(st[k]) for k in ("u", "v", "w", "p")})
+        print(name, its, flush=True)
+    with open(os.path.join(HERE, "c2_128_digest.json"), "w") as fh:
+        json.dump(out, fh, indent=1)
+
+
 def main():
     ref = Reference()
     ref.set_threads(1)
+    if sys.argv[1:] == ["c2"]:
+        c2_fixture(ref)
+        return
     kernels_fixture(ref)
     steps_fixture(ref)
     c1_fixture(ref)
+    c2_fixture(ref)
     for fn in sorted(os.listdir(HERE)):
         print(fn, os.path.getsize(os.path.join(HERE, fn)))
 
