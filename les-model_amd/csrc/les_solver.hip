@@ -733,7 +733,11 @@ struct PairGeom {
 
 // TB threads per workgroup.  The pair never closes an application (a
 // reducing last step runs single), so it reduces nothing.
-template <typename T, bool FIRST, int CODED, int V, int P, int TB>
+// R: LDS ring depth in planes.  With 3 a second barrier per plane keeps step
+// A of the next plane from overwriting what step B still reads; with 4 the
+// plane A(t+1) overwrites was last read by B(t-2), which every wave finished
+// before the barrier of plane t, so one barrier per plane suffices.
+template <typename T, bool FIRST, int CODED, int V, int P, int TB, int R>
 __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T> A,
                                                   const T* __restrict__ r,
                                                   const T* __restrict__ z_in,
@@ -745,7 +749,7 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
 {
     if (is_done(sc)) return;
     extern __shared__ unsigned char les_smem[];
-    T* zs = reinterpret_cast<T*>(les_smem);  // [3][P*rpp][nz]
+    T* zs = reinterpret_cast<T*>(les_smem);  // [R][P*rpp][nz]
     const int lb = logical_block(g.grid);
     const int jb = lb % g.njb, ic = lb / g.njb;
     const int j0 = jb * g.J;
@@ -802,16 +806,16 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                     rnew[pp][v] = rv[v];
                     if (CODED == 1) cnew[pp][v] = cd[v];
                 }
-                vstore<T, V>(zs + (t + 3) % 3 * plane_sz + a * d.nz + kk, zn);
+                vstore<T, V>(zs + (t + R) % R * plane_sz + a * d.nz + kk, zn);
             }
         }
         __syncthreads();
         // ---- B at plane t-1 from the ring ----
         const int iB = t - 1;
         if (iB >= ia && iB < ib) {
-            const T* zc_pl = zs + (iB + 3) % 3 * plane_sz;
-            const T* zm_pl = zs + (iB + 2) % 3 * plane_sz;
-            const T* zp_pl = zs + (iB + 4) % 3 * plane_sz;
+            const T* zc_pl = zs + (iB + R) % R * plane_sz;
+            const T* zm_pl = zs + (iB + R - 1) % R * plane_sz;
+            const T* zp_pl = zs + (iB + 1) % R * plane_sz;
 #pragma unroll
             for (int pp = 0; pp < P; ++pp) {
                 const int q = pp * g.rpp + sub;
@@ -844,7 +848,7 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                 vstore<T, V>(z_out + id, zn);
             }
         }
-        __syncthreads();
+        if (R == 3) __syncthreads();
 #pragma unroll
         for (int pp = 0; pp < P; ++pp)
 #pragma unroll
@@ -1345,10 +1349,18 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
 #ifndef LES_PAIR_PASSES_F64
 #define LES_PAIR_PASSES_F64 3
 #endif
+#ifndef LES_PAIR_RING_F32
+#define LES_PAIR_RING_F32 4
+#endif
+#ifndef LES_PAIR_RING_F64
+#define LES_PAIR_RING_F64 4
+#endif
 template <typename T>
 constexpr int pair_passes() { return sizeof(T) == 4 ? LES_PAIR_PASSES_F32 : LES_PAIR_PASSES_F64; }
 template <typename T>
 constexpr int pair_block() { return sizeof(T) == 4 ? LES_PAIR_BLOCK_F32 : LES_PAIR_BLOCK_F64; }
+template <typename T>
+constexpr int pair_ring() { return sizeof(T) == 4 ? LES_PAIR_RING_F32 : LES_PAIR_RING_F64; }
 
 template <typename T>
 bool cheb_pair_geom(const Dims& d, PairGeom& g, int target_grid)
@@ -1360,7 +1372,7 @@ bool cheb_pair_geom(const Dims& d, PairGeom& g, int target_grid)
     g.rpp = TB / g.tpr;
     g.J = P * g.rpp - 2;
     if (g.J < 2) return false;
-    if ((size_t)3 * P * g.rpp * d.nz * sizeof(T) > kPairMaxLds) return false;
+    if ((size_t)pair_ring<T>() * P * g.rpp * d.nz * sizeof(T) > kPairMaxLds) return false;
     g.njb = (d.ny + g.J - 1) / g.J;
     if (target_grid <= 0) target_grid = sizeof(T) == 4 ? kPairTargetGridF32 : kPairTargetGridF64;
     int nxc = std::max(1, std::min(d.nx, target_grid / g.njb));
@@ -1383,13 +1395,14 @@ hipError_t launch_cheb_pair(hipStream_t s, const Dims& d, Stencil<T> A, const T*
                             double beta_a, double alpha_a, double beta_b, double alpha_b,
                             int first, SolveScalars* sc, int target_grid)
 {
-    constexpr int V = vec_width<T>(), P = pair_passes<T>(), TB = pair_block<T>();
+    constexpr int V = vec_width<T>(), P = pair_passes<T>(), TB = pair_block<T>(),
+                  R = pair_ring<T>();
     PairGeom g;
     if (!cheb_pair_geom<T>(d, g, target_grid)) return hipErrorNotSupported;
-    const size_t lds = (size_t)3 * P * g.rpp * d.nz * sizeof(T);
+    const size_t lds = (size_t)R * P * g.rpp * d.nz * sizeof(T);
 #define LES_PAIR(F, CD)                                                                         \
-    hipLaunchKernelGGL((k_cheb_pair<T, F, CD, V, P, TB>), dim3(g.grid), dim3(TB), lds, s, d, g, \
-                       A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a, beta_b,         \
+    hipLaunchKernelGGL((k_cheb_pair<T, F, CD, V, P, TB, R>), dim3(g.grid), dim3(TB), lds, s, d, \
+                       g, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a, beta_b,      \
                        alpha_b, sc)
     if (first) {
         if (A.code) LES_PAIR(true, 1);
