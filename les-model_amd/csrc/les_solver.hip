@@ -91,6 +91,36 @@ __device__ __forceinline__ void vstore(T* __restrict__ p, const T (&in)[V])
     }
 }
 
+// Streaming (non-temporal) forms for data that is read or written once per
+// launch; LES_NT selects them (0: ordinary loads and stores).
+#ifndef LES_NT
+#define LES_NT 0
+#endif
+template <typename T, int V>
+__device__ __forceinline__ void vload_s(const T* __restrict__ p, T (&out)[V])
+{
+    if constexpr (LES_NT && V > 1) {
+        const vec_t<T, V> x = __builtin_nontemporal_load(reinterpret_cast<const vec_t<T, V>*>(p));
+#pragma unroll
+        for (int v = 0; v < V; ++v) out[v] = x[v];
+    } else {
+        vload<T, V>(p, out);
+    }
+}
+
+template <typename T, int V>
+__device__ __forceinline__ void vstore_s(T* __restrict__ p, const T (&in)[V])
+{
+    if constexpr (LES_NT && V > 1) {
+        vec_t<T, V> x;
+#pragma unroll
+        for (int v = 0; v < V; ++v) x[v] = in[v];
+        __builtin_nontemporal_store(x, reinterpret_cast<vec_t<T, V>*>(p));
+    } else {
+        vstore<T, V>(p, in);
+    }
+}
+
 // Cells per thread of the streaming kernels: 16-byte vectors when the row
 // length allows them, else 1.
 template <typename T>
@@ -550,8 +580,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Dims d, Chunking ch, TX* 
             xv[v] = xv[v] + ax * (TX)pv[v];
             rv[v] = rv[v] + ar * qv[v];
         }
-        vstore<TX, V>(x + id, xv);
-        vstore<T, V>(r + id, rv);
+        vstore_s<TX, V>(x + id, xv);
+        vstore_s<T, V>(r + id, rv);
         T dv[V];
         if (diag) vload<T, V>(diag + id, dv);
 #pragma unroll
@@ -782,14 +812,14 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                 const int i = t, k = kk, id = (i * d.ny + j) * d.nz + kk;
                 Nbr<T, V> nb;
                 T rv[V], az[V], pp_[V], zc[V];
-                vload<T, V>(r + id, rv);
+                vload_s<T, V>(r + id, rv);
                 if (FIRST) {
                     load_nbr<T, V>(r, d, id, i, j, k, [&](T x) { return T(0) + a0 * x; }, nb);
 #pragma unroll
                     for (int v = 0; v < V; ++v) pp_[v] = rv[v];
                 } else {
                     load_nbr<T, V>(z_in, d, id, i, j, k, [](T x) { return x; }, nb);
-                    vload<T, V>(p_in + id, pp_);
+                    vload_s<T, V>(p_in + id, pp_);
                 }
 #pragma unroll
                 for (int v = 0; v < V; ++v) zc[v] = nb.c[v];
@@ -844,8 +874,8 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                     pn[v] = res + btb * pkeep[pp][v];
                     zn[v] = nb.c[v] + atb * pn[v];
                 }
-                vstore<T, V>(p_out + id, pn);
-                vstore<T, V>(z_out + id, zn);
+                vstore_s<T, V>(p_out + id, pn);
+                vstore_s<T, V>(z_out + id, zn);
             }
         }
         if (R == 3) __syncthreads();
@@ -948,8 +978,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_mixed(Dims d, Chunking ch
             rv[v] = rv[v] + ar * qv[v];
             acc[0] += (double)rv[v] * (double)rv[v];
         }
-        vstore<float, V>(x + id, xv);
-        vstore<float, V>(r + id, rv);
+        vstore_s<float, V>(x + id, xv);
+        vstore_s<float, V>(r + id, rv);
     }
     block_partial<1>(acc, red.partials, lb);
 }
