@@ -181,6 +181,38 @@ def cpu_baseline(m, st0, solid, bounds, threads, step0, gpu_iters_per_step):
                        f"{gpu_iters_per_step:.0f} iterations per step")
 
 
+def adjoint_timing():
+    """The differentiation workload (96^3, FP64, 20 fixed-dt steps with
+    checkpoints every 5, pressure to 1e-13 relative): wall time of the
+    rollout and of the adjoint (which recomputes each segment)."""
+    import torch
+    import lesgpu
+    import les_workloads as wl
+    n = 96
+    m = wl.mesh(n)
+    g = lesgpu.LesGpu.from_mesh(m)
+    g.set_mask(np.zeros(n ** 3, dtype=np.uint8))
+    u, v, w = wl.spectral_velocity(n, 0)
+    for name, a in zip("uvw", (u, v, w)):
+        g.set_field(name, a)
+    g.velocity_bcs()
+    g.build_stencil()
+    prm = g.adj_params(dt=1e-3, nsteps=20, checkpoint_every=5, Cs=0.17)
+    g.synchronize()
+    t0 = time.perf_counter()
+    fwd = g.rollout(prm)
+    t1 = time.perf_counter()
+    ub = [np.ones(n ** 3) * m["dx"] ** 3 for _ in range(3)]
+    bwd = g.adjoint(prm, *ub)
+    t2 = time.perf_counter()
+    g.close()
+    return {"workload": "96^3 FP64, 20 steps dt=1e-3, checkpoints every 5, Jacobi-PCG in the "
+                        "mirror-weighted inner product to 1e-13 rel",
+            "rollout_s": t1 - t0, "adjoint_s": t2 - t1, "rollout_pcg_iters": fwd["iters"],
+            "adjoint_pcg_iters": bwd["iters"], "unconverged": fwd["unconverged"] +
+            bwd["unconverged"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +225,8 @@ def main():
                     help="precision:stencil pairs; precision fp64|mixed, stencil arrays|coded")
     ap.add_argument("--check-every", type=int, default=0)
     ap.add_argument("--step0", type=int, default=21)
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the timing of the differentiation workload")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="launch the plain FP64 copy kernel this many times first "
                          "(reference size for HBM byte counters)")
@@ -311,6 +345,10 @@ def main():
                 a, b = R["state"][k], F["state"][k]
                 err[k] = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
             out["mixed_max_rel_err_vs_fp64"] = err
+    if not args.no_adjoint:
+        log("differentiation workload: 96^3 rollout + adjoint")
+        out["adjoint"] = adjoint_timing()
+        log(f"  rollout {out['adjoint']['rollout_s']:.2f} s, adjoint {out['adjoint']['adjoint_s']:.2f} s")
     if not args.no_cpu_baseline:
         log(f"CPU baseline: 1 reference step on {args.cpu_threads} threads")
         try:
