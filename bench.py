@@ -213,6 +213,79 @@ def adjoint_timing():
             bwd["unconverged"]}
 
 
+def io_timing(g, m, ref_sample=True):
+    """The next-row paths on the workload's grid, from the fields the timed
+    run left on the device: the statistics update kernel (HIP events around
+    each launch), the checkpoint save and load, and the VTK writer (blocking
+    and background), with files written under TMPDIR and removed.  The
+    reference's VTK writer is timed on a 64^3 sample and scaled by the cell
+    count (bounded CPU work)."""
+    import shutil
+    import tempfile
+    import lesgpu
+    N = m["nx"] * m["ny"] * m["nz"]
+    out = {}
+    g.stats_init(0)
+    for s in range(2):
+        g.stats_update(1 + s)
+    g.kernel_timing(lesgpu.KT_STATS)
+    reps = 20
+    for s in range(reps):
+        g.stats_update(3 + s)
+    n_l, ms, _ = g.kernel_timing_read()
+    g.kernel_timing(lesgpu.KT_NONE)
+    us = 1e3 * ms / max(n_l, 1)
+    gbs = 192.0 * N / (us * 1e-6) / 1e9
+    out["stats_update"] = {"kernel": "k_stats_update", "avg_us": us, "launches": n_l,
+                           "bytes_per_cell": 192, "achieved_gbs": gbs,
+                           "frac": gbs / HBM_PEAK_GBS}
+    d = tempfile.mkdtemp(prefix="les_io_", dir=os.environ.get("TMPDIR"))
+    try:
+        g.synchronize()
+        t0 = time.perf_counter()
+        g.save_state(1.0, 1e-3, 100, d)
+        t1 = time.perf_counter()
+        g.load_state(d)
+        t2 = time.perf_counter()
+        size = os.path.getsize(os.path.join(d, "checkpoint.data"))
+        out["checkpoint"] = {"bytes": size, "save_s": t1 - t0, "load_s": t2 - t1,
+                             "save_gbs": size / (t1 - t0) / 1e9,
+                             "load_gbs": size / (t2 - t1) / 1e9}
+        os.remove(os.path.join(d, "checkpoint.data"))
+        path = os.path.join(d, "out.vtk")
+        t0 = time.perf_counter()
+        g.write_field_vtk(100, path)
+        t1 = time.perf_counter()
+        vsize = os.path.getsize(path)
+        os.remove(path)
+        g.synchronize()
+        t2 = time.perf_counter()
+        g.write_field_vtk(200, path, background=True)
+        t3 = time.perf_counter()
+        g.io_wait()
+        t4 = time.perf_counter()
+        out["vtk"] = {"bytes": vsize, "threads": int(os.environ.get("LES_IO_THREADS", "0")) or
+                      min(16, os.cpu_count() or 1), "write_s": t1 - t0,
+                      "background_call_s": t3 - t2, "background_total_s": t4 - t2}
+        if ref_sample:
+            from cpu_checkers import REF_SO, Reference
+            if os.path.exists(REF_SO):
+                import les_workloads as wl
+                n = 64
+                ms_ = wl.mesh(n)
+                r = np.random.default_rng(0)
+                f = {k: r.standard_normal(n ** 3) for k in ("u", "v", "w", "p", "nu_t")}
+                f["solid"] = np.zeros(n ** 3, np.uint8)
+                t0 = time.perf_counter()
+                Reference().write_field_vtk(d, f, ms_, 0)
+                t1 = time.perf_counter()
+                out["vtk"]["reference_writer_s_scaled"] = (t1 - t0) * N / n ** 3
+                out["vtk"]["reference_sample"] = "write_field_vtk (io.cpp:225) on 64^3, 1 thread"
+    finally:
+        shutil.rmtree(d, ignore_errors=True)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -227,6 +300,8 @@ def main():
     ap.add_argument("--step0", type=int, default=21)
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the timing of the differentiation workload")
+    ap.add_argument("--no-io", action="store_true",
+                    help="skip the statistics / checkpoint / VTK timings")
     ap.add_argument("--copy-calibration", type=int, default=0,
                     help="launch the plain FP64 copy kernel this many times first "
                          "(reference size for HBM byte counters)")
@@ -349,6 +424,10 @@ def main():
         log("differentiation workload: 96^3 rollout + adjoint")
         out["adjoint"] = adjoint_timing()
         log(f"  rollout {out['adjoint']['rollout_s']:.2f} s, adjoint {out['adjoint']['adjoint_s']:.2f} s")
+    if not args.no_io:
+        log("statistics, checkpoint and VTK output on the workload's grid")
+        out["io"] = io_timing(g, m, ref_sample=not args.no_cpu_baseline)
+        log(f"  {json.dumps(out['io'])}")
     if not args.no_cpu_baseline:
         log(f"CPU baseline: 1 reference step on {args.cpu_threads} threads")
         try:

@@ -18,7 +18,8 @@
  *   - A context owns all device memory; the caller owns host buffers.
  *     ptr_kind says whether a caller pointer is host (0) or device (1).
  *   - Every function returns 0 on success or a negative LES_ERR_* code;
- *     les_last_error() gives the message.  Non-convergence of a solver is
+ *     les_last_error() gives the message (les_last_error(NULL) for a call
+ *     made without a context).  Non-convergence of a solver is
  *     not an error (it is reported in the result struct, mirroring
  *     pressure.cpp:411-413).
  *   - Calls are ordered on the context's stream.  Calls that return host
@@ -39,6 +40,7 @@ extern "C" {
 #define LES_ERR_HIP (-2)
 #define LES_ERR_STATE (-3)
 #define LES_ERR_NOMEM (-4)
+#define LES_ERR_IO (-5)
 
 typedef struct les_ctx les_ctx;
 
@@ -261,12 +263,70 @@ int les_rollout(les_ctx* ctx, const les_adj_params* prm, les_adj_stats* stats);
 int les_adjoint(les_ctx* ctx, const les_adj_params* prm, double* ubar, double* vbar,
                 double* wbar, int ptr_kind, les_adj_stats* stats);
 
+/* ---- checkpoint, VTK output and time statistics (src/io.cpp, src/statistics.cpp) ----
+ * The files are byte-identical to the reference writers' for the same field
+ * values.  A NULL path or directory selects the reference's own relative
+ * one.  Unlike the reference, which prints a message and writes nothing, a
+ * file that cannot be opened or written returns LES_ERR_IO. */
+
+/* save_state (io.cpp:14-59): <dir>/checkpoint.meta and <dir>/checkpoint.data
+ * (dir NULL: "checkpoint") from the context's u, v, w, p, nu_t and mask.
+ * The fields stream through two pinned staging chunks; synchronises. */
+int les_save_state(les_ctx* ctx, const char* dir, double time_simulated, double dt,
+                   int next_step_to_run);
+/* load_state (io.cpp:61-120) into the context's u, v, w, p, nu_t and mask.
+ * *next_step receives the meta file's step, or 0 where the reference
+ * returns 0: no meta file (time and dt untouched), no data file, or nothing
+ * read for the mask.  A short data file changes the fields as the reference
+ * changes its arrays: a section read in part is overwritten in part, later
+ * sections not at all, and the mask is set to (byte == 1) where it was read
+ * and 0 beyond.  The stencil is not rebuilt.  Synchronises. */
+int les_load_state(les_ctx* ctx, const char* dir, double* time_simulated, double* dt,
+                   int* next_step);
+
+/* write_field_vtk (io.cpp:225-256) from the context's fields (path NULL:
+ * "output/output_<step>.vtk").  Synchronises; the formatting runs on several
+ * host threads (LES_IO_THREADS, default the hardware's, at most 16). */
+int les_write_field_vtk(les_ctx* ctx, const char* path, int step);
+/* The same file written in the background: the fields are snapshotted on the
+ * device, ordered on the context's stream, and a host thread copies, formats
+ * and writes them while further work runs.  A later VTK call waits for it;
+ * les_io_wait waits and returns the first background failure. */
+int les_write_field_vtk_async(les_ctx* ctx, const char* path, int step);
+int les_io_wait(les_ctx* ctx);
+/* The same file from host arrays; no context, no device work. */
+int les_write_field_vtk_host(const char* path, int step, int nx, int ny, int nz, double dx,
+                             double dy, double dz, const double* u, const double* v,
+                             const double* w, const double* p, const double* nu_t,
+                             const uint8_t* solid);
+
+/* Time statistics on the device (statistics.cpp:70-131): ten running sums
+ * per cell, in this slot order. */
+enum {
+    LES_ST_U = 0, LES_ST_V, LES_ST_W, LES_ST_P,
+    LES_ST_UU, LES_ST_VV, LES_ST_WW, LES_ST_UV, LES_ST_UW, LES_ST_VW, LES_ST_COUNT
+};
+/* init_statistics: zeroes the sums; samples are taken for steps > start_step. */
+int les_stats_init(les_ctx* ctx, int start_step);
+/* update_statistics: adds the context's u, v, w, p (and their products) when
+ * current_step > start_step. */
+int les_stats_update(les_ctx* ctx, int current_step);
+/* finalize_and_save (statistics.cpp:133-186): the means and Reynolds
+ * stresses are formed on the device and written to path (NULL:
+ * "output/statistics_final.vtk"; "": formed, not written).  Nothing happens
+ * when no sample was taken. */
+int les_stats_finalize(les_ctx* ctx, const char* path);
+/* One slot: the running sum (final 0) or, after les_stats_finalize, the mean
+ * (slots U..P) or stress (UU..VW) (final 1). */
+int les_stats_get(les_ctx* ctx, int which, int final, double* out, int ptr_kind);
+int les_stats_samples(les_ctx* ctx, int* num_samples);
+
 /* ---- diagnostics (no reference counterpart) ----
  * Brackets every launch of one kernel class with HIP events on the
  * context's stream so its average duration can be read back.  Starting
  * resets the record; LES_KT_NONE stops. */
 enum { LES_KT_NONE = 0, LES_KT_CHEB_STEP = 1, LES_KT_PCG_MATVEC = 2, LES_KT_PCG_UPDATE = 3,
-       LES_KT_MOMENTUM = 4, LES_KT_IC_SOLVE = 5, LES_KT_CHEB_PAIR = 6 };
+       LES_KT_MOMENTUM = 4, LES_KT_IC_SOLVE = 5, LES_KT_CHEB_PAIR = 6, LES_KT_STATS = 7 };
 int les_kernel_timing(les_ctx* ctx, int kernel_class);
 /* launches recorded, their summed duration in ms, and the bytes per cell the
  * class is charged (element size of the run) are returned; synchronises. */

@@ -13,13 +13,16 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <fstream>
 #include <new>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/les_gpu.h"
 #include "les_adjoint.hpp"
+#include "les_io.hpp"
 #include "les_mixed.hpp"
 #include "les_physics.hpp"
 
@@ -74,6 +77,20 @@ struct les_ctx {
     double roll_dt = 0, roll_nu = 0, roll_rho = 0, roll_Cs = 0;
     bool cheb_pairs = true;           // fuse Chebyshev steps in pairs where the grid allows
     int pair_grid = 0;                // workgroups of a pair launch (0: built-in default)
+    // checkpoint, VTK output, statistics
+    void* io_stage[2] = {};           // pinned staging chunks of the checkpoint streams
+    hipEvent_t io_ev[2] = {};         // completion of each chunk's copy
+    double* stat_sum[kStatArrays] = {};   // running sums (device)
+    double* stat_out[kStatArrays] = {};   // means and stresses (device)
+    bool have_stats = false, stat_final = false;
+    int stat_start = 0, stat_samples = 0;
+    unsigned char* vtk_snap = nullptr;    // device snapshot: u, v, w, p, nu_t, mask
+    std::vector<unsigned char> vtk_host;  // host copy of the same
+    hipStream_t io_stream = nullptr;      // the writer thread's copies
+    hipEvent_t snap_ev = nullptr;
+    std::thread io_thread;
+    int io_rc = 0;                        // first failure of a background write
+    std::string io_err;
     // kernel timing diagnostics
     int kt_class = 0;
     int kt_elem = 0;
@@ -414,6 +431,7 @@ int les_ctx_create(int nx, int ny, int nz, double dx, double dy, double dz, les_
 int les_ctx_destroy(les_ctx* c)
 {
     if (!c) return LES_OK;
+    if (c->io_thread.joinable()) c->io_thread.join();
     if (c->stream) hipStreamSynchronize(c->stream);
     auto fr = [](void* p) {
         if (p) hipFree(p);
@@ -446,6 +464,17 @@ int les_ctx_destroy(les_ctx* c)
     for (auto ev : c->ev)
         if (ev) hipEventDestroy(ev);
     for (auto ev : c->kt_events) hipEventDestroy(ev);
+    for (int q = 0; q < 2; ++q) {
+        if (c->io_stage[q]) hipHostFree(c->io_stage[q]);
+        if (c->io_ev[q]) hipEventDestroy(c->io_ev[q]);
+    }
+    for (int a = 0; a < kStatArrays; ++a) {
+        fr(c->stat_sum[a]);
+        fr(c->stat_out[a]);
+    }
+    fr(c->vtk_snap);
+    if (c->snap_ev) hipEventDestroy(c->snap_ev);
+    if (c->io_stream) hipStreamDestroy(c->io_stream);
     if (c->own) hipStreamDestroy(c->own);
     delete c;
     return LES_OK;
@@ -1719,6 +1748,372 @@ int les_diag_copy(les_ctx* c, int reps)
     for (int r = 0; r < reps; ++r)
         HIPCHK(c, (launch_convert<double, double>(c->stream, c->d.N, c->t1, c->t2)));
     HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------------
+// checkpoint (io.cpp:14-120), VTK output (io.cpp:225-256) and time
+// statistics (statistics.cpp:70-186) of the device-resident state
+// ------------------------------------------------------------------
+namespace {
+
+constexpr size_t kIoChunk = size_t(32) << 20;  // bytes per staging chunk
+
+int ensure_stage(les_ctx* c)
+{
+    for (int q = 0; q < 2; ++q) {
+        if (!c->io_stage[q]) HIPCHK(c, hipHostMalloc(&c->io_stage[q], kIoChunk, 0));
+        if (!c->io_ev[q]) HIPCHK(c, hipEventCreateWithFlags(&c->io_ev[q], hipEventDisableTiming));
+    }
+    return LES_OK;
+}
+
+std::string join_path(const char* dir, const char* def_dir, const char* name)
+{
+    return std::string(dir ? dir : def_dir) + "/" + name;
+}
+
+// Appends `bytes` device bytes at src to f through the two staging chunks:
+// the copy of chunk k+1 runs while chunk k is written.  mask01 writes each
+// byte as 1 if nonzero, else 0 (Field::is_solid as io.cpp:71-74 stores it).
+int device_to_file(les_ctx* c, FILE* f, const void* src, size_t bytes, bool mask01,
+                   const std::string& path)
+{
+    const size_t nch = (bytes + kIoChunk - 1) / kIoChunk;
+    auto enqueue = [&](size_t k) -> hipError_t {
+        const size_t off = k * kIoChunk, len = std::min(kIoChunk, bytes - off);
+        hipError_t e = hipMemcpyAsync(c->io_stage[k & 1], (const char*)src + off, len,
+                                      hipMemcpyDeviceToHost, c->stream);
+        return e == hipSuccess ? hipEventRecord(c->io_ev[k & 1], c->stream) : e;
+    };
+    if (nch) HIPCHK(c, enqueue(0));
+    for (size_t k = 0; k < nch; ++k) {
+        if (k + 1 < nch) HIPCHK(c, enqueue(k + 1));
+        HIPCHK(c, hipEventSynchronize(c->io_ev[k & 1]));
+        const size_t len = std::min(kIoChunk, bytes - k * kIoChunk);
+        unsigned char* b = (unsigned char*)c->io_stage[k & 1];
+        if (mask01)
+            for (size_t i = 0; i < len; ++i) b[i] = b[i] ? 1 : 0;
+        if (std::fwrite(b, 1, len, f) != len)
+            return fail(c, LES_ERR_IO, "write to " + path + " failed");
+    }
+    return LES_OK;
+}
+
+// Reads up to `bytes` from `in` into device memory at dst the way one
+// std::istream::read of that size fills an array: a short read overwrites
+// the prefix it got and leaves the stream failed, so later reads get
+// nothing.  mask: the section is the reference's zero-initialised byte
+// buffer turned into is_solid = (byte == 1) (io.cpp:103-107), so every byte
+// is written, 0 past the data.  *got receives the bytes read.
+int file_to_device(les_ctx* c, std::ifstream& in, void* dst, size_t bytes, bool mask,
+                   size_t* got)
+{
+    size_t total = 0;
+    for (size_t off = 0, k = 0; off < bytes; off += kIoChunk, ++k) {
+        const size_t len = std::min(kIoChunk, bytes - off);
+        HIPCHK(c, hipEventSynchronize(c->io_ev[k & 1]));  // the chunk's last copy is done
+        char* b = (char*)c->io_stage[k & 1];
+        size_t n = 0;
+        if (in) {
+            in.read(b, (std::streamsize)len);
+            n = (size_t)in.gcount();
+        }
+        total += n;
+        size_t up = n;
+        if (mask) {
+            for (size_t i = 0; i < n; ++i) b[i] = b[i] == 1 ? 1 : 0;
+            std::memset(b + n, 0, len - n);
+            up = len;
+        }
+        if (up) {
+            HIPCHK(c, hipMemcpyAsync((char*)dst + off, b, up, hipMemcpyHostToDevice, c->stream));
+            HIPCHK(c, hipEventRecord(c->io_ev[k & 1], c->stream));
+        }
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    *got = total;
+    return LES_OK;
+}
+
+VtkGrid vtk_grid(const les_ctx* c) { return VtkGrid{c->d.nx, c->d.ny, c->d.nz, c->dx, c->dy, c->dz}; }
+
+constexpr int kVtkFields[5] = {LES_F_U, LES_F_V, LES_F_W, LES_F_P, LES_F_NU_T};
+
+// Waits for a background VTK write; its failure, if any, is kept for les_io_wait.
+void join_writer(les_ctx* c)
+{
+    if (c->io_thread.joinable()) c->io_thread.join();
+}
+
+int write_vtk_from_host(les_ctx* c, const std::string& path, int step)
+{
+    const size_t N = (size_t)c->d.N;
+    const double* h = (const double*)c->vtk_host.data();
+    std::string err;
+    if (vtk_write_fields(path, step, vtk_grid(c), h, h + N, h + 2 * N, h + 3 * N, h + 4 * N,
+                         c->vtk_host.data() + 5 * N * sizeof(double), err))
+        return fail(c, LES_ERR_IO, err);
+    return LES_OK;
+}
+
+int ensure_vtk_host(les_ctx* c)
+{
+    const size_t bytes = (size_t)c->d.N * (5 * sizeof(double) + 1);
+    if (c->vtk_host.size() != bytes) {
+        try {
+            c->vtk_host.resize(bytes);
+        } catch (const std::bad_alloc&) {
+            return fail(c, LES_ERR_NOMEM, "host buffer for the VTK fields");
+        }
+    }
+    return LES_OK;
+}
+
+std::string vtk_path(const char* path, int step)
+{
+    return path ? std::string(path) : "output/output_" + std::to_string(step) + ".vtk";
+}
+
+int ensure_stats(les_ctx* c, double** arr)
+{
+    int rc;
+    for (int a = 0; a < kStatArrays; ++a)
+        if ((rc = alloc(c, &arr[a], c->d.N)) != LES_OK) return rc;
+    return LES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int les_save_state(les_ctx* c, const char* dir, double time_simulated, double dt,
+                   int next_step_to_run)
+{
+    if (!c) return LES_ERR_ARG;
+    int rc;
+    if ((rc = ensure_stage(c)) != LES_OK) return rc;
+    std::string err;
+    if (checkpoint_write_meta(join_path(dir, "checkpoint", "checkpoint.meta"), next_step_to_run,
+                              time_simulated, dt, err))
+        return fail(c, LES_ERR_IO, err);
+    const std::string path = join_path(dir, "checkpoint", "checkpoint.data");
+    FILE* f = std::fopen(path.c_str(), "wb");
+    if (!f) return fail(c, LES_ERR_IO, "cannot open " + path + " for writing");
+    for (int q : kVtkFields)
+        if ((rc = device_to_file(c, f, c->f[q], nbytes(c, 8), false, path)) != LES_OK) {
+            std::fclose(f);
+            return rc;
+        }
+    rc = device_to_file(c, f, c->solid, nbytes(c, 1), true, path);
+    if (std::fclose(f) != 0 && rc == LES_OK) rc = fail(c, LES_ERR_IO, "closing " + path + " failed");
+    return rc;
+}
+
+int les_load_state(les_ctx* c, const char* dir, double* time_simulated, double* dt,
+                   int* next_step)
+{
+    if (!c || !time_simulated || !dt || !next_step) return fail(c, LES_ERR_ARG, "NULL argument");
+    *next_step = 0;
+    std::ifstream meta(join_path(dir, "checkpoint", "checkpoint.meta"));
+    if (!meta) return LES_OK;  // no checkpoint: start from scratch
+    int start = 0;
+    meta >> start;
+    meta >> *time_simulated;
+    meta >> *dt;
+    meta.close();
+    std::ifstream data(join_path(dir, "checkpoint", "checkpoint.data"), std::ios::binary);
+    if (!data) {
+        c->err = "found checkpoint.meta but not checkpoint.data";
+        return LES_OK;
+    }
+    int rc;
+    if ((rc = ensure_stage(c)) != LES_OK) return rc;
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    size_t got = 0;
+    for (int q : kVtkFields)
+        if ((rc = file_to_device(c, data, c->f[q], nbytes(c, 8), false, &got)) != LES_OK) return rc;
+    if ((rc = file_to_device(c, data, c->solid, nbytes(c, 1), true, &got)) != LES_OK) return rc;
+    c->have_roll = false;
+    if (got == 0) {
+        c->err = "error reading checkpoint data: the file might be corrupt";
+        return LES_OK;
+    }
+    *next_step = start;
+    return LES_OK;
+}
+
+int les_write_field_vtk(les_ctx* c, const char* path, int step)
+{
+    if (!c) return LES_ERR_ARG;
+    join_writer(c);
+    int rc;
+    if ((rc = ensure_vtk_host(c)) != LES_OK) return rc;
+    const size_t N = (size_t)c->d.N;
+    unsigned char* h = c->vtk_host.data();
+    for (int q = 0; q < 5; ++q)
+        HIPCHK(c, hipMemcpyAsync(h + q * N * sizeof(double), c->f[kVtkFields[q]], N * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(h + 5 * N * sizeof(double), c->solid, N, hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    return write_vtk_from_host(c, vtk_path(path, step), step);
+}
+
+int les_write_field_vtk_async(les_ctx* c, const char* path, int step)
+{
+    if (!c) return LES_ERR_ARG;
+    join_writer(c);
+    int rc;
+    if ((rc = ensure_vtk_host(c)) != LES_OK) return rc;
+    const size_t N = (size_t)c->d.N, fb = N * sizeof(double);
+    if (!c->vtk_snap && (rc = alloc(c, &c->vtk_snap, 5 * fb + N, false)) != LES_OK) return rc;
+    if (!c->io_stream) HIPCHK(c, hipStreamCreateWithFlags(&c->io_stream, hipStreamNonBlocking));
+    if (!c->snap_ev) HIPCHK(c, hipEventCreateWithFlags(&c->snap_ev, hipEventDisableTiming));
+    // the snapshot is ordered after the work already queued on the context's
+    // stream and before whatever is queued after this call
+    for (int q = 0; q < 5; ++q)
+        HIPCHK(c, hipMemcpyAsync(c->vtk_snap + q * fb, c->f[kVtkFields[q]], fb,
+                                 hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->vtk_snap + 5 * fb, c->solid, N, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHK(c, hipEventRecord(c->snap_ev, c->stream));
+    const std::string p = vtk_path(path, step);
+    try {
+        c->io_thread = std::thread([c, p, step, N, fb]() {
+            hipError_t e = hipSetDevice(c->device);
+            if (e == hipSuccess) e = hipStreamWaitEvent(c->io_stream, c->snap_ev, 0);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(c->vtk_host.data(), c->vtk_snap, 5 * fb + N,
+                                   hipMemcpyDeviceToHost, c->io_stream);
+            if (e == hipSuccess) e = hipStreamSynchronize(c->io_stream);
+            std::string err;
+            if (e != hipSuccess) {
+                err = std::string("background VTK copy: ") + hipGetErrorString(e);
+            } else {
+                const double* h = (const double*)c->vtk_host.data();
+                vtk_write_fields(p, step, vtk_grid(c), h, h + N, h + 2 * N, h + 3 * N, h + 4 * N,
+                                 c->vtk_host.data() + 5 * fb, err);
+            }
+            if (!err.empty() && c->io_rc == 0) {
+                c->io_rc = e != hipSuccess ? LES_ERR_HIP : LES_ERR_IO;
+                c->io_err = err;
+            }
+        });
+    } catch (const std::system_error& ex) {
+        return fail(c, LES_ERR_STATE, std::string("cannot start the writer thread: ") + ex.what());
+    }
+    return LES_OK;
+}
+
+int les_io_wait(les_ctx* c)
+{
+    if (!c) return LES_ERR_ARG;
+    join_writer(c);
+    const int rc = c->io_rc;
+    if (rc != LES_OK) fail(c, rc, c->io_err);
+    c->io_rc = 0;
+    c->io_err.clear();
+    return rc;
+}
+
+int les_write_field_vtk_host(const char* path, int step, int nx, int ny, int nz, double dx,
+                             double dy, double dz, const double* u, const double* v,
+                             const double* w, const double* p, const double* nu_t,
+                             const uint8_t* solid)
+{
+    if (!u || !v || !w || !p || !nu_t || !solid || nx < 1 || ny < 1 || nz < 1)
+        return fail(nullptr, LES_ERR_ARG, "bad grid or NULL array");
+    if ((long long)nx * ny * nz >= (1LL << 31))
+        return fail(nullptr, LES_ERR_ARG, "grid exceeds 2^31 cells (int indices)");
+    std::string err;
+    if (vtk_write_fields(vtk_path(path, step), step, VtkGrid{nx, ny, nz, dx, dy, dz}, u, v, w, p,
+                         nu_t, solid, err))
+        return fail(nullptr, LES_ERR_IO, err);
+    return LES_OK;
+}
+
+int les_stats_init(les_ctx* c, int start_step)
+{
+    if (!c) return LES_ERR_ARG;
+    int rc;
+    if ((rc = ensure_stats(c, c->stat_sum)) != LES_OK) return rc;
+    for (int a = 0; a < kStatArrays; ++a)
+        HIPCHK(c, hipMemsetAsync(c->stat_sum[a], 0, nbytes(c, 8), c->stream));
+    c->stat_start = start_step;
+    c->stat_samples = 0;
+    c->have_stats = true;
+    c->stat_final = false;
+    return LES_OK;
+}
+
+int les_stats_update(les_ctx* c, int current_step)
+{
+    if (!c) return LES_ERR_ARG;
+    if (!c->have_stats) return fail(c, LES_ERR_STATE, "les_stats_init has not been called");
+    if (current_step <= c->stat_start) return LES_OK;
+    ++c->stat_samples;
+    StatArrays S;
+    for (int a = 0; a < kStatArrays; ++a) S.s[a] = c->stat_sum[a];
+    KtScope kt(c, LES_KT_STATS, 8);
+    HIPCHK(c, launch_stats_update(c->stream, c->d.N, c->f[LES_F_U], c->f[LES_F_V], c->f[LES_F_W],
+                                  c->f[LES_F_P], S));
+    return LES_OK;
+}
+
+int les_stats_finalize(les_ctx* c, const char* path)
+{
+    if (!c) return LES_ERR_ARG;
+    if (!c->have_stats) return fail(c, LES_ERR_STATE, "les_stats_init has not been called");
+    if (c->stat_samples == 0) return LES_OK;  // statistics.cpp:135-138: nothing to save
+    int rc;
+    if ((rc = ensure_stats(c, c->stat_out)) != LES_OK) return rc;
+    StatArrays S, O;
+    for (int a = 0; a < kStatArrays; ++a) {
+        S.s[a] = c->stat_sum[a];
+        O.s[a] = c->stat_out[a];
+    }
+    const double n_inv = 1.0 / static_cast<double>(c->stat_samples);
+    HIPCHK(c, launch_stats_finalize(c->stream, c->d.N, S, n_inv, O));
+    c->stat_final = true;
+    if (path && !*path) return LES_OK;  // "": compute only
+    const size_t N = (size_t)c->d.N;
+    std::vector<double> h;
+    try {
+        h.resize(kStatArrays * N);
+    } catch (const std::bad_alloc&) {
+        return fail(c, LES_ERR_NOMEM, "host buffer for the statistics");
+    }
+    const double* m[kStatArrays];
+    for (int a = 0; a < kStatArrays; ++a) {
+        HIPCHK(c, hipMemcpyAsync(h.data() + a * N, c->stat_out[a], N * sizeof(double),
+                                 hipMemcpyDeviceToHost, c->stream));
+        m[a] = h.data() + a * N;
+    }
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    std::string err;
+    if (vtk_write_statistics(path ? path : "output/statistics_final.vtk", vtk_grid(c), m, err))
+        return fail(c, LES_ERR_IO, err);
+    return LES_OK;
+}
+
+int les_stats_get(les_ctx* c, int which, int final, double* out, int ptr_kind)
+{
+    if (!c || !out || which < 0 || which >= kStatArrays)
+        return fail(c, LES_ERR_ARG, "bad statistics slot or NULL out");
+    if (!c->have_stats || (final && !c->stat_final))
+        return fail(c, LES_ERR_STATE, final ? "statistics not finalised" : "no statistics");
+    const double* src = final ? c->stat_out[which] : c->stat_sum[which];
+    HIPCHK(c, hipMemcpyAsync(out, src, nbytes(c, 8), kind_out(ptr_kind), c->stream));
+    if (!ptr_kind) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return LES_OK;
+}
+
+int les_stats_samples(les_ctx* c, int* num_samples)
+{
+    if (!c || !num_samples) return fail(c, LES_ERR_ARG, "NULL argument");
+    *num_samples = c->have_stats ? c->stat_samples : 0;
     return LES_OK;
 }
 

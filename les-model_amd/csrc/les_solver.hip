@@ -138,9 +138,47 @@ struct Nbr {
     T km, kp;  // k-1 of the first cell, k+1 of the last
 };
 
-template <typename T, int V, typename F>
-__device__ __forceinline__ void load_nbr(const T* __restrict__ x, const Dims& d, int id, int i,
-                                         int j, int k, F f, Nbr<T, V>& nb)
+// The k-1 / k+V neighbours of a thread's cells are the last / first cell of
+// the adjacent lanes in the same row.  LES_KSHFL takes them from those lanes
+// (ds_bpermute): when a row lies inside one wave (nz / V a power of two up to
+// 64) no memory access is left; when rows cross wave edges, SPAN kernels
+// load at the edge lanes only and the others keep the two scalar loads
+// (a divergent load between the unrolled passes of the fused Chebyshev pair
+// costs more than it saves).  Every lane of a row must execute this
+// together: callers skip whole rows only.  ld(n) gives the value at flat
+// index n where it comes from memory.
+#ifndef LES_KSHFL
+#define LES_KSHFL 1
+#endif
+template <typename T, int V, bool SPAN, typename L>
+__device__ __forceinline__ void row_edges(const Dims& d, int id, int k, const T (&c)[V], T& km,
+                                          T& kp, L ld)
+{
+    km = T(0);
+    kp = T(0);
+    const int tpr = d.nz / V;
+    const bool inwave = tpr <= 64 && (tpr & (tpr - 1)) == 0;
+    if (LES_KSHFL && (inwave || SPAN)) {
+        const int lane = (int)threadIdx.x & 63;
+        const T up = __shfl_up(c[V - 1], 1, 64);
+        const T dn = __shfl_down(c[0], 1, 64);
+        if (inwave) {
+            if (k > 0) km = up;
+            if (k + V < d.nz) kp = dn;
+        } else {
+            if (k > 0) km = lane > 0 ? up : ld(id - 1);
+            if (k + V < d.nz) kp = lane < 63 ? dn : ld(id + V);
+        }
+    } else {
+        if (k > 0) km = ld(id - 1);
+        if (k + V < d.nz) kp = ld(id + V);
+    }
+}
+
+// The centre and the four i/j neighbour vectors (zero outside the grid).
+template <typename T, int V>
+__device__ __forceinline__ void load_nbr5(const T* __restrict__ x, const Dims& d, int id, int i,
+                                          int j, Nbr<T, V>& nb)
 {
     vload<T, V>(x + id, nb.c);
 #pragma unroll
@@ -149,8 +187,14 @@ __device__ __forceinline__ void load_nbr(const T* __restrict__ x, const Dims& d,
     if (i < d.nx - 1) vload<T, V>(x + id + d.sij, nb.e);
     if (j > 0) vload<T, V>(x + id - d.nz, nb.s);
     if (j < d.ny - 1) vload<T, V>(x + id + d.nz, nb.n);
-    nb.km = k > 0 ? x[id - 1] : T(0);
-    nb.kp = k + V < d.nz ? x[id + V] : T(0);
+}
+
+template <typename T, int V, bool SPAN = true, typename F>
+__device__ __forceinline__ void load_nbr(const T* __restrict__ x, const Dims& d, int id, int i,
+                                         int j, int k, F f, Nbr<T, V>& nb)
+{
+    load_nbr5<T, V>(x, d, id, i, j, nb);
+    row_edges<T, V, SPAN>(d, id, k, nb.c, nb.km, nb.kp, [&](int n) { return x[n]; });
 #pragma unroll
     for (int v = 0; v < V; ++v) {
         nb.c[v] = f(nb.c[v]);
@@ -523,8 +567,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Sten
             load_nbr<T, V>(p, d, id, i, j, k, [](T x) { return x; }, nb);
         } else {
             Nbr<T, V> nz_, np_;
-            load_nbr<T, V>(z, d, id, i, j, k, [](T x) { return x; }, nz_);
-            load_nbr<T, V>(p_old, d, id, i, j, k, [](T x) { return x; }, np_);
+            load_nbr5<T, V>(z, d, id, i, j, nz_);
+            load_nbr5<T, V>(p_old, d, id, i, j, np_);
             auto comb = [&](T zz, T pp, int n) {
                 if (PU == 1) return zz + beta * pp;
                 return zz / diag[n] + beta * pp;
@@ -537,8 +581,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Sten
                 nb.s[v] = j > 0 ? comb(nz_.s[v], np_.s[v], id - d.nz + v) : T(0);
                 nb.n[v] = j < d.ny - 1 ? comb(nz_.n[v], np_.n[v], id + d.nz + v) : T(0);
             }
-            nb.km = k > 0 ? comb(nz_.km, np_.km, id - 1) : T(0);
-            nb.kp = k + V < d.nz ? comb(nz_.kp, np_.kp, id + V) : T(0);
+            row_edges<T, V, true>(d, id, k, nb.c, nb.km, nb.kp,
+                                  [&](int n) { return comb(z[n], p_old[n], n); });
             vstore_s<T, V>(p + id, nb.c);
         }
         T qv[V];
@@ -815,11 +859,12 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                 T rv[V], az[V], pp_[V], zc[V];
                 vload_s<T, V>(r + id, rv);
                 if (FIRST) {
-                    load_nbr<T, V>(r, d, id, i, j, k, [&](T x) { return T(0) + a0 * x; }, nb);
+                    load_nbr<T, V, false>(r, d, id, i, j, k,
+                                          [&](T x) { return T(0) + a0 * x; }, nb);
 #pragma unroll
                     for (int v = 0; v < V; ++v) pp_[v] = rv[v];
                 } else {
-                    load_nbr<T, V>(z_in, d, id, i, j, k, [](T x) { return x; }, nb);
+                    load_nbr<T, V, false>(z_in, d, id, i, j, k, [](T x) { return x; }, nb);
                     vload_s<T, V>(p_in + id, pp_);
                 }
 #pragma unroll
@@ -862,8 +907,8 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                 if (i < d.nx - 1) vload<T, V>(zp_pl + a * d.nz + kk, nb.e);
                 if (j > 0) vload<T, V>(zc_pl + (a - 1) * d.nz + kk, nb.s);
                 if (j < d.ny - 1) vload<T, V>(zc_pl + (a + 1) * d.nz + kk, nb.n);
-                nb.km = k > 0 ? zc_pl[a * d.nz + kk - 1] : T(0);
-                nb.kp = k + V < d.nz ? zc_pl[a * d.nz + kk + V] : T(0);
+                row_edges<T, V, false>(d, id, k, nb.c, nb.km, nb.kp,
+                                       [&](int n) { return zc_pl[a * d.nz + kk + (n - id)]; });
                 T az[V];
                 apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az,
                                      CODED == 1 ? ckeep[pp] : nullptr);

@@ -28,9 +28,13 @@ PC_NONE, PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
 SOLVER_PCG, SOLVER_SOR_LEX, SOLVER_SOR_RB, SOLVER_JACOBI = 0, 1, 2, 3
 KT_NONE, KT_CHEB_STEP, KT_PCG_MATVEC, KT_PCG_UPDATE, KT_MOMENTUM, KT_IC_SOLVE = 0, 1, 2, 3, 4, 5
-KT_CHEB_PAIR = 6
+KT_CHEB_PAIR, KT_STATS = 6, 7
 IP_EUCLID, IP_MIRROR = 0, 1
 OPT_CHEB_PAIRS, OPT_PAIR_GRID = 1, 2
+# statistics slots (statistics.h:15-31): means of u, v, w, p, then the sums
+# of the six products
+STATS = ("u", "v", "w", "p", "uu", "vv", "ww", "uv", "uw", "vw")
+STAT_ID = {n: i for i, n in enumerate(STATS)}
 
 DP = C.POINTER(C.c_double)
 
@@ -142,6 +146,18 @@ def load_library(path: str = LIB_PATH):
     L.les_kernel_timing.argtypes = [vp, C.c_int]
     L.les_kernel_timing_read.argtypes = [vp, C.POINTER(C.c_int), DP, C.POINTER(C.c_int)]
     L.les_diag_copy.argtypes = [vp, C.c_int]
+    cp = C.c_char_p
+    L.les_save_state.argtypes = [vp, cp, C.c_double, C.c_double, C.c_int]
+    L.les_load_state.argtypes = [vp, cp, DP, DP, C.POINTER(C.c_int)]
+    L.les_write_field_vtk.argtypes = [vp, cp, C.c_int]
+    L.les_write_field_vtk_async.argtypes = [vp, cp, C.c_int]
+    L.les_io_wait.argtypes = [vp]
+    L.les_write_field_vtk_host.argtypes = [cp, C.c_int] + [C.c_int] * 3 + [C.c_double] * 3 + [vp] * 6
+    L.les_stats_init.argtypes = [vp, C.c_int]
+    L.les_stats_update.argtypes = [vp, C.c_int]
+    L.les_stats_finalize.argtypes = [vp, cp]
+    L.les_stats_get.argtypes = [vp, C.c_int, C.c_int, vp, C.c_int]
+    L.les_stats_samples.argtypes = [vp, C.POINTER(C.c_int)]
     _lib = L
     return L
 
@@ -160,6 +176,23 @@ def _ptr(a, dtype=np.float64):
     if not a.is_contiguous():
         raise ValueError("tensor must be contiguous")
     return C.c_void_p(a.data_ptr()), (LES_PTR_DEVICE if a.is_cuda else LES_PTR_HOST)
+
+
+def _path(p):
+    return None if p is None else os.fsencode(p)
+
+
+def write_field_vtk_host(path, step, m, u, v, w, p, nu_t, solid):
+    """write_field_vtk (io.cpp:225-256) from host arrays with the library's
+    multi-threaded writer; no GPU is used."""
+    L = load_library()
+    arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (u, v, w, p, nu_t)]
+    sm = np.ascontiguousarray(solid, dtype=np.uint8)
+    rc = L.les_write_field_vtk_host(_path(path), int(step), m["nx"], m["ny"], m["nz"], m["dx"],
+                                    m["dy"], m["dz"], *[_host_ptr(a, np.float64) for a in arrs],
+                                    _host_ptr(sm, np.uint8))
+    if rc != 0:
+        raise LesError(f"les_write_field_vtk_host: {L.les_last_error(None).decode()} ({rc})")
 
 
 class LesGpu:
@@ -240,6 +273,51 @@ class LesGpu:
 
     def set_stream(self, stream_handle):
         self._chk(self.lib.les_set_stream(self.h, C.c_void_p(stream_handle)), "les_set_stream")
+
+    # ---- checkpoint, VTK output, statistics (io.cpp, statistics.cpp) ----
+    def save_state(self, time_simulated, dt, next_step_to_run, directory=None):
+        """save_state (io.cpp:14-59) from the device fields."""
+        self._chk(self.lib.les_save_state(self.h, _path(directory), float(time_simulated),
+                                          float(dt), int(next_step_to_run)), "les_save_state")
+
+    def load_state(self, directory=None, time_simulated=0.0, dt=0.0):
+        """load_state (io.cpp:61-120): returns (step, time_simulated, dt);
+        step 0 where the reference returns 0 (time and dt then as it leaves
+        them)."""
+        t, d, s = C.c_double(time_simulated), C.c_double(dt), C.c_int(0)
+        self._chk(self.lib.les_load_state(self.h, _path(directory), C.byref(t), C.byref(d),
+                                          C.byref(s)), "les_load_state")
+        return s.value, t.value, d.value
+
+    def write_field_vtk(self, step, path=None, background=False):
+        """write_field_vtk (io.cpp:225-256) from the device fields; with
+        background=True the call returns once the fields are snapshotted on
+        the device and io_wait() joins the writer."""
+        f = self.lib.les_write_field_vtk_async if background else self.lib.les_write_field_vtk
+        self._chk(f(self.h, _path(path), int(step)), "les_write_field_vtk")
+
+    def io_wait(self):
+        self._chk(self.lib.les_io_wait(self.h), "les_io_wait")
+
+    def stats_init(self, start_step):
+        self._chk(self.lib.les_stats_init(self.h, int(start_step)), "les_stats_init")
+
+    def stats_update(self, current_step):
+        self._chk(self.lib.les_stats_update(self.h, int(current_step)), "les_stats_update")
+
+    def stats_finalize(self, path=None):
+        self._chk(self.lib.les_stats_finalize(self.h, _path(path)), "les_stats_finalize")
+
+    def stats_get(self, name, final=False):
+        out = np.empty(self.N)
+        self._chk(self.lib.les_stats_get(self.h, STAT_ID[name], 1 if final else 0,
+                                         _host_ptr(out, np.float64), 0), "les_stats_get")
+        return out
+
+    def stats_samples(self):
+        n = C.c_int(0)
+        self._chk(self.lib.les_stats_samples(self.h, C.byref(n)), "les_stats_samples")
+        return n.value
 
     # ---- operator ----
     def build_stencil(self, ignore_mask=False):

@@ -1224,3 +1224,42 @@ void orc_adjoint(double* ubar, double* vbar, double* wbar, const double* traj,
     free(su); free(sv); free(sw);
     free(W);
 }
+
+/* ---- time statistics ----
+ * update_statistics (statistics.cpp:98-131): per cell, each sum gains the
+ * value or product; the reference's loop order over cells is irrelevant
+ * (cells are independent). */
+void orc_stats_update(double* const sums[10], const double* u, const double* v, const double* w,
+                      const double* p, long n)
+{
+    for (long i = 0; i < n; ++i) {
+        const double a = u[i], b = v[i], c = w[i], d = p[i];
+        sums[0][i] += a;
+        sums[1][i] += b;
+        sums[2][i] += c;
+        sums[3][i] += d;
+        sums[4][i] += a * a;
+        sums[5][i] += b * b;
+        sums[6][i] += c * c;
+        sums[7][i] += a * b;
+        sums[8][i] += a * c;
+        sums[9][i] += b * c;
+    }
+}
+
+/* finalize_and_save's arithmetic (statistics.cpp:141-176): n_inv = 1/k, the
+ * means sum*n_inv, the stresses (sum*n_inv) - mean_a*mean_b. */
+void orc_stats_finalize(double* const out[10], double* const sums[10], int num_samples, long n)
+{
+    const double n_inv = 1.0 / (double)num_samples;
+    static const int pa[6] = {0, 1, 2, 0, 0, 1}, pb[6] = {0, 1, 2, 1, 2, 2};
+    for (long i = 0; i < n; ++i) {
+        double mean[4];
+        for (int q = 0; q < 4; ++q) {
+            mean[q] = sums[q][i] * n_inv;
+            out[q][i] = mean[q];
+        }
+        for (int s = 0; s < 6; ++s)
+            out[4 + s][i] = (sums[4 + s][i] * n_inv) - (mean[pa[s]] * mean[pb[s]]);
+    }
+}

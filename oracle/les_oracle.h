@@ -173,6 +173,13 @@ void orc_adj_nut_bcs(double* braw, const double* bnut, const uint8_t* solid, con
 void orc_adj_sgs(double* bu, double* bv, double* bw, const double* braw, const double* u,
                  const double* v, const double* w, const orc_mesh* m, double Cs);
 
+/* ---- time statistics (statistics.cpp:98-131, 133-176) ----
+ * sums[10] in the order u, v, w, p, uu, vv, ww, uv, uw, vw. */
+void orc_stats_update(double* const sums[10], const double* u, const double* v, const double* w,
+                      const double* p, long n);
+/* means (slots 0-3) and Reynolds stresses (4-9) from the sums of k samples */
+void orc_stats_finalize(double* const out[10], double* const sums[10], int num_samples, long n);
+
 #ifdef __cplusplus
 }
 #endif

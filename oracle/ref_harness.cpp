@@ -8,11 +8,15 @@
 // function named in its comment, and copies the results back.
 #include <cstdint>
 #include <cstring>
+#include <string>
 #include <vector>
 #include <omp.h>
+#include <unistd.h>
 
 #include "field.h"
+#include "io.h"
 #include "pressure.h"
+#include "statistics.h"
 
 double compute_adaptive_dt(const Field& field, const Mesh& mesh, double nu);
 void compute_sgs_stress(Field& f, Stress& sgs, const Mesh& m);
@@ -49,6 +53,33 @@ void get_stencil(double* const* a, const StencilMatrix& A, size_t n)
 {
     get(a[0], A.A_c, n); get(a[1], A.A_n, n); get(a[2], A.A_s, n); get(a[3], A.A_e, n);
     get(a[4], A.A_w, n); get(a[5], A.A_t, n); get(a[6], A.A_b, n);
+}
+
+// The reference's writers use paths relative to the working directory
+// (checkpoint/..., output/...): run them with `dir` as that directory.
+struct InDir {
+    std::string prev;
+    bool ok = false;
+    explicit InDir(const char* dir)
+    {
+        char b[4096];
+        if (getcwd(b, sizeof b)) prev = b;
+        ok = !prev.empty() && chdir(dir) == 0;
+    }
+    ~InDir()
+    {
+        if (ok && chdir(prev.c_str()) != 0) std::abort();
+    }
+};
+
+Field mk_field(const Mesh& m, const double* u, const double* v, const double* w, const double* p,
+               const double* nu_t, const uint8_t* solid)
+{
+    Field f(m.nx, m.ny, m.nz);
+    size_t n = ncell(m);
+    put(f.u, u, n); put(f.v, v, n); put(f.w, w, n); put(f.p, p, n); put(f.nu_t, nu_t, n);
+    put_mask(f, solid, n);
+    return f;
 }
 
 }  // namespace
@@ -439,6 +470,81 @@ int ref_run_steps(double* u, double* v, double* w, double* un, double* vn, doubl
     get(un, f.u_new, n); get(vn, f.v_new, n); get(wn, f.w_new, n);
     get(p, f.p, n); get(p_rhs, f.p_rhs, n); get(nu_t, f.nu_t, n);
     return 0;
+}
+
+// save_state (io.cpp:14) with `dir` as the working directory (it must hold
+// checkpoint/).  Returns -1 if dir cannot be entered.
+int ref_save_state(const char* dir, const double* u, const double* v, const double* w,
+                   const double* p, const double* nu_t, const uint8_t* solid, int nx, int ny,
+                   int nz, double time_simulated, double dt, int next_step)
+{
+    InDir in(dir);
+    if (!in.ok) return -1;
+    Mesh m = mk_mesh(nx, ny, nz, 1, 1, 1);
+    Field f = mk_field(m, u, v, w, p, nu_t, solid);
+    save_state(f, m, time_simulated, dt, next_step);
+    return 0;
+}
+
+// load_state (io.cpp:61) in `dir`: the arrays are the Field's previous
+// content on entry and what load_state leaves in it on return.
+int ref_load_state(const char* dir, double* u, double* v, double* w, double* p, double* nu_t,
+                   uint8_t* solid, int nx, int ny, int nz, double* time_simulated, double* dt)
+{
+    InDir in(dir);
+    if (!in.ok) return -1;
+    Mesh m = mk_mesh(nx, ny, nz, 1, 1, 1);
+    Field f = mk_field(m, u, v, w, p, nu_t, solid);
+    int step = load_state(f, m, *time_simulated, *dt);
+    size_t n = ncell(m);
+    get(u, f.u, n); get(v, f.v, n); get(w, f.w, n); get(p, f.p, n); get(nu_t, f.nu_t, n);
+    for (size_t i = 0; i < n; ++i) solid[i] = f.is_solid[i] ? 1 : 0;
+    return step;
+}
+
+// write_field_vtk (io.cpp:225) in `dir` (which must hold output/).
+int ref_write_field_vtk(const char* dir, const double* u, const double* v, const double* w,
+                        const double* p, const double* nu_t, const uint8_t* solid, int nx, int ny,
+                        int nz, double dx, double dy, double dz, int step)
+{
+    InDir in(dir);
+    if (!in.ok) return -1;
+    Mesh m = mk_mesh(nx, ny, nz, dx, dy, dz);
+    Field f = mk_field(m, u, v, w, p, nu_t, solid);
+    write_field_vtk(f, m, step);
+    return 0;
+}
+
+// init_statistics, then update_statistics (statistics.cpp:98) with state s
+// (u[s], v[s], w[s], p[s]) at step steps[s] for s < nstates, then -- when
+// dir is given -- finalize_and_save (statistics.cpp:133) to
+// dir/output/statistics_final.vtk.  sums10 receives the ten running sums in
+// the order u, v, w, p, uu, vv, ww, uv, uw, vw; returns num_samples.
+int ref_statistics(const char* dir, int nx, int ny, int nz, double dx, double dy, double dz,
+                   int start_step, int nstates, const int* steps, const double* const* u,
+                   const double* const* v, const double* const* w, const double* const* p,
+                   double* const* sums10)
+{
+    Mesh m = mk_mesh(nx, ny, nz, dx, dy, dz);
+    size_t n = ncell(m);
+    Statistics st;
+    init_statistics(st, m, start_step);
+    Field f(nx, ny, nz);
+    for (int s = 0; s < nstates; ++s) {
+        put(f.u, u[s], n); put(f.v, v[s], n); put(f.w, w[s], n); put(f.p, p[s], n);
+        update_statistics(st, f, steps[s]);
+    }
+    const std::vector<double>* src[10] = {&st.sum_field.u, &st.sum_field.v, &st.sum_field.w,
+                                          &st.sum_field.p, &st.uu_sum, &st.vv_sum, &st.ww_sum,
+                                          &st.uv_sum, &st.uw_sum, &st.vw_sum};
+    for (int a = 0; a < 10; ++a)
+        if (sums10 && sums10[a]) get(sums10[a], *src[a], n);
+    if (dir) {
+        InDir in(dir);
+        if (!in.ok) return -1;
+        finalize_and_save(st, m, "output/statistics_final.vtk");
+    }
+    return st.num_samples;
 }
 
 }  // extern "C"

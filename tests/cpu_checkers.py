@@ -44,6 +44,10 @@ def _p6(arrs):
     return (DP * 6)(*[_p(a) for a in arrs])
 
 
+def _p10(arrs):
+    return (DP * 10)(*[_p(a) for a in arrs])
+
+
 class OrcMesh(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double)]
@@ -75,6 +79,7 @@ def _ncell(m):
 
 
 STATE_KEYS = ("u", "v", "w", "u_new", "v_new", "w_new", "p", "p_rhs", "nu_t")
+IO_KEYS = ("u", "v", "w", "p", "nu_t")   # the fields a checkpoint holds (io.cpp:42-46)
 
 
 class Oracle:
@@ -118,6 +123,16 @@ class Oracle:
 
     def set_threads(self, n):
         self.lib.orc_set_threads(int(n))
+
+    # ---- time statistics (statistics.cpp:98-176) ----
+    def stats_update(self, sums10, u, v, w, p):
+        self.lib.orc_stats_update(_p10(sums10), _p(u), _p(v), _p(w), _p(p), C.c_long(len(u)))
+
+    def stats_finalize(self, sums10, num_samples):
+        out = [np.empty_like(sums10[0]) for _ in range(10)]
+        self.lib.orc_stats_finalize(_p10(out), _p10(sums10), int(num_samples),
+                                    C.c_long(len(sums10[0])))
+        return out
 
     @staticmethod
     def _m(m):
@@ -335,12 +350,62 @@ class Reference:
         L.ref_sgs.argtypes = [DP, C.POINTER(DP), DP, DP, DP] + md
         L.ref_momentum.argtypes = [DP] * 7 + md + [C.c_double, C.c_double]
         L.ref_correct.argtypes = [DP, DP, DP, DP] + md + [C.c_double, C.c_double]
+        io = [DP] * 5 + [U8P, C.c_int, C.c_int, C.c_int]
+        L.ref_save_state.argtypes = [C.c_char_p] + io + [C.c_double, C.c_double, C.c_int]
+        L.ref_load_state.argtypes = [C.c_char_p] + io + [DP, DP]
+        L.ref_write_field_vtk.argtypes = [C.c_char_p] + io + [C.c_double] * 3 + [C.c_int]
+        L.ref_statistics.argtypes = [C.c_char_p] + md + [C.c_int, C.c_int, IP] + [
+            C.POINTER(DP)] * 4 + [C.POINTER(DP)]
         L.ref_run_steps.argtypes = [DP] * 9 + [U8P, C.POINTER(DP)] + md + [
             C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
             C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, IP, DP]
 
     def set_threads(self, n):
         self.lib.ref_set_threads(int(n))
+
+    # ---- checkpoint, VTK and statistics writers (io.cpp, statistics.cpp),
+    # run with `directory` as the working directory they write under ----
+    def save_state(self, directory, f, m, time_simulated, dt, next_step):
+        os.makedirs(os.path.join(directory, "checkpoint"), exist_ok=True)
+        rc = self.lib.ref_save_state(os.fsencode(directory), *[_p(f[k]) for k in IO_KEYS],
+                                     _p(f["solid"]), m["nx"], m["ny"], m["nz"],
+                                     C.c_double(time_simulated), C.c_double(dt), int(next_step))
+        assert rc == 0, directory
+
+    def load_state(self, directory, f, m, time_simulated=0.0, dt=0.0):
+        """Returns (step, time, dt, fields) as load_state leaves them, the
+        fields starting from copies of f."""
+        out = {k: np.array(f[k], copy=True) for k in IO_KEYS + ("solid",)}
+        t, d = C.c_double(time_simulated), C.c_double(dt)
+        step = self.lib.ref_load_state(os.fsencode(directory), *[_p(out[k]) for k in IO_KEYS],
+                                       _p(out["solid"]), m["nx"], m["ny"], m["nz"], C.byref(t),
+                                       C.byref(d))
+        return step, t.value, d.value, out
+
+    def write_field_vtk(self, directory, f, m, step):
+        os.makedirs(os.path.join(directory, "output"), exist_ok=True)
+        rc = self.lib.ref_write_field_vtk(os.fsencode(directory), *[_p(f[k]) for k in IO_KEYS],
+                                          _p(f["solid"]), m["nx"], m["ny"], m["nz"], m["dx"],
+                                          m["dy"], m["dz"], int(step))
+        assert rc == 0, directory
+        return os.path.join(directory, "output", "output_%d.vtk" % step)
+
+    def statistics(self, directory, m, start_step, steps, states):
+        """init_statistics + update_statistics per (step, state) +
+        finalize_and_save into directory/output (when directory is given).
+        Returns (num_samples, sums10)."""
+        n = _ncell(m)
+        sums = [np.empty(n) for _ in range(10)]
+        k = len(steps)
+        seq = {q: (DP * k)(*[_p(s[q]) for s in states]) for q in ("u", "v", "w", "p")}
+        if directory is not None:
+            os.makedirs(os.path.join(directory, "output"), exist_ok=True)
+        ns = self.lib.ref_statistics(None if directory is None else os.fsencode(directory),
+                                     m["nx"], m["ny"], m["nz"], m["dx"], m["dy"], m["dz"],
+                                     int(start_step), k, (C.c_int * k)(*steps), seq["u"],
+                                     seq["v"], seq["w"], seq["p"], _p10(sums))
+        assert ns >= 0
+        return ns, sums
 
     @staticmethod
     def _md(m):
