@@ -870,18 +870,21 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res,
                 } else {
                     // p = M^-1 r + beta p_old, fused into q = A p
                     const bool jac = pc == LES_PC_JACOBI;
+                    // (with the previous iteration's x += alpha p_old)
                     HIPCHK(c, launch_pcg_matvec_pu<double>(s, d, A, pb[cur], jac ? r : z,
                                                            jac ? c->A[SC] : nullptr, pb[cur ^ 1],
-                                                           q, c->red, sc, wgt));
+                                                           q, c->red, sc, wgt, x));
                     cur ^= 1;
                     p = pb[cur];
                 }
             }
+            // r -= alpha q; x's term waits for the next matvec
             if (pc == LES_PC_JACOBI) {
-                HIPCHK(c, (launch_pcg_update<double, double>(s, d, x, r, p, q, c->A[SC], c->red, sc,
-                                                             wgt)));
+                HIPCHK(c, (launch_pcg_update<double, double>(s, d, nullptr, r, nullptr, q, c->A[SC],
+                                                             c->red, sc, wgt)));
             } else {
-                HIPCHK(c, (launch_pcg_update<double, double>(s, d, x, r, p, q, nullptr, c->red, sc)));
+                HIPCHK(c, (launch_pcg_update<double, double>(s, d, nullptr, r, nullptr, q, nullptr,
+                                                             c->red, sc)));
                 if ((rc = precond_fp64(c, pc, r, z, prm->lambda_min, prm->lambda_max, prm->degree,
                                        EPI_BETA, sc, A, wgt)) != LES_OK)
                     return rc;
@@ -902,6 +905,8 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res,
         if (prm->max_iter <= 0) finished = true;
         ++batch;
     }
+    // the last update's x term (its matvec never ran)
+    HIPCHK(c, launch_pcg_x_pending<double>(s, d, x, pb[0], pb[1], sc));
     HIPCHK(c, hipMemcpyAsync(&c->sc_host[0], sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     const SolveScalars& h = c->sc_host[0];
@@ -1046,15 +1051,17 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
                 if (issued + it == 0) {
                     HIPCHK(c, launch_pcg_matvec<float>(s, d, A32, pb[cur], w.q, c->red, sc));
                 } else {
-                    // p = z + beta p_old fused into q = A p
+                    // p = z + beta p_old fused into q = A p, with the
+                    // previous iteration's x32 += alpha p_old
                     HIPCHK(c, launch_pcg_matvec_pu<float>(s, d, A32, pb[cur], w.z, nullptr,
-                                                          pb[cur ^ 1], w.q, c->red, sc));
+                                                          pb[cur ^ 1], w.q, c->red, sc, nullptr,
+                                                          w.x));
                     cur ^= 1;
                 }
             }
             {
                 KtScope kt(c, LES_KT_PCG_UPDATE, 4);
-                HIPCHK(c, launch_pcg_update_mixed(s, d, w.x, w.r, pb[cur], w.q, c->red, sc));
+                HIPCHK(c, launch_pcg_update_mixed(s, d, nullptr, w.r, nullptr, w.q, c->red, sc));
             }
             // The replacement kernels run gated on the device's request; they
             // are enqueued on the iterations where a request can stand: the
@@ -1062,7 +1069,7 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
             // residual is near the tolerance or a request was seen pending.
             const int itn = issued + it + 1;
             if (near || itn % init.reliable_every == 0 || itn >= prm->max_iter) {
-                HIPCHK(c, launch_mixed_fold(s, d, x64, w.x, 0, sc));
+                HIPCHK(c, launch_mixed_fold(s, d, x64, w.x, pb[0], pb[1], 0, sc));
                 HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.r, 0, 0, c->red, sc,
                                                 nullptr));
             }
@@ -1092,7 +1099,7 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
         ++batch;
     }
     // fold whatever correction is left and measure the FP64 residual
-    HIPCHK(c, launch_mixed_fold(s, d, x64, w.x, 1, sc));
+    HIPCHK(c, launch_mixed_fold(s, d, x64, w.x, pb[0], pb[1], 1, sc));
     HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.q, 0, 1, c->red, sc, c->dsc));
     HIPCHK(c, hipMemcpyAsync(&c->sc_host[0], sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(c->hsc, c->dsc, 8, hipMemcpyDeviceToHost, s));

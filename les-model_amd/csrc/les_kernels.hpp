@@ -125,13 +125,21 @@ hipError_t launch_pcg_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T
 // The same with the previous iteration's direction update fused in:
 // p = z + beta p_old (diag == nullptr) or p = r/diag + beta p_old (z = r,
 // Jacobi), formed at every stencil point and written to p (distinct from
-// p_old); q = A p.
+// p_old); q = A p.  x non-null: also the previous iteration's deferred
+// x += alpha p_old.
 template <typename T>
 hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, const T* p_old,
                                 const T* z, const T* diag, T* p, T* q, RedScratch red,
-                                SolveScalars* sc, const double* wgt = nullptr);
-// x += alpha p; r -= alpha q; ||r|| test (pressure.cpp:381-392).  With
-// jacobi_diag != nullptr also r.(r/A_c) and beta (pressure.cpp:396-402).
+                                SolveScalars* sc, const double* wgt = nullptr, T* x = nullptr);
+// x += alpha p_last after the last iteration when x's update is deferred to
+// the matvec (p_last = p0 for even iterations, p1 for odd; nothing when no
+// iteration ran)
+template <typename T>
+hipError_t launch_pcg_x_pending(hipStream_t s, const Dims& d, T* x, const T* p0, const T* p1,
+                                SolveScalars* sc);
+// x += alpha p; r -= alpha q; ||r|| test (pressure.cpp:381-392); x null:
+// r only (x deferred to the next matvec).  With jacobi_diag != nullptr also
+// r.(r/A_c) and beta (pressure.cpp:396-402).
 template <typename T, typename TX>
 hipError_t launch_pcg_update(hipStream_t s, const Dims& d, TX* x, T* r, const T* p, const T* q,
                              const T* jacobi_diag, RedScratch red, SolveScalars* sc,
@@ -192,9 +200,10 @@ hipError_t launch_ic_bwd_level(hipStream_t s, const Dims& d, int level, T* z, co
                                const T* Lw, const T* Ls, const T* Lb, const SolveScalars* sc);
 
 // ---- mixed precision (no reference counterpart) ----
-// x64 += x32; x32 = 0  (when force or sc->reliable_now)
-hipError_t launch_mixed_fold(hipStream_t s, const Dims& d, double* x64, float* x32, int force,
-                             SolveScalars* sc);
+// x64 += x32 + alpha p_last; x32 = -alpha p_last (when force or
+// sc->reliable_now): the fold with the last update's deferred x term
+hipError_t launch_mixed_fold(hipStream_t s, const Dims& d, double* x64, float* x32,
+                             const float* p0, const float* p1, int force, SolveScalars* sc);
 // r64 = b - A64 x64 -> r32 = (float) r64, ||r64||^2 (and ||b||^2 when init);
 // runs when force, init or sc->reliable_now.  Epilogue EPI_INIT_MIXED when
 // init, else EPI_RELIABLE (force && !init: EPI_STORE of ||r64|| into out).

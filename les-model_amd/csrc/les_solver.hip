@@ -546,6 +546,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(Dims d, Chunking ch, Stenci
 // in): 0 p given; 1 p = z + beta p_old; 2 p = r / diag + beta p_old
 // (Jacobi), formed at every stencil point from z (or r, diag) and p_old,
 // written to p at the cell itself; p_old and p are distinct buffers.
+// x (PU != 0, may be null): the previous iteration's x += alpha p_old,
+// deferred from its update kernel to here, where p_old is read anyway;
+// alpha is still that iteration's (the next one is formed after this
+// kernel), and the arithmetic is the update's.
 template <typename T, int CODED, int V, int PU>
 __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Stencil<T> A,
                                                        const T* __restrict__ p_old,
@@ -554,11 +558,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Sten
                                                        T* __restrict__ p,
                                                        T* __restrict__ q, RedScratch red,
                                                        const SolveScalars* sc,
-                                                       const double* __restrict__ wgt)
+                                                       const double* __restrict__ wgt,
+                                                       T* __restrict__ x)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
     const T beta = (T)sc->beta;
+    const T ax = (T)sc->alpha;
     double acc[1] = {0.0};
     FOR_VCELLS(d, ch, lb, V)
     {
@@ -584,6 +590,13 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Sten
             row_edges<T, V, true>(d, id, k, nb.c, nb.km, nb.kp,
                                   [&](int n) { return comb(z[n], p_old[n], n); });
             vstore_s<T, V>(p + id, nb.c);
+            if (x) {
+                T xv[V];
+                vload<T, V>(x + id, xv);
+#pragma unroll
+                for (int v = 0; v < V; ++v) xv[v] = xv[v] + ax * np_.c[v];
+                vstore_s<T, V>(x + id, xv);
+            }
         }
         T qv[V];
         apply7v<CODED, T, V>(A, d, id, i, j, k, nb, qv);
@@ -614,18 +627,20 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(Dims d, Chunking ch, TX* 
     double acc[2] = {0.0, 0.0};
     FOR_VCELLS(d, ch, lb, V)
     {
-        TX xv[V];
-        T rv[V], pv[V], qv[V];
-        vload<TX, V>(x + id, xv);
+        T rv[V], qv[V];
+        if (x) {  // null: x is updated by the next iteration's matvec
+            TX xv[V];
+            T pv[V];
+            vload<TX, V>(x + id, xv);
+            vload<T, V>(p + id, pv);
+#pragma unroll
+            for (int v = 0; v < V; ++v) xv[v] = xv[v] + ax * (TX)pv[v];
+            vstore_s<TX, V>(x + id, xv);
+        }
         vload<T, V>(r + id, rv);
-        vload<T, V>(p + id, pv);
         vload<T, V>(q + id, qv);
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-            xv[v] = xv[v] + ax * (TX)pv[v];
-            rv[v] = rv[v] + ar * qv[v];
-        }
-        vstore_s<TX, V>(x + id, xv);
+        for (int v = 0; v < V; ++v) rv[v] = rv[v] + ar * qv[v];
         vstore_s<T, V>(r + id, rv);
         T dv[V];
         if (diag) vload<T, V>(diag + id, dv);
@@ -961,18 +976,46 @@ __global__ __launch_bounds__(kBlock) void k_cheb_scale(Dims d, Chunking ch,
 // ------------------------------------------------------------------
 // mixed precision: FP32 iteration with FP64 reliable residual replacement
 // ------------------------------------------------------------------
+// x64 += x32 + alpha p_last, the last update's x term, which the next
+// matvec would otherwise add (p_last: the direction of iteration iter-1,
+// p0 for even iterations, p1 for odd ones).  x32 is left at
+// -(alpha p_last), so that next matvec brings it to exactly 0 -- the value
+// an eager update followed by this fold would leave.
 __global__ __launch_bounds__(kBlock) void k_mixed_fold(Dims d, Chunking ch,
                                                        double* __restrict__ x64,
-                                                       float* __restrict__ x32, int gate,
+                                                       float* __restrict__ x32,
+                                                       const float* __restrict__ p0,
+                                                       const float* __restrict__ p1, int gate,
                                                        const SolveScalars* sc)
 {
     if (!gate_open(gate, sc)) return;
     const int lb = logical_block(ch.grid);
+    const int it = sc->iter;
+    const float ax = (float)sc->alpha;
+    const float* pl = ((it - 1) & 1) ? p1 : p0;
     FOR_CELLS(d, ch, lb)
     {
-        x64[id] = x64[id] + (double)x32[id];
-        x32[id] = 0.0f;
+        const float ap = it > 0 ? ax * pl[id] : 0.0f;
+        const float xs = it > 0 ? x32[id] + ap : x32[id];
+        x64[id] = x64[id] + (double)xs;
+        x32[id] = -ap;
     }
+}
+
+// x += alpha p_last (FP64 solve): the last update's x term, deferred to the
+// next matvec, which does not run once the solve has stopped.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_x_pending(Dims d, Chunking ch, T* __restrict__ x,
+                                                          const T* __restrict__ p0,
+                                                          const T* __restrict__ p1,
+                                                          const SolveScalars* sc)
+{
+    const int it = sc->iter;
+    if (it <= 0) return;
+    const int lb = logical_block(ch.grid);
+    const T ax = (T)sc->alpha;
+    const T* pl = ((it - 1) & 1) ? p1 : p0;
+    FOR_CELLS(d, ch, lb) x[id] = x[id] + ax * pl[id];
 }
 
 template <int CODED>
@@ -1013,18 +1056,22 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_mixed(Dims d, Chunking ch
     double acc[1] = {0.0};
     FOR_VCELLS(d, ch, lb, V)
     {
-        float xv[V], rv[V], pv[V], qv[V];
-        vload<float, V>(x + id, xv);
+        float rv[V], qv[V];
+        if (x) {  // null: x is updated by the next iteration's matvec
+            float xv[V], pv[V];
+            vload<float, V>(x + id, xv);
+            vload<float, V>(p + id, pv);
+#pragma unroll
+            for (int v = 0; v < V; ++v) xv[v] = xv[v] + ax * pv[v];
+            vstore_s<float, V>(x + id, xv);
+        }
         vload<float, V>(r + id, rv);
-        vload<float, V>(p + id, pv);
         vload<float, V>(q + id, qv);
 #pragma unroll
         for (int v = 0; v < V; ++v) {
-            xv[v] = xv[v] + ax * pv[v];
             rv[v] = rv[v] + ar * qv[v];
             acc[0] += (double)rv[v] * (double)rv[v];
         }
-        vstore_s<float, V>(x + id, xv);
         vstore_s<float, V>(r + id, rv);
     }
     block_partial<1>(acc, red.partials, lb);
@@ -1257,13 +1304,13 @@ hipError_t launch_pcg_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T
                              RedScratch red, SolveScalars* sc, const double* wgt)
 {
     return launch_pcg_matvec_pu<T>(s, d, A, nullptr, nullptr, nullptr, const_cast<T*>(p), q, red,
-                                   sc, wgt);
+                                   sc, wgt, nullptr);
 }
 
 template <typename T>
 hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, const T* p_old,
                                 const T* z, const T* diag, T* p, T* q, RedScratch red,
-                                SolveScalars* sc, const double* wgt)
+                                SolveScalars* sc, const double* wgt, T* x)
 {
     constexpr int W = vec_width<T>();
     const int V = pick_width<T>(d);
@@ -1271,7 +1318,7 @@ hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, cons
     const int pu = p_old == nullptr ? 0 : (diag ? 2 : 1);
 #define LES_MV(CD, VV, PU)                                                                      \
     hipLaunchKernelGGL((k_pcg_matvec<T, CD, VV, PU>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, \
-                       A, p_old, z, diag, p, q, red, sc, wgt)
+                       A, p_old, z, diag, p, q, red, sc, wgt, x)
 #define LES_MV_V(CD, PU)                 \
     do {                                 \
         if (V == W) LES_MV(CD, W, PU);   \
@@ -1534,14 +1581,26 @@ hipError_t launch_ic_bwd_level(hipStream_t s, const Dims& d, int level, T* z, co
     return hipGetLastError();
 }
 
-hipError_t launch_mixed_fold(hipStream_t s, const Dims& d, double* x64, float* x32, int force,
-                             SolveScalars* sc)
+hipError_t launch_mixed_fold(hipStream_t s, const Dims& d, double* x64, float* x32,
+                             const float* p0, const float* p1, int force, SolveScalars* sc)
 {
     const Chunking ch = make_chunking(d);
-    hipLaunchKernelGGL(k_mixed_fold, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x64, x32,
+    hipLaunchKernelGGL(k_mixed_fold, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x64, x32, p0, p1,
                        force ? (int)GATE_NONE : (int)GATE_RELIABLE, sc);
     return hipGetLastError();
 }
+
+template <typename T>
+hipError_t launch_pcg_x_pending(hipStream_t s, const Dims& d, T* x, const T* p0, const T* p1,
+                                SolveScalars* sc)
+{
+    const Chunking ch = make_chunking(d);
+    hipLaunchKernelGGL(k_pcg_x_pending<T>, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, p0, p1,
+                       sc);
+    return hipGetLastError();
+}
+template hipError_t launch_pcg_x_pending<double>(hipStream_t, const Dims&, double*, const double*,
+                                                 const double*, SolveScalars*);
 
 hipError_t launch_mixed_residual(hipStream_t s, const Dims& d, Stencil<double> A, const double* b,
                                  const double* x64, float* r32, int init, int force,
@@ -1607,7 +1666,7 @@ hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double
                                              T*, RedScratch, SolveScalars*, const double*);     \
     template hipError_t launch_pcg_matvec_pu<T>(hipStream_t, const Dims&, Stencil<T>, const T*, \
                                                 const T*, const T*, T*, T*, RedScratch,         \
-                                                SolveScalars*, const double*);                  \
+                                                SolveScalars*, const double*, T*);              \
     template hipError_t launch_pcg_p_jacobi<T>(hipStream_t, const Dims&, const T*, const T*,    \
                                                T*, int, int, RedScratch, SolveScalars*,         \
                                                const double*);                                  \
