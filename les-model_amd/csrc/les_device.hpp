@@ -159,11 +159,24 @@ __device__ __forceinline__ void reduce_partials(const double* partials, int grid
     double s[NR];
 #pragma unroll
     for (int r = 0; r < NR; ++r) s[r] = 0.0;
-    for (int b = threadIdx.x; b < grid; b += kBlock) {
+    // thread t takes partials t, t + kBlock, t + 2 kBlock, ... in that
+    // order; the loads of a chunk are all issued before the first add, so
+    // their latencies overlap instead of adding up
+    constexpr int kU = kMaxGrid / kBlock;
+    for (int b0 = 0; b0 < grid; b0 += kU * kBlock) {
+        double x[kU][NR];
 #pragma unroll
-        for (int r = 0; r < NR; ++r) {
-            const double x = partials[b * kMaxRed + r];
-            s[r] = MAX ? (s[r] > x ? s[r] : x) : s[r] + x;
+        for (int u = 0; u < kU; ++u) {
+            const int b = b0 + (int)threadIdx.x + u * kBlock;
+#pragma unroll
+            for (int r = 0; r < NR; ++r) x[u][r] = b < grid ? partials[b * kMaxRed + r] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < kU; ++u) {
+            if (b0 + (int)threadIdx.x + u * kBlock >= grid) break;
+#pragma unroll
+            for (int r = 0; r < NR; ++r)
+                s[r] = MAX ? (s[r] > x[u][r] ? s[r] : x[u][r]) : s[r] + x[u][r];
         }
     }
     block_reduce<NR, MAX>(s);
