@@ -304,7 +304,7 @@ int cheb_chain(les_ctx* c, const Stencil<T>& A, const T* r, T* zout, T* za, T* z
         HIPCHK(c, launch_cheb_scale<T>(s, d, r, zout, a0, epi, c->red, sc, wgt));
         return LES_OK;
     }
-    const bool pairs = c->cheb_pairs && cheb_pair_supported<T>(d, c->pair_grid);
+    const bool pairs = c->cheb_pairs && cheb_pair_supported<T>(d, c->pair_grid, A.code != nullptr);
     const T* zin = nullptr;
     T* zbuf[2] = {za, zb};
     int nz = 0;

@@ -32,7 +32,8 @@ constexpr int kMaxRed = 4;       // sums a fused kernel may reduce at once
 // LDS limits the pair kernel to 4-6 workgroups per CU, so more workgroups
 // only add the halo planes each i-run recomputes).
 constexpr int kPairTargetGridF32 = 1024;
-constexpr int kPairTargetGridF64 = 512;
+constexpr int kPairTargetGridF64Code = 512;
+constexpr int kPairTargetGridF64Arrays = 256;
 constexpr size_t kPairMaxLds = 64 * 1024;              // LDS ring limit of that kernel
 
 struct Dims {

@@ -173,11 +173,12 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
                             const double* wgt = nullptr);
 // Two Chebyshev steps in one pass (k_cheb_pair): steps (beta_a, alpha_a) and
 // (beta_b, alpha_b); p_in/p_out and z_in/z_out distinct; no reduction.  Not
-// every grid shape is supported (cheb_pair_supported); the caller then runs
-// two launch_cheb_step.  target_grid: about how many workgroups to spread
-// the grid over (0: the built-in default).
+// every grid shape is supported (cheb_pair_supported; the geometry depends
+// on whether A is coded); the caller then runs two launch_cheb_step.
+// target_grid: about how many workgroups to spread the grid over (0: the
+// built-in default).
 template <typename T>
-bool cheb_pair_supported(const Dims& d, int target_grid);
+bool cheb_pair_supported(const Dims& d, int target_grid, bool coded);
 template <typename T>
 hipError_t launch_cheb_pair(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
                             const T* z_in, T* z_out, const T* p_in, T* p_out, double alpha0,

@@ -1454,23 +1454,33 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
     return hipSuccess;
 }
 
-// Workgroup size and row passes per thread of the pair kernel (J = passes
-// * rows-per-pass - 2 core rows per workgroup), per element type.
-// (Measured at 256^3: 256 threads x 2 passes beats 512 x 1 in FP32, 256 x 3
-// and 512 x 2 are even in FP64.  Issuing all passes' loads before their
-// arithmetic saves a memory round trip per plane but costs occupancy: 3%
-// faster in FP64, 14% slower in FP32, so neither does it.)
+// Geometry of the pair kernel per element type and operator storage (CD 1:
+// the code, 0: the seven arrays): workgroup size TB, row passes per thread P
+// (J = P * rows-per-pass - 2 core rows per workgroup), LDS ring depth R and
+// the number of workgroups aimed for.  Measured at 256^3 on MI355X:
+//   FP32: 256 x 2, 1024 workgroups; 512 x 1 and 256 x 3 are slower;
+//   FP64, code: 512 x 2, 512 workgroups -- 10% faster than 256 x 3;
+//   FP64, arrays: 1024 x 1, 256 workgroups -- 4% faster than 256 x 3.
+// Issuing all passes' loads before their arithmetic saves a memory round
+// trip per plane but costs occupancy (3% faster in FP64, 14% slower in
+// FP32), so no configuration does it.
 #ifndef LES_PAIR_BLOCK_F32
 #define LES_PAIR_BLOCK_F32 256
 #endif
 #ifndef LES_PAIR_PASSES_F32
 #define LES_PAIR_PASSES_F32 2
 #endif
-#ifndef LES_PAIR_BLOCK_F64
-#define LES_PAIR_BLOCK_F64 256
+#ifndef LES_PAIR_BLOCK_F64C
+#define LES_PAIR_BLOCK_F64C 512
 #endif
-#ifndef LES_PAIR_PASSES_F64
-#define LES_PAIR_PASSES_F64 3
+#ifndef LES_PAIR_PASSES_F64C
+#define LES_PAIR_PASSES_F64C 2
+#endif
+#ifndef LES_PAIR_BLOCK_F64A
+#define LES_PAIR_BLOCK_F64A 1024
+#endif
+#ifndef LES_PAIR_PASSES_F64A
+#define LES_PAIR_PASSES_F64A 1
 #endif
 #ifndef LES_PAIR_RING_F32
 #define LES_PAIR_RING_F32 4
@@ -1478,26 +1488,31 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
 #ifndef LES_PAIR_RING_F64
 #define LES_PAIR_RING_F64 4
 #endif
-template <typename T>
-constexpr int pair_passes() { return sizeof(T) == 4 ? LES_PAIR_PASSES_F32 : LES_PAIR_PASSES_F64; }
-template <typename T>
-constexpr int pair_block() { return sizeof(T) == 4 ? LES_PAIR_BLOCK_F32 : LES_PAIR_BLOCK_F64; }
-template <typename T>
-constexpr int pair_ring() { return sizeof(T) == 4 ? LES_PAIR_RING_F32 : LES_PAIR_RING_F64; }
+template <typename T, int CD>
+struct PairCfg {
+    static constexpr bool F32 = sizeof(T) == 4;
+    static constexpr int TB = F32 ? LES_PAIR_BLOCK_F32 : (CD ? LES_PAIR_BLOCK_F64C : LES_PAIR_BLOCK_F64A);
+    static constexpr int P = F32 ? LES_PAIR_PASSES_F32
+                                 : (CD ? LES_PAIR_PASSES_F64C : LES_PAIR_PASSES_F64A);
+    static constexpr int R = F32 ? LES_PAIR_RING_F32 : LES_PAIR_RING_F64;
+    static constexpr int GRID = F32 ? kPairTargetGridF32
+                                    : (CD ? kPairTargetGridF64Code : kPairTargetGridF64Arrays);
+};
 
-template <typename T>
+template <typename T, int CD>
 bool cheb_pair_geom(const Dims& d, PairGeom& g, int target_grid)
 {
-    constexpr int V = vec_width<T>(), P = pair_passes<T>(), TB = pair_block<T>();
+    using C = PairCfg<T, CD>;
+    constexpr int V = vec_width<T>(), P = C::P, TB = C::TB;
     if (d.nz % V) return false;
     g.tpr = d.nz / V;
     if (g.tpr > TB || TB % g.tpr) return false;
     g.rpp = TB / g.tpr;
     g.J = P * g.rpp - 2;
     if (g.J < 2) return false;
-    if ((size_t)pair_ring<T>() * P * g.rpp * d.nz * sizeof(T) > kPairMaxLds) return false;
+    if ((size_t)C::R * P * g.rpp * d.nz * sizeof(T) > kPairMaxLds) return false;
     g.njb = (d.ny + g.J - 1) / g.J;
-    if (target_grid <= 0) target_grid = sizeof(T) == 4 ? kPairTargetGridF32 : kPairTargetGridF64;
+    if (target_grid <= 0) target_grid = C::GRID;
     int nxc = std::max(1, std::min(d.nx, target_grid / g.njb));
     g.irange = (d.nx + nxc - 1) / nxc;
     nxc = (d.nx + g.irange - 1) / g.irange;
@@ -1506,10 +1521,32 @@ bool cheb_pair_geom(const Dims& d, PairGeom& g, int target_grid)
 }
 
 template <typename T>
-bool cheb_pair_supported(const Dims& d, int target_grid)
+bool cheb_pair_supported(const Dims& d, int target_grid, bool coded)
 {
     PairGeom g;
-    return cheb_pair_geom<T>(d, g, target_grid);
+    return coded ? cheb_pair_geom<T, 1>(d, g, target_grid)
+                 : cheb_pair_geom<T, 0>(d, g, target_grid);
+}
+
+template <typename T, int CD>
+hipError_t launch_cheb_pair_cd(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
+                               const T* z_in, T* z_out, const T* p_in, T* p_out, double alpha0,
+                               double beta_a, double alpha_a, double beta_b, double alpha_b,
+                               int first, SolveScalars* sc, int target_grid)
+{
+    using C = PairCfg<T, CD>;
+    constexpr int V = vec_width<T>(), P = C::P, TB = C::TB, R = C::R;
+    PairGeom g;
+    if (!cheb_pair_geom<T, CD>(d, g, target_grid)) return hipErrorNotSupported;
+    const size_t lds = (size_t)R * P * g.rpp * d.nz * sizeof(T);
+#define LES_PAIR(F)                                                                             \
+    hipLaunchKernelGGL((k_cheb_pair<T, F, CD, V, P, TB, R>), dim3(g.grid), dim3(TB), lds, s, d, \
+                       g, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a, beta_b,      \
+                       alpha_b, sc)
+    if (first) LES_PAIR(true);
+    else LES_PAIR(false);
+#undef LES_PAIR
+    return hipGetLastError();
 }
 
 template <typename T>
@@ -1518,24 +1555,11 @@ hipError_t launch_cheb_pair(hipStream_t s, const Dims& d, Stencil<T> A, const T*
                             double beta_a, double alpha_a, double beta_b, double alpha_b,
                             int first, SolveScalars* sc, int target_grid)
 {
-    constexpr int V = vec_width<T>(), P = pair_passes<T>(), TB = pair_block<T>(),
-                  R = pair_ring<T>();
-    PairGeom g;
-    if (!cheb_pair_geom<T>(d, g, target_grid)) return hipErrorNotSupported;
-    const size_t lds = (size_t)R * P * g.rpp * d.nz * sizeof(T);
-#define LES_PAIR(F, CD)                                                                         \
-    hipLaunchKernelGGL((k_cheb_pair<T, F, CD, V, P, TB, R>), dim3(g.grid), dim3(TB), lds, s, d, \
-                       g, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a, beta_b,      \
-                       alpha_b, sc)
-    if (first) {
-        if (A.code) LES_PAIR(true, 1);
-        else LES_PAIR(true, 0);
-    } else {
-        if (A.code) LES_PAIR(false, 1);
-        else LES_PAIR(false, 0);
-    }
-#undef LES_PAIR
-    return hipGetLastError();
+    if (A.code)
+        return launch_cheb_pair_cd<T, 1>(s, d, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a,
+                                         alpha_a, beta_b, alpha_b, first, sc, target_grid);
+    return launch_cheb_pair_cd<T, 0>(s, d, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a,
+                                     alpha_a, beta_b, alpha_b, first, sc, target_grid);
 }
 
 template <typename T>
@@ -1679,7 +1703,7 @@ hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double
                                             RedScratch, SolveScalars*, const double*);          \
     template hipError_t launch_cheb_scale<T>(hipStream_t, const Dims&, const T*, T*, double,    \
                                              int, RedScratch, SolveScalars*, const double*);    \
-    template bool cheb_pair_supported<T>(const Dims&, int);                                     \
+    template bool cheb_pair_supported<T>(const Dims&, int, bool);                               \
     template hipError_t launch_cheb_pair<T>(hipStream_t, const Dims&, Stencil<T>, const T*,     \
                                             const T*, T*, const T*, T*, double, double, double, \
                                             double, double, int, SolveScalars*, int);           \
