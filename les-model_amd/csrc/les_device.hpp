@@ -28,12 +28,11 @@ constexpr int kWaves = kBlock / 64;
 constexpr int kXcd = 8;          // XCDs on MI355X
 constexpr int kMaxGrid = 4096;   // workgroups of a streaming kernel (multiple of kXcd)
 constexpr int kMaxRed = 4;       // sums a fused kernel may reduce at once
-// Workgroups of a fused Chebyshev pair launch (measured at 256^3 on MI355X:
-// LDS limits the pair kernel to 4-6 workgroups per CU, so more workgroups
-// only add the halo planes each i-run recomputes).
-constexpr int kPairTargetGridF32 = 1024;
-constexpr int kPairTargetGridF64Code = 512;
-constexpr int kPairTargetGridF64Arrays = 256;
+// Workgroups of a fused Chebyshev pair launch: one round at two workgroups
+// per CU (measured at 256^3 on MI355X; more workgroups only add the halo
+// planes each i-run recomputes).
+constexpr int kPairTargetGridCode = 512;     // 512-thread workgroups
+constexpr int kPairTargetGridArrays = 256;   // 1024-thread workgroups
 constexpr size_t kPairMaxLds = 64 * 1024;              // LDS ring limit of that kernel
 
 struct Dims {

@@ -1454,49 +1454,37 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
     return hipSuccess;
 }
 
-// Geometry of the pair kernel per element type and operator storage (CD 1:
-// the code, 0: the seven arrays): workgroup size TB, row passes per thread P
-// (J = P * rows-per-pass - 2 core rows per workgroup), LDS ring depth R and
-// the number of workgroups aimed for.  Measured at 256^3 on MI355X:
-//   FP32: 256 x 2, 1024 workgroups; 512 x 1 and 256 x 3 are slower;
-//   FP64, code: 512 x 2, 512 workgroups -- 10% faster than 256 x 3;
-//   FP64, arrays: 1024 x 1, 256 workgroups -- 4% faster than 256 x 3.
-// Issuing all passes' loads before their arithmetic saves a memory round
-// trip per plane but costs occupancy (3% faster in FP64, 14% slower in
-// FP32), so no configuration does it.
-#ifndef LES_PAIR_BLOCK_F32
-#define LES_PAIR_BLOCK_F32 256
+// Geometry of the pair kernel per operator storage (CD 1: the code, 0: the
+// seven arrays): workgroup size TB, row passes per thread P (J = P *
+// rows-per-pass - 2 core rows per workgroup), LDS ring depth R and the
+// number of workgroups aimed for -- one round at the two workgroups per CU
+// that the 64 KiB ring allows.  Measured at 256^3 on MI355X, FP32 and FP64
+// alike: with the code, 512 x 2 (J = 14 FP32, 6 FP64) beats 256 x 2 (FP32)
+// and 256 x 3 (FP64) by 8-10%; with the arrays, 1024 x 1 beats them by
+// 4-12%; aiming for more workgroups than one round is slower.  Issuing all
+// passes' loads before their arithmetic saves a memory round trip per plane
+// but costs occupancy, so no configuration does it.
+#ifndef LES_PAIR_BLOCK_CODE
+#define LES_PAIR_BLOCK_CODE 512
 #endif
-#ifndef LES_PAIR_PASSES_F32
-#define LES_PAIR_PASSES_F32 2
+#ifndef LES_PAIR_PASSES_CODE
+#define LES_PAIR_PASSES_CODE 2
 #endif
-#ifndef LES_PAIR_BLOCK_F64C
-#define LES_PAIR_BLOCK_F64C 512
+#ifndef LES_PAIR_BLOCK_ARRAYS
+#define LES_PAIR_BLOCK_ARRAYS 1024
 #endif
-#ifndef LES_PAIR_PASSES_F64C
-#define LES_PAIR_PASSES_F64C 2
+#ifndef LES_PAIR_PASSES_ARRAYS
+#define LES_PAIR_PASSES_ARRAYS 1
 #endif
-#ifndef LES_PAIR_BLOCK_F64A
-#define LES_PAIR_BLOCK_F64A 1024
-#endif
-#ifndef LES_PAIR_PASSES_F64A
-#define LES_PAIR_PASSES_F64A 1
-#endif
-#ifndef LES_PAIR_RING_F32
-#define LES_PAIR_RING_F32 4
-#endif
-#ifndef LES_PAIR_RING_F64
-#define LES_PAIR_RING_F64 4
+#ifndef LES_PAIR_RING
+#define LES_PAIR_RING 4
 #endif
 template <typename T, int CD>
 struct PairCfg {
-    static constexpr bool F32 = sizeof(T) == 4;
-    static constexpr int TB = F32 ? LES_PAIR_BLOCK_F32 : (CD ? LES_PAIR_BLOCK_F64C : LES_PAIR_BLOCK_F64A);
-    static constexpr int P = F32 ? LES_PAIR_PASSES_F32
-                                 : (CD ? LES_PAIR_PASSES_F64C : LES_PAIR_PASSES_F64A);
-    static constexpr int R = F32 ? LES_PAIR_RING_F32 : LES_PAIR_RING_F64;
-    static constexpr int GRID = F32 ? kPairTargetGridF32
-                                    : (CD ? kPairTargetGridF64Code : kPairTargetGridF64Arrays);
+    static constexpr int TB = CD ? LES_PAIR_BLOCK_CODE : LES_PAIR_BLOCK_ARRAYS;
+    static constexpr int P = CD ? LES_PAIR_PASSES_CODE : LES_PAIR_PASSES_ARRAYS;
+    static constexpr int R = LES_PAIR_RING;
+    static constexpr int GRID = CD ? kPairTargetGridCode : kPairTargetGridArrays;
 };
 
 template <typename T, int CD>
