@@ -26,7 +26,15 @@ namespace les {
 constexpr int kBlock = 256;      // threads per workgroup (4 wave64)
 constexpr int kWaves = kBlock / 64;
 constexpr int kXcd = 8;          // XCDs on MI355X
-constexpr int kMaxGrid = 4096;   // workgroups of a streaming kernel (multiple of kXcd)
+// Workgroups of a streaming kernel (multiple of kXcd): one round at four
+// 256-thread workgroups per CU, each streaming its rows in many passes.
+// Measured at 256^3 on MI355X over 512-8192: 1024 is fastest in FP32 and
+// FP64 (fused matvec 119 -> 112 us FP32, 250 -> 207 us FP64; single
+// Chebyshev step 90 -> 78 us against 4096).
+#ifndef LES_MAX_GRID
+#define LES_MAX_GRID 1024
+#endif
+constexpr int kMaxGrid = LES_MAX_GRID;
 constexpr int kMaxRed = 4;       // sums a fused kernel may reduce at once
 // Workgroups of a fused Chebyshev pair launch: one round at two workgroups
 // per CU (measured at 256^3 on MI355X; more workgroups only add the halo
