@@ -5,7 +5,8 @@
 // Work decomposition.  A "row" is one (i, j) line of nz contiguous cells.
 // Each workgroup owns a contiguous range of rows; threads run along k, so
 // the row's i and j are uniform across the wavefront (scalar ALU) and no
-// per-cell division is needed.  When nz < 256 one pass covers several rows.
+// per-cell division is needed.  When a row has fewer cells than the
+// workgroup has threads, one pass covers several rows.
 // The workgroup -> row-range map is XCD-aware: MI355X dispatches
 // workgroups round-robin over its 8 XCDs, so workgroups b, b+8, b+16, ...
 // (one XCD) receive consecutive row ranges and their i+-1 / j+-1 neighbours
@@ -23,16 +24,21 @@
 
 namespace les {
 
-constexpr int kBlock = 256;      // threads per workgroup (4 wave64)
+// Threads per workgroup of the row kernels (16 wave64).  With kMaxGrid
+// below, a streaming kernel runs one 1024-thread workgroup per CU, each
+// walking a long contiguous range of rows.  Measured at 256^3 on MI355X
+// against 256 threads x 1024 workgroups: single Chebyshev step 80 -> 72 us
+// (5.2 TB/s), fused matvec 115 -> 103 us (FP32); larger grids of smaller
+// workgroups were slower still.
+#ifndef LES_BLOCK
+#define LES_BLOCK 1024
+#endif
+constexpr int kBlock = LES_BLOCK;
 constexpr int kWaves = kBlock / 64;
 constexpr int kXcd = 8;          // XCDs on MI355X
-// Workgroups of a streaming kernel (multiple of kXcd): one round at four
-// 256-thread workgroups per CU, each streaming its rows in many passes.
-// Measured at 256^3 on MI355X over 512-8192: 1024 is fastest in FP32 and
-// FP64 (fused matvec 119 -> 112 us FP32, 250 -> 207 us FP64; single
-// Chebyshev step 90 -> 78 us against 4096).
+// Workgroups of a streaming kernel (multiple of kXcd): one per CU.
 #ifndef LES_MAX_GRID
-#define LES_MAX_GRID 1024
+#define LES_MAX_GRID 256
 #endif
 constexpr int kMaxGrid = LES_MAX_GRID;
 constexpr int kMaxRed = 4;       // sums a fused kernel may reduce at once
@@ -170,7 +176,7 @@ __device__ __forceinline__ void reduce_partials(const double* partials, int grid
     // thread t takes partials t, t + kBlock, t + 2 kBlock, ... in that
     // order; the loads of a chunk are all issued before the first add, so
     // their latencies overlap instead of adding up
-    constexpr int kU = kMaxGrid / kBlock;
+    constexpr int kU = kMaxGrid > kBlock ? kMaxGrid / kBlock : 1;
     for (int b0 = 0; b0 < grid; b0 += kU * kBlock) {
         double x[kU][NR];
 #pragma unroll
