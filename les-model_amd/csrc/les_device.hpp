@@ -42,12 +42,25 @@ constexpr int kXcd = 8;          // XCDs on MI355X
 #endif
 constexpr int kMaxGrid = LES_MAX_GRID;
 constexpr int kMaxRed = 4;       // sums a fused kernel may reduce at once
-// Workgroups of a fused Chebyshev pair launch: one round at two workgroups
-// per CU (measured at 256^3 on MI355X; more workgroups only add the halo
-// planes each i-run recomputes).
-constexpr int kPairTargetGridCode = 512;     // 512-thread workgroups
-constexpr int kPairTargetGridArrays = 256;   // 1024-thread workgroups
-constexpr size_t kPairMaxLds = 64 * 1024;              // LDS ring limit of that kernel
+// Workgroups of a fused Chebyshev pair launch: one round of what its LDS
+// ring and registers allow per CU (measured at 256^3 on MI355X; more
+// workgroups only add the halo planes each i-run recomputes).
+#ifndef LES_PAIR_GRID_CODE32
+#define LES_PAIR_GRID_CODE32 512
+#endif
+#ifndef LES_PAIR_GRID_CODE64
+#define LES_PAIR_GRID_CODE64 256
+#endif
+#ifndef LES_PAIR_GRID_ARRAYS
+#define LES_PAIR_GRID_ARRAYS 256
+#endif
+#ifndef LES_PAIR_MAX_LDS
+#define LES_PAIR_MAX_LDS (160 * 1024)
+#endif
+constexpr int kPairTargetGridCode32 = LES_PAIR_GRID_CODE32;  // 512 threads, 2 per CU
+constexpr int kPairTargetGridCode64 = LES_PAIR_GRID_CODE64;  // 1024 threads, 1 per CU
+constexpr int kPairTargetGridArrays = LES_PAIR_GRID_ARRAYS;  // 1024 threads, 1 per CU
+constexpr size_t kPairMaxLds = LES_PAIR_MAX_LDS;  // LDS of one workgroup on gfx950
 
 struct Dims {
     int nx, ny, nz;

@@ -1454,18 +1454,25 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
     return hipSuccess;
 }
 
-// Geometry of the pair kernel per operator storage (CD 1: the code, 0: the
-// seven arrays): workgroup size TB, row passes per thread P (J = P *
-// rows-per-pass - 2 core rows per workgroup), LDS ring depth R and the
-// number of workgroups aimed for -- one round at the two workgroups per CU
-// that the 64 KiB ring allows.  Measured at 256^3 on MI355X, FP32 and FP64
-// alike: with the code, 512 x 2 (J = 14 FP32, 6 FP64) beats 256 x 2 (FP32)
-// and 256 x 3 (FP64) by 8-10%; with the arrays, 1024 x 1 beats them by
-// 4-12%; aiming for more workgroups than one round is slower.  Issuing all
-// passes' loads before their arithmetic saves a memory round trip per plane
-// but costs occupancy, so no configuration does it.
-#ifndef LES_PAIR_BLOCK_CODE
-#define LES_PAIR_BLOCK_CODE 512
+// Geometry of the pair kernel per element type and operator storage (CD 1:
+// the code, 0: the seven arrays): workgroup size TB, row passes per thread P
+// (J = P * rows-per-pass - 2 core rows per workgroup), LDS ring depth R and
+// the number of workgroups aimed for -- one round of what the ring and the
+// registers allow per CU.  Measured at 256^3 on MI355X:
+//   FP32, code:   512 x 2 (J = 14, 64 KiB ring, 2 per CU); 256 x 2, 768 x 2
+//                 and 1024 x 2 are 2-8% slower;
+//   FP64, code:   1024 x 2 (J = 14, 128 KiB ring, 1 per CU), 12% faster than
+//                 512 x 2 and 20% faster than 256 x 3;
+//   arrays, both: 1024 x 1 (J = 14 FP32, 6 FP64, 64 KiB), 4-12% faster
+//                 than 256 x 2 / 256 x 3 and than 1024 x 2.
+// Aiming for more workgroups than one round is slower.  Issuing all passes'
+// loads before their arithmetic saves a memory round trip per plane but
+// costs occupancy, so no configuration does it.
+#ifndef LES_PAIR_BLOCK_CODE32
+#define LES_PAIR_BLOCK_CODE32 512
+#endif
+#ifndef LES_PAIR_BLOCK_CODE64
+#define LES_PAIR_BLOCK_CODE64 1024
 #endif
 #ifndef LES_PAIR_PASSES_CODE
 #define LES_PAIR_PASSES_CODE 2
@@ -1481,10 +1488,13 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
 #endif
 template <typename T, int CD>
 struct PairCfg {
-    static constexpr int TB = CD ? LES_PAIR_BLOCK_CODE : LES_PAIR_BLOCK_ARRAYS;
+    static constexpr bool F32 = sizeof(T) == 4;
+    static constexpr int TB = CD ? (F32 ? LES_PAIR_BLOCK_CODE32 : LES_PAIR_BLOCK_CODE64)
+                                 : LES_PAIR_BLOCK_ARRAYS;
     static constexpr int P = CD ? LES_PAIR_PASSES_CODE : LES_PAIR_PASSES_ARRAYS;
     static constexpr int R = LES_PAIR_RING;
-    static constexpr int GRID = CD ? kPairTargetGridCode : kPairTargetGridArrays;
+    static constexpr int GRID = CD ? (F32 ? kPairTargetGridCode32 : kPairTargetGridCode64)
+                                   : kPairTargetGridArrays;
 };
 
 template <typename T, int CD>
@@ -1527,6 +1537,14 @@ hipError_t launch_cheb_pair_cd(hipStream_t s, const Dims& d, Stencil<T> A, const
     PairGeom g;
     if (!cheb_pair_geom<T, CD>(d, g, target_grid)) return hipErrorNotSupported;
     const size_t lds = (size_t)R * P * g.rpp * d.nz * sizeof(T);
+    if (lds > 64 * 1024) {
+        // rings above the default dynamic LDS limit need the kernel's opt-in
+        hipError_t e = hipFuncSetAttribute(
+            reinterpret_cast<const void*>(first ? &k_cheb_pair<T, true, CD, V, P, TB, R>
+                                                : &k_cheb_pair<T, false, CD, V, P, TB, R>),
+            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
 #define LES_PAIR(F)                                                                             \
     hipLaunchKernelGGL((k_cheb_pair<T, F, CD, V, P, TB, R>), dim3(g.grid), dim3(TB), lds, s, d, \
                        g, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a, beta_b,      \
