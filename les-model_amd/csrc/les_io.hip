@@ -25,7 +25,10 @@ namespace les {
 namespace {
 
 // ---- statistics kernels: two cells per thread (16-byte loads) ----
-constexpr int kStatGrid = 2048;
+#ifndef LES_STAT_GRID
+#define LES_STAT_GRID 2048
+#endif
+constexpr int kStatGrid = LES_STAT_GRID;
 
 __global__ __launch_bounds__(kBlock) void k_stats_update(int n, const double* __restrict__ u,
                                                          const double* __restrict__ v,
