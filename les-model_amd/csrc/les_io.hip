@@ -24,9 +24,11 @@ namespace les {
 
 namespace {
 
-// ---- statistics kernels: two cells per thread (16-byte loads) ----
+// ---- statistics kernels: two cells per thread (16-byte loads), one
+// workgroup per CU on a grid-stride loop (measured at 256^3 on MI355X: 256
+// workgroups 638 us, 2048 workgroups 679 us) ----
 #ifndef LES_STAT_GRID
-#define LES_STAT_GRID 2048
+#define LES_STAT_GRID 256
 #endif
 constexpr int kStatGrid = LES_STAT_GRID;
 
