@@ -171,7 +171,19 @@ def cpu_baseline(m, st0, solid, bounds, threads, step0, gpu_iters_per_step):
     t_iter = max(t_full - t_short, 0.0) / max(it_full - it_short, 1)
     t_fixed = max(t_short - it_short * t_iter, 0.0)
     t_step = t_fixed + gpu_iters_per_step * t_iter
-    return dict(value=1.0 / t_step, unit="steps/s", cores=threads, kind=kind,
+    # the same on one thread, from the step cut at 2 and at 4 iterations
+    chk.set_threads(1)
+    t2, i2 = one_step(2)
+    t4, i4 = one_step(4)
+    chk.set_threads(threads)
+    t1_iter = max(t4 - t2, 0.0) / max(i4 - i2, 1)
+    t1_fixed = max(t2 - i2 * t1_iter, 0.0)
+    single = dict(value=1.0 / (t1_fixed + gpu_iters_per_step * t1_iter), unit="steps/s", cores=1,
+                  pcg_iters_per_s=1.0 / t1_iter if t1_iter > 0 else None,
+                  sample=f"the same step cut at {i2} and {i4} iterations on 1 thread "
+                         f"({t2:.1f} s, {t4:.1f} s): {t1_fixed:.2f} s fixed + "
+                         f"{1e3 * t1_iter:.0f} ms per iteration, scaled the same way")
+    return dict(value=1.0 / t_step, unit="steps/s", cores=threads, kind=kind, single_thread=single,
                 pcg_iters_per_s=1.0 / t_iter if t_iter > 0 else None,
                 sample=f"step {step0} of the {m['nx']}^3 workload from the same initial state, "
                        f"PCG-Chebyshev degree 8 FP64: to 1e-8 ({it_full} iterations, "
@@ -407,6 +419,12 @@ def main():
                           "cheb_frac": r["cheb_gbs"] / HBM_PEAK_GBS,
                           "cheb_bytes_per_cell": r["cheb_bpc"]} for mname, r in res.items()},
     }
+    rf = out["roofline"]
+    if rf["traffic"]:
+        # counter bytes below the algorithmic count would mean the Infinity
+        # Cache served compulsory traffic: flagged
+        rf["traffic_over_algorithmic"] = rf["traffic"] / rf["algorithmic_bytes_per_launch"]
+        rf["traffic_below_algorithmic"] = rf["traffic_over_algorithmic"] < 1.0
     fp64_plain = "fp64:arrays" if "fp64:arrays" in res else next(
         (mm for mm in res if mm.startswith("fp64")), None)
     if fp64_plain is not None:
