@@ -371,9 +371,16 @@ def test_cheb_pairs_identical(orc, precision, fmt, shape, degree):
             g.build_stencil()
             g.set_field("b", b)
             g.set_field("p", f[4])
+            g.kernel_timing(lesgpu.KT_CHEB_PAIR)
             r = g.pcg(precond=PC_CHEBYSHEV, lmin=3 * np.pi ** 2, lmax=lmax, degree=degree,
                       max_iter=12, tol=0.0, precision=precision, reliable_every=5,
                       stencil_format=fmt)
+            launches = g.kernel_timing_read()[0]
+            g.kernel_timing(lesgpu.KT_NONE)
+            # every shape here fits every pair geometry, so the fused path
+            # ran -- except at degree 3, whose only pair would end on the
+            # reducing last step, which always runs single
+            assert (launches > 0) == bool(pairs and degree >= 4), (pairs, launches)
             outs.append((r, g.get_field("p"), g.pcg_trace(13)))
     (ra, xa, ta), (rb, xb, tb) = outs
     assert ra == rb
