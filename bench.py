@@ -66,6 +66,25 @@ def cheb_bytes_per_cell(degree, elem, coded=False):
     return (first + (m - 1) * later) / m
 
 
+def pcg_bytes_per_iter(degree, elem, coded, pairs):
+    """Algorithmic bytes per cell of one PCG-Chebyshev iteration as the
+    library runs it: the preconditioner application (fused pairs where they
+    apply, the reducing last step single), the matvec with the direction
+    update and the previous iteration's x update fused in (p_old, z, the
+    operator, x -> p, q, x) and the r update (r, q -> r).  The finalize
+    kernels and the gated FP64 residual replacement (every 50 iterations)
+    are left out."""
+    op = 2 if coded else 7 * elem
+    first, later = 3 * elem + op, 5 * elem + op
+    steps = degree - 1
+    if pairs:
+        npairs = steps // 2 - (1 if steps % 2 == 0 else 0)
+        cheb = first + (npairs - 1) * later + (steps - 2 * npairs) * later
+    else:
+        cheb = first + (steps - 1) * later
+    return cheb + (6 * elem + op) + 3 * elem
+
+
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "chebyshev_256_traffic.json")
 
 
@@ -298,6 +317,15 @@ def io_timing(g, m, ref_sample=True):
     return out
 
 
+def pcg_hbm(r, elem, cells):
+    """Whole-iteration PCG bandwidth of one mode: algorithmic bytes per
+    iteration times iterations per second of the timed steps (their physics
+    counted in the time), against the 8 TB/s peak."""
+    bpc = pcg_bytes_per_iter(8, elem, r["coded"], r["pairs"])
+    gbs = bpc * cells * float(np.sum(r["iters"])) / r["seconds"] / 1e9
+    return {"bytes_per_cell_per_iter": bpc, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -412,8 +440,10 @@ def main():
                      "algorithmic_bytes_per_launch": bpc * N,
                      "bytes_per_cell": bpc, "cells": N, "launches": n_launch,
                      "avg_us": 1e3 * kt_ms / n_launch if n_launch else None},
+        "pcg_hbm": pcg_hbm(R, elem, N),
         "modes": {mname: {"steps_per_s": args.steps / r["seconds"],
                           "pcg_iters_per_step": float(np.mean(r["iters"])),
+                          "pcg_hbm": pcg_hbm(r, r["elem"], N),
                           "cheb_kernel": "k_cheb_pair" if r["pairs"] else "k_cheb_step",
                           "cheb_gbs": r["cheb_gbs"],
                           "cheb_frac": r["cheb_gbs"] / HBM_PEAK_GBS,
