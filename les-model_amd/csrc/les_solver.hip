@@ -805,7 +805,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(Dims d, Chunking ch, Stenc
 //      z_{k+2} = z_{k+1} + alpha_{k+1} p_{k+1}
 // A workgroup owns J consecutive j-rows (whole rows, all k) of a range of
 // i-planes and marches along i.  For plane t it runs A on its rows plus one
-// halo row on each side into a 3-plane LDS ring of z_{k+1}, then B on plane
+// halo row on each side into an R-plane LDS ring of z_{k+1}, then B on plane
 // t-1 from that ring; p_k of its own rows stays in registers from one plane
 // to the next.  z_{k+1} and p_k never reach HBM.  Each cell's arithmetic is
 // exactly k_cheb_step's, so results are bit-identical to two single steps.
