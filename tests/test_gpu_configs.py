@@ -6,6 +6,9 @@
   (tests/golden/c2_128_digest.json, written by make_golden.py).  GS/SOR is
   bit-exact (sha256 of every field); the PCG solvers agree to the
   summation-order bar.
+* 256^3, 5% Bernoulli solids (the measured workload): two steps in plain
+  FP64 and in mixed precision, each to 1e-8 relative in FP64; the mixed
+  fields agree with the FP64 ones to 1e-6 of their maxima.
 * 512^3, mixed precision (20 random boxes): one step of PCG-Chebyshev
   degree 8 on Lanczos bounds reaches 1e-8 relative in FP64 (checked by an
   FP64 residual computed with the 7-array operator) -- the largest
@@ -132,3 +135,39 @@ def test_c5_512_mixed_step():
         b = g.get_field("b")
         res = b - g.matvec(p)
         assert np.linalg.norm(res) <= 1.01e-8 * np.linalg.norm(b)
+
+
+def test_c3_256_mixed_against_fp64():
+    """256^3 with 5% Bernoulli solids (the measured workload): two steps
+    (21, 22, the first under the adaptive dt) in plain FP64 (7 arrays) and in
+    mixed precision (coded operator).  Each reaches 1e-8 relative in FP64,
+    and the mixed velocity and pressure agree with the FP64 ones to 1e-6 of
+    their maxima."""
+    n = 256
+    m = wl.mesh(n)
+    N = n ** 3
+    u, v, w = wl.spectral_velocity(n, 0)
+    solid = wl.bernoulli_mask(n, 0.05, 0)
+    out = {}
+    for prec, fmt in ((lesgpu.PREC_FP64, lesgpu.STENCIL_ARRAYS),
+                      (lesgpu.PREC_MIXED, lesgpu.STENCIL_CODED)):
+        with lesgpu.LesGpu.from_mesh(m) as g:
+            g.set_mask(solid)
+            st = {k: np.zeros(N) for k in lesgpu.FIELDS[:9]}
+            st["u"], st["v"], st["w"] = u, v, w
+            g.set_state(st)
+            g.velocity_bcs()
+            g.build_stencil()
+            lo, hi = g.lanczos_bounds(20, seed=0)
+            for s in (21, 22):
+                r = g.step(step=s, nu=1e-3, rho=1.2, Cs=0.17, precond=lesgpu.PC_CHEBYSHEV,
+                           lmin=lo, lmax=1.05 * hi, degree=8, max_iter=20000, tol=1e-8,
+                           tol_rel=True, precision=prec, reliable_every=50, stencil_format=fmt)
+                assert r["converged"] == 1, (prec, s, r)
+                assert r["true_res_norm"] <= 1e-8 * r["b_norm"], (prec, s, r)
+            out[prec] = {k: g.get_field(k) for k in ("u", "v", "w", "p")}
+    for k in ("u", "v", "w", "p"):
+        a, b = out[lesgpu.PREC_MIXED][k], out[lesgpu.PREC_FP64][k]
+        err = float(np.max(np.abs(a - b)) / np.max(np.abs(b)))
+        print(f"[256^3] mixed vs FP64 {k}: {err:.2e}", flush=True)
+        assert err <= 1e-6, (k, err)
