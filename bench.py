@@ -76,12 +76,13 @@ def pcg_bytes_per_iter(degree, elem, coded, pairs):
     are left out."""
     op = 2 if coded else 7 * elem
     first, later = 3 * elem + op, 5 * elem + op
+    last = 4 * elem + op  # the reducing last step stores z only
     steps = degree - 1
     if pairs:
         npairs = steps // 2 - (1 if steps % 2 == 0 else 0)
-        cheb = first + (npairs - 1) * later + (steps - 2 * npairs) * later
+        cheb = first + (npairs - 1) * later + (steps - 2 * npairs - 1) * later + last
     else:
-        cheb = first + (steps - 1) * later
+        cheb = first + (steps - 2) * later + last
     return cheb + (6 * elem + op) + 3 * elem
 
 

@@ -785,7 +785,8 @@ __global__ __launch_bounds__(kBlock) void k_cheb_step(Dims d, Chunking ch, Stenc
             pn[v] = res + bt * pp[v];
             zn[v] = zc[v] + at * pn[v];
         }
-        vstore_s<T, V>(p + id, pn);
+        // the reducing step is an application's last: its p is never read
+        if (!REDUCE) vstore_s<T, V>(p + id, pn);
         vstore_s<T, V>(z_out + id, zn);
         if (REDUCE) {
 #pragma unroll
