@@ -6,6 +6,8 @@
 // differs (deterministic two-stage reductions, les_device.hpp).
 #include "les_kernels.hpp"
 
+#include <type_traits>
+
 #include <math.h>
 
 #include <algorithm>
@@ -212,7 +214,8 @@ __device__ __forceinline__ void load_nbr(const T* __restrict__ x, const Dims& d,
 template <int CODED, typename T, int V>
 __device__ __forceinline__ void apply7v(const Stencil<T>& A, const Dims& d, int id, int i, int j,
                                         int k, const Nbr<T, V>& nb, T (&q)[V],
-                                        const uint16_t* codes = nullptr)
+                                        const uint16_t* codes = nullptr,
+                                        const T (*coef)[V] = nullptr)
 {
     T cc[V], cw[V], ce[V], cs[V], cn[V], cb[V], ct[V];
     if (CODED == 1) {
@@ -233,6 +236,17 @@ __device__ __forceinline__ void apply7v(const Stencil<T>& A, const Dims& d, int 
             cn[v] = mult<T>((c >> 6) & 3u, A.oy, A.oy2);
             cb[v] = mult<T>((c >> 8) & 3u, A.oz, A.oz2);
             ct[v] = mult<T>((c >> 10) & 3u, A.oz, A.oz2);
+        }
+    } else if (coef) {  // the seven coefficients, given in StencilMatrix member order
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            cc[v] = coef[SC][v];
+            cn[v] = coef[SN][v];
+            cs[v] = coef[SS][v];
+            ce[v] = coef[SE][v];
+            cw[v] = coef[SW][v];
+            ct[v] = coef[ST][v];
+            cb[v] = coef[SB][v];
         }
     } else {
         vload<T, V>(A.a[SC] + id, cc);
@@ -828,7 +842,7 @@ struct PairGeom {
 // A of the next plane from overwriting what step B still reads; with 4 the
 // plane A(t+1) overwrites was last read by B(t-2), which every wave finished
 // before the barrier of plane t, so one barrier per plane suffices.
-template <typename T, bool FIRST, int CODED, int V, int P, int TB, int R>
+template <typename T, bool FIRST, int CODED, int V, int P, int TB, int R, bool CC>
 __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T> A,
                                                   const T* __restrict__ r,
                                                   const T* __restrict__ z_in,
@@ -841,6 +855,11 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
     if (is_done(sc)) return;
     extern __shared__ unsigned char les_smem[];
     T* zs = reinterpret_cast<T*>(les_smem);  // [R][P*rpp][nz]
+    // CC (arrays format): the core rows' seven coefficients of the plane
+    // step B works on, [7][J][nz] after the ring.  Step A keeps the ones it
+    // loads in registers and stores them here once step B of the previous
+    // plane has read its own; every thread reads and writes only its cells.
+    T* cfs = zs + R * P * g.rpp * d.nz;
     const int lb = logical_block(g.grid);
     const int jb = lb % g.njb, ic = lb / g.njb;
     const int j0 = jb * g.J;
@@ -854,6 +873,10 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
     // of a plane to step B of the same plane one iteration later
     T pkeep[P][V], pnew[P][V], rkeep[P][V], rnew[P][V];
     uint16_t ckeep[P][V], cnew[P][V];
+    T cfnew[CC ? P : 1][7][V];
+    bool cfval[CC ? P : 1];
+#pragma unroll
+    for (int pp = 0; pp < (CC ? P : 1); ++pp) cfval[pp] = false;
 #pragma unroll
     for (int pp = 0; pp < P; ++pp)
 #pragma unroll
@@ -887,7 +910,21 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                 for (int v = 0; v < V; ++v) zc[v] = nb.c[v];
                 uint16_t cd[V];
                 if (CODED == 1) vload<uint16_t, V>(A.code + id, cd);
-                apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az, CODED == 1 ? cd : nullptr);
+                if (CC) {
+                    T cf[7][V];
+#pragma unroll
+                    for (int c = 0; c < 7; ++c) vload<T, V>(A.a[c] + id, cf[c]);
+                    apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az, nullptr, cf);
+                    if (q < g.J) {
+#pragma unroll
+                        for (int c = 0; c < 7; ++c)
+#pragma unroll
+                            for (int v = 0; v < V; ++v) cfnew[CC ? pp : 0][c][v] = cf[c][v];
+                        cfval[CC ? pp : 0] = true;
+                    }
+                } else {
+                    apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az, CODED == 1 ? cd : nullptr);
+                }
                 T zn[V];
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
@@ -926,8 +963,15 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                 row_edges<T, V, false>(d, id, k, nb.c, nb.km, nb.kp,
                                        [&](int n) { return zc_pl[a * d.nz + kk + (n - id)]; });
                 T az[V];
-                apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az,
-                                     CODED == 1 ? ckeep[pp] : nullptr);
+                if (CC) {
+                    T cf[7][V];
+#pragma unroll
+                    for (int c = 0; c < 7; ++c) vload<T, V>(cfs + (c * g.J + q) * d.nz + kk, cf[c]);
+                    apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az, nullptr, cf);
+                } else {
+                    apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az,
+                                         CODED == 1 ? ckeep[pp] : nullptr);
+                }
                 T pn[V], zn[V];
                 const T* rv = rkeep[pp];
 #pragma unroll
@@ -938,6 +982,16 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                 }
                 vstore_s<T, V>(p_out + id, pn);
                 vstore_s<T, V>(z_out + id, zn);
+            }
+        }
+        if (CC) {
+#pragma unroll
+            for (int pp = 0; pp < (CC ? P : 1); ++pp) {
+                if (!cfval[pp]) continue;
+                const int q = pp * g.rpp + sub;
+#pragma unroll
+                for (int c = 0; c < 7; ++c) vstore<T, V>(cfs + (c * g.J + q) * d.nz + kk, cfnew[pp][c]);
+                cfval[pp] = false;
             }
         }
         if (R == 3) __syncthreads();
@@ -1487,9 +1541,16 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
 #ifndef LES_PAIR_RING
 #define LES_PAIR_RING 4
 #endif
+#ifndef LES_PAIR_CARRY_COEF64
+#define LES_PAIR_CARRY_COEF64 1
+#endif
 template <typename T, int CD>
 struct PairCfg {
     static constexpr bool F32 = sizeof(T) == 4;
+    // FP64 arrays: carry the core rows' coefficients from step A to step B
+    // through LDS (one 1024-thread workgroup per CU leaves the room) instead
+    // of reading the seven arrays again
+    static constexpr bool CC = !CD && !F32 && LES_PAIR_CARRY_COEF64;
     static constexpr int TB = CD ? (F32 ? LES_PAIR_BLOCK_CODE32 : LES_PAIR_BLOCK_CODE64)
                                  : LES_PAIR_BLOCK_ARRAYS;
     static constexpr int P = CD ? LES_PAIR_PASSES_CODE : LES_PAIR_PASSES_ARRAYS;
@@ -1497,6 +1558,14 @@ struct PairCfg {
     static constexpr int GRID = CD ? (F32 ? kPairTargetGridCode32 : kPairTargetGridCode64)
                                    : kPairTargetGridArrays;
 };
+
+// LDS of one pair workgroup: the ring, plus the carried coefficients (cc)
+template <typename T, int CD>
+size_t pair_lds(const Dims& d, const PairGeom& g, bool cc)
+{
+    using C = PairCfg<T, CD>;
+    return ((size_t)C::R * C::P * g.rpp + (cc ? 7 * (size_t)g.J : 0)) * d.nz * sizeof(T);
+}
 
 template <typename T, int CD>
 bool cheb_pair_geom(const Dims& d, PairGeom& g, int target_grid)
@@ -1509,7 +1578,7 @@ bool cheb_pair_geom(const Dims& d, PairGeom& g, int target_grid)
     g.rpp = TB / g.tpr;
     g.J = P * g.rpp - 2;
     if (g.J < 2) return false;
-    if ((size_t)C::R * P * g.rpp * d.nz * sizeof(T) > kPairMaxLds) return false;
+    if (pair_lds<T, CD>(d, g, false) > kPairMaxLds) return false;
     g.njb = (d.ny + g.J - 1) / g.J;
     if (target_grid <= 0) target_grid = C::GRID;
     int nxc = std::max(1, std::min(d.nx, target_grid / g.njb));
@@ -1537,21 +1606,35 @@ hipError_t launch_cheb_pair_cd(hipStream_t s, const Dims& d, Stencil<T> A, const
     constexpr int V = vec_width<T>(), P = C::P, TB = C::TB, R = C::R;
     PairGeom g;
     if (!cheb_pair_geom<T, CD>(d, g, target_grid)) return hipErrorNotSupported;
-    const size_t lds = (size_t)R * P * g.rpp * d.nz * sizeof(T);
+    // carry the coefficients where configured and where they fit
+    const bool cc = C::CC && pair_lds<T, CD>(d, g, true) <= kPairMaxLds;
+    const size_t lds = pair_lds<T, CD>(d, g, cc);
+    auto kernel = [&](auto F, auto CCv) {
+        return reinterpret_cast<const void*>(
+            &k_cheb_pair<T, decltype(F)::value, CD, V, P, TB, R, decltype(CCv)::value>);
+    };
+    using t_ = std::true_type;
+    using f_ = std::false_type;
+    using c_ = std::integral_constant<bool, C::CC>;  // no carrying variant where CC is off
+    const void* fn = first ? (cc ? kernel(t_{}, c_{}) : kernel(t_{}, f_{}))
+                           : (cc ? kernel(f_{}, c_{}) : kernel(f_{}, f_{}));
     if (lds > 64 * 1024) {
         // rings above the default dynamic LDS limit need the kernel's opt-in
-        hipError_t e = hipFuncSetAttribute(
-            reinterpret_cast<const void*>(first ? &k_cheb_pair<T, true, CD, V, P, TB, R>
-                                                : &k_cheb_pair<T, false, CD, V, P, TB, R>),
-            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
         if (e != hipSuccess) return e;
     }
-#define LES_PAIR(F)                                                                             \
-    hipLaunchKernelGGL((k_cheb_pair<T, F, CD, V, P, TB, R>), dim3(g.grid), dim3(TB), lds, s, d, \
-                       g, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a, beta_b,      \
+#define LES_PAIR(F, K)                                                                          \
+    hipLaunchKernelGGL((k_cheb_pair<T, F, CD, V, P, TB, R, K>), dim3(g.grid), dim3(TB), lds, s, \
+                       d, g, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a, beta_b,   \
                        alpha_b, sc)
-    if (first) LES_PAIR(true);
-    else LES_PAIR(false);
+    if (first) {
+        if (cc) LES_PAIR(true, C::CC);
+        else LES_PAIR(true, false);
+    } else {
+        if (cc) LES_PAIR(false, C::CC);
+        else LES_PAIR(false, false);
+    }
 #undef LES_PAIR
     return hipGetLastError();
 }
