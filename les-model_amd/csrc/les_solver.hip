@@ -1518,8 +1518,11 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
 //                 and 1024 x 2 are 2-8% slower;
 //   FP64, code:   1024 x 2 (J = 14, 128 KiB ring, 1 per CU), 12% faster than
 //                 512 x 2 and 20% faster than 256 x 3;
-//   arrays, both: 1024 x 1 (J = 14 FP32, 6 FP64, 64 KiB), 4-12% faster
-//                 than 256 x 2 / 256 x 3 and than 1024 x 2.
+//   arrays, both: 768 x 1 (J = 10 FP32, 4 FP64) with the core rows'
+//                 coefficients carried through LDS (118 / 104 KiB, 1 per
+//                 CU): 15% (FP32) and 20% (FP64) faster than 1024 x 1
+//                 re-reading them; 512, 896 and 768 x 2 are slower in one
+//                 precision or both.
 // Aiming for more workgroups than one round is slower.  Issuing all passes'
 // loads before their arithmetic saves a memory round trip per plane but
 // costs occupancy, so no configuration does it.
@@ -1533,7 +1536,7 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
 #define LES_PAIR_PASSES_CODE 2
 #endif
 #ifndef LES_PAIR_BLOCK_ARRAYS
-#define LES_PAIR_BLOCK_ARRAYS 1024
+#define LES_PAIR_BLOCK_ARRAYS 768
 #endif
 #ifndef LES_PAIR_PASSES_ARRAYS
 #define LES_PAIR_PASSES_ARRAYS 1
@@ -1544,13 +1547,16 @@ hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T*
 #ifndef LES_PAIR_CARRY_COEF64
 #define LES_PAIR_CARRY_COEF64 1
 #endif
+#ifndef LES_PAIR_CARRY_COEF32
+#define LES_PAIR_CARRY_COEF32 1
+#endif
 template <typename T, int CD>
 struct PairCfg {
     static constexpr bool F32 = sizeof(T) == 4;
-    // FP64 arrays: carry the core rows' coefficients from step A to step B
-    // through LDS (one 1024-thread workgroup per CU leaves the room) instead
-    // of reading the seven arrays again
-    static constexpr bool CC = !CD && !F32 && LES_PAIR_CARRY_COEF64;
+    // arrays format: carry the core rows' coefficients from step A to step
+    // B through LDS (one workgroup per CU leaves the room) instead of
+    // reading the seven arrays again
+    static constexpr bool CC = !CD && (F32 ? LES_PAIR_CARRY_COEF32 : LES_PAIR_CARRY_COEF64);
     static constexpr int TB = CD ? (F32 ? LES_PAIR_BLOCK_CODE32 : LES_PAIR_BLOCK_CODE64)
                                  : LES_PAIR_BLOCK_ARRAYS;
     static constexpr int P = CD ? LES_PAIR_PASSES_CODE : LES_PAIR_PASSES_ARRAYS;
