@@ -188,9 +188,8 @@ Stencil<T> coded_stencil(const les_ctx* c)
     s.ox = (T)(-idx2);
     s.oy = (T)(-idy2);
     s.oz = (T)(-idz2);
-    s.ox2 = (T)(-idx2 + -idx2);
-    s.oy2 = (T)(-idy2 + -idy2);
-    s.oz2 = (T)(-idz2 + -idz2);
+    // a mirrored face's (-idx2 + -idx2) rounded to T is exactly 2 * s.ox:
+    // the kernels form it as the multiplier times s.ox
     return s;
 }
 

@@ -29,8 +29,7 @@ struct Stencil {
     const T* a[7];
     const uint16_t* code;  // non-null: coefficients come from the code
     T cfluid;              // centre of a fluid row
-    T ox, oy, oz;          // -1/dx^2, -1/dy^2, -1/dz^2
-    T ox2, oy2, oz2;       // 2*(-1/dx^2) ...
+    T ox, oy, oz;          // -1/dx^2, -1/dy^2, -1/dz^2; a doubled face is 2*ox ...
 };
 
 enum { CODE_IDENTITY = 1 << 12 };

@@ -19,11 +19,26 @@ namespace {
 // ------------------------------------------------------------------
 // operator application
 // ------------------------------------------------------------------
+// A coded off-diagonal coefficient: multiplier m (0, 1 or 2) times the base
+// coefficient o = (T)(-1/h^2).  The reference's doubled coefficient is
+// 2 * (-1/h^2) rounded to T, which equals 2 * o exactly (scaling by two
+// commutes with rounding), so m * o is exact; the fused form adds +0 so that
+// m = 0 gives +0 (a plain product would give -0 for o < 0).
 template <typename T>
-__device__ __forceinline__ T mult(unsigned m, T one, T two)
+__device__ __forceinline__ T mult(unsigned m, T o)
 {
-    return m == 0u ? T(0) : (m == 1u ? one : two);
+    if constexpr (sizeof(T) == 4)
+        return __builtin_fmaf((float)m, o, 0.0f);
+    else
+        return __builtin_fma((double)m, o, 0.0);
 }
+
+// The value of a neighbour outside the grid.  With the coded operator the
+// coefficient of a missing neighbour is +0, so its product is -0, and adding
+// -0 leaves every sum unchanged (-0 included): the coded stencil needs no
+// bounds guards.  The arrays format keeps the reference's guards.
+template <typename T>
+__device__ __forceinline__ constexpr T outside() { return -T(0); }
 
 // The seven coefficients of a cell.  CODED selects the storage at compile
 // time (1 coded, 0 arrays; the run-time A.code must agree), -1 at run time.
@@ -34,12 +49,12 @@ __device__ __forceinline__ void coefs7(const Stencil<T>& A, int id, T& c, T& w, 
     if (CODED == 1 || (CODED == -1 && A.code)) {
         const unsigned cd = A.code[id];
         c = (cd & CODE_IDENTITY) ? T(1) : A.cfluid;
-        w = mult<T>(cd & 3u, A.ox, A.ox2);
-        e = mult<T>((cd >> 2) & 3u, A.ox, A.ox2);
-        s = mult<T>((cd >> 4) & 3u, A.oy, A.oy2);
-        n = mult<T>((cd >> 6) & 3u, A.oy, A.oy2);
-        b = mult<T>((cd >> 8) & 3u, A.oz, A.oz2);
-        t = mult<T>((cd >> 10) & 3u, A.oz, A.oz2);
+        w = mult<T>(cd & 3u, A.ox);
+        e = mult<T>((cd >> 2) & 3u, A.ox);
+        s = mult<T>((cd >> 4) & 3u, A.oy);
+        n = mult<T>((cd >> 6) & 3u, A.oy);
+        b = mult<T>((cd >> 8) & 3u, A.oz);
+        t = mult<T>((cd >> 10) & 3u, A.oz);
     } else {
         c = A.a[SC][id]; w = A.a[SW][id]; e = A.a[SE][id]; s = A.a[SS][id];
         n = A.a[SN][id]; b = A.a[SB][id]; t = A.a[ST][id];
@@ -156,8 +171,8 @@ template <typename T, int V, bool SPAN, typename L>
 __device__ __forceinline__ void row_edges(const Dims& d, int id, int k, const T (&c)[V], T& km,
                                           T& kp, L ld)
 {
-    km = T(0);
-    kp = T(0);
+    km = outside<T>();
+    kp = outside<T>();
     const int tpr = d.nz / V;
     const bool inwave = tpr <= 64 && (tpr & (tpr - 1)) == 0;
     if (LES_KSHFL && (inwave || SPAN)) {
@@ -177,14 +192,15 @@ __device__ __forceinline__ void row_edges(const Dims& d, int id, int k, const T 
     }
 }
 
-// The centre and the four i/j neighbour vectors (zero outside the grid).
+// The centre and the four i/j neighbour vectors (outside<T>() outside the
+// grid).
 template <typename T, int V>
 __device__ __forceinline__ void load_nbr5(const T* __restrict__ x, const Dims& d, int id, int i,
                                           int j, Nbr<T, V>& nb)
 {
     vload<T, V>(x + id, nb.c);
 #pragma unroll
-    for (int v = 0; v < V; ++v) nb.w[v] = nb.e[v] = nb.s[v] = nb.n[v] = T(0);
+    for (int v = 0; v < V; ++v) nb.w[v] = nb.e[v] = nb.s[v] = nb.n[v] = outside<T>();
     if (i > 0) vload<T, V>(x + id - d.sij, nb.w);
     if (i < d.nx - 1) vload<T, V>(x + id + d.sij, nb.e);
     if (j > 0) vload<T, V>(x + id - d.nz, nb.s);
@@ -197,16 +213,18 @@ __device__ __forceinline__ void load_nbr(const T* __restrict__ x, const Dims& d,
 {
     load_nbr5<T, V>(x, d, id, i, j, nb);
     row_edges<T, V, SPAN>(d, id, k, nb.c, nb.km, nb.kp, [&](int n) { return x[n]; });
+    // f goes over loaded values only: outside the grid the value stays
+    // outside<T>() (an identity f folds the selects away)
 #pragma unroll
     for (int v = 0; v < V; ++v) {
         nb.c[v] = f(nb.c[v]);
-        nb.w[v] = f(nb.w[v]);
-        nb.e[v] = f(nb.e[v]);
-        nb.s[v] = f(nb.s[v]);
-        nb.n[v] = f(nb.n[v]);
+        if (i > 0) nb.w[v] = f(nb.w[v]);
+        if (i < d.nx - 1) nb.e[v] = f(nb.e[v]);
+        if (j > 0) nb.s[v] = f(nb.s[v]);
+        if (j < d.ny - 1) nb.n[v] = f(nb.n[v]);
     }
-    nb.km = f(nb.km);
-    nb.kp = f(nb.kp);
+    if (k > 0) nb.km = f(nb.km);
+    if (k + V < d.nz) nb.kp = f(nb.kp);
 }
 
 // q = A x at cells id .. id+V-1: per cell exactly apply7's operand order
@@ -230,12 +248,12 @@ __device__ __forceinline__ void apply7v(const Stencil<T>& A, const Dims& d, int 
         for (int v = 0; v < V; ++v) {
             const unsigned c = cd[v];
             cc[v] = (c & CODE_IDENTITY) ? T(1) : A.cfluid;
-            cw[v] = mult<T>(c & 3u, A.ox, A.ox2);
-            ce[v] = mult<T>((c >> 2) & 3u, A.ox, A.ox2);
-            cs[v] = mult<T>((c >> 4) & 3u, A.oy, A.oy2);
-            cn[v] = mult<T>((c >> 6) & 3u, A.oy, A.oy2);
-            cb[v] = mult<T>((c >> 8) & 3u, A.oz, A.oz2);
-            ct[v] = mult<T>((c >> 10) & 3u, A.oz, A.oz2);
+            cw[v] = mult<T>(c & 3u, A.ox);
+            ce[v] = mult<T>((c >> 2) & 3u, A.ox);
+            cs[v] = mult<T>((c >> 4) & 3u, A.oy);
+            cn[v] = mult<T>((c >> 6) & 3u, A.oy);
+            cb[v] = mult<T>((c >> 8) & 3u, A.oz);
+            ct[v] = mult<T>((c >> 10) & 3u, A.oz);
         }
     } else if (coef) {  // the seven coefficients, given in StencilMatrix member order
 #pragma unroll
@@ -256,6 +274,20 @@ __device__ __forceinline__ void apply7v(const Stencil<T>& A, const Dims& d, int 
         vload<T, V>(A.a[SN] + id, cn);
         vload<T, V>(A.a[SB] + id, cb);
         vload<T, V>(A.a[ST] + id, ct);
+    }
+    if (CODED == 1) {  // neighbours outside the grid hold outside<T>()
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            T val = cc[v] * nb.c[v];
+            val += cw[v] * nb.w[v];
+            val += ce[v] * nb.e[v];
+            val += cs[v] * nb.s[v];
+            val += cn[v] * nb.n[v];
+            val += cb[v] * (v == 0 ? nb.km : nb.c[v - 1]);
+            val += ct[v] * (v == V - 1 ? nb.kp : nb.c[v + 1]);
+            q[v] = val;
+        }
+        return;
     }
 #pragma unroll
     for (int v = 0; v < V; ++v) {
@@ -596,10 +628,10 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Sten
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 nb.c[v] = comb(nz_.c[v], np_.c[v], id + v);
-                nb.w[v] = i > 0 ? comb(nz_.w[v], np_.w[v], id - d.sij + v) : T(0);
-                nb.e[v] = i < d.nx - 1 ? comb(nz_.e[v], np_.e[v], id + d.sij + v) : T(0);
-                nb.s[v] = j > 0 ? comb(nz_.s[v], np_.s[v], id - d.nz + v) : T(0);
-                nb.n[v] = j < d.ny - 1 ? comb(nz_.n[v], np_.n[v], id + d.nz + v) : T(0);
+                nb.w[v] = i > 0 ? comb(nz_.w[v], np_.w[v], id - d.sij + v) : outside<T>();
+                nb.e[v] = i < d.nx - 1 ? comb(nz_.e[v], np_.e[v], id + d.sij + v) : outside<T>();
+                nb.s[v] = j > 0 ? comb(nz_.s[v], np_.s[v], id - d.nz + v) : outside<T>();
+                nb.n[v] = j < d.ny - 1 ? comb(nz_.n[v], np_.n[v], id + d.nz + v) : outside<T>();
             }
             row_edges<T, V, true>(d, id, k, nb.c, nb.km, nb.kp,
                                   [&](int n) { return comb(z[n], p_old[n], n); });
@@ -884,6 +916,11 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
             pkeep[pp][v] = pnew[pp][v] = rkeep[pp][v] = rnew[pp][v] = T(0);
             ckeep[pp][v] = cnew[pp][v] = 0;
         }
+    // the coded step B reads its i/j neighbours from the ring unguarded:
+    // rows and planes outside the grid hold outside<T>() there
+    T pad[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) pad[v] = outside<T>();
     for (int t = ia - 1; t <= ib && ia < d.nx; ++t) {
         // ---- A at plane t: z_{k+1} into the ring, p_k of core rows ----
         if (t >= 0 && t < d.nx) {
@@ -892,7 +929,10 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                 const int q = pp * g.rpp + sub;
                 const int a = q < g.J ? q + 1 : (q == g.J ? 0 : g.J + 1);
                 const int j = j0 - 1 + a;
-                if (j < 0 || j >= d.ny) continue;
+                if (j < 0 || j >= d.ny) {
+                    if (CODED == 1) vstore<T, V>(zs + (t + R) % R * plane_sz + a * d.nz + kk, pad);
+                    continue;
+                }
                 const int i = t, k = kk, id = (i * d.ny + j) * d.nz + kk;
                 Nbr<T, V> nb;
                 T rv[V], az[V], pp_[V], zc[V];
@@ -937,6 +977,13 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                 }
                 vstore<T, V>(zs + (t + R) % R * plane_sz + a * d.nz + kk, zn);
             }
+        } else if (CODED == 1) {
+#pragma unroll
+            for (int pp = 0; pp < P; ++pp) {
+                const int q = pp * g.rpp + sub;
+                const int a = q < g.J ? q + 1 : (q == g.J ? 0 : g.J + 1);
+                vstore<T, V>(zs + (t + R) % R * plane_sz + a * d.nz + kk, pad);
+            }
         }
         __syncthreads();
         // ---- B at plane t-1 from the ring ----
@@ -954,12 +1001,19 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                 const int i = iB, k = kk, id = (i * d.ny + j) * d.nz + kk;
                 Nbr<T, V> nb;
                 vload<T, V>(zc_pl + a * d.nz + kk, nb.c);
+                if (CODED == 1) {
+                    vload<T, V>(zm_pl + a * d.nz + kk, nb.w);
+                    vload<T, V>(zp_pl + a * d.nz + kk, nb.e);
+                    vload<T, V>(zc_pl + (a - 1) * d.nz + kk, nb.s);
+                    vload<T, V>(zc_pl + (a + 1) * d.nz + kk, nb.n);
+                } else {
 #pragma unroll
-                for (int v = 0; v < V; ++v) nb.w[v] = nb.e[v] = nb.s[v] = nb.n[v] = T(0);
-                if (i > 0) vload<T, V>(zm_pl + a * d.nz + kk, nb.w);
-                if (i < d.nx - 1) vload<T, V>(zp_pl + a * d.nz + kk, nb.e);
-                if (j > 0) vload<T, V>(zc_pl + (a - 1) * d.nz + kk, nb.s);
-                if (j < d.ny - 1) vload<T, V>(zc_pl + (a + 1) * d.nz + kk, nb.n);
+                    for (int v = 0; v < V; ++v) nb.w[v] = nb.e[v] = nb.s[v] = nb.n[v] = T(0);
+                    if (i > 0) vload<T, V>(zm_pl + a * d.nz + kk, nb.w);
+                    if (i < d.nx - 1) vload<T, V>(zp_pl + a * d.nz + kk, nb.e);
+                    if (j > 0) vload<T, V>(zc_pl + (a - 1) * d.nz + kk, nb.s);
+                    if (j < d.ny - 1) vload<T, V>(zc_pl + (a + 1) * d.nz + kk, nb.n);
+                }
                 row_edges<T, V, false>(d, id, k, nb.c, nb.km, nb.kp,
                                        [&](int n) { return zc_pl[a * d.nz + kk + (n - id)]; });
                 T az[V];
