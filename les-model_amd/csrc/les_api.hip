@@ -497,7 +497,7 @@ int les_set_option(les_ctx* c, int option, int value)
         c->cheb_pairs = value != 0;
         return LES_OK;
     case LES_OPT_PAIR_GRID:
-        if (value < 0 || value > kMaxGrid) return fail(c, LES_ERR_ARG, "pair grid out of range");
+        if (value < 0 || value > kPairMaxGrid) return fail(c, LES_ERR_ARG, "pair grid out of range");
         c->pair_grid = value;
         return LES_OK;
     default:

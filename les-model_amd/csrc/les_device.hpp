@@ -59,7 +59,8 @@ constexpr int kMaxRed = 4;       // sums a fused kernel may reduce at once
 #endif
 constexpr int kPairTargetGridCode32 = LES_PAIR_GRID_CODE32;  // 512 threads, 2 per CU
 constexpr int kPairTargetGridCode64 = LES_PAIR_GRID_CODE64;  // 1024 threads, 1 per CU
-constexpr int kPairTargetGridArrays = LES_PAIR_GRID_ARRAYS;  // 1024 threads, 1 per CU
+constexpr int kPairTargetGridArrays = LES_PAIR_GRID_ARRAYS;  // 768 threads, 1 per CU
+constexpr int kPairMaxGrid = 4096;  // largest target a caller may set (the pair reduces nothing)
 constexpr size_t kPairMaxLds = LES_PAIR_MAX_LDS;  // LDS of one workgroup on gfx950
 
 struct Dims {

@@ -359,15 +359,18 @@ def test_mixed_precision_vector_rows(orc, shape):
 def test_cheb_pairs_identical(orc, precision, fmt, shape, degree):
     """Fused pairs of Chebyshev steps reproduce single steps bit for bit:
     fixed-iteration solves and their residual histories are identical.  The
-    last shape gives each workgroup a run of several i-planes."""
+    last shape gives each workgroup a run of several i-planes; the pair runs
+    at its default grid, at 8 workgroups (long i-runs) and at 4096 (one
+    plane per workgroup)."""
     m, solid, f = random_case(*shape, frac=0.05)
     A = orc.build_stencil(solid, m)
     lmax = orc.lambda_max(A, m, 20)
     b = orc.build_b(orc.divergence(f[0], f[1], f[2], solid, m, 1e-3, 1.2), solid, m)
     outs = []
-    for pairs in (0, 1):
+    for pairs, grid in ((0, 0), (1, 0), (1, 8), (1, 4096)):
         with ctx_for(m, solid) as g:
             g.set_option(lesgpu.OPT_CHEB_PAIRS, pairs)
+            g.set_option(lesgpu.OPT_PAIR_GRID, grid)
             g.build_stencil()
             g.set_field("b", b)
             g.set_field("p", f[4])
@@ -382,11 +385,23 @@ def test_cheb_pairs_identical(orc, precision, fmt, shape, degree):
             # reducing last step, which always runs single
             assert (launches > 0) == bool(pairs and degree >= 4), (pairs, launches)
             outs.append((r, g.get_field("p"), g.pcg_trace(13)))
-    (ra, xa, ta), (rb, xb, tb) = outs
-    assert ra == rb
-    assert bits_equal(xa, xb)
-    if precision == lesgpu.PREC_FP64:
-        assert bits_equal(ta, tb)
+    ra, xa, ta = outs[0]
+    for rb, xb, tb in outs[1:]:
+        assert ra == rb
+        assert bits_equal(xa, xb)
+        if precision == lesgpu.PREC_FP64:
+            assert bits_equal(ta, tb)
+
+
+def test_pair_grid_option_range():
+    """The pair grid option takes 0 (the built-in default) to 4096."""
+    m, solid, _ = random_case(4, 8, 8, 1, frac=0.0)
+    with ctx_for(m, solid) as g:
+        g.set_option(lesgpu.OPT_PAIR_GRID, 4096)
+        g.set_option(lesgpu.OPT_PAIR_GRID, 0)
+        for bad in (-1, 4097):
+            with pytest.raises(lesgpu.LesError, match="pair grid out of range"):
+                g.set_option(lesgpu.OPT_PAIR_GRID, bad)
 
 
 def test_lanczos_closed_form():
