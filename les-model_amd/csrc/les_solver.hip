@@ -1060,6 +1060,136 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
     }
 }
 
+// ------------------------------------------------------------------
+// q = A p with the direction update fused (PU 0: p given; PU 1:
+// p = z + beta p_old, stored) and the deferred x update, marching along i
+// like the Chebyshev pair: for plane t a workgroup forms p on its J rows and
+// one halo row on each side into an R-plane LDS ring (once per cell, where
+// k_pcg_matvec forms it again at every neighbour), then applies A at plane
+// t-1 from the ring.  Every cell's arithmetic is k_pcg_matvec's, so p, q and
+// x are bit-identical to it; only the order in which p.q is summed differs.
+// One kBlock workgroup per CU and at most kMaxGrid of them, so the partials
+// fit the finalize.  p_old, z and p must be distinct buffers.
+// ------------------------------------------------------------------
+template <typename T, int CODED, int V, int PU, int P, int R>
+__global__ __launch_bounds__(kBlock) void k_pcg_matvec_ring(Dims d, PairGeom g, Stencil<T> A,
+                                                            const T* __restrict__ p_old,
+                                                            const T* __restrict__ z,
+                                                            T* __restrict__ p,
+                                                            T* __restrict__ q, RedScratch red,
+                                                            const SolveScalars* sc,
+                                                            const double* __restrict__ wgt,
+                                                            T* __restrict__ x)
+{
+    if (sc->done) return;
+    extern __shared__ unsigned char les_smem[];
+    T* ps = reinterpret_cast<T*>(les_smem);  // [R][P*rpp][nz]
+    const int lb = logical_block(g.grid);
+    const int jb = lb % g.njb, ic = lb / g.njb;
+    const int j0 = jb * g.J;
+    const int ia = ic * g.irange, ib = min(ia + g.irange, d.nx);
+    const int sub = (int)threadIdx.x / g.tpr;
+    const int kk = ((int)threadIdx.x - sub * g.tpr) * V;
+    const int plane_sz = P * g.rpp * d.nz;
+    const T beta = (T)sc->beta;
+    const T ax = (T)sc->alpha;
+    double acc[1] = {0.0};
+    // p_old of the core rows, kept for the x update of the same cells one
+    // plane later
+    T okeep[P][V], onew[P][V];
+    T pad[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) pad[v] = outside<T>();
+#pragma unroll
+    for (int pp = 0; pp < P; ++pp)
+#pragma unroll
+        for (int v = 0; v < V; ++v) okeep[pp][v] = onew[pp][v] = T(0);
+    for (int t = ia - 1; t <= ib && ia < d.nx; ++t) {
+        // ---- p at plane t into the ring (the coded operator reads rows and
+        // planes outside the grid as outside<T>() from it) ----
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp) {
+            const int qr = pp * g.rpp + sub;
+            const int a = qr < g.J ? qr + 1 : (qr == g.J ? 0 : g.J + 1);
+            const int j = j0 - 1 + a;
+            T* slot = ps + (t + R) % R * plane_sz + a * d.nz + kk;
+            if (t < 0 || t >= d.nx || j < 0 || j >= d.ny) {
+                if (CODED == 1) vstore<T, V>(slot, pad);
+                continue;
+            }
+            const int id = (t * d.ny + j) * d.nz + kk;
+            T pv[V];
+            if (PU == 0) {
+                vload<T, V>(p + id, pv);
+            } else {
+                T zv[V], ov[V];
+                vload_s<T, V>(z + id, zv);
+                vload_s<T, V>(p_old + id, ov);
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    pv[v] = zv[v] + beta * ov[v];
+                    onew[pp][v] = ov[v];
+                }
+                if (qr < g.J && t >= ia && t < ib) vstore_s<T, V>(p + id, pv);
+            }
+            vstore<T, V>(slot, pv);
+        }
+        __syncthreads();
+        // ---- q = A p at plane t-1 from the ring ----
+        const int iB = t - 1;
+        if (iB >= ia && iB < ib) {
+            const T* pc_pl = ps + (iB + R) % R * plane_sz;
+            const T* pm_pl = ps + (iB + R - 1) % R * plane_sz;
+            const T* pp_pl = ps + (iB + 1) % R * plane_sz;
+#pragma unroll
+            for (int pp = 0; pp < P; ++pp) {
+                const int qr = pp * g.rpp + sub;
+                if (qr >= g.J) continue;
+                const int a = qr + 1, j = j0 + qr;
+                if (j >= d.ny) continue;
+                const int i = iB, k = kk, id = (i * d.ny + j) * d.nz + kk;
+                Nbr<T, V> nb;
+                vload<T, V>(pc_pl + a * d.nz + kk, nb.c);
+                if (CODED == 1) {
+                    vload<T, V>(pm_pl + a * d.nz + kk, nb.w);
+                    vload<T, V>(pp_pl + a * d.nz + kk, nb.e);
+                    vload<T, V>(pc_pl + (a - 1) * d.nz + kk, nb.s);
+                    vload<T, V>(pc_pl + (a + 1) * d.nz + kk, nb.n);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) nb.w[v] = nb.e[v] = nb.s[v] = nb.n[v] = T(0);
+                    if (i > 0) vload<T, V>(pm_pl + a * d.nz + kk, nb.w);
+                    if (i < d.nx - 1) vload<T, V>(pp_pl + a * d.nz + kk, nb.e);
+                    if (j > 0) vload<T, V>(pc_pl + (a - 1) * d.nz + kk, nb.s);
+                    if (j < d.ny - 1) vload<T, V>(pc_pl + (a + 1) * d.nz + kk, nb.n);
+                }
+                row_edges<T, V, false>(d, id, k, nb.c, nb.km, nb.kp,
+                                       [&](int n) { return pc_pl[a * d.nz + kk + (n - id)]; });
+                T qv[V];
+                apply7v<CODED, T, V>(A, d, id, i, j, k, nb, qv);
+                vstore_s<T, V>(q + id, qv);
+                if (PU == 1 && x) {
+                    T xv[V];
+                    vload<T, V>(x + id, xv);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) xv[v] = xv[v] + ax * okeep[pp][v];
+                    vstore_s<T, V>(x + id, xv);
+                }
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const double pq = (double)nb.c[v] * (double)qv[v];
+                    acc[0] += wgt ? wgt[id + v] * pq : pq;
+                }
+            }
+        }
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp)
+#pragma unroll
+            for (int v = 0; v < V; ++v) okeep[pp][v] = onew[pp][v];
+    }
+    block_partial<1>(acc, red.partials, lb);
+}
+
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_cheb_scale(Dims d, Chunking ch,
                                                        const T* __restrict__ r,
@@ -1416,6 +1546,44 @@ hipError_t launch_pcg_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T
                                    sc, wgt, nullptr);
 }
 
+// The ring form of the matvec (k_pcg_matvec_ring): passes of rows per
+// workgroup (FP32 rows fill a wave at nz = 256, FP64 rows two), ring depth.
+#ifndef LES_MATVEC_RING
+#define LES_MATVEC_RING 1
+#endif
+template <typename T>
+struct MvRingCfg {
+    static constexpr int P = sizeof(T) == 4 ? 1 : 2;
+    static constexpr int R = 4;
+};
+
+template <typename T>
+size_t mv_ring_lds(const Dims& d, const PairGeom& g)
+{
+    return (size_t)MvRingCfg<T>::R * MvRingCfg<T>::P * g.rpp * d.nz * sizeof(T);
+}
+
+// Workgroups: njb j-blocks times i-runs, at most kMaxGrid in all.
+template <typename T>
+bool mv_ring_geom(const Dims& d, PairGeom& g)
+{
+    constexpr int V = vec_width<T>(), P = MvRingCfg<T>::P;
+    if (d.nz % V) return false;
+    g.tpr = d.nz / V;
+    if (g.tpr > kBlock || kBlock % g.tpr) return false;
+    g.rpp = kBlock / g.tpr;
+    g.J = P * g.rpp - 2;
+    if (g.J < 2) return false;
+    if (mv_ring_lds<T>(d, g) > kPairMaxLds) return false;
+    g.njb = (d.ny + g.J - 1) / g.J;
+    if (g.njb > kMaxGrid) return false;
+    int nxc = std::max(1, std::min(d.nx, kMaxGrid / g.njb));
+    g.irange = (d.nx + nxc - 1) / nxc;
+    nxc = (d.nx + g.irange - 1) / g.irange;
+    g.grid = ((g.njb * nxc + kXcd - 1) / kXcd) * kXcd;  // <= kMaxGrid, a multiple of kXcd
+    return true;
+}
+
 template <typename T>
 hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, const T* p_old,
                                 const T* z, const T* diag, T* p, T* q, RedScratch red,
@@ -1423,8 +1591,37 @@ hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, cons
 {
     constexpr int W = vec_width<T>();
     const int V = pick_width<T>(d);
-    const Chunking ch = make_chunking(d, V);
     const int pu = p_old == nullptr ? 0 : (diag ? 2 : 1);
+    PairGeom g;
+    if (LES_MATVEC_RING && pu != 2 && V == W && mv_ring_geom<T>(d, g)) {
+        constexpr int P = MvRingCfg<T>::P, R = MvRingCfg<T>::R;
+        const size_t lds = mv_ring_lds<T>(d, g);
+        const bool cd = A.code != nullptr;
+        const void* fn =
+            cd ? (pu ? reinterpret_cast<const void*>(&k_pcg_matvec_ring<T, 1, W, 1, P, R>)
+                     : reinterpret_cast<const void*>(&k_pcg_matvec_ring<T, 1, W, 0, P, R>))
+               : (pu ? reinterpret_cast<const void*>(&k_pcg_matvec_ring<T, 0, W, 1, P, R>)
+                     : reinterpret_cast<const void*>(&k_pcg_matvec_ring<T, 0, W, 0, P, R>));
+        if (lds > 64 * 1024) {
+            hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                               (int)lds);
+            if (e != hipSuccess) return e;
+        }
+#define LES_MVR(CD, PU)                                                                         \
+    hipLaunchKernelGGL((k_pcg_matvec_ring<T, CD, W, PU, P, R>), dim3(g.grid), dim3(kBlock), lds, \
+                       s, d, g, A, p_old, z, p, q, red, sc, wgt, x)
+        if (cd) {
+            if (pu) LES_MVR(1, 1);
+            else LES_MVR(1, 0);
+        } else {
+            if (pu) LES_MVR(0, 1);
+            else LES_MVR(0, 0);
+        }
+#undef LES_MVR
+        if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+        return launch_finalize(s, g.grid, 1, false, EPI_ALPHA, GATE_DONE, red, sc, nullptr);
+    }
+    const Chunking ch = make_chunking(d, V);
 #define LES_MV(CD, VV, PU)                                                                      \
     hipLaunchKernelGGL((k_pcg_matvec<T, CD, VV, PU>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, \
                        A, p_old, z, diag, p, q, red, sc, wgt, x)
