@@ -229,11 +229,38 @@ __device__ __forceinline__ void load_nbr(const T* __restrict__ x, const Dims& d,
 
 // q = A x at cells id .. id+V-1: per cell exactly apply7's operand order
 // and guards.
-template <int CODED, typename T, int V>
+// The coded coefficients as LDS lookup tables: (w, e) by code bits 0-3,
+// (s, n) by bits 4-7, (b, t, centre) by bits 8-12.  Filled with the same
+// operations as the decode in apply7v, so every coefficient is bit-identical;
+// a cell then costs three LDS reads in place of about twenty VALU operations.
+template <typename T>
+struct alignas(4 * sizeof(T)) CodeTab {
+    vec_t<T, 2> x[16], y[16];
+    vec_t<T, 4> z[32];
+};
+
+// Threads 0..31 fill the table; the caller synchronises before use.
+template <typename T>
+__device__ __forceinline__ void code_tab_fill(CodeTab<T>& tb, const Stencil<T>& A)
+{
+    const unsigned t = threadIdx.x;
+    if (t < 16) {
+        tb.x[t] = vec_t<T, 2>{mult<T>(t & 3u, A.ox), mult<T>((t >> 2) & 3u, A.ox)};
+        tb.y[t] = vec_t<T, 2>{mult<T>(t & 3u, A.oy), mult<T>((t >> 2) & 3u, A.oy)};
+    }
+    if (t < 32) {
+        const unsigned cd = t << 8;
+        tb.z[t] = vec_t<T, 4>{mult<T>(t & 3u, A.oz), mult<T>((t >> 2) & 3u, A.oz),
+                              (cd & CODE_IDENTITY) ? T(1) : A.cfluid, T(0)};
+    }
+}
+
+template <int CODED, typename T, int V, bool TAB = false>
 __device__ __forceinline__ void apply7v(const Stencil<T>& A, const Dims& d, int id, int i, int j,
                                         int k, const Nbr<T, V>& nb, T (&q)[V],
                                         const uint16_t* codes = nullptr,
-                                        const T (*coef)[V] = nullptr)
+                                        const T (*coef)[V] = nullptr,
+                                        const CodeTab<T>* tab = nullptr)
 {
     T cc[V], cw[V], ce[V], cs[V], cn[V], cb[V], ct[V];
     if (CODED == 1) {
@@ -244,16 +271,32 @@ __device__ __forceinline__ void apply7v(const Stencil<T>& A, const Dims& d, int 
         } else {
             vload<uint16_t, V>(A.code + id, cd);
         }
+        if constexpr (TAB) {
 #pragma unroll
-        for (int v = 0; v < V; ++v) {
-            const unsigned c = cd[v];
-            cc[v] = (c & CODE_IDENTITY) ? T(1) : A.cfluid;
-            cw[v] = mult<T>(c & 3u, A.ox);
-            ce[v] = mult<T>((c >> 2) & 3u, A.ox);
-            cs[v] = mult<T>((c >> 4) & 3u, A.oy);
-            cn[v] = mult<T>((c >> 6) & 3u, A.oy);
-            cb[v] = mult<T>((c >> 8) & 3u, A.oz);
-            ct[v] = mult<T>((c >> 10) & 3u, A.oz);
+            for (int v = 0; v < V; ++v) {
+                const unsigned c = cd[v];
+                const vec_t<T, 2> ex = tab->x[c & 15u], ey = tab->y[(c >> 4) & 15u];
+                const vec_t<T, 4> ez = tab->z[(c >> 8) & 31u];
+                cw[v] = ex[0];
+                ce[v] = ex[1];
+                cs[v] = ey[0];
+                cn[v] = ey[1];
+                cb[v] = ez[0];
+                ct[v] = ez[1];
+                cc[v] = ez[2];
+            }
+        } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const unsigned c = cd[v];
+                cc[v] = (c & CODE_IDENTITY) ? T(1) : A.cfluid;
+                cw[v] = mult<T>(c & 3u, A.ox);
+                ce[v] = mult<T>((c >> 2) & 3u, A.ox);
+                cs[v] = mult<T>((c >> 4) & 3u, A.oy);
+                cn[v] = mult<T>((c >> 6) & 3u, A.oy);
+                cb[v] = mult<T>((c >> 8) & 3u, A.oz);
+                ct[v] = mult<T>((c >> 10) & 3u, A.oz);
+            }
         }
     } else if (coef) {  // the seven coefficients, given in StencilMatrix member order
 #pragma unroll
@@ -874,8 +917,17 @@ struct PairGeom {
 // A of the next plane from overwriting what step B still reads; with 4 the
 // plane A(t+1) overwrites was last read by B(t-2), which every wave finished
 // before the barrier of plane t, so one barrier per plane suffices.
+#ifndef LES_PAIR_CODE_TAB
+#define LES_PAIR_CODE_TAB 1
+#endif
+// Waves per SIMD the pair keeps resident: two 512-thread workgroups per CU,
+// or one of 768 / 1024 threads (the register budget follows: 128 VGPRs at 4).
+template <int TB>
+constexpr int pair_waves_per_simd() { return TB <= 512 ? 4 : TB / 256; }
+
 template <typename T, bool FIRST, int CODED, int V, int P, int TB, int R, bool CC>
-__global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T> A,
+__global__ __launch_bounds__(TB, pair_waves_per_simd<TB>()) void k_cheb_pair(Dims d, PairGeom g,
+                                                                           Stencil<T> A,
                                                   const T* __restrict__ r,
                                                   const T* __restrict__ z_in,
                                                   T* __restrict__ z_out,
@@ -887,6 +939,14 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
     if (is_done(sc)) return;
     extern __shared__ unsigned char les_smem[];
     T* zs = reinterpret_cast<T*>(les_smem);  // [R][P*rpp][nz]
+    // the coded operator's coefficients come from LDS lookup tables
+    const CodeTab<T>* tab = nullptr;
+    if constexpr (CODED == 1 && LES_PAIR_CODE_TAB) {
+        __shared__ CodeTab<T> tab_s;
+        code_tab_fill(tab_s, A);
+        __syncthreads();
+        tab = &tab_s;
+    }
     // CC (arrays format): the core rows' seven coefficients of the plane
     // step B works on, [7][J][nz] after the ring.  Step A keeps the ones it
     // loads in registers and stores them here once step B of the previous
@@ -963,7 +1023,8 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                         cfval[CC ? pp : 0] = true;
                     }
                 } else {
-                    apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az, CODED == 1 ? cd : nullptr);
+                    apply7v<CODED, T, V, CODED == 1 && LES_PAIR_CODE_TAB>(
+                        A, d, id, i, j, k, nb, az, CODED == 1 ? cd : nullptr, nullptr, tab);
                 }
                 T zn[V];
 #pragma unroll
@@ -1023,8 +1084,8 @@ __global__ __launch_bounds__(TB) void k_cheb_pair(Dims d, PairGeom g, Stencil<T>
                     for (int c = 0; c < 7; ++c) vload<T, V>(cfs + (c * g.J + q) * d.nz + kk, cf[c]);
                     apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az, nullptr, cf);
                 } else {
-                    apply7v<CODED, T, V>(A, d, id, i, j, k, nb, az,
-                                         CODED == 1 ? ckeep[pp] : nullptr);
+                    apply7v<CODED, T, V, CODED == 1 && LES_PAIR_CODE_TAB>(
+                        A, d, id, i, j, k, nb, az, CODED == 1 ? ckeep[pp] : nullptr, nullptr, tab);
                 }
                 T pn[V], zn[V];
                 const T* rv = rkeep[pp];
