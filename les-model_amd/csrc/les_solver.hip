@@ -1896,7 +1896,9 @@ bool cheb_pair_geom(const Dims& d, PairGeom& g, int target_grid)
     g.rpp = TB / g.tpr;
     g.J = P * g.rpp - 2;
     if (g.J < 2) return false;
-    if (pair_lds<T, CD>(d, g, false) > kPairMaxLds) return false;
+    // the coded pair also holds its decode tables in static LDS
+    const size_t tab = CD && LES_PAIR_CODE_TAB ? sizeof(CodeTab<T>) : 0;
+    if (pair_lds<T, CD>(d, g, false) + tab > kPairMaxLds) return false;
     g.njb = (d.ny + g.J - 1) / g.J;
     if (target_grid <= 0) target_grid = C::GRID;
     int nxc = std::max(1, std::min(d.nx, target_grid / g.njb));
