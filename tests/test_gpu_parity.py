@@ -436,6 +436,37 @@ def test_pcg_ring_matvec(orc, pc):
             assert np.allclose(outs[0][1], tr_o, rtol=1e-7, atol=0)
 
 
+@pytest.mark.parametrize("pc", [PC_NONE, PC_CHEBYSHEV])
+def test_coded_negative_zero(orc, pc):
+    """The coded kernels sum all seven terms unguarded and pad the neighbours
+    outside the grid with -0 (adding the -0 product of a missing neighbour
+    leaves any sum unchanged).  Blocks of -0.0 in b and x0, against the
+    domain faces, put signed zeros through those kernels (the ring matvec
+    and the fused pairs); coded and arrays solves must still agree bit for
+    bit, in FP64 and mixed precision."""
+    m, solid, f = random_case(20, 16, 64, 11, frac=0.05)
+    A = orc.build_stencil(solid, m)
+    lmax = orc.lambda_max(A, m, 20)
+    b = orc.build_b(f[3], solid, m).reshape(20, 16, 64)
+    b[:, :, :8] = -0.0
+    b[:6] = -0.0
+    x0 = np.zeros_like(b)
+    x0[:, :5] = -0.0
+    b, x0 = b.ravel(), x0.ravel()
+    assert np.signbit(b).any() and np.signbit(x0).any()
+    for precision in (lesgpu.PREC_FP64, lesgpu.PREC_MIXED):
+        outs = []
+        for fmt in (lesgpu.STENCIL_ARRAYS, lesgpu.STENCIL_CODED):
+            with ctx_for(m, solid) as g:
+                g.build_stencil()
+                g.set_field("b", b)
+                g.set_field("p", x0)
+                g.pcg(precond=pc, lmin=3 * np.pi ** 2, lmax=lmax, degree=6, max_iter=6, tol=0.0,
+                      precision=precision, reliable_every=4, stencil_format=fmt)
+                outs.append((g.get_field("p"), g.pcg_trace(7)))
+        assert bits_equal(outs[0][0], outs[1][0]) and bits_equal(outs[0][1], outs[1][1])
+
+
 def test_lanczos_closed_form():
     """On an obstacle-free grid the largest eigenvalue of A is known in
     closed form; 40 Lanczos steps approach it from below."""
