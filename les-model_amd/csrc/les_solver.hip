@@ -227,8 +227,6 @@ __device__ __forceinline__ void load_nbr(const T* __restrict__ x, const Dims& d,
     if (k + V < d.nz) nb.kp = f(nb.kp);
 }
 
-// q = A x at cells id .. id+V-1: per cell exactly apply7's operand order
-// and guards.
 // The coded coefficients as LDS lookup tables: (w, e) by code bits 0-3,
 // (s, n) by bits 4-7, (b, t, centre) by bits 8-12.  Filled with the same
 // operations as the decode in apply7v, so every coefficient is bit-identical;
@@ -255,6 +253,11 @@ __device__ __forceinline__ void code_tab_fill(CodeTab<T>& tb, const Stencil<T>& 
     }
 }
 
+// q = A x at cells id .. id+V-1, per cell in apply7's operand order.  The
+// arrays format keeps apply7's guards; the coded format sums all seven terms,
+// its callers padding neighbours outside the grid with outside<T>().  The
+// coefficients come from the code (decoded, or looked up in tab when TAB),
+// from coef (carried arrays, StencilMatrix member order) or from A's arrays.
 template <int CODED, typename T, int V, bool TAB = false>
 __device__ __forceinline__ void apply7v(const Stencil<T>& A, const Dims& d, int id, int i, int j,
                                         int k, const Nbr<T, V>& nb, T (&q)[V],
@@ -911,31 +914,32 @@ struct PairGeom {
     int grid;          // workgroups (multiple of kXcd)
 };
 
-// TB threads per workgroup.  The pair never closes an application (a
-// reducing last step runs single), so it reduces nothing.
-// R: LDS ring depth in planes.  With 3 a second barrier per plane keeps step
-// A of the next plane from overwriting what step B still reads; with 4 the
-// plane A(t+1) overwrites was last read by B(t-2), which every wave finished
-// before the barrier of plane t, so one barrier per plane suffices.
-// Wave priority during one phase of each plane (s_setprio 2, else 0):
-// 1 step A (the loads), 2 step B, 0 neither, -1 by element type (measured
-// at 256^3: A for FP32, 1.4 % faster; B for FP64, 1.7 % faster).
-// The ring matvec looks the coded coefficients up in LDS tables as the pair
-// does (measured at 256^3: 90.1 -> 88.7 us in FP32).
+// The coded coefficients of the pair (and of the ring matvec, LES_MV_TAB)
+// come from LDS lookup tables (measured at 256^3 in FP32: pair 85.2 -> 83.8
+// us, ring matvec 90.1 -> 88.7 us).
+#ifndef LES_PAIR_CODE_TAB
+#define LES_PAIR_CODE_TAB 1
+#endif
 #ifndef LES_MV_TAB
 #define LES_MV_TAB 1
 #endif
+// Wave priority during one phase of each plane (s_setprio 2, else 0):
+// 1 step A (the loads), 2 step B, 0 neither, -1 by element type (measured
+// at 256^3: A for FP32, 1.4 % faster; B for FP64, 1.7 % faster).
 #ifndef LES_PAIR_PRIO
 #define LES_PAIR_PRIO -1
-#endif
-#ifndef LES_PAIR_CODE_TAB
-#define LES_PAIR_CODE_TAB 1
 #endif
 // Waves per SIMD the pair keeps resident: two 512-thread workgroups per CU,
 // or one of 768 / 1024 threads (the register budget follows: 128 VGPRs at 4).
 template <int TB>
 constexpr int pair_waves_per_simd() { return TB <= 512 ? 4 : TB / 256; }
 
+// TB threads per workgroup.  The pair never closes an application (a
+// reducing last step runs single), so it reduces nothing.
+// R: LDS ring depth in planes.  With 3 a second barrier per plane keeps step
+// A of the next plane from overwriting what step B still reads; with 4 the
+// plane A(t+1) overwrites was last read by B(t-2), which every wave finished
+// before the barrier of plane t, so one barrier per plane suffices.
 template <typename T, bool FIRST, int CODED, int V, int P, int TB, int R, bool CC>
 __global__ __launch_bounds__(TB, pair_waves_per_simd<TB>()) void k_cheb_pair(Dims d, PairGeom g,
                                                                            Stencil<T> A,
