@@ -929,10 +929,10 @@ struct PairGeom {
 #ifndef LES_PAIR_PRIO
 #define LES_PAIR_PRIO -1
 #endif
-// Waves per SIMD the pair keeps resident: two 512-thread workgroups per CU,
-// or one of 768 / 1024 threads (the register budget follows: 128 VGPRs at 4).
+// Waves per SIMD the pair keeps resident: two workgroups per CU up to 512
+// threads, one above (the register budget follows: 128 VGPRs at 4 waves).
 template <int TB>
-constexpr int pair_waves_per_simd() { return TB <= 512 ? 4 : TB / 256; }
+constexpr int pair_waves_per_simd() { return TB <= 512 ? TB / 128 : TB / 256; }
 
 // TB threads per workgroup.  The pair never closes an application (a
 // reducing last step runs single), so it reduces nothing.
