@@ -197,6 +197,25 @@ class Oracle:
                               max_iter, tol, _p(tr))
         return x, it, tr[: it + 1]
 
+    def mirror_weights(self, A, m):
+        """W = 1/2 per direction in which a row of A is mirrored."""
+        W = np.zeros(_ncell(m))
+        self.lib.orc_mirror_weights.argtypes = [DP, C.POINTER(DP), C.POINTER(OrcMesh)]
+        self.lib.orc_mirror_weights(_p(W), _p7(A), self._m(m))
+        return W
+
+    def pcg_w(self, A, b, x0, m, precond, W, lmin=0.0, lmax=0.0, degree=5, max_iter=1000,
+              tol=1e-6):
+        """orc_pcg with r.z and p.q in the W-weighted inner product."""
+        self.lib.orc_pcg_w.argtypes = [C.POINTER(DP), DP, DP, C.c_int, C.c_int, C.c_int, C.c_int,
+                                       C.POINTER(DP), C.c_double, C.c_double, C.c_int, C.c_int,
+                                       C.c_double, DP, DP]
+        x = x0.copy()
+        tr = np.zeros(max_iter + 1)
+        it = self.lib.orc_pcg_w(_p7(A), _p(b), _p(x), *_dims(m), precond, _p7(A), lmin, lmax,
+                                degree, max_iter, tol, _p(tr), _p(W))
+        return x, it, tr[: it + 1]
+
     def pressure_bcs(self, p, solid, m):
         p = p.copy()
         self.lib.orc_apply_pressure_bcs(_p(p), _p(solid), self._m(m))

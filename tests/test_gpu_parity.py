@@ -467,6 +467,29 @@ def test_coded_negative_zero(orc, pc):
         assert bits_equal(outs[0][0], outs[1][0]) and bits_equal(outs[0][1], outs[1][1])
 
 
+@pytest.mark.parametrize("pc", [PC_JACOBI, PC_CHEBYSHEV])
+def test_pcg_mirror_inner_product_ring(orc, pc):
+    """PCG with r.z and p.q in the mirror-weighted inner product (the
+    adjoint's solves, FP64) on a grid whose rows the ring matvec takes, so
+    its weighted p.q path runs: fixed-iteration solves follow the oracle's
+    weighted PCG."""
+    m, solid, f = random_case(20, 16, 64, 13, frac=0.05)
+    A = orc.build_stencil(solid, m)
+    lmax = orc.lambda_max(A, m, 20)
+    b = orc.build_b(orc.divergence(f[0], f[1], f[2], solid, m, 1e-3, 1.2), solid, m)
+    W = orc.mirror_weights(A, m)
+    kw = dict(lmin=3 * np.pi ** 2, lmax=lmax, degree=4)
+    xo, io, tr_o = orc.pcg_w(A, b, f[4], m, pc, W, max_iter=8, tol=0.0, **kw)
+    with ctx_for(m, solid) as g:
+        g.build_stencil()
+        g.set_field("b", b)
+        g.set_field("p", f[4])
+        res = g.pcg(precond=pc, max_iter=8, tol=0.0, inner_product=lesgpu.IP_MIRROR, **kw)
+        assert res["iters"] == io == 8
+        assert rel_err(g.get_field("p"), xo) < FP64_REL_TOL
+        assert np.allclose(g.pcg_trace(9), tr_o, rtol=1e-7, atol=0)
+
+
 def test_lanczos_closed_form():
     """On an obstacle-free grid the largest eigenvalue of A is known in
     closed form; 40 Lanczos steps approach it from below."""
