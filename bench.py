@@ -1,25 +1,34 @@
 #!/usr/bin/env python3
 """LES steps/s at 256^3 on one MI355X: FP64 vs mixed-precision PCG to 1e-8.
 
-Workload (mixed-precision configuration of the reference's LES step):
+Workload (the mixed-precision configuration of the reference's LES step):
   256^3 unit cube, spectral velocity field (E(k) ~ k^-5/3, rms 1, seed 0),
   5% of cells solid (Bernoulli, seed 0), p = 0, nu = 1e-3, rho = 1.2,
-  Cs = 0.17, steps numbered from 21 so the reference's adaptive time step
-  (main.cpp:115-119: min(CFL 0.4, viscous 0.2, 1e-4)) sets dt -- at 256^3
-  the fixed dt = 1e-3 of steps 0-20 puts the explicit scheme at CFL ~1.3 and
-  the field diverges within 3 steps, in the reference's arithmetic as here,
-  pressure by PCG-Chebyshev degree 8 on bounds from 20 Lanczos steps,
-  stopping at ||r|| <= 1e-8 ||b||; mixed precision replaces the residual in
-  FP64 every 50 iterations.
+  Cs = 0.17.  The pressure operator A is built before the obstacle mask, as
+  the shipped driver builds it (main.cpp:65-81): built from the mask, its
+  identity rows hold p = 0 inside every solid cell next to mirror-Neumann
+  fluid rows, correct_velocity (main.cpp:671-714) differences p across that
+  jump, and the reference's own step map diverges within a few steps.
+  Steps are numbered from 21, so the reference's adaptive time step
+  (main.cpp:115-119) sets dt: the fixed dt = 1e-3 of steps 0-20 puts the
+  explicit scheme at CFL ~1.3 at 256^3.  Pressure by PCG-Chebyshev degree 8
+  on bounds from 20 Lanczos steps, to ||r|| <= 1e-8 ||b||; mixed precision
+  replaces the residual in FP64 every 50 iterations.
 One step = Smagorinsky nu_t -> momentum (+ lagged divergence/RHS) -> PCG ->
 projection, all on the GPU with fields resident in HBM.
 
-`value` is the mixed-precision run's steps/s; the FP64 run on the same
-inputs is reported beside it.  Prints one JSON line on stdout; progress goes
-to stderr.
+Every step's PCG iterations, dt and max |u| are recorded; a mode whose
+trajectory fails (a solve that does not converge, a non-finite scalar, or
+max |u| growing past GROWTH_LIMIT times its first value) is refused: its
+timing is dropped and its entry carries the error.  The JSON line is printed
+in every case; the exit status is non-zero when the headline mode failed.
+
+`value` is the fastest mixed-precision mode's steps/s; every mode's line is
+reported beside it.  Prints one JSON line on stdout; progress goes to stderr.
 """
 import argparse
 import json
+import math
 import os
 import platform
 import subprocess
@@ -33,6 +42,8 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+METRIC = "LES steps/s at 256^3, FP64 vs mixed PCG to 1e-8 rel.; PCG HBM GB/s vs peak"
+GROWTH_LIMIT = 4.0   # max |u| may grow to this multiple of its value at the first step
 
 
 def log(*a):
@@ -123,32 +134,66 @@ def make_workload(n, seed=0):
     return wl.mesh(n), (u, v, w), solid
 
 
-def run_mode(g, st0, precision, stencil, bounds, steps, warmup, kt_class, pcg_kw, step0):
+def step_record(s, r):
+    return {"step": s, "iters": int(r["iters"]), "dt": r["dt"], "speed_max": r["speed_max"],
+            "converged": int(r["converged"]), "fault": int(r["fault"]),
+            "true_rel_res": r["true_res_norm"] / r["b_norm"] if r["b_norm"] > 0 else 0.0}
+
+
+def trajectory_error(rec, speed0):
+    """Why a step's record refuses the trajectory, or None."""
+    if rec["fault"]:
+        return f"step {rec['step']}: non-finite PCG scalar"
+    if not rec["converged"]:
+        return f"step {rec['step']}: PCG did not converge in {rec['iters']} iterations"
+    sm = rec["speed_max"]
+    if sm >= 0 and not math.isfinite(sm):
+        return f"step {rec['step']}: max |u| is not finite"
+    if sm >= 0 and speed0 > 0 and sm > GROWTH_LIMIT * speed0:
+        return (f"step {rec['step']}: max |u| {sm:.3g} exceeds {GROWTH_LIMIT:g}x its first "
+                f"value {speed0:.3g} (diverging)")
+    return None
+
+
+def run_mode(g, st0, precision, stencil, bounds, steps, warmup, kt_class, pcg_kw, step0,
+             keep_start_state=False):
     import torch
     import lesgpu
     g.set_state(st0)
-    its, info = [], []
     lmin, lmax = bounds
     kw = dict(nu=1e-3, rho=1.2, Cs=0.17, precond=lesgpu.PC_CHEBYSHEV, lmin=lmin, lmax=lmax,
               degree=8, max_iter=20000, tol=1e-8, tol_rel=True, precision=precision,
               reliable_every=50, stencil_format=stencil, **pcg_kw)
+    traj = []
+    speed0 = None
     for s in range(step0, step0 + warmup):
-        r = g.step(step=s, **kw)
-        log(f"  warmup step {s}: iters {r['iters']} conv {r['converged']}")
+        rec = step_record(s, g.step(step=s, **kw))
+        traj.append(rec)
+        speed0 = rec["speed_max"] if speed0 is None else speed0
+        log(f"  warmup step {s}: iters {rec['iters']} dt {rec['dt']:.3g} "
+            f"max|u| {rec['speed_max']:.3f}")
+        err = trajectory_error(rec, speed0)
+        if err:
+            return dict(error=err, trajectory=traj)
+    start_state = g.get_state() if keep_start_state else None
     g.synchronize()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
+    out = []
     for s in range(step0 + warmup, step0 + warmup + steps):
-        r = g.step(step=s, **kw)
-        its.append(r["iters"])
-        info.append(r)
+        out.append((s, g.step(step=s, **kw)))
     g.synchronize()
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    for r in info:
-        if not r["converged"]:
-            raise RuntimeError(f"PCG did not converge: {r}")
+    for s, r in out:
+        rec = step_record(s, r)
+        traj.append(rec)
+        speed0 = rec["speed_max"] if speed0 is None else speed0
+        err = trajectory_error(rec, speed0)
+        if err:
+            return dict(error=err, trajectory=traj)
     state = g.get_state()
+    last_speed = float(max(np.max(np.abs(state[k])) for k in ("u", "v", "w")))
     # the kernel's own duration, from HIP events around each of its launches
     # in one more step (the events are kept out of the timed steps: each
     # record adds a few microseconds between launches)
@@ -156,68 +201,65 @@ def run_mode(g, st0, precision, stencil, bounds, steps, warmup, kt_class, pcg_kw
     g.step(step=step0 + warmup + steps, **kw)
     kt = g.kernel_timing_read()
     g.kernel_timing(lesgpu.KT_NONE)
-    return dict(seconds=t1 - t0, iters=its, last=info[-1], kt=kt, state=state)
+    its = [rec["iters"] for rec in traj[warmup:]]
+    return dict(seconds=t1 - t0, iters=its, trajectory=traj, kt=kt, state=state,
+                start_state=start_state, final_component_max=last_speed)
 
 
-def cpu_baseline(m, st0, solid, bounds, threads, step0, gpu_iters_per_step):
-    """The same workload on the host cores: the reference solver built from
-    its own sources (oracle/_ref), else the oracle.  A bounded sample: one
-    step from the same initial state run to 1e-8, and the same step cut at
-    2 PCG iterations, give the per-iteration and fixed (physics) costs; the
-    rate is then scaled to the GPU run's mean PCG iterations per step."""
+def cpu_baseline(m, state, solid, bounds, threads, step, gpu_iters, single_thread=False):
+    """The reference solver built from its own sources (oracle/_ref), else
+    the oracle, timed on the host cores on the step the GPU timed first:
+    from the GPU's FP64 state at that step, with the same stencil order,
+    Chebyshev bounds and tolerance.  One whole step to 1e-8 (std::chrono
+    around the step inside the reference harness)."""
     from cpu_checkers import Reference, Oracle, REF_SO, PC_CHEBYSHEV
     lmin, lmax = bounds
-    kw = dict(lmin=lmin, lmax=lmax, degree=8, tol=1e-8, tol_rel=True, nu=1e-3, rho=1.2)
+    kw = dict(lmin=lmin, lmax=lmax, degree=8, tol=1e-8, tol_rel=True, nu=1e-3, rho=1.2,
+              max_iter=20000)
     if os.path.exists(REF_SO):
         chk, kind = Reference(), "reference"
     else:
         chk, kind = Oracle(), "port"
-    chk.set_threads(threads)
-    A = None if kind == "reference" else chk.build_stencil(solid, m)
+    # the shipped driver's order: A built before the mask (main.cpp:65-81)
+    A = chk.build_stencil(np.zeros_like(solid), m)
 
-    def one_step(max_iter):
+    def one_step(nthreads):
+        chk.set_threads(nthreads)
+        secs = []
         t0 = time.perf_counter()
         if kind == "reference":
             # the reference fixes Cs = 0.18 (main.cpp:425); the cost is the same
-            _, its, _ = chk.run_steps(st0, solid, m, None, 1, step0=step0, solver=4,
-                                      max_iter=max_iter, **kw)
+            _, its, dts = chk.run_steps(state, solid, m, A, 1, step0=step, solver=4,
+                                        step_seconds=secs, **kw)
         else:
-            _, its, _ = chk.run_steps(st0, solid, m, A, 1, step0=step0, precond=PC_CHEBYSHEV,
-                                      Cs=0.17, max_iter=max_iter, **kw)
-        return time.perf_counter() - t0, its[0]
+            _, its, dts = chk.run_steps(state, solid, m, A, 1, step0=step, precond=PC_CHEBYSHEV,
+                                        Cs=0.17, **kw)
+            secs = [time.perf_counter() - t0]
+        return float(secs[0]), int(its[0]), float(dts[0])
 
-    t_full, it_full = one_step(20000)
-    t_short, it_short = one_step(2)
-    t_iter = max(t_full - t_short, 0.0) / max(it_full - it_short, 1)
-    t_fixed = max(t_short - it_short * t_iter, 0.0)
-    t_step = t_fixed + gpu_iters_per_step * t_iter
-    # the same on one thread, from the step cut at 2 and at 4 iterations
-    chk.set_threads(1)
-    t2, i2 = one_step(2)
-    t4, i4 = one_step(4)
+    t, it, dt = one_step(threads)
+    out = dict(value=1.0 / t, unit="steps/s", cores=threads, kind=kind,
+               nproc=os.cpu_count(), cpu_model=cpu_model(),
+               seconds_per_step=t, pcg_iters=it, gpu_pcg_iters=gpu_iters,
+               pcg_iters_per_s=it / t,
+               sample=f"step {step} of the {m['nx']}^3 workload (the first step the GPU timed) "
+                      f"from the GPU's FP64 state at that step: the reference's "
+                      f"solve_pressure_pcg_chebyshev(degree 8) on the same Lanczos bounds to "
+                      f"1e-8 relative, A built before the mask; {it} PCG iterations "
+                      f"(GPU: {gpu_iters}), dt {dt:.3g}, {t:.1f} s on {threads} OpenMP threads "
+                      f"of {cpu_model()}")
+    if single_thread:
+        t1, it1, _ = one_step(1)
+        out["single_thread"] = dict(value=1.0 / t1, unit="steps/s", cores=1, seconds_per_step=t1,
+                                    pcg_iters=it1)
     chk.set_threads(threads)
-    t1_iter = max(t4 - t2, 0.0) / max(i4 - i2, 1)
-    t1_fixed = max(t2 - i2 * t1_iter, 0.0)
-    single = dict(value=1.0 / (t1_fixed + gpu_iters_per_step * t1_iter), unit="steps/s", cores=1,
-                  pcg_iters_per_s=1.0 / t1_iter if t1_iter > 0 else None,
-                  sample=f"the same step cut at {i2} and {i4} iterations on 1 thread "
-                         f"({t2:.1f} s, {t4:.1f} s): {t1_fixed:.2f} s fixed + "
-                         f"{1e3 * t1_iter:.0f} ms per iteration, scaled the same way")
-    return dict(value=1.0 / t_step, unit="steps/s", cores=threads, kind=kind, single_thread=single,
-                pcg_iters_per_s=1.0 / t_iter if t_iter > 0 else None,
-                sample=f"step {step0} of the {m['nx']}^3 workload from the same initial state, "
-                       f"PCG-Chebyshev degree 8 FP64: to 1e-8 ({it_full} iterations, "
-                       f"{t_full:.1f} s) and cut at {it_short} iterations ({t_short:.1f} s) on "
-                       f"{threads} threads of {cpu_model()}; {t_fixed:.2f} s fixed + "
-                       f"{1e3 * t_iter:.0f} ms per iteration, scaled to the GPU run's "
-                       f"{gpu_iters_per_step:.0f} iterations per step")
+    return out
 
 
 def adjoint_timing():
     """The differentiation workload (96^3, FP64, 20 fixed-dt steps with
     checkpoints every 5, pressure to 1e-13 relative): wall time of the
     rollout and of the adjoint (which recomputes each segment)."""
-    import torch
     import lesgpu
     import les_workloads as wl
     n = 96
@@ -327,14 +369,43 @@ def pcg_hbm(r, elem, cells):
     return {"bytes_per_cell_per_iter": bpc, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
 
 
+def parse_mode(mode):
+    parts = mode.split(":")
+    prec_s = parts[0]
+    sten_s = parts[1] if len(parts) > 1 else "coded"
+    kern_s = parts[2] if len(parts) > 2 else "pairs"
+    return prec_s, sten_s, kern_s != "single"
+
+
+def base_line(args, m=None):
+    grid = [args.n] * 3 if m is None else [m["nx"], m["ny"], m["nz"]]
+    return {
+        "metric": METRIC, "value": None, "unit": "steps/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
+        "data": "synthetic (seeded spectral velocity, seeded 5% solid mask)",
+        "config": {"workload": f"LES step {grid[0]}^3, spectral velocity, 5% Bernoulli solids, "
+                               f"A built before the mask (main.cpp:65-81), PCG-Chebyshev(8) on "
+                               f"20-step Lanczos bounds to 1e-8 rel, steps {args.step0 + args.warmup}"
+                               f"-{args.step0 + args.warmup + args.steps - 1}",
+                   "grid": grid, "solid_fraction": 0.05, "precond": "chebyshev", "degree": 8,
+                   "reliable_every": 50, "first_step": args.step0,
+                   "stencil_order": "A before the mask (main.cpp:65-81)",
+                   "dt": "adaptive (main.cpp:208-250)"},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
+    ap.add_argument("--cpu-threads", type=int,
+                    default=int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1))
+    ap.add_argument("--cpu-single-thread", action="store_true",
+                    help="also time the CPU baseline step on one thread (about 1.5 min at 256^3)")
     ap.add_argument("--modes", default="fp64:arrays,mixed:arrays,fp64:coded,mixed:coded",
                     help="precision:stencil pairs; precision fp64|mixed, stencil arrays|coded")
     ap.add_argument("--check-every", type=int, default=0)
@@ -352,12 +423,26 @@ def main():
               "(run with --gpus 1)", file=sys.stderr)
         sys.exit(2)
 
+    out = base_line(args)
+    try:
+        rc = measure(args, out)
+    except Exception as e:  # reported in the line, never swallowed silently
+        import traceback
+        traceback.print_exc()
+        out["error"] = repr(e)
+        rc = 1
+    print(json.dumps(out), flush=True)
+    sys.exit(rc)
+
+
+def measure(args, out):
     import torch
     import lesgpu
     assert torch.cuda.is_available(), "no GPU"
     torch.cuda.set_device(0)
 
     m, (u, v, w), solid = make_workload(args.n)
+    out.update(base_line(args, m))
     N = m["nx"] * m["ny"] * m["nz"]
     g = lesgpu.LesGpu.from_mesh(m)
     g.set_mask(solid)
@@ -366,7 +451,7 @@ def main():
     g.set_state(st0)
     g.velocity_bcs()          # main.cpp:102
     st0 = g.get_state()
-    g.build_stencil()
+    g.build_stencil(ignore_mask=True)   # main.cpp:65-81: A before the mask
     if args.copy_calibration:
         g.diag_copy(args.copy_calibration)
         log(f"copy calibration: {args.copy_calibration} x k_convert<double,double> of "
@@ -376,24 +461,26 @@ def main():
     bounds = (rmin, 1.05 * rmax)
     log(f"Lanczos(20): ritz [{rmin:.4g}, {rmax:.4g}] -> Chebyshev [{bounds[0]:.4g}, "
         f"{bounds[1]:.4g}] in {time.perf_counter() - t0:.2f} s")
+    out["config"]["lanczos_bounds"] = list(bounds)
 
     modes = args.modes.split(",")
     res = {}
     pcg_kw = dict(check_every=args.check_every)
+    fp64_for_cpu = next((md for md in modes if md.startswith("fp64")), None)
     for mode in modes:
-        prec_s, sten_s, kern_s = (mode.split(":") + ["coded", "pairs"][len(mode.split(":")) - 1:])[:3]
+        prec_s, sten_s, pairs = parse_mode(mode)
         prec = lesgpu.PREC_MIXED if prec_s == "mixed" else lesgpu.PREC_FP64
         sten = lesgpu.STENCIL_CODED if sten_s == "coded" else lesgpu.STENCIL_ARRAYS
-        pairs = kern_s != "single"
         g.set_option(lesgpu.OPT_CHEB_PAIRS, 1 if pairs else 0)
         log(f"mode {mode}: {args.warmup} warmup + {args.steps} timed steps")
-        res[mode] = run_mode(g, st0, prec, sten, bounds, args.steps, args.warmup,
-                             lesgpu.KT_CHEB_PAIR if pairs else lesgpu.KT_CHEB_STEP, pcg_kw,
-                             args.step0)
-        r = res[mode]
-        r["elem"] = 4 if prec_s == "mixed" else 8
-        r["coded"] = sten_s == "coded"
-        r["pairs"] = pairs
+        r = run_mode(g, st0, prec, sten, bounds, args.steps, args.warmup,
+                     lesgpu.KT_CHEB_PAIR if pairs else lesgpu.KT_CHEB_STEP, pcg_kw, args.step0,
+                     keep_start_state=(mode == fp64_for_cpu and not args.no_cpu_baseline))
+        r.update(elem=4 if prec_s == "mixed" else 8, coded=sten_s == "coded", pairs=pairs)
+        res[mode] = r
+        if "error" in r:
+            log(f"  {mode}: REFUSED: {r['error']}")
+            continue
         n_l, ms, _ = r["kt"]
         r["cheb_bpc"] = (cheb_pair_bytes_per_cell if pairs else cheb_bytes_per_cell)(
             8, r["elem"], r["coded"])
@@ -401,91 +488,100 @@ def main():
         log(f"  {mode}: {args.steps / r['seconds']:.3f} steps/s, iters/step "
             f"{np.mean(r['iters']):.1f}, {'pair' if pairs else 'step'} {n_l} launches avg "
             f"{1e3 * ms / max(n_l, 1):.1f} us = {r['cheb_gbs']:.0f} GB/s algorithmic")
-    head = max((m for m in res if m.startswith("mixed")),
-               key=lambda m: args.steps / res[m]["seconds"], default=modes[0])
-    R = res[head]
-    steps_per_s = args.steps / R["seconds"]
-    n_launch, kt_ms, elem = R["kt"]
-    bpc = R["cheb_bpc"]
-    achieved = R["cheb_gbs"]
 
-    out = {
-        "metric": "LES steps/s at 256^3, FP64 vs mixed PCG to 1e-8 rel.; PCG HBM GB/s vs peak",
-        "value": steps_per_s,
-        "unit": "steps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": 1e3 * R["seconds"] / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32+f64" if head.startswith("mixed") else "f64",
-        "data": "synthetic (seeded spectral velocity, seeded 5% solid mask)",
-        "config": {"workload": f"LES step {m['nx']}^3, PCG-Chebyshev(8) on 20-step Lanczos "
-                               f"bounds to 1e-8 rel, {head}",
-                   "grid": [m["nx"], m["ny"], m["nz"]], "solid_fraction": 0.05,
-                   "precond": "chebyshev", "degree": 8, "reliable_every": 50,
-                   "first_step": args.step0, "dt": "adaptive (main.cpp:208-250)",
-                   "lanczos_bounds": list(bounds), "mode": head},
-        "pcg_iters_per_step": float(np.mean(R["iters"])),
-        "pcg_iters_per_s": float(np.sum(R["iters"]) / R["seconds"]),
-        "roofline": {"bound": "hbm",
-                     "kernel": f"{'k_cheb_pair' if R['pairs'] else 'k_cheb_step'}"
-                               f"<{'float' if elem == 4 else 'double'}>"
-                               f"{' coded' if R['coded'] else ''}",
-                     "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(f"{head}:{'pairs' if R['pairs'] else 'single'}",
-                                                 m["nx"]),
-                     "algorithmic_bytes_per_launch": bpc * N,
-                     "bytes_per_cell": bpc, "cells": N, "launches": n_launch,
-                     "avg_us": 1e3 * kt_ms / n_launch if n_launch else None},
-        "pcg_hbm": pcg_hbm(R, elem, N),
-        "modes": {mname: {"steps_per_s": args.steps / r["seconds"],
-                          "pcg_iters_per_step": float(np.mean(r["iters"])),
-                          "pcg_hbm": pcg_hbm(r, r["elem"], N),
-                          "cheb_kernel": "k_cheb_pair" if r["pairs"] else "k_cheb_step",
-                          "cheb_gbs": r["cheb_gbs"],
-                          "cheb_frac": r["cheb_gbs"] / HBM_PEAK_GBS,
-                          "cheb_bytes_per_cell": r["cheb_bpc"]} for mname, r in res.items()},
-    }
-    rf = out["roofline"]
-    if rf["traffic"]:
-        # counter bytes below the algorithmic count would mean the Infinity
-        # Cache served compulsory traffic: flagged
-        rf["traffic_over_algorithmic"] = rf["traffic"] / rf["algorithmic_bytes_per_launch"]
-        rf["traffic_below_algorithmic"] = rf["traffic_over_algorithmic"] < 1.0
-    fp64_plain = "fp64:arrays" if "fp64:arrays" in res else next(
-        (mm for mm in res if mm.startswith("fp64")), None)
-    if fp64_plain is not None:
-        F = res[fp64_plain]
-        out["fp64_plain_mode"] = fp64_plain
-        out["fp64_steps_per_s"] = args.steps / F["seconds"]
-        if head.startswith("mixed"):
-            out["mixed_vs_fp64_speedup"] = steps_per_s / out["fp64_steps_per_s"]
-            err = {}
-            for k in ("u", "v", "w", "p"):
-                a, b = R["state"][k], F["state"][k]
-                err[k] = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
-            out["mixed_max_rel_err_vs_fp64"] = err
+    def mode_entry(r):
+        e = {"trajectory": r["trajectory"]}
+        if "error" in r:
+            e["error"] = r["error"]
+            return e
+        e.update(steps_per_s=args.steps / r["seconds"],
+                 pcg_iters_per_step=float(np.mean(r["iters"])),
+                 pcg_hbm=pcg_hbm(r, r["elem"], N),
+                 cheb_kernel="k_cheb_pair" if r["pairs"] else "k_cheb_step",
+                 cheb_gbs=r["cheb_gbs"], cheb_frac=r["cheb_gbs"] / HBM_PEAK_GBS,
+                 cheb_bytes_per_cell=r["cheb_bpc"],
+                 final_component_max=r["final_component_max"])
+        return e
+
+    out["modes"] = {mname: mode_entry(r) for mname, r in res.items()}
+    ok = [md for md in res if "error" not in res[md]]
+    mixed_ok = [md for md in ok if md.startswith("mixed")]
+    pool = mixed_ok or ok
+    head = max(pool, key=lambda md: args.steps / res[md]["seconds"]) if pool else None
+    head_wanted = "mixed" if any(md.startswith("mixed") for md in modes) else modes[0]
+    rc = 0
+    if head is None or not head.startswith(head_wanted.split(":")[0]):
+        out["error"] = "the headline (mixed-precision) mode failed: " + "; ".join(
+            f"{md}: {res[md]['error']}" for md in res if "error" in res[md])
+        rc = 1
+    if head is not None:
+        R = res[head]
+        steps_per_s = args.steps / R["seconds"]
+        n_launch, kt_ms, elem = R["kt"]
+        bpc = R["cheb_bpc"]
+        achieved = R["cheb_gbs"]
+        out.update({
+            "value": steps_per_s,
+            "ms_per_step": 1e3 * R["seconds"] / args.steps,
+            "dtype": "f32+f64" if head.startswith("mixed") else "f64",
+            "pcg_iters_per_step": float(np.mean(R["iters"])),
+            "pcg_iters_per_s": float(np.sum(R["iters"]) / R["seconds"]),
+            "roofline": {"bound": "hbm",
+                         "kernel": f"{'k_cheb_pair' if R['pairs'] else 'k_cheb_step'}"
+                                   f"<{'float' if elem == 4 else 'double'}>"
+                                   f"{' coded' if R['coded'] else ''}",
+                         "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                         "frac": achieved / HBM_PEAK_GBS,
+                         "traffic": measured_traffic(
+                             f"{head}:{'pairs' if R['pairs'] else 'single'}", m["nx"]),
+                         "algorithmic_bytes_per_launch": bpc * N,
+                         "bytes_per_cell": bpc, "cells": N, "launches": n_launch,
+                         "avg_us": 1e3 * kt_ms / n_launch if n_launch else None},
+            "pcg_hbm": pcg_hbm(R, elem, N),
+        })
+        out["config"]["mode"] = head
+        out["config"]["workload"] += f", {head}"
+        rf = out["roofline"]
+        if rf["traffic"]:
+            # counter bytes below the algorithmic count would mean the
+            # Infinity Cache served compulsory traffic: flagged
+            rf["traffic_over_algorithmic"] = rf["traffic"] / rf["algorithmic_bytes_per_launch"]
+            rf["traffic_below_algorithmic"] = rf["traffic_over_algorithmic"] < 1.0
+        fp64_plain = "fp64:arrays" if "fp64:arrays" in ok else next(
+            (md for md in ok if md.startswith("fp64")), None)
+        if fp64_plain is not None:
+            F = res[fp64_plain]
+            out["fp64_plain_mode"] = fp64_plain
+            out["fp64_steps_per_s"] = args.steps / F["seconds"]
+            if head.startswith("mixed"):
+                out["mixed_vs_fp64_speedup"] = steps_per_s / out["fp64_steps_per_s"]
+                err = {}
+                for k in ("u", "v", "w", "p"):
+                    a, b = R["state"][k], F["state"][k]
+                    err[k] = float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+                out["mixed_max_rel_err_vs_fp64"] = err
     if not args.no_adjoint:
         log("differentiation workload: 96^3 rollout + adjoint")
         out["adjoint"] = adjoint_timing()
-        log(f"  rollout {out['adjoint']['rollout_s']:.2f} s, adjoint {out['adjoint']['adjoint_s']:.2f} s")
+        log(f"  rollout {out['adjoint']['rollout_s']:.2f} s, adjoint "
+            f"{out['adjoint']['adjoint_s']:.2f} s")
     if not args.no_io:
         log("statistics, checkpoint and VTK output on the workload's grid")
         out["io"] = io_timing(g, m, ref_sample=not args.no_cpu_baseline)
         log(f"  {json.dumps(out['io'])}")
-    if not args.no_cpu_baseline:
-        log(f"CPU baseline: 1 reference step on {args.cpu_threads} threads")
+    g.close()
+    F = res.get(fp64_for_cpu) if fp64_for_cpu else None
+    if not args.no_cpu_baseline and F is not None and F.get("start_state") is not None:
+        step = args.step0 + args.warmup
+        log(f"CPU baseline: step {step} of the reference on {args.cpu_threads} threads")
         try:
-            out["cpu_baseline"] = cpu_baseline(m, st0, solid, bounds, args.cpu_threads,
-                                               args.step0, float(np.mean(R["iters"])))
+            out["cpu_baseline"] = cpu_baseline(m, F["start_state"], solid, bounds,
+                                               args.cpu_threads, step, F["iters"][0],
+                                               single_thread=args.cpu_single_thread)
+            log(f"  {out['cpu_baseline']['sample']}")
         except Exception as e:  # reported, not hidden
             out["cpu_baseline"] = {"error": repr(e)}
-    g.close()
-    print(json.dumps(out), flush=True)
+    return rc
 
 
 if __name__ == "__main__":

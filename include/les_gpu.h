@@ -222,6 +222,11 @@ typedef struct les_step_stats {
     double dt;
     les_pcg_result pcg;
     int sweeps;
+    /* max |u| and max nu_t at the start of the step, from the reductions
+     * compute_adaptive_dt makes (main.cpp:217-243); -1 on steps whose dt
+     * is fixed (no reduction is run) */
+    double speed_max;
+    double nu_t_max;
 } les_step_stats;
 
 int les_step(les_ctx* ctx, const les_step_params* prm, les_step_stats* stats);

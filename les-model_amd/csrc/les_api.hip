@@ -1099,12 +1099,14 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     }
     // fold whatever correction is left and measure the FP64 residual
     HIPCHK(c, launch_mixed_fold(s, d, x64, w.x, pb[0], pb[1], 1, sc));
-    HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.q, 0, 1, c->red, sc, c->dsc));
+    HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, nullptr, 0, 1, c->red, sc, c->dsc));
     HIPCHK(c, hipMemcpyAsync(&c->sc_host[0], sc, sizeof(SolveScalars), hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipMemcpyAsync(c->hsc, c->dsc, 8, hipMemcpyDeviceToHost, s));
     HIPCHK(c, hipStreamSynchronize(s));
     const SolveScalars& h = c->sc_host[0];
-    res->iters = h.converged ? h.iter : prm->max_iter;
+    // reference convention (pressure.cpp:411-413) except on a fault, where
+    // the iteration that met the non-finite scalar is reported
+    res->iters = (h.converged || h.fault) ? h.iter : prm->max_iter;
     res->converged = h.converged;
     res->res_norm = h.err;
     res->res0_norm = h.err0;
@@ -1446,11 +1448,17 @@ int les_step(les_ctx* c, const les_step_params* prm, les_step_stats* stats)
 {
     if (!c || !prm) return fail(c, LES_ERR_ARG, "NULL argument");
     les_step_stats st{};
+    st.speed_max = -1.0;
+    st.nu_t_max = -1.0;
     int rc;
     double dt;
     if (prm->dt_fixed > 0) dt = prm->dt_fixed;
     else if (prm->step <= 20) dt = 1e-3;
-    else if ((rc = les_adaptive_dt(c, prm->nu, &dt)) != LES_OK) return rc;
+    else {
+        if ((rc = les_adaptive_dt(c, prm->nu, &dt)) != LES_OK) return rc;
+        st.speed_max = std::sqrt(c->hsc[0]);  // the reductions les_adaptive_dt read back
+        st.nu_t_max = c->hsc[1];
+    }
     st.dt = dt;
     const PhysConsts k = make_consts(c, dt, prm->nu, prm->rho, prm->Cs);
     hipStream_t s = c->stream;

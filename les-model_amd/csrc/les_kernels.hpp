@@ -60,6 +60,9 @@ struct SolveScalars {
     int restart_pending; // 1: the next beta is 0 (p restarts from z)
     int restarts;
     double true_err;     // ||b - A x64|| of the last replacement
+    double rscale;       // mixed: power of two the FP32 residual carries (set at
+                         // iteration 0 from ||b - A x0||, so that ||r32|| ~ 1:
+                         // no FP32 overflow or subnormals whatever b's scale)
     double* trace;       // optional: ||r_k|| for k = 0..max_iter
 };
 

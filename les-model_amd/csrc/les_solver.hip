@@ -388,10 +388,22 @@ __device__ void epilogue(int epi, const double* tot, SolveScalars* sc, double* o
     case EPI_INIT:
         epi_init(tot, sc);
         break;
-    case EPI_INIT_MIXED:
+    case EPI_INIT_MIXED: {
         epi_init(tot, sc);
         sc->true_err = sc->err0;
+        // r32 = r64 * 2^-e with ||r64|| = m 2^e, 0.5 <= m < 1: the FP32
+        // iteration is linear in r, so a power-of-two scale changes no
+        // mantissa bit of any in-range value, only keeps the range in FP32
+        double s = 1.0;
+        const double e = sc->err0;
+        if (e > 0.0 && e <= 1.7976931348623157e308) {
+            int ex = 0;
+            frexp(e, &ex);
+            s = ldexp(1.0, -ex);
+        }
+        sc->rscale = s;
         break;
+    }
     case EPI_RHO:
         sc->rho = tot[0];
         break;
@@ -435,7 +447,7 @@ __device__ void epilogue(int epi, const double* tot, SolveScalars* sc, double* o
         const int it = sc->iter + 1;
         sc->iter = it;
         sc->rr = tot[0];
-        const double e = sqrt(tot[0]);
+        const double e = sqrt(tot[0]) / sc->rscale;  // r32 carries the scale
         sc->err = e;
         if (sc->trace) sc->trace[it] = e;
         if (!(e == e)) {
@@ -1320,12 +1332,13 @@ __global__ __launch_bounds__(kBlock) void k_mixed_fold(Dims d, Chunking ch,
     const int lb = logical_block(ch.grid);
     const int it = sc->iter;
     const float ax = (float)sc->alpha;
+    const double unscale = 1.0 / sc->rscale;  // exact: a power of two
     const float* pl = ((it - 1) & 1) ? p1 : p0;
     FOR_CELLS(d, ch, lb)
     {
         const float ap = it > 0 ? ax * pl[id] : 0.0f;
         const float xs = it > 0 ? x32[id] + ap : x32[id];
-        x64[id] = x64[id] + (double)xs;
+        x64[id] = x64[id] + (double)xs * unscale;
         x32[id] = -ap;
     }
 }
@@ -1355,17 +1368,18 @@ __global__ __launch_bounds__(kBlock) void k_mixed_residual(Dims d, Chunking ch, 
 {
     if (!gate_open(gate, sc)) return;
     const int lb = logical_block(ch.grid);
+    const double rs = r32 ? sc->rscale : 1.0;
     double acc[2] = {0.0, 0.0};
     FOR_CELLS(d, ch, lb)
     {
         const double q = apply7<CODED>(A, d, id, i, j, k, [&](int n) { return x[n]; });
         const double bv = b[id];
         const double rv = bv - q;
-        r32[id] = (float)rv;
+        if (r32) r32[id] = (float)(rv * rs);
         acc[0] += rv * rv;
         acc[1] += bv * bv;
     }
-    block_partial<2>(acc, red.partials, lb);
+    if (red.partials) block_partial<2>(acc, red.partials, lb);
 }
 
 template <int V>
@@ -2070,14 +2084,24 @@ hipError_t launch_mixed_residual(hipStream_t s, const Dims& d, Stencil<double> A
 {
     const Chunking ch = make_chunking(d);
     const int gate = (init || force) ? (int)GATE_NONE : (int)GATE_RELIABLE;
-    if (A.code)
-        hipLaunchKernelGGL((k_mixed_residual<1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, b,
-                           x64, r32, gate, red, sc);
-    else
-        hipLaunchKernelGGL((k_mixed_residual<0>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, b,
-                           x64, r32, gate, red, sc);
-    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
-    const int epi = init ? EPI_INIT_MIXED : (force ? EPI_STORE_SQRT : EPI_RELIABLE);
+    auto pass = [&](float* dst, RedScratch rd) {
+        if (A.code)
+            hipLaunchKernelGGL((k_mixed_residual<1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A,
+                               b, x64, dst, gate, rd, sc);
+        else
+            hipLaunchKernelGGL((k_mixed_residual<0>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A,
+                               b, x64, dst, gate, rd, sc);
+        return hipGetLastError();
+    };
+    if (init) {
+        // ||b - A x0|| first: it sets the scale r32 carries; then r32
+        if (pass(nullptr, red) != hipSuccess) return hipErrorLaunchFailure;
+        hipError_t e = launch_finalize(s, ch.grid, 2, false, EPI_INIT_MIXED, gate, red, sc, out);
+        if (e != hipSuccess) return e;
+        return pass(r32, RedScratch{nullptr});
+    }
+    if (pass(r32, red) != hipSuccess) return hipErrorLaunchFailure;
+    const int epi = force ? EPI_STORE_SQRT : EPI_RELIABLE;
     return launch_finalize(s, ch.grid, 2, false, epi, gate, red, sc, out);
 }
 

@@ -64,7 +64,8 @@ class StepParams(C.Structure):
 
 
 class StepStats(C.Structure):
-    _fields_ = [("dt", C.c_double), ("pcg", PcgResult), ("sweeps", C.c_int)]
+    _fields_ = [("dt", C.c_double), ("pcg", PcgResult), ("sweeps", C.c_int),
+                ("speed_max", C.c_double), ("nu_t_max", C.c_double)]
 
 
 class AdjParams(C.Structure):
@@ -438,7 +439,8 @@ class LesGpu:
                          sor_sweeps)
         st = StepStats()
         self._chk(self.lib.les_step(self.h, C.byref(prm), C.byref(st)), "les_step")
-        return {"dt": st.dt, "sweeps": st.sweeps, **st.pcg.as_dict()}
+        return {"dt": st.dt, "sweeps": st.sweeps, "speed_max": st.speed_max,
+                "nu_t_max": st.nu_t_max, **st.pcg.as_dict()}
 
     # ---- differentiation (no reference counterpart) ----
     @classmethod

@@ -12,12 +12,16 @@
  *     (oracle/Makefile -> oracle/_ref/libref.so), bit for bit on one thread,
  *     function by function and for whole steps;
  *   - against the committed fixtures in tests/golden/ written by that build;
- *   - against closed forms: the separable cosine eigenvectors of the
- *     obstacle-free stencil, exact central differences of quadratic fields,
- *     LL^T = A on the lower pattern for IC(0), dense solves on tiny grids.
- *   orc_solve_pressure_rbsor has no reference counterpart; it is pinned by
- *   convergence to the same fixed point as orc_solve_pressure_gs and by a
- *   brute-force numpy restatement on a tiny grid.
+ *   - against closed forms (tests/test_oracle_closed_forms.py): the
+ *     separable cosine eigenvectors of the obstacle-free stencil, exact
+ *     central and one-sided differences of polynomial fields (SGS, momentum,
+ *     divergence, projection), L L^T = A on the diagonal and lower pattern
+ *     for IC(0), PCG against a dense solve on a 10^3 grid.
+ *   orc_solve_pressure_rbsor has no reference counterpart; the same test
+ *   file pins it bit for bit against a vectorised numpy restatement of its
+ *   two colour half-sweeps (1, 2 and 7 sweeps, and the early-exit count),
+ *   and pins its converged result, with orc_solve_pressure_gs's, against
+ *   the dense fixed point of the sweep-plus-pressure-BC map.
  */
 #include "les_oracle.h"
 

@@ -6,6 +6,7 @@
 // Each wrapper copies raw arrays into the reference's own types (Field,
 // StencilMatrix, Vector; src/field.h, src/pressure.h), calls the reference
 // function named in its comment, and copies the results back.
+#include <chrono>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -416,7 +417,7 @@ int ref_run_steps(double* u, double* v, double* w, double* un, double* vn, doubl
                   int ny, int nz, double dx, double dy, double dz, double nu, double rho,
                   int step0, int nsteps, double dt_fixed, int solver, double lmin, double lmax,
                   int degree, int max_iter, double tol, int tol_rel, int gs_iters,
-                  int* iters_out, double* dt_out)
+                  int* iters_out, double* dt_out, double* secs_out)
 {
     Mesh m = mk_mesh(nx, ny, nz, dx, dy, dz);
     size_t n = ncell(m);
@@ -435,6 +436,8 @@ int ref_run_steps(double* u, double* v, double* w, double* un, double* vn, doubl
     std::vector<double*> l7 = {L.A_c.data(), L.A_n.data(), L.A_s.data(), L.A_e.data(),
                                L.A_w.data(), L.A_t.data(), L.A_b.data()};
     for (int s = 0; s < nsteps; ++s) {
+        // wall time of the step as the driver loop runs it (main.cpp:115-144)
+        const auto t_start = std::chrono::steady_clock::now();
         int step = step0 + s;
         double dt;
         if (dt_fixed > 0) dt = dt_fixed;
@@ -465,6 +468,9 @@ int ref_run_steps(double* u, double* v, double* w, double* un, double* vn, doubl
         if (iters_out) iters_out[s] = it;
         correct_velocity(f, m, dt, rho);
         apply_velocity_BCs(f, m);
+        if (secs_out)
+            secs_out[s] = std::chrono::duration<double>(std::chrono::steady_clock::now() -
+                                                        t_start).count();
     }
     get(u, f.u, n); get(v, f.v, n); get(w, f.w, n);
     get(un, f.u_new, n); get(vn, f.v_new, n); get(wn, f.w_new, n);

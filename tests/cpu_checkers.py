@@ -377,7 +377,7 @@ class Reference:
             C.POINTER(DP)] * 4 + [C.POINTER(DP)]
         L.ref_run_steps.argtypes = [DP] * 9 + [U8P, C.POINTER(DP)] + md + [
             C.c_double, C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_double,
-            C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, IP, DP]
+            C.c_double, C.c_int, C.c_int, C.c_double, C.c_int, C.c_int, IP, DP, DP]
 
     def set_threads(self, n):
         self.lib.ref_set_threads(int(n))
@@ -542,14 +542,22 @@ class Reference:
 
     def run_steps(self, state, solid, m, A, nsteps, step0=0, solver=0, nu=1e-3, rho=1.2,
                   dt_fixed=0.0, lmin=0.0, lmax=0.0, degree=5, max_iter=1000, tol=1e-6,
-                  tol_rel=False, gs_iters=200):
+                  tol_rel=False, gs_iters=200, step_seconds=None):
+        """The driver's step loop (main.cpp:115-144) from `state`; A None:
+        the reference's build_stencil from `solid`.  With a list as
+        step_seconds, each step's wall time (std::chrono around the step
+        inside the harness) is appended to it."""
         st = {k: state[k].copy() for k in STATE_KEYS}
         iters = np.zeros(nsteps, dtype=np.int32)
         dts = np.zeros(nsteps)
+        secs = np.zeros(nsteps)
         self.lib.ref_run_steps(*[_p(st[k]) for k in STATE_KEYS], _p(np.ascontiguousarray(solid)),
                                _p7(A) if A is not None else None, *self._md(m), nu, rho, step0,
                                nsteps, dt_fixed, solver, lmin, lmax, degree, max_iter, tol,
-                               1 if tol_rel else 0, gs_iters, iters.ctypes.data_as(IP), _p(dts))
+                               1 if tol_rel else 0, gs_iters, iters.ctypes.data_as(IP), _p(dts),
+                               _p(secs))
+        if step_seconds is not None:
+            step_seconds.extend(float(x) for x in secs)
         return st, list(iters), list(dts)
 
 

@@ -352,8 +352,45 @@ def test_mixed_precision_vector_rows(orc, shape):
         assert rel_err(x, x_ref) < 1e-5
 
 
+@pytest.mark.parametrize("pc", [PC_JACOBI, PC_CHEBYSHEV])
+@pytest.mark.parametrize("shape", [None, (10, 9, 36, 7)])
+def test_mixed_precision_rhs_scale(orc, pc, shape):
+    """The FP32 residual carries a power-of-two scale set from ||b - A x0||,
+    so a right-hand side far outside FP32's range (||b|| ~ 1e43 and 1e-35:
+    unscaled, the first overflows FP32 and the second decays into its
+    subnormals) solves exactly as b does: same iteration and replacement
+    counts, and the solution scaled by the same power of two, bit for bit."""
+    if shape is None:
+        m, solid, f = box_case(32, 5)
+    else:
+        m, solid, f = random_case(*shape, frac=0.05)
+    A = orc.build_stencil(solid, m)
+    lmax = orc.lambda_max(A, m, 20)
+    b = orc.build_b(orc.divergence(f[0], f[1], f[2], solid, m, 1e-3, 1.2), solid, m)
+    kw = dict(precond=pc, lmin=3 * np.pi ** 2, lmax=lmax, degree=8, max_iter=5000, tol=1e-8,
+              tol_rel=True, precision=lesgpu.PREC_MIXED, reliable_every=50)
+    out = {}
+    with ctx_for(m, solid) as g:
+        g.build_stencil()
+        for e in (0, 130, -130):
+            g.set_field("b", np.ldexp(b, e))
+            g.set_field("p", np.zeros_like(b))
+            res = g.pcg(**kw)
+            out[e] = (res, g.get_field("p"))
+    r0, x0 = out[0]
+    assert r0["converged"] == 1 and r0["fault"] == 0, r0
+    assert np.linalg.norm(b - orc.matvec(x0, A, m)) < 1.01e-8 * np.linalg.norm(b)
+    for e in (130, -130):
+        r, x = out[e]
+        assert r["fault"] == 0 and r["converged"] == 1, (e, r)
+        assert (r["iters"], r["replacements"], r["restarts"]) == \
+            (r0["iters"], r0["replacements"], r0["restarts"]), (e, r, r0)
+        assert bits_equal(np.ldexp(x, -e), x0), e
+        assert r["true_res_norm"] == np.ldexp(r0["true_res_norm"], e), e
+
+
 @pytest.mark.parametrize("precision", [lesgpu.PREC_FP64, lesgpu.PREC_MIXED])
-@pytest.mark.parametrize("fmt", [lesgpu.STENCIL_ARRAYS, lesgpu.STENCIL_CODED])
+@pytest.mark.parametrize("fmt",[lesgpu.STENCIL_ARRAYS, lesgpu.STENCIL_CODED])
 @pytest.mark.parametrize("shape,degree", [((6, 10, 256, 8), 8), ((7, 23, 32, 9), 5),
                                           ((5, 14, 256, 10), 3), ((64, 120, 256, 11), 8)])
 def test_cheb_pairs_identical(orc, precision, fmt, shape, degree):
