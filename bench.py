@@ -3,8 +3,13 @@
 
 Workload (the mixed-precision configuration of the reference's LES step):
   256^3 unit cube, spectral velocity field (E(k) ~ k^-5/3, rms 1, seed 0),
-  5% of cells solid (Bernoulli, seed 0), p = 0, nu = 1e-3, rho = 1.2,
-  Cs = 0.17.  The pressure operator A is built before the obstacle mask, as
+  5% of cells solid (Bernoulli, seed 0), p = 0, nu = 5e-3, rho = 1.2,
+  Cs = 0.17.  The configuration leaves nu open; at nu = 1e-3 the
+  reference's forward-Euler step with central advection (main.cpp:563-659)
+  is unstable on this field: dt = 1e-4 exceeds the FTCS limit
+  2 nu_eff / |u|^2 ~ 8e-5, and max |u| grows about 14% a step from step
+  ~27 (nu = 2e-3 likewise); at nu = 5e-3 it stays bounded for 100 steps.
+  The pressure operator A is built before the obstacle mask, as
   the shipped driver builds it (main.cpp:65-81): built from the mask, its
   identity rows hold p = 0 inside every solid cell next to mirror-Neumann
   fluid rows, correct_velocity (main.cpp:671-714) differences p across that
@@ -156,12 +161,12 @@ def trajectory_error(rec, speed0):
 
 
 def run_mode(g, st0, precision, stencil, bounds, steps, warmup, kt_class, pcg_kw, step0,
-             keep_start_state=False):
+             nu, keep_start_state=False):
     import torch
     import lesgpu
     g.set_state(st0)
     lmin, lmax = bounds
-    kw = dict(nu=1e-3, rho=1.2, Cs=0.17, precond=lesgpu.PC_CHEBYSHEV, lmin=lmin, lmax=lmax,
+    kw = dict(nu=nu, rho=1.2, Cs=0.17, precond=lesgpu.PC_CHEBYSHEV, lmin=lmin, lmax=lmax,
               degree=8, max_iter=20000, tol=1e-8, tol_rel=True, precision=precision,
               reliable_every=50, stencil_format=stencil, **pcg_kw)
     traj = []
@@ -206,7 +211,7 @@ def run_mode(g, st0, precision, stencil, bounds, steps, warmup, kt_class, pcg_kw
                 start_state=start_state, final_component_max=last_speed)
 
 
-def cpu_baseline(m, state, solid, bounds, threads, step, gpu_iters, single_thread=False):
+def cpu_baseline(m, state, solid, bounds, threads, step, gpu_iters, nu, single_thread=False):
     """The reference solver built from its own sources (oracle/_ref), else
     the oracle, timed on the host cores on the step the GPU timed first:
     from the GPU's FP64 state at that step, with the same stencil order,
@@ -214,7 +219,7 @@ def cpu_baseline(m, state, solid, bounds, threads, step, gpu_iters, single_threa
     around the step inside the reference harness)."""
     from cpu_checkers import Reference, Oracle, REF_SO, PC_CHEBYSHEV
     lmin, lmax = bounds
-    kw = dict(lmin=lmin, lmax=lmax, degree=8, tol=1e-8, tol_rel=True, nu=1e-3, rho=1.2,
+    kw = dict(lmin=lmin, lmax=lmax, degree=8, tol=1e-8, tol_rel=True, nu=nu, rho=1.2,
               max_iter=20000)
     if os.path.exists(REF_SO):
         chk, kind = Reference(), "reference"
@@ -385,10 +390,12 @@ def base_line(args, m=None):
         "scaling": "weak", "vs_baseline": None, "dtype": "f32+f64",
         "data": "synthetic (seeded spectral velocity, seeded 5% solid mask)",
         "config": {"workload": f"LES step {grid[0]}^3, spectral velocity, 5% Bernoulli solids, "
+                               f"nu {args.nu:g}, "
                                f"A built before the mask (main.cpp:65-81), PCG-Chebyshev(8) on "
                                f"20-step Lanczos bounds to 1e-8 rel, steps {args.step0 + args.warmup}"
                                f"-{args.step0 + args.warmup + args.steps - 1}",
-                   "grid": grid, "solid_fraction": 0.05, "precond": "chebyshev", "degree": 8,
+                   "grid": grid, "solid_fraction": 0.05, "nu": args.nu, "rho": 1.2, "Cs": 0.17,
+                   "precond": "chebyshev", "degree": 8,
                    "reliable_every": 50, "first_step": args.step0,
                    "stencil_order": "A before the mask (main.cpp:65-81)",
                    "dt": "adaptive (main.cpp:208-250)"},
@@ -401,6 +408,7 @@ def main():
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--nu", type=float, default=5e-3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int,
                     default=int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1))
@@ -475,7 +483,7 @@ def measure(args, out):
         log(f"mode {mode}: {args.warmup} warmup + {args.steps} timed steps")
         r = run_mode(g, st0, prec, sten, bounds, args.steps, args.warmup,
                      lesgpu.KT_CHEB_PAIR if pairs else lesgpu.KT_CHEB_STEP, pcg_kw, args.step0,
-                     keep_start_state=(mode == fp64_for_cpu and not args.no_cpu_baseline))
+                     args.nu, keep_start_state=(mode == fp64_for_cpu and not args.no_cpu_baseline))
         r.update(elem=4 if prec_s == "mixed" else 8, coded=sten_s == "coded", pairs=pairs)
         res[mode] = r
         if "error" in r:
@@ -577,7 +585,7 @@ def measure(args, out):
         try:
             out["cpu_baseline"] = cpu_baseline(m, F["start_state"], solid, bounds,
                                                args.cpu_threads, step, F["iters"][0],
-                                               single_thread=args.cpu_single_thread)
+                                               args.nu, single_thread=args.cpu_single_thread)
             log(f"  {out['cpu_baseline']['sample']}")
         except Exception as e:  # reported, not hidden
             out["cpu_baseline"] = {"error": repr(e)}

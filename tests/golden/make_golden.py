@@ -5,9 +5,11 @@ solver, compiled from its own sources by oracle/Makefile (oracle/_ref/).
 Run from the repository root where the reference build exists:
     make -C oracle ref && python tests/golden/make_golden.py
 
-Everything runs on one OpenMP thread (the reference's reductions and its
-power-iteration start vector are deterministic only there).  The fixtures
-hold inputs and the reference's outputs:
+Everything but the C2 steps runs on one OpenMP thread (the reference's
+reductions and its power-iteration start vector are deterministic only
+there); the C2 steps may run on more (its fixture is replayed on the GPU at
+its recorded iteration counts and compared to a tolerance, never bit for
+bit).  The fixtures hold inputs and the reference's outputs:
 
   kernels_10x9x8.npz   every hot-path function once, on a 10x9x8 grid with
                        unequal spacings and a seeded solid mask that has
@@ -18,13 +20,16 @@ hold inputs and the reference's outputs:
                        the adaptive time step)
   c1_digest.json       the CPU-runnable configuration (32^3, U(-1,1) seed 0,
                        centred 8^3 obstacle, 10 steps, PCG-Jacobi to 1e-8
-                       relative): per-step PCG iterations and dt, and final
-                       field digests (sums, norms, 1024 seeded probes, sha256)
-  c2_128_digest.json   the plain GPU configuration's inputs at 128^3, two
-                       steps with each of its pressure solvers (PCG-Jacobi,
-                       PCG-IC(0), PCG-Chebyshev 4, GS/SOR 200 sweeps): per-step
-                       iterations (sweeps for GS) and dt, final field digests
-                       (`python tests/golden/make_golden.py c2` remakes only it)
+                       relative, A built before the mask as main.cpp:65-81):
+                       per-step PCG iterations and dt, and final field
+                       digests (sums, norms, 1024 seeded probes, sha256)
+                       (`make_golden.py c1` remakes only it)
+  c2_128_digest.json   the plain GPU configuration at 128^3, 50 steps with
+                       each of its pressure solvers (PCG-Jacobi, PCG-IC(0),
+                       PCG-Chebyshev 4, GS/SOR 200 sweeps): per-step
+                       iterations and dt, field digests every 10 steps
+                       (`make_golden.py c2 THREADS [RUN ...]` remakes it, the
+                       steps on THREADS OpenMP threads)
 """
 import hashlib
 import json
@@ -111,59 +116,92 @@ def digest(a):
 
 
 def c1_fixture(ref):
+    """C1 in the shipped driver's order (A built before the obstacle mask,
+    main.cpp:65-81), one thread: the oracle reproduces it bit for bit."""
     m, (u, v, w), solid = wl.config_c1()
     N = 32 ** 3
     st0 = initial_state(u, v, w, N, solid, m, ref)
-    st, its, dts = ref.run_steps(st0, solid, m, None, 10, step0=0, solver=1, nu=1e-3, rho=1.2,
+    A = ref.build_stencil(np.zeros_like(solid), m)
+    st, its, dts = ref.run_steps(st0, solid, m, A, 10, step0=0, solver=1, nu=1e-3, rho=1.2,
                                  max_iter=20000, tol=1e-8, tol_rel=True)
     out = dict(config="32^3 U(-1,1) seed 0, centred 8^3 box, nu=1e-3, Cs=0.18 (reference), "
-                      "dt=1e-3, 10 steps, PCG-Jacobi to 1e-8 relative",
+                      "dt=1e-3, 10 steps, PCG-Jacobi to 1e-8 relative, A built before the mask "
+                      "(main.cpp:65-81)",
                iters=[int(i) for i in its], dts=[float(x) for x in dts],
                fields={k: digest(st[k]) for k in STATE_KEYS})
     with open(os.path.join(HERE, "c1_digest.json"), "w") as fh:
         json.dump(out, fh, indent=1)
 
 
-def c2_fixture(ref):
-    """The plain GPU configuration's inputs at 128^3 (spectral velocity seed 0,
-    two random 10-20 cell boxes, seed 1), two driver steps from step 0 with
-    each pressure solver the configuration names: PCG-Jacobi, PCG-IC(0) (20
-    iterations: it does not converge on this operator), PCG-Chebyshev degree
-    4 on lambda from estimate_lambda_max and 3 pi^2, and GS/SOR 200 sweeps."""
+C2_STEPS = 50
+C2_EVERY = 10
+
+
+def c2_fixture(ref, threads):
+    """The plain GPU configuration at 128^3 (spectral velocity seed 0, two
+    random 10-20 cell boxes, seed 1, nu = 1e-3), A built before the mask as
+    the shipped driver does (main.cpp:65-81), C2_STEPS driver steps from
+    step 0 with each pressure solver the configuration names: PCG-Jacobi,
+    PCG-IC(0) (20 iterations a step: it does not converge on this operator),
+    PCG-Chebyshev 4 on lambda from estimate_lambda_max (one thread, Q8) and
+    3 pi^2, and GS/SOR 200 sweeps.  The steps run on `threads` OpenMP
+    threads; each run records its per-step iteration counts (sweeps for
+    GS are not returned) and dt, and field digests every C2_EVERY steps,
+    so a GPU run can replay the same iteration counts."""
     n = 128
     m = wl.mesh(n)
     N = n ** 3
     solid = wl.random_boxes_mask(n, 2, 1)
     u, v, w = wl.spectral_velocity(n, 0)
     st0 = initial_state(u, v, w, N, solid, m, ref)
-    A = ref.build_stencil(solid, m)
+    A = ref.build_stencil(np.zeros_like(solid), m)
+    ref.set_threads(1)
     lmax = ref.lambda_max(A, m, 20)
-    out = dict(config="128^3 spectral seed 0, 2 random boxes (seed 1), nu=1e-3, Cs=0.18 "
-                      "(reference), dt=1e-3, 2 steps per solver", lambda_max=lmax, runs={})
+    ref.set_threads(threads)
+    out = dict(config=f"128^3 spectral seed 0, 2 random boxes (seed 1), nu=1e-3, Cs=0.18 "
+                      f"(reference), dt=1e-3 to step 20 then adaptive, {C2_STEPS} steps per "
+                      f"solver, A built before the mask (main.cpp:65-81), {threads} threads",
+               lambda_max=lmax, steps=C2_STEPS, every=C2_EVERY, runs={})
     runs = {"pcg_jacobi": dict(solver=1, max_iter=20000, tol=1e-8, tol_rel=True),
             "pcg_ic0_20": dict(solver=2, max_iter=20, tol=0.0, tol_rel=False),
             "pcg_cheb4": dict(solver=4, max_iter=20000, tol=1e-8, tol_rel=True,
                               lmin=3 * np.pi ** 2, lmax=lmax, degree=4),
             "gs_200": dict(solver=5, gs_iters=200)}
-    for name, kw in runs.items():
-        st, its, dts = ref.run_steps(st0, solid, m, None, 2, step0=0, nu=1e-3, rho=1.2, **kw)
-        out["runs"][name] = dict(iters=[int(i) for i in its], dts=[float(x) for x in dts],
-                                 fields={k: digest(st[k]) for k in ("u", "v", "w", "p")})
-        print(name, its, flush=True)
-    with open(os.path.join(HERE, "c2_128_digest.json"), "w") as fh:
-        json.dump(out, fh, indent=1)
+    names = sys.argv[3:] or list(runs)
+    path = os.path.join(HERE, "c2_128_digest.json")
+    if os.path.exists(path):
+        old = json.load(open(path))
+        if old.get("steps") == C2_STEPS:
+            out["runs"] = {k: v for k, v in old["runs"].items() if k not in names}
+    for name in names:
+        kw = runs[name]
+        st = st0
+        its_all, dts_all, snaps = [], [], {}
+        for s0 in range(0, C2_STEPS, C2_EVERY):
+            st, its, dts = ref.run_steps(st, solid, m, A, C2_EVERY, step0=s0, nu=1e-3, rho=1.2,
+                                         **kw)
+            its_all += [int(i) for i in its]
+            dts_all += [float(x) for x in dts]
+            snaps[str(s0 + C2_EVERY)] = {k: digest(st[k]) for k in ("u", "v", "w", "p")}
+            print(name, s0 + C2_EVERY, its[-3:], flush=True)
+        out["runs"][name] = dict(iters=its_all, dts=dts_all, fields=snaps)
+        with open(path, "w") as fh:
+            json.dump(out, fh, indent=1)
 
 
 def main():
     ref = Reference()
     ref.set_threads(1)
-    if sys.argv[1:] == ["c2"]:
-        c2_fixture(ref)
+    if sys.argv[1:2] == ["c2"]:
+        c2_fixture(ref, int(sys.argv[2]) if len(sys.argv) > 2 else 1)
+        return
+    if sys.argv[1:2] == ["c1"]:
+        c1_fixture(ref)
         return
     kernels_fixture(ref)
     steps_fixture(ref)
     c1_fixture(ref)
-    c2_fixture(ref)
+    c2_fixture(ref, 1)
     for fn in sorted(os.listdir(HERE)):
         print(fn, os.path.getsize(os.path.join(HERE, fn)))
 

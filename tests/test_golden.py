@@ -87,11 +87,12 @@ def test_steps16(orc):
 
 
 def test_c1_digest(orc):
-    """The CPU-runnable configuration: 32^3, 10 steps, PCG-Jacobi to 1e-8."""
+    """The CPU-runnable configuration: 32^3, 10 steps, PCG-Jacobi to 1e-8,
+    A built before the mask (main.cpp:65-81)."""
     with open(os.path.join(GOLD, "c1_digest.json")) as fh:
         D = json.load(fh)
     m, (u, v, w), solid = wl.config_c1()
-    A = orc.build_stencil(solid, m)
+    A = orc.build_stencil(np.zeros_like(solid), m)
     st0 = initial_state(u, v, w, 32 ** 3, solid, m, orc)
     out, its, dts = orc.run_steps(st0, solid, m, A, 10, precond=PC_JACOBI, nu=1e-3, rho=1.2,
                                   Cs=0.18, max_iter=20000, tol=1e-8, tol_rel=True)

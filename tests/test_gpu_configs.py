@@ -1,16 +1,18 @@
 """The configurations at their own sizes on the GPU.
 
-* 128^3, plain FP64 (spectral velocity seed 0, two random boxes): two
-  driver steps with each pressure solver the configuration names, against
-  the reference build's per-step iteration counts and final-field digests
+* 128^3, plain FP64 (spectral velocity seed 0, two random boxes, A built
+  before the mask): the configured 50 driver steps with each pressure
+  solver the configuration names, against the reference build's per-step
+  iteration counts and field digests every 10 steps
   (tests/golden/c2_128_digest.json, written by make_golden.py).  GS/SOR is
-  bit-exact (sha256 of every field); the PCG solvers agree to the
-  summation-order bar.
+  bit-exact (sha256 of every field); the PCG solvers replay the reference's
+  iteration counts and agree to 1e-10, and PCG-Jacobi solved to tolerance
+  agrees to 1e-7.
 * 256^3, 5% Bernoulli solids (the measured workload, A built before the
-  mask as main.cpp:65-81 does): 25 steps in plain FP64 and in mixed
-  precision, each to 1e-8 relative in FP64; the mixed fields agree with the
-  FP64 ones to 1e-6 of their maxima every fifth step.
-* 512^3 (20 random boxes, spectral field): 10 steps of mixed-precision
+  mask as main.cpp:65-81 does, nu = 5e-3): the configured 100 steps in FP64
+  and in mixed precision, each to 1e-8 relative in FP64; the mixed fields
+  agree with the FP64 ones to 1e-6 of their maxima every tenth step.
+* 512^3 (20 random boxes, spectral field, nu = 5e-3): 10 steps of mixed-precision
   PCG-Chebyshev degree 8 on Lanczos bounds to 1e-8 relative in FP64
   (checked again with the 7-array operator), and the IC(0) factor and
   preconditioner application bit-identical to the oracle at that size.
@@ -51,64 +53,92 @@ def c2():
     return D, m, solid, (u, v, w)
 
 
-def run_c2(c2, name, **step_kw):
+def run_c2(c2, name, replay, **step_kw):
+    """The C2 run `name` on the GPU for the fixture's step count, A built
+    before the mask (main.cpp:65-81).  replay: every PCG runs exactly the
+    reference's iteration count of that step (tol = 0).  Returns the
+    fixture's run, the per-step stats and the fields at every checkpoint
+    step of the fixture."""
     D, m, solid, (u, v, w) = c2
+    R = D["runs"][name]
     N = m["nx"] * m["ny"] * m["nz"]
+    snaps, stats = {}, []
     with lesgpu.LesGpu.from_mesh(m) as g:
         g.set_mask(solid)
         st = {k: np.zeros(N) for k in lesgpu.FIELDS[:9]}
         st["u"], st["v"], st["w"] = u, v, w
         g.set_state(st)
         g.velocity_bcs()
-        g.build_stencil()
+        g.build_stencil(ignore_mask=True)
         if step_kw.get("precond") == lesgpu.PC_IC0:
             g.ic0_factor()
         if step_kw.get("precond") == lesgpu.PC_CHEBYSHEV:
             lam = g.power_lambda_max(20)
             assert abs(lam - D["lambda_max"]) <= 1e-12 * D["lambda_max"]
-        stats = [g.step(step=s, nu=1e-3, rho=1.2, Cs=0.18, **step_kw) for s in range(2)]
-        out = {k: g.get_field(k) for k in ("u", "v", "w", "p")}
-    return D["runs"][name], stats, out
+        for s in range(D["steps"]):
+            kw = dict(step_kw)
+            if replay:
+                kw.update(max_iter=R["iters"][s], tol=0.0, tol_rel=False)
+            r = g.step(step=s, nu=1e-3, rho=1.2, Cs=0.18, **kw)
+            assert r["fault"] == 0, (name, s, r)
+            stats.append(r)
+            if str(s + 1) in R["fields"]:
+                snaps[str(s + 1)] = {k: g.get_field(k) for k in ("u", "v", "w", "p")}
+    return R, stats, snaps
 
 
-def test_c2_pcg_jacobi(c2):
-    R, stats, out = run_c2(c2, "pcg_jacobi", precond=lesgpu.PC_JACOBI, max_iter=20000,
-                           tol=1e-8, tol_rel=True)
-    for s, ref_it in zip(stats, R["iters"]):
-        assert s["converged"] == 1 and abs(s["iters"] - ref_it) <= 1, (s["iters"], ref_it)
-    for k, a in out.items():
-        ok, e = field_close(a, R["fields"][k], 1e-9)
-        assert ok, (k, e)
+def assert_snaps_close(R, snaps, tol, label):
+    worst = 0.0
+    for step, fields in snaps.items():
+        errs = {}
+        for k, a in fields.items():
+            ok, e = field_close(a, R["fields"][step][k], tol)
+            errs[k] = max(e)
+            assert ok, (label, step, k, e)
+        worst = max(worst, max(errs.values()))
+        print(f"[C2 {label} step {step}] " + ", ".join(f"{k} {e:.1e}" for k, e in errs.items()),
+              flush=True)
+    return worst
 
 
-def test_c2_pcg_cheb4(c2):
+def test_c2_pcg_jacobi_to_tolerance(c2):
+    """PCG-Jacobi to 1e-8 relative for the 50 steps: the GPU sums dot
+    products in another order, so a step may stop one iteration earlier or
+    later than the reference's; fields agree to 1e-7."""
+    R, stats, snaps = run_c2(c2, "pcg_jacobi", False, precond=lesgpu.PC_JACOBI,
+                             max_iter=20000, tol=1e-8, tol_rel=True)
+    for s, (st, ref_it) in enumerate(zip(stats, R["iters"])):
+        assert st["converged"] == 1 and abs(st["iters"] - ref_it) <= 1, (s, st["iters"], ref_it)
+    assert_snaps_close(R, snaps, 1e-7, "jacobi tol")
+
+
+@pytest.mark.parametrize("name", ["pcg_jacobi", "pcg_cheb4", "pcg_ic0_20"])
+def test_c2_replay_within_1e10(c2, name):
+    """The 50 steps with each PCG replaying the reference's iteration count
+    of that step (tol = 0; IC(0) runs 20 iterations a step in both, since it
+    does not converge on this operator): velocity and pressure agree with
+    the reference's to 1e-10 of each field's maximum (probes) and in the
+    2-norm at every tenth step."""
     D = c2[0]
-    R, stats, out = run_c2(c2, "pcg_cheb4", precond=lesgpu.PC_CHEBYSHEV, max_iter=20000,
-                           tol=1e-8, tol_rel=True, lmin=3 * np.pi ** 2, lmax=D["lambda_max"],
-                           degree=4)
-    for s, ref_it in zip(stats, R["iters"]):
-        assert s["converged"] == 1 and abs(s["iters"] - ref_it) <= 1, (s["iters"], ref_it)
-    for k, a in out.items():
-        ok, e = field_close(a, R["fields"][k], 1e-9)
-        assert ok, (k, e)
-
-
-def test_c2_pcg_ic0_fixed_iterations(c2):
-    """IC(0) in hyperplane order; it stagnates on this operator in the
-    reference as here, so the solve is compared after 20 iterations."""
-    R, stats, out = run_c2(c2, "pcg_ic0_20", precond=lesgpu.PC_IC0, max_iter=20, tol=0.0)
-    assert [s["iters"] for s in stats] == R["iters"] == [20, 20]
-    for k, a in out.items():
-        ok, e = field_close(a, R["fields"][k], 1e-7)
-        assert ok, (k, e)
+    kw = {"pcg_jacobi": dict(precond=lesgpu.PC_JACOBI),
+          "pcg_cheb4": dict(precond=lesgpu.PC_CHEBYSHEV, lmin=3 * np.pi ** 2,
+                            lmax=D["lambda_max"], degree=4),
+          "pcg_ic0_20": dict(precond=lesgpu.PC_IC0)}[name]
+    R, stats, snaps = run_c2(c2, name, True, **kw)
+    assert [st["iters"] for st in stats] == R["iters"]
+    assert_snaps_close(R, snaps, 1e-10, name)
 
 
 def test_c2_sor_bitexact(c2):
-    R, stats, out = run_c2(c2, "gs_200", solver=lesgpu.SOLVER_SOR_LEX, omega=1.7,
-                           sor_sweeps=200)
-    for k, a in out.items():
-        assert hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest() == \
-            R["fields"][k]["sha256"], k
+    """GS/SOR 200 sweeps a step (solve_pressure_GS in place of the PCG,
+    main.cpp:133) for the 50 steps: every checkpoint's fields bit for bit
+    (sha256)."""
+    R, stats, snaps = run_c2(c2, "gs_200", False, solver=lesgpu.SOLVER_SOR_LEX, omega=1.7,
+                             sor_sweeps=200)
+    for step, fields in snaps.items():
+        for k, a in fields.items():
+            assert hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest() == \
+                R["fields"][step][k]["sha256"], (step, k)
 
 
 def spectral_ctx(m, solid, seed=0):
@@ -128,7 +158,8 @@ def spectral_ctx(m, solid, seed=0):
 
 def test_c5_512_mixed_steps():
     """512^3 (134M cells) from the seeded spectral field with 20 seeded solid
-    boxes: 10 steps of mixed-precision PCG-Chebyshev degree 8 on Lanczos
+    boxes, nu = 5e-3 as the 256^3 workload: 10 of the configuration's 50
+    steps of mixed-precision PCG-Chebyshev degree 8 on Lanczos
     bounds, each to 1e-8 relative in FP64 (the FP64 residual checked again
     with the 7-array operator after the last), with max |u| bounded."""
     n = 512
@@ -138,7 +169,7 @@ def test_c5_512_mixed_steps():
         lo, hi = g.lanczos_bounds(20, seed=0)
         speeds = []
         for s in range(21, 31):
-            r = g.step(step=s, nu=1e-3, rho=1.2, Cs=0.17, precond=lesgpu.PC_CHEBYSHEV, lmin=lo,
+            r = g.step(step=s, nu=5e-3, rho=1.2, Cs=0.17, precond=lesgpu.PC_CHEBYSHEV, lmin=lo,
                        lmax=1.05 * hi, degree=8, max_iter=20000, tol=1e-8, tol_rel=True,
                        precision=lesgpu.PREC_MIXED, reliable_every=50)
             print(f"[512^3 mixed step {s}] dt {r['dt']:.3e}, {r['iters']} iterations, max|u| "
@@ -187,31 +218,33 @@ def test_c5_512_ic0_factor_and_apply_bitexact():
 
 
 def test_c3_256_mixed_against_fp64():
-    """256^3 with 5% Bernoulli solids (the measured workload, A built before
-    the mask): 25 steps (21-45, under the adaptive dt) in plain FP64 (7
-    arrays) and in mixed precision (coded operator).  Every solve reaches
-    1e-8 relative in FP64, max |u| stays bounded, and the mixed velocity and
-    pressure agree with the FP64 ones to 1e-6 of their maxima after every
-    fifth step."""
+    """256^3 with 5% Bernoulli solids (the measured workload: A built before
+    the mask, nu = 5e-3 -- at 1e-3 the reference's explicit step is
+    unstable on this field): the configured 100 steps (21-120, under the
+    adaptive dt) in FP64 and in mixed precision, both with the coded
+    operator (bit-identical to the seven arrays).  Every solve reaches 1e-8
+    relative in FP64, max |u| stays bounded, and the mixed velocity and
+    pressure agree with the FP64 ones to 1e-6 of their maxima at every tenth
+    step; the per-field errors are printed."""
     n = 256
     m = wl.mesh(n)
     solid = wl.bernoulli_mask(n, 0.05, 0)
     snaps = {}
-    for prec, fmt in ((lesgpu.PREC_FP64, lesgpu.STENCIL_ARRAYS),
-                      (lesgpu.PREC_MIXED, lesgpu.STENCIL_CODED)):
+    for prec in (lesgpu.PREC_FP64, lesgpu.PREC_MIXED):
         with spectral_ctx(m, solid) as g:
             lo, hi = g.lanczos_bounds(20, seed=0)
-            for s in range(21, 46):
-                r = g.step(step=s, nu=1e-3, rho=1.2, Cs=0.17, precond=lesgpu.PC_CHEBYSHEV,
+            for s in range(21, 121):
+                r = g.step(step=s, nu=5e-3, rho=1.2, Cs=0.17, precond=lesgpu.PC_CHEBYSHEV,
                            lmin=lo, lmax=1.05 * hi, degree=8, max_iter=20000, tol=1e-8,
-                           tol_rel=True, precision=prec, reliable_every=50, stencil_format=fmt)
+                           tol_rel=True, precision=prec, reliable_every=50,
+                           stencil_format=lesgpu.STENCIL_CODED)
                 assert r["converged"] == 1 and r["fault"] == 0, (prec, s, r)
                 assert r["true_res_norm"] <= 1e-8 * r["b_norm"], (prec, s, r)
-                assert r["speed_max"] < 20.0, (prec, s, r)
-                if (s - 20) % 5 == 0:
+                assert r["speed_max"] < 10.0, (prec, s, r)
+                if s % 10 == 0:
                     snaps[(prec, s)] = {k: g.get_field(k) for k in ("u", "v", "w", "p")}
     worst = 0.0
-    for s in range(25, 46, 5):
+    for s in range(30, 121, 10):
         errs = {}
         for k in ("u", "v", "w", "p"):
             a, b = snaps[(lesgpu.PREC_MIXED, s)][k], snaps[(lesgpu.PREC_FP64, s)][k]

@@ -74,13 +74,12 @@ def test_kernels_fixture():
         assert bits_equal(np.stack([g.get_field(k) for k in ("u", "v", "w")]), K["correct"])
 
 
-def test_c1_against_reference_digest():
+def c1_gpu_steps(replay):
     """The CPU-runnable configuration (32^3, U(-1,1) seed 0, centred 8^3
-    obstacle, 10 steps, PCG-Jacobi to 1e-8 relative) on the GPU against the
-    reference build's per-step iteration counts and final-field digests.
-    Tolerance: the GPU sums dot products in another order, so iteration
-    counts may differ by one near the threshold and fields by roughly the
-    solver tolerance."""
+    obstacle, A built before the mask as main.cpp:65-81 does, 10 steps of
+    PCG-Jacobi to 1e-8 relative) on the GPU.  replay: each step's PCG runs
+    exactly the reference's iteration count of that step (tol = 0), the
+    iterate the reference stopped at."""
     with open(os.path.join(GOLD, "c1_digest.json")) as fh:
         D = json.load(fh)
     m, (u, v, w), solid = wl.config_c1()
@@ -91,20 +90,52 @@ def test_c1_against_reference_digest():
         st["u"], st["v"], st["w"] = u, v, w
         g.set_state(st)
         g.velocity_bcs()
-        g.build_stencil()
+        g.build_stencil(ignore_mask=True)
         its = []
         for s in range(10):
-            r = g.step(step=s, nu=1e-3, rho=1.2, Cs=0.18, precond=lesgpu.PC_JACOBI,
-                       max_iter=20000, tol=1e-8, tol_rel=True)
-            assert r["converged"] == 1
+            kw = dict(max_iter=D["iters"][s], tol=0.0) if replay else \
+                dict(max_iter=20000, tol=1e-8, tol_rel=True)
+            r = g.step(step=s, nu=1e-3, rho=1.2, Cs=0.18, precond=lesgpu.PC_JACOBI, **kw)
+            assert r["fault"] == 0
+            assert replay or r["converged"] == 1
             its.append(r["iters"])
         out = g.get_state()
+    return D, its, out
+
+
+def digest_err(a, dg):
+    """max |a - ref| over the digest's probes relative to the field's max,
+    and the relative difference of the 2-norms."""
+    scale = max(abs(dg["max"]), abs(dg["min"]), 1e-300)
+    probe = a[np.array(dg["probe_idx"])]
+    err = np.max(np.abs(probe - np.array(dg["probe"]))) / scale
+    return float(err), abs(float(np.linalg.norm(a)) - dg["l2"]) / max(dg["l2"], 1e-300)
+
+
+def test_c1_against_reference_digest():
+    """Solved to tolerance: the GPU sums dot products in another order, so
+    iteration counts may differ by one near the threshold and fields by
+    roughly the solver tolerance."""
+    D, its, out = c1_gpu_steps(replay=False)
     for a, b in zip(its, D["iters"]):
         assert abs(a - b) <= 1, (its, D["iters"])
     for k, dg in D["fields"].items():
-        a = out[k]
-        scale = max(abs(dg["max"]), abs(dg["min"]), 1e-300)
-        probe = a[np.array(dg["probe_idx"])]
-        err = np.max(np.abs(probe - np.array(dg["probe"]))) / scale
-        assert err < 1e-7, (k, err)
-        assert abs(np.linalg.norm(a) - dg["l2"]) <= 1e-8 * dg["l2"], k
+        err, l2 = digest_err(out[k], dg)
+        assert err < 1e-7 and l2 <= 1e-8, (k, err, l2)
+
+
+def test_c1_replay_within_1e10():
+    """Replaying the reference's per-step iteration counts removes the
+    tolerance test's +-1 iteration: all nine fields after 10 whole steps
+    agree with the reference's to 1e-10 of each field's maximum (probes) and
+    in the 2-norm; what is left is the summation order of the dot
+    products."""
+    D, its, out = c1_gpu_steps(replay=True)
+    assert its == D["iters"]
+    worst = {}
+    for k, dg in D["fields"].items():
+        err, l2 = digest_err(out[k], dg)
+        worst[k] = (err, l2)
+        assert err < 1e-10 and l2 < 1e-10, (k, err, l2)
+    print("[C1 replay] " + ", ".join(f"{k} {e:.1e}/{l:.1e}" for k, (e, l) in worst.items()),
+          flush=True)
