@@ -112,6 +112,17 @@ int les_get_stencil(les_ctx* ctx, int which, double* const coef[7], int ptr_kind
 int les_matvec(les_ctx* ctx, int which, const double* x, double* y, int ptr_kind);
 /* parallel_dot (pressure.cpp:962), deterministic order. */
 int les_dot(les_ctx* ctx, const double* a, const double* b, int ptr_kind, double* out);
+/* No reference counterpart: les_dot for callers that cannot allocate
+ * (code under HIP graph capture, serving loops, harnesses that forbid
+ * hipMalloc).  No context; a, b are device arrays of an nx*ny*nz grid and
+ * the sum, bit-identical to les_dot's on the same grid, is written to the
+ * device double *out, ordered on `hip_stream` (NULL: the null stream)
+ * without synchronising.  Allocates nothing: the reduction's partial sums
+ * live in a static 8 KiB device array, one per GPU, shared by every call on
+ * that GPU, so calls that overlap on one GPU race.  Grids of more than
+ * 2^31-1 cells are rejected (LES_ERR_ARG).  Errors: les_last_error(NULL). */
+int les_dot_noalloc(int nx, int ny, int nz, const double* a, const double* b, double* out,
+                    void* hip_stream);
 
 /* ---- preconditioners ---- */
 /* build_ic_preconditioner (pressure.cpp:495): L from the context's A. */

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <mutex>
 #include <new>
 #include <random>
 #include <string>
@@ -670,6 +671,69 @@ int les_dot(les_ctx* c, const double* a, const double* b, int ptr_kind, double* 
     HIPCHK(c, hipMemcpyAsync(c->hsc, c->dsc, 8, hipMemcpyDeviceToHost, c->stream));
     HIPCHK(c, hipStreamSynchronize(c->stream));
     *out = c->hsc[0];
+    return LES_OK;
+}
+
+}  // extern "C"
+
+// ---------------- parallel_dot without allocation ----------------
+// Partial sums of les_dot_noalloc's reduction: one row of kMaxRed per
+// workgroup, kMaxGrid rows (8 KiB).  A file-scope device array, so the
+// entry point needs no allocation; each GPU holds its own copy for the life
+// of the process.  Every call on a GPU uses that GPU's copy: calls that
+// overlap on one GPU (different streams or threads) race on it.  Each call
+// writes every partial row its finalize reads, so nothing is cleared.
+static __device__ __align__(256) double scratch[kMaxGrid * kMaxRed];
+
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of the current device's copy of `scratch`, looked up
+// once per device (the array's name is a host-side stand-in, not a device
+// address).
+int scratch_of_current_device(double** out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return fail(nullptr, LES_ERR_HIP, std::string("hipGetDevice: ") + hipGetErrorString(e));
+    if (dev < 0 || dev >= kMaxDevices) return fail(nullptr, LES_ERR_ARG, "device index out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(nullptr, LES_ERR_HIP,
+                        std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[dev] = static_cast<double*>(p);
+    }
+    *out = g_scratch_ptr[dev];
+    return LES_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int les_dot_noalloc(int nx, int ny, int nz, const double* a, const double* b, double* out,
+                    void* stream)
+{
+    if (!a || !b || !out) return fail(nullptr, LES_ERR_ARG, "NULL argument");
+    if (nx < 1 || ny < 1 || nz < 1) return fail(nullptr, LES_ERR_ARG, "grid dimensions must be >= 1");
+    const long long n = (long long)nx * ny * nz;
+    if (n > 2147483647LL)
+        return fail(nullptr, LES_ERR_ARG,
+                    "grid of more than 2^31-1 cells: cell indices are 32-bit (field.h:40)");
+    double* scr = nullptr;
+    int rc;
+    if ((rc = scratch_of_current_device(&scr)) != LES_OK) return rc;
+    const Dims d{nx, ny, nz, ny * nz, (int)n};
+    // the reduction's grid never exceeds kMaxGrid, the rows scratch holds
+    if (make_chunking(d).grid > kMaxGrid) return fail(nullptr, LES_ERR_STATE, "reduction grid too large");
+    const hipError_t e = launch_dot(static_cast<hipStream_t>(stream), d, a, b, RedScratch{scr}, out);
+    if (e != hipSuccess) return fail(nullptr, LES_ERR_HIP, std::string("les_dot_noalloc: ") + hipGetErrorString(e));
     return LES_OK;
 }
 

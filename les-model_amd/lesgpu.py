@@ -123,6 +123,7 @@ def load_library(path: str = LIB_PATH):
     L.les_get_stencil.argtypes = [vp, C.c_int, C.POINTER(vp), C.c_int]
     L.les_matvec.argtypes = [vp, C.c_int, vp, vp, C.c_int]
     L.les_dot.argtypes = [vp, vp, vp, C.c_int, DP]
+    L.les_dot_noalloc.argtypes = [C.c_int, C.c_int, C.c_int, vp, vp, vp, vp]
     L.les_ic0_factor.argtypes = [vp]
     L.les_precond_apply.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_double, C.c_double,
                                     C.c_int]
@@ -181,6 +182,21 @@ def _ptr(a, dtype=np.float64):
 
 def _path(p):
     return None if p is None else os.fsencode(p)
+
+
+def dot_noalloc(nx, ny, nz, a, b, out, stream=None):
+    """les_dot_noalloc: <a, b> of two device arrays (torch CUDA tensors)
+    into the device tensor `out` (first element), ordered on `stream` (a
+    hipStream_t handle; None: the null stream), with no allocation."""
+    L = load_library()
+    for t in (a, b, out):
+        if not getattr(t, "is_cuda", False) or not t.is_contiguous():
+            raise ValueError("les_dot_noalloc takes contiguous device tensors")
+    rc = L.les_dot_noalloc(int(nx), int(ny), int(nz), C.c_void_p(a.data_ptr()),
+                           C.c_void_p(b.data_ptr()), C.c_void_p(out.data_ptr()),
+                           C.c_void_p(stream))
+    if rc != 0:
+        raise LesError(f"les_dot_noalloc: {L.les_last_error(None).decode()} ({rc})")
 
 
 def write_field_vtk_host(path, step, m, u, v, w, p, nu_t, solid):
