@@ -12,6 +12,15 @@
 
 #include <math.h>
 
+#include <algorithm>
+#include <cmath>
+
+// 1: the step's SGS and momentum kernels march along i (k_phys_march);
+// 0: the row kernels run.
+#ifndef LES_PHYS_MARCH
+#define LES_PHYS_MARCH 1
+#endif
+
 namespace les {
 
 namespace {
@@ -473,6 +482,438 @@ __global__ __launch_bounds__(kBlock) void k_sor_colour(Dims d, Chunking ch, doub
 
 int face_grid(const Dims& d) { return (face_count(d) + kBlock - 1) / kBlock; }
 
+// ------------------------------------------------------------------
+// i-marching forms of k_sgs (with the nu_t BCs) and k_momentum (MOM_STEP)
+// ------------------------------------------------------------------
+// A workgroup owns R consecutive rows j0 .. j0+R-1 (every k) of a run of
+// i-planes and marches along i.  Each thread owns V consecutive cells of
+// one row.  The u, v, w values of its cells at planes t-1, t, t+1 sit in
+// registers (plane t+2 is loaded while plane t is computed); plane t, with
+// one halo row on each side, sits in LDS for the j and k neighbours (two
+// buffers, one barrier per plane).  Each value is read from HBM once per
+// i-run; the halo rows are the own rows of the neighbouring j-blocks of the
+// same i-run, which run on the same XCD, so they come mostly from L2.
+// Per cell, the arithmetic and its operand order are those of sgs_at and
+// mom_at above, so the results are bit-identical to the row kernels'.
+struct MarchGeom {
+    int tpr;    // threads per row (nz / V)
+    int R;      // rows per workgroup
+    int nj;     // j-blocks
+    int ni;     // i-runs
+    int ilen;   // planes per i-run (the last run ends at nx-1)
+    int work;   // nj * ni
+    int grid;   // work rounded up to a multiple of kXcd
+    int block;  // R * tpr
+};
+
+#ifndef LES_MARCH_GRID
+#define LES_MARCH_GRID 256
+#endif
+constexpr int kMarchGrid = LES_MARCH_GRID;  // workgroups aimed at (one per CU)
+// cells per thread along k (2: 16-byte loads where nz is even)
+#ifndef LES_MARCH_V_SGS
+#define LES_MARCH_V_SGS 2
+#endif
+#ifndef LES_MARCH_V_MOM
+#define LES_MARCH_V_MOM 1
+#endif
+constexpr int kMarchBlock = 1024;
+// threads of a march workgroup aimed at (rows R = this / threads per row)
+#ifndef LES_MARCH_THREADS
+#define LES_MARCH_THREADS 1024
+#endif
+
+template <int V>
+size_t march_lds(const Dims& d, const MarchGeom& g)
+{
+    return 2 * 3 * (size_t)(g.R + 2) * d.nz * sizeof(double);
+}
+
+template <int V>
+bool march_geom(const Dims& d, MarchGeom& g)
+{
+    if (d.nz % V) return false;
+    g.tpr = d.nz / V;
+    if (g.tpr > LES_MARCH_THREADS / 2) return false;  // R >= 2: a thread loads one halo row at most
+    g.R = std::min(LES_MARCH_THREADS / g.tpr, d.ny);
+    if (g.R < 2) return false;
+    g.block = g.R * g.tpr;
+    g.nj = (d.ny + g.R - 1) / g.R;
+    int ni = std::max(1, std::min(d.nx, kMarchGrid / g.nj));
+    g.ilen = (d.nx + ni - 1) / ni;
+    ni = (d.nx + g.ilen - 1) / g.ilen;
+    // the outlet plane takes the value computed at nx-2, so no i-run may
+    // start at nx-1: the last run absorbs that plane
+    if (ni > 1 && (ni - 1) * g.ilen == d.nx - 1) --ni;
+    g.ni = ni;
+    g.work = g.nj * g.ni;
+    g.grid = ((g.work + kXcd - 1) / kXcd) * kXcd;
+    return march_lds<V>(d, g) <= kPairMaxLds;
+}
+
+template <int V>
+__device__ __forceinline__ void ld_v(const double* __restrict__ p, double (&o)[V])
+{
+    if constexpr (V == 2) {
+        const double2 x = *reinterpret_cast<const double2*>(p);
+        o[0] = x.x;
+        o[1] = x.y;
+    } else {
+        o[0] = *p;
+    }
+}
+
+template <int V>
+__device__ __forceinline__ void st_v(double* __restrict__ p, const double (&a)[V])
+{
+    if constexpr (V == 2) {
+        *reinterpret_cast<double2*>(p) = make_double2(a[0], a[1]);
+    } else {
+        *p = a[0];
+    }
+}
+
+// nu_t from the 18 neighbour values of one cell (sgs_at's expressions).
+// P2: every divisor 2 d is a power of two, so x / (2 d) and x * (1/(2 d))
+// are the same real number -- the reciprocal is exact -- and round to the
+// same double; the multiply replaces the long division sequence.
+template <bool P2>
+__device__ __forceinline__ double grad_div(double x, double two_d, double inv_2d)
+{
+    return P2 ? x * inv_2d : x / two_d;
+}
+
+template <bool P2>
+__device__ __forceinline__ double sgs_from(const PhysConsts& c, double ue, double uw, double un,
+                                           double us, double ut, double ub, double ve, double vw,
+                                           double vn, double vs, double vt, double vb, double we,
+                                           double ww, double wn, double ws, double wt, double wb)
+{
+    const double dudx = grad_div<P2>(ue - uw, c.two_dx, c.inv_2dx);
+    const double dudy = grad_div<P2>(un - us, c.two_dy, c.inv_2dy);
+    const double dudz = grad_div<P2>(ut - ub, c.two_dz, c.inv_2dz);
+    const double dvdx = grad_div<P2>(ve - vw, c.two_dx, c.inv_2dx);
+    const double dvdy = grad_div<P2>(vn - vs, c.two_dy, c.inv_2dy);
+    const double dvdz = grad_div<P2>(vt - vb, c.two_dz, c.inv_2dz);
+    const double dwdx = grad_div<P2>(we - ww, c.two_dx, c.inv_2dx);
+    const double dwdy = grad_div<P2>(wn - ws, c.two_dy, c.inv_2dy);
+    const double dwdz = grad_div<P2>(wt - wb, c.two_dz, c.inv_2dz);
+    const double Sxx = dudx, Syy = dvdy, Szz = dwdz;
+    const double Sxy = 0.5 * (dudy + dvdx);
+    const double Sxz = 0.5 * (dudz + dwdx);
+    const double Syz = 0.5 * (dvdz + dwdy);
+    const double Smag = sqrt(2.0 * (Sxx * Sxx + Syy * Syy + Szz * Szz) +
+                             4.0 * (Sxy * Sxy + Sxz * Sxz + Syz * Syz));
+    return c.csd2 * Smag;
+}
+
+// one momentum component from its 7 stencil values (mom_at's expressions)
+__device__ __forceinline__ double mom_from(const PhysConsts& c, double fc, double fe, double fw,
+                                           double fn, double fs, double ft, double fb, double uc,
+                                           double vc, double wc, double nu_eff)
+{
+    const double ddx = (fe - fw) * c.inv_2dx;
+    const double ddy = (fn - fs) * c.inv_2dy;
+    const double ddz = (ft - fb) * c.inv_2dz;
+    const double adv = uc * ddx + vc * ddy + wc * ddz;
+    const double lap = (fe - 2 * fc + fw) * c.inv_dx2 + (fn - 2 * fc + fs) * c.inv_dy2 +
+                       (ft - 2 * fc + fb) * c.inv_dz2;
+    return fc + c.dt * (-adv + nu_eff * lap);
+}
+
+// The inputs of one plane of a thread's cells: u, v, w (index 0..2).
+template <int V>
+struct PlaneVals {
+    double f[3][V];
+};
+
+// Per-plane extras: the halo row's u, v, w, and (own cells) nu_t and the
+// mask bytes, loaded one plane ahead.
+template <int V>
+struct PlaneAux {
+    double h[3][V];
+    double nt[V];
+    uint32_t sm;   // mask bytes of the V cells, unpacked where used
+};
+
+template <int V>
+__device__ __forceinline__ uint32_t ld_mask(const uint8_t* __restrict__ p)
+{
+    if constexpr (V == 2) return *reinterpret_cast<const uint16_t*>(p);
+    else return *p;
+}
+
+// MOM = false: k_sgs with apply_bcs (o0 = nu_t).  MOM = true: k_momentum in
+// MOM_STEP mode (o0..o2 = the provisional velocity, p_rhs and b optional).
+// The loads for plane t+2 (own cells) and t+1 (halo row, nu_t, mask) are
+// issued at the top of plane t and not touched before its end (the mask
+// stays packed until it is used), so each has the plane's LDS stores,
+// barrier and arithmetic to arrive.
+template <int V, bool MOM, bool P2>
+__global__ __launch_bounds__(kMarchBlock) void k_phys_march(
+    Dims d, MarchGeom g, PhysConsts c, const double* __restrict__ u, const double* __restrict__ v,
+    const double* __restrict__ w, const double* __restrict__ nu_t,
+    const uint8_t* __restrict__ solid, double* __restrict__ o0, double* __restrict__ o1,
+    double* __restrict__ o2, double* __restrict__ p_rhs, double* __restrict__ b)
+{
+    extern __shared__ unsigned char les_march_smem[];
+    double* lds = reinterpret_cast<double*>(les_march_smem);
+    const int lb = logical_block(gridDim.x);
+    if (lb >= g.work) return;  // padding of the grid to a multiple of kXcd
+    const int jb = lb % g.nj, ic = lb / g.nj;
+    const int j0 = jb * g.R;
+    const int i0 = ic * g.ilen;
+    const int i1 = (ic == g.ni - 1) ? d.nx : min(d.nx, i0 + g.ilen);
+    const int r = threadIdx.x / g.tpr;
+    const int k = (threadIdx.x - r * g.tpr) * V;
+    const int j = j0 + r;
+    const bool live = j < d.ny;
+    const int rows = g.R + 2;
+    const int plane = 3 * rows * d.nz;  // doubles per LDS buffer
+    // the halo row this thread loads: j0-1 (first row's threads) or j0+R
+    // (last row's threads, when that row exists)
+    const int hj = (r == 0 && j0 > 0) ? j0 - 1
+                 : (r == g.R - 1 && j0 + g.R < d.ny) ? j0 + g.R : -1;
+    const int hrow = (r == 0) ? 0 : g.R + 1;
+    const bool wall_row = (j == 0 || j == d.ny - 1);
+    const long sij = d.sij;
+    const long col = (long)j * d.nz + k;
+    const long hcol = (long)hj * d.nz + k;
+    const double* const src[3] = {u, v, w};
+
+    auto load_own = [&](int i, PlaneVals<V>& P) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) ld_v<V>(src[a] + i * sij + col, P.f[a]);
+    };
+    auto load_aux = [&](int i, PlaneAux<V>& X) {
+        if (live) {
+            X.sm = ld_mask<V>(solid + i * sij + col);
+            if (MOM) ld_v<V>(nu_t + i * sij + col, X.nt);
+        }
+        if (hj >= 0) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) ld_v<V>(src[a] + i * sij + hcol, X.h[a]);
+        }
+    };
+
+    double keep[MOM ? 3 : 1][V];  // raw values of the previous plane (the outlet copies them)
+#pragma unroll
+    for (int q = 0; q < V; ++q)
+#pragma unroll
+        for (int a = 0; a < (MOM ? 3 : 1); ++a) keep[a][q] = 0.0;
+
+    // one plane: P, C, X = planes t-1, t, t+1 of the own cells; Y receives
+    // plane t+2; A holds plane t's extras, B receives plane t+1's
+    auto body = [&](int t, const PlaneVals<V>& P, const PlaneVals<V>& C, const PlaneVals<V>& X,
+                    PlaneVals<V>& Y, const PlaneAux<V>& A, PlaneAux<V>& B) {
+        double* L = lds + ((t - i0) & 1) * plane;
+        if (t + 1 < i1) {
+            if (live && t + 2 < d.nx) load_own(t + 2, Y);
+            load_aux(t + 1, B);
+        }
+        // plane t into LDS: own rows, then the halo row
+        if (live) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) st_v<V>(L + (a * rows + r + 1) * d.nz + k, C.f[a]);
+        }
+        if (hj >= 0) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) st_v<V>(L + (a * rows + hrow) * d.nz + k, A.h[a]);
+        }
+        __syncthreads();
+        if (!live) return;
+        const double* Lu = L + (0 * rows + r + 1) * d.nz;
+        const double* Lv = L + (1 * rows + r + 1) * d.nz;
+        const double* Lw = L + (2 * rows + r + 1) * d.nz;
+        const long id0 = t * sij + col;
+        double r0[V], r1[V], r2[V], rhs[V], bq[V];
+#pragma unroll
+        for (int q = 0; q < V; ++q) {
+            const int kk = k + q;
+            const bool wall = wall_row || kk == 0 || kk == d.nz - 1;
+            const bool sol = (A.sm >> (8 * q)) & 0xff;
+            const double cu = C.f[0][q], cv = C.f[1][q], cw = C.f[2][q];
+            // neighbours (only read where the reference's stencil reads them)
+            const double ub = (q > 0) ? C.f[0][q - 1] : (kk > 0 ? Lu[kk - 1] : 0.0);
+            const double ut = (q < V - 1) ? C.f[0][q + 1] : (kk < d.nz - 1 ? Lu[kk + 1] : 0.0);
+            const double vb = (q > 0) ? C.f[1][q - 1] : (kk > 0 ? Lv[kk - 1] : 0.0);
+            const double vt = (q < V - 1) ? C.f[1][q + 1] : (kk < d.nz - 1 ? Lv[kk + 1] : 0.0);
+            const double wb = (q > 0) ? C.f[2][q - 1] : (kk > 0 ? Lw[kk - 1] : 0.0);
+            const double wt = (q < V - 1) ? C.f[2][q + 1] : (kk < d.nz - 1 ? Lw[kk + 1] : 0.0);
+            const double us = j > 0 ? Lu[kk - d.nz] : 0.0;
+            const double un = j < d.ny - 1 ? Lu[kk + d.nz] : 0.0;
+            const double vs = j > 0 ? Lv[kk - d.nz] : 0.0;
+            const double vn = j < d.ny - 1 ? Lv[kk + d.nz] : 0.0;
+            const double ws = j > 0 ? Lw[kk - d.nz] : 0.0;
+            const double wn = j < d.ny - 1 ? Lw[kk + d.nz] : 0.0;
+            const double ue = X.f[0][q], uw = P.f[0][q];
+            const double ve = X.f[1][q], vw = P.f[1][q];
+            const double we = X.f[2][q], ww = P.f[2][q];
+            if constexpr (!MOM) {
+                double out;
+                if (wall || t == 0) {
+                    out = 0.0;
+                } else if (t == d.nx - 1) {
+                    out = keep[0][q];
+                } else {
+                    const double raw = sgs_from<P2>(c, ue, uw, un, us, ut, ub, ve, vw, vn, vs, vt,
+                                                    vb, we, ww, wn, ws, wt, wb);
+                    keep[0][q] = raw;
+                    out = sol ? 0.0 : raw;
+                }
+                r0[q] = out;
+            } else {
+                if (wall) {
+                    r0[q] = r1[q] = r2[q] = 0.0;
+                } else if (t == 0) {
+                    r0[q] = 1.0;  // stored below only where the cell is fluid
+                    r1[q] = r2[q] = 0.0;
+                } else if (t == d.nx - 1) {
+                    r0[q] = keep[0][q];
+                    r1[q] = keep[1][q];
+                    r2[q] = keep[2][q];
+                } else {
+                    const double nu_eff = c.nu + A.nt[q];
+                    const double a0 = mom_from(c, cu, ue, uw, un, us, ut, ub, cu, cv, cw, nu_eff);
+                    const double a1 = mom_from(c, cv, ve, vw, vn, vs, vt, vb, cu, cv, cw, nu_eff);
+                    const double a2 = mom_from(c, cw, we, ww, wn, ws, wt, wb, cu, cv, cw, nu_eff);
+                    keep[0][q] = a0;
+                    keep[1][q] = a1;
+                    keep[2][q] = a2;
+                    r0[q] = sol ? 0.0 : a0;
+                    r1[q] = sol ? 0.0 : a1;
+                    r2[q] = sol ? 0.0 : a2;
+                }
+                // lagged divergence of (u, v, w) (calculate_divergence) and b
+                // (build_b_vector: -div on fluid cells, +0 elsewhere)
+                double rv = 0.0, bv = 0.0;
+                if (!sol && t != d.nx - 1) {
+                    double dudx, dvdy, dwdz;
+                    if (t == 0) dudx = (ue - cu) * c.div_ix1;
+                    else dudx = (ue - uw) * c.div_ix;
+                    if (j == 0) dvdy = (vn - cv) * c.div_iy1;
+                    else if (j == d.ny - 1) dvdy = (cv - vs) * c.div_iy1;
+                    else dvdy = (vn - vs) * c.div_iy;
+                    if (kk == 0) dwdz = (wt - cw) * c.div_iz1;
+                    else if (kk == d.nz - 1) dwdz = (cw - wb) * c.div_iz1;
+                    else dwdz = (wt - wb) * c.div_iz;
+                    rv = c.div_scale * (dudx + dvdy + dwdz);
+                    bv = -rv;
+                }
+                rhs[q] = rv;
+                bq[q] = bv;
+            }
+        }
+        if constexpr (!MOM) {
+            st_v<V>(o0 + id0, r0);
+        } else {
+            if (t == 0) {
+                // solid inlet cells keep the buffer's previous content
+#pragma unroll
+                for (int q = 0; q < V; ++q) {
+                    const int kk = k + q;
+                    const bool wall = wall_row || kk == 0 || kk == d.nz - 1;
+                    if (wall || !((A.sm >> (8 * q)) & 0xff)) {
+                        o0[id0 + q] = r0[q];
+                        o1[id0 + q] = r1[q];
+                        o2[id0 + q] = r2[q];
+                    }
+                }
+            } else {
+                st_v<V>(o0 + id0, r0);
+                st_v<V>(o1 + id0, r1);
+                st_v<V>(o2 + id0, r2);
+            }
+            if (p_rhs) st_v<V>(p_rhs + id0, rhs);
+            if (b) st_v<V>(b + id0, bq);
+        }
+    };
+
+    PlaneVals<V> P, C, X;
+    PlaneAux<V> A;
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int q = 0; q < V; ++q) P.f[a][q] = C.f[a][q] = X.f[a][q] = A.h[a][q] = 0.0;
+#pragma unroll
+    for (int q = 0; q < V; ++q) A.nt[q] = 0.0;
+    A.sm = 0;
+    if (live) {
+        if (i0 > 0) load_own(i0 - 1, P);
+        load_own(i0, C);
+        if (i0 + 1 < d.nx) load_own(i0 + 1, X);
+    }
+    load_aux(i0, A);
+    for (int t = i0; t < i1; ++t) {
+        PlaneVals<V> Y;
+        PlaneAux<V> B;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int q = 0; q < V; ++q) Y.f[a][q] = B.h[a][q] = 0.0;
+#pragma unroll
+        for (int q = 0; q < V; ++q) B.nt[q] = 0.0;
+        B.sm = 0;
+        body(t, P, C, X, Y, A, B);
+        // the register roles move one plane on; the copies wait for the
+        // loads issued at the top of this plane, after its work
+        P = C;
+        C = X;
+        X = Y;
+        A = B;
+    }
+}
+
+// x is a positive power of two (a normal number whose significand is 1)
+inline bool pow2(double x)
+{
+    int e = 0;
+    return x > 0.0 && std::isfinite(x) && std::frexp(x, &e) == 0.5 &&
+           x >= 2.2250738585072014e-308;
+}
+
+template <bool MOM>
+hipError_t launch_march(hipStream_t s, const Dims& d, const PhysConsts& c, const double* u,
+                        const double* v, const double* w, const double* nu_t,
+                        const uint8_t* solid, double* o0, double* o1, double* o2, double* p_rhs,
+                        double* b, bool& ran)
+{
+    ran = false;
+    MarchGeom g;
+    int V = 0;
+    if ((MOM ? LES_MARCH_V_MOM : LES_MARCH_V_SGS) >= 2 && march_geom<2>(d, g)) V = 2;
+    else if (march_geom<1>(d, g)) V = 1;
+    else return hipSuccess;   // the row kernel runs
+    const size_t lds = V == 2 ? march_lds<2>(d, g) : march_lds<1>(d, g);
+    // SGS divides its differences by 2 dx, 2 dy, 2 dz: by an exact
+    // reciprocal multiply where all three are powers of two
+    const bool p2 = !MOM && pow2(c.two_dx) && pow2(c.two_dy) && pow2(c.two_dz) &&
+                    c.inv_2dx * c.two_dx == 1.0 && c.inv_2dy * c.two_dy == 1.0 &&
+                    c.inv_2dz * c.two_dz == 1.0;
+    const void* fn =
+        V == 2 ? (p2 ? reinterpret_cast<const void*>(&k_phys_march<2, MOM, true>)
+                     : reinterpret_cast<const void*>(&k_phys_march<2, MOM, false>))
+               : (p2 ? reinterpret_cast<const void*>(&k_phys_march<1, MOM, true>)
+                     : reinterpret_cast<const void*>(&k_phys_march<1, MOM, false>));
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+#define LES_MARCH_LAUNCH(VV, PP)                                                                \
+    hipLaunchKernelGGL((k_phys_march<VV, MOM, PP>), dim3(g.grid), dim3(g.block), lds, s, d, g, \
+                       c, u, v, w, nu_t, solid, o0, o1, o2, p_rhs, b)
+    if (V == 2) {
+        if (p2) LES_MARCH_LAUNCH(2, true);
+        else LES_MARCH_LAUNCH(2, false);
+    } else {
+        if (p2) LES_MARCH_LAUNCH(1, true);
+        else LES_MARCH_LAUNCH(1, false);
+    }
+#undef LES_MARCH_LAUNCH
+    ran = true;
+    return hipGetLastError();
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------
@@ -485,6 +926,12 @@ hipError_t launch_sgs(hipStream_t s, const Dims& d, const PhysConsts& c, const d
     const Chunking ch = make_chunking(d);
     RedScratch rd = red;
     if (!(apply_bcs && nut_max)) rd.partials = nullptr;
+    if (LES_PHYS_MARCH && apply_bcs && !tau.t[0] && !rd.partials) {
+        bool ran = false;
+        const hipError_t e = launch_march<false>(s, d, c, u, v, w, nullptr, solid, nu_t, nullptr,
+                                                 nullptr, nullptr, nullptr, ran);
+        if (e != hipSuccess || ran) return e;
+    }
     hipLaunchKernelGGL(k_sgs, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, c, u, v, w, solid, nu_t,
                        tau, apply_bcs, rd);
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
@@ -498,6 +945,12 @@ hipError_t launch_momentum(hipStream_t s, const Dims& d, const PhysConsts& c, co
                            const uint8_t* solid, double* un, double* vn, double* wn,
                            double* p_rhs, double* b, int mode)
 {
+    if (LES_PHYS_MARCH && mode == MOM_STEP) {
+        bool ran = false;
+        const hipError_t e = launch_march<true>(s, d, c, u, v, w, nu_t, solid, un, vn, wn, p_rhs,
+                                                b, ran);
+        if (e != hipSuccess || ran) return e;
+    }
     const Chunking ch = make_chunking(d);
     hipLaunchKernelGGL(k_momentum, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, c, u, v, w, nu_t,
                        solid, un, vn, wn, p_rhs, b, mode);

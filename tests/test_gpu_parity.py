@@ -237,6 +237,43 @@ def test_stationary_bitexact(orc, shape):
             assert bits_equal(g.get_field("p"), po), it
 
 
+@pytest.mark.parametrize("shape", SHAPES + [(9, 11, 256, 8), (6, 19, 64, 9), (8, 16, 64, 10, "iso"),
+                                   (16, 11, 256, 11, "iso")])
+@pytest.mark.parametrize("stencil_first", [False, True])
+def test_step_physics_bitexact(orc, shape, stencil_first):
+    """A whole driver step with the PCG held at 0 iterations, so no
+    reduction enters: SGS with its BCs, momentum with the velocity BCs and
+    the lagged divergence / b (the i-marching kernels where the grid allows
+    them, the row kernels elsewhere), then the projection with p as it
+    stands.  Every field bit for bit against the oracle, from step 21 (the
+    adaptive dt) and with inlet solid cells.  Shapes with nz = 256 and 64
+    over several j-blocks and i-runs exercise the march's halo rows; the
+    "iso" shapes have power-of-two spacings (dx = 1/nx ...), where the SGS
+    gradients multiply by the exact reciprocal of 2 dx instead of dividing."""
+    m, solid, f = random_case(*shape[:4], anisotropic=len(shape) < 5)
+    solid = solid.copy()
+    nz = m["nz"]
+    solid[[nz + 1, 2 * nz + 2]] = 1  # inlet plane (i = 0)
+    N = m["nx"] * m["ny"] * nz
+    A = orc.build_stencil(np.zeros_like(solid) if stencil_first else solid, m)
+    st0 = initial_state(f[0], f[1], f[2], N, solid, m, orc)
+    st0["p"] = 0.1 * f[3]
+    st0["u_new"], st0["v_new"], st0["w_new"] = f[4], f[5], f[6]
+    kw = dict(max_iter=0, tol=0.0)
+    so, _, do = orc.run_steps(st0, solid, m, A, 1, step0=21, precond=PC_JACOBI, nu=1e-3,
+                              rho=1.2, Cs=0.17, **kw)
+    with ctx_for(m, solid) as g:
+        g.set_state(st0)
+        g.build_stencil(ignore_mask=stencil_first)
+        r = g.step(step=21, nu=1e-3, rho=1.2, Cs=0.17, precond=PC_JACOBI, **kw)
+        assert r["dt"] == do[0]
+        out = g.get_state()
+        b = g.get_field("b")
+    for k in ("u", "v", "w", "p", "nu_t", "u_new", "v_new", "w_new", "p_rhs"):
+        assert bits_equal(out[k], so[k]), k
+    assert bits_equal(b, orc.build_b(so["p_rhs"], solid, m))
+
+
 @pytest.mark.parametrize("pc", [PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV])
 def test_steps(orc, pc):
     """Whole LES steps (main.cpp:115-144) from step 18 so the run crosses
