@@ -163,7 +163,10 @@ def c2_fixture(ref, threads):
                       f"solver, A built before the mask (main.cpp:65-81), {threads} threads",
                lambda_max=lmax, steps=C2_STEPS, every=C2_EVERY, runs={})
     runs = {"pcg_jacobi": dict(solver=1, max_iter=20000, tol=1e-8, tol_rel=True),
-            "pcg_ic0_20": dict(solver=2, max_iter=20, tol=0.0, tol_rel=False),
+            # IC(0) does not converge on this operator: at 20 iterations a
+            # step the velocity reaches ~1e93 by step 10 and NaN by step 20
+            # in the reference, so its run stops at step 6, digests every step
+            "pcg_ic0_20": dict(solver=2, max_iter=20, tol=0.0, tol_rel=False, steps=6, every=1),
             "pcg_cheb4": dict(solver=4, max_iter=20000, tol=1e-8, tol_rel=True,
                               lmin=3 * np.pi ** 2, lmax=lmax, degree=4),
             "gs_200": dict(solver=5, gs_iters=200)}
@@ -174,17 +177,19 @@ def c2_fixture(ref, threads):
         if old.get("steps") == C2_STEPS:
             out["runs"] = {k: v for k, v in old["runs"].items() if k not in names}
     for name in names:
-        kw = runs[name]
+        kw = dict(runs[name])
+        steps, every = kw.pop("steps", C2_STEPS), kw.pop("every", C2_EVERY)
         st = st0
         its_all, dts_all, snaps = [], [], {}
-        for s0 in range(0, C2_STEPS, C2_EVERY):
-            st, its, dts = ref.run_steps(st, solid, m, A, C2_EVERY, step0=s0, nu=1e-3, rho=1.2,
+        for s0 in range(0, steps, every):
+            st, its, dts = ref.run_steps(st, solid, m, A, every, step0=s0, nu=1e-3, rho=1.2,
                                          **kw)
             its_all += [int(i) for i in its]
             dts_all += [float(x) for x in dts]
-            snaps[str(s0 + C2_EVERY)] = {k: digest(st[k]) for k in ("u", "v", "w", "p")}
-            print(name, s0 + C2_EVERY, its[-3:], flush=True)
-        out["runs"][name] = dict(iters=its_all, dts=dts_all, fields=snaps)
+            snaps[str(s0 + every)] = {k: digest(st[k]) for k in ("u", "v", "w", "p")}
+            print(name, s0 + every, its[-3:], flush=True)
+        out["runs"][name] = dict(iters=its_all, dts=dts_all, fields=snaps, steps=steps,
+                                 every=every)
         with open(path, "w") as fh:
             json.dump(out, fh, indent=1)
 

@@ -75,7 +75,7 @@ def run_c2(c2, name, replay, **step_kw):
         if step_kw.get("precond") == lesgpu.PC_CHEBYSHEV:
             lam = g.power_lambda_max(20)
             assert abs(lam - D["lambda_max"]) <= 1e-12 * D["lambda_max"]
-        for s in range(D["steps"]):
+        for s in range(R.get("steps", D["steps"])):
             kw = dict(step_kw)
             if replay:
                 kw.update(max_iter=R["iters"][s], tol=0.0, tol_rel=False)
@@ -112,13 +112,26 @@ def test_c2_pcg_jacobi_to_tolerance(c2):
     assert_snaps_close(R, snaps, 1e-7, "jacobi tol")
 
 
-@pytest.mark.parametrize("name", ["pcg_jacobi", "pcg_cheb4", "pcg_ic0_20"])
+@pytest.mark.parametrize("name", ["pcg_jacobi", "pcg_cheb4"])
 def test_c2_replay_within_1e10(c2, name):
     """The 50 steps with each PCG replaying the reference's iteration count
-    of that step (tol = 0; IC(0) runs 20 iterations a step in both, since it
-    does not converge on this operator): velocity and pressure agree with
-    the reference's to 1e-10 of each field's maximum (probes) and in the
-    2-norm at every tenth step."""
+    of that step (tol = 0): velocity and pressure agree with the reference's
+    to 1e-10 of each field's maximum (probes) and in the 2-norm at every
+    tenth step."""
+    _c2_replay(c2, name, 1e-10)
+
+
+def test_c2_ic0_fixed_iterations(c2):
+    """PCG-IC(0) (the true factor, hyperplane-ordered) at 20 iterations a
+    step.  IC(0) of this non-symmetric operator does not make CG converge,
+    and the reference's flow explodes under it (max |u| 13, 35, 100, 870,
+    2.5e4, 1.5e7 over steps 1-6, NaN by step 20), so the fixture holds the
+    first 6 steps, digested every step.  The GPU replays them and agrees to
+    1e-10 through that growth."""
+    _c2_replay(c2, "pcg_ic0_20", 1e-10)
+
+
+def _c2_replay(c2, name, bar):
     D = c2[0]
     kw = {"pcg_jacobi": dict(precond=lesgpu.PC_JACOBI),
           "pcg_cheb4": dict(precond=lesgpu.PC_CHEBYSHEV, lmin=3 * np.pi ** 2,
@@ -126,7 +139,7 @@ def test_c2_replay_within_1e10(c2, name):
           "pcg_ic0_20": dict(precond=lesgpu.PC_IC0)}[name]
     R, stats, snaps = run_c2(c2, name, True, **kw)
     assert [st["iters"] for st in stats] == R["iters"]
-    assert_snaps_close(R, snaps, 1e-10, name)
+    assert_snaps_close(R, snaps, bar, name)
 
 
 def test_c2_sor_bitexact(c2):
