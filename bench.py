@@ -102,7 +102,7 @@ def pcg_bytes_per_iter(degree, elem, coded, pairs):
     return cheb + (6 * elem + op) + 3 * elem
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "chebyshev_256_traffic.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "les_step_256_traffic.json")
 
 
 def measured_traffic(mode, n):
