@@ -1154,6 +1154,224 @@ __global__ __launch_bounds__(TB, pair_waves_per_simd<TB>()) void k_cheb_pair(Dim
 }
 
 // ------------------------------------------------------------------
+// The fused Chebyshev pair with its input staged (coded operator)
+// ------------------------------------------------------------------
+// k_cheb_pair's step A fetches z_k at the cell and its four i/j neighbours
+// from global memory (mostly L2 hits) when it needs them.  Here each thread
+// owns one A-row (the J core rows and the halo row on each side) and V cells,
+// keeps z_k of its cells at planes t-1, t, t+1 in registers, and gets the
+// j neighbours from an LDS copy of plane t (its A-rows plus the rows j0-2 and
+// j0+J+1).  z_k of plane t+2, the extra rows of plane t+2, and r, p and the
+// code of plane t+1 are loaded while plane t is computed, and the LDS copy of
+// plane t+1 is written from registers before the plane's barrier (two
+// buffers), so every global load has a plane of work to arrive and one
+// barrier per plane still orders all LDS use.  One 1024-thread workgroup per
+// CU.  Step B and every cell's arithmetic are k_cheb_pair's, so the results
+// are bit-identical.  Rows must lie inside one wave (k-edges by shuffle).
+template <typename T, bool FIRST, int V, int R>
+__global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
+    Dims d, PairGeom g, Stencil<T> A, const T* __restrict__ r, const T* __restrict__ z_in,
+    T* __restrict__ z_out, const T* __restrict__ p_in, T* __restrict__ p_out, double alpha0,
+    double beta_a, double alpha_a, double beta_b, double alpha_b, const SolveScalars* sc)
+{
+    if (is_done(sc)) return;
+    extern __shared__ unsigned char les_smem[];
+    const int arows = g.J + 2;  // A-rows j0-1 .. j0+J
+    const int zrows = arows + 2;
+    T* zs = reinterpret_cast<T*>(les_smem);        // ring of z_{k+1}: [R][arows][nz]
+    T* zi = zs + (size_t)R * arows * d.nz;         // plane t of z_k: [2][zrows][nz]
+    __shared__ CodeTab<T> tab_s;
+    code_tab_fill(tab_s, A);
+    __syncthreads();
+    const CodeTab<T>* tab = &tab_s;
+    const int lb = logical_block(g.grid);
+    const int jb = lb % g.njb, ic = lb / g.njb;
+    const int j0 = jb * g.J;
+    const int ia = ic * g.irange, ib = min(ia + g.irange, d.nx);
+    if (ia >= d.nx) return;  // padding of the grid (the whole workgroup)
+    const int a = (int)threadIdx.x / g.tpr;  // A-row
+    const int kk = ((int)threadIdx.x - a * g.tpr) * V;
+    const int j = j0 - 1 + a;
+    const bool jin = j >= 0 && j < d.ny;
+    const bool core = a >= 1 && a <= g.J && j < d.ny;
+    // the extra input row this thread also loads: j0-2 (A-row 0) or j0+J+1
+    // (the last A-row); its LDS row index and whether it lies in the grid
+    const int xa = (a == 0) ? 0 : (a == arows - 1 ? zrows - 1 : -1);
+    const int xj = (a == 0) ? j0 - 2 : j0 + g.J + 1;
+    const bool xin = xa >= 0 && xj >= 0 && xj < d.ny;
+    const long sij = d.sij;
+    const long col = (long)j * d.nz + kk, xcol = (long)xj * d.nz + kk;
+    const int zplane = zrows * d.nz, rplane = arows * d.nz;
+    const T a0 = (T)alpha0, bta = (T)beta_a, ata = (T)alpha_a, btb = (T)beta_b, atb = (T)alpha_b;
+    constexpr int prio = LES_PAIR_PRIO >= 0 ? LES_PAIR_PRIO : (sizeof(T) == 4 ? 1 : 2);
+    T pad[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) pad[v] = outside<T>();
+    // z_k as stored (r for the first pair, scaled where used), outside<T>()
+    // outside the grid
+    const T* zsrc = FIRST ? r : z_in;
+    auto load_z = [&](int i, bool in, long c0, T (&o)[V]) {
+        if (in && i >= 0 && i < d.nx) vload<T, V>(zsrc + i * sij + c0, o);
+        else {
+#pragma unroll
+            for (int v = 0; v < V; ++v) o[v] = pad[v];
+        }
+    };
+    // the value step A uses: alpha0 r in the first pair for loaded values
+    auto zval = [&](bool loaded, const T (&x)[V], T (&o)[V]) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) o[v] = (FIRST && loaded) ? T(0) + a0 * x[v] : x[v];
+    };
+    auto write_zi = [&](int i, const T (&own)[V], const T (&ext)[V]) {
+        T* zt = zi + (i & 1) * zplane;
+        const bool pin = i >= 0 && i < d.nx;
+        T o[V];
+        zval(pin && jin, own, o);
+        vstore<T, V>(zt + (a + 1) * d.nz + kk, o);
+        if (xa >= 0) {
+            zval(pin && xin, ext, o);
+            vstore<T, V>(zt + xa * d.nz + kk, o);
+        }
+    };
+
+    T zw[V], zc[V], ze[V], xz[V];            // z_k at t-1, t, t+1; extra row at t+1
+    T rc[V], pc[V];                          // r and p of plane t (non-first)
+    uint16_t cc[V];                          // code of plane t
+    T pkeep[V], rkeep[V], pnew[V], rnew[V];  // p_k, r of core rows for step B
+    uint16_t ckeep[V], cnew[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        rc[v] = pc[v] = pkeep[v] = rkeep[v] = pnew[v] = rnew[v] = T(0);
+        cc[v] = ckeep[v] = cnew[v] = 0;
+    }
+    const int t0 = ia - 1;
+    load_z(t0 - 1, jin, col, zw);
+    load_z(t0, jin, col, zc);
+    load_z(t0 + 1, jin, col, ze);
+    {
+        T x0[V];
+        load_z(t0, xin, xcol, x0);
+        load_z(t0 + 1, xin, xcol, xz);
+        write_zi(t0, zc, x0);
+    }
+    if (jin && t0 >= 0) {
+        if (!FIRST) {
+            vload_s<T, V>(r + t0 * sij + col, rc);
+            vload_s<T, V>(p_in + t0 * sij + col, pc);
+        }
+        vload<uint16_t, V>(A.code + t0 * sij + col, cc);
+    }
+    __syncthreads();
+    for (int t = t0; t <= ib; ++t) {
+        if (prio == 1) __builtin_amdgcn_s_setprio(2);
+        // ---- loads for the next planes ----
+        T zn[V], xn[V], rn[V], pn_[V];
+        uint16_t cn[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            rn[v] = pn_[v] = T(0);
+            xn[v] = pad[v];
+            cn[v] = 0;
+        }
+        load_z(t + 2, jin, col, zn);
+        if (xa >= 0) load_z(t + 2, xin, xcol, xn);
+        if (jin && t + 1 < d.nx) {
+            if (!FIRST) {
+                vload_s<T, V>(r + (t + 1) * sij + col, rn);
+                vload_s<T, V>(p_in + (t + 1) * sij + col, pn_);
+            }
+            vload<uint16_t, V>(A.code + (t + 1) * sij + col, cn);
+        }
+        // ---- A at plane t: z_{k+1} into the ring, p_k of core rows ----
+        T* ring_t = zs + (t + R) % R * rplane + a * d.nz + kk;
+        if (t >= 0 && t < d.nx && jin) {
+            Nbr<T, V> nb;
+            zval(true, zc, nb.c);
+            if (t > 0) zval(true, zw, nb.w);
+            else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) nb.w[v] = pad[v];
+            }
+            if (t < d.nx - 1) zval(true, ze, nb.e);
+            else {
+#pragma unroll
+                for (int v = 0; v < V; ++v) nb.e[v] = pad[v];
+            }
+            const T* zt = zi + (t & 1) * zplane;
+            vload<T, V>(zt + a * d.nz + kk, nb.s);
+            vload<T, V>(zt + (a + 2) * d.nz + kk, nb.n);
+            const long id = t * sij + col;
+            row_edges<T, V, false>(d, (int)id, kk, nb.c, nb.km, nb.kp,
+                                   [&](int n) { return zt[(a + 1) * d.nz + kk + (n - (int)id)]; });
+            T az[V];
+            apply7v<1, T, V, true>(A, d, (int)id, t, j, kk, nb, az, cc, nullptr, tab);
+            const T* rv = FIRST ? zc : rc;
+            const T* pp = FIRST ? zc : pc;
+            T zo[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const T res = rv[v] - az[v];
+                const T pk = res + bta * pp[v];
+                zo[v] = nb.c[v] + ata * pk;
+                pnew[v] = pk;
+                rnew[v] = rv[v];
+                cnew[v] = cc[v];
+            }
+            vstore<T, V>(ring_t, zo);
+        } else {
+            vstore<T, V>(ring_t, pad);
+        }
+        // ---- plane t+1 of z_k into the other LDS buffer ----
+        write_zi(t + 1, ze, xz);
+        if (prio == 1) __builtin_amdgcn_s_setprio(0);
+        __syncthreads();
+        if (prio == 2) __builtin_amdgcn_s_setprio(2);
+        // ---- B at plane t-1 from the ring ----
+        const int iB = t - 1;
+        if (core && iB >= ia && iB < ib) {
+            const T* zc_pl = zs + (iB + R) % R * rplane;
+            const T* zm_pl = zs + (iB + R - 1) % R * rplane;
+            const T* zp_pl = zs + (iB + 1) % R * rplane;
+            Nbr<T, V> nb;
+            vload<T, V>(zc_pl + a * d.nz + kk, nb.c);
+            vload<T, V>(zm_pl + a * d.nz + kk, nb.w);
+            vload<T, V>(zp_pl + a * d.nz + kk, nb.e);
+            vload<T, V>(zc_pl + (a - 1) * d.nz + kk, nb.s);
+            vload<T, V>(zc_pl + (a + 1) * d.nz + kk, nb.n);
+            const long id = iB * sij + col;
+            row_edges<T, V, false>(d, (int)id, kk, nb.c, nb.km, nb.kp,
+                                   [&](int n) { return zc_pl[a * d.nz + kk + (n - (int)id)]; });
+            T az[V];
+            apply7v<1, T, V, true>(A, d, (int)id, iB, j, kk, nb, az, ckeep, nullptr, tab);
+            T pn[V], zo[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const T res = rkeep[v] - az[v];
+                pn[v] = res + btb * pkeep[v];
+                zo[v] = nb.c[v] + atb * pn[v];
+            }
+            vstore_s<T, V>(p_out + id, pn);
+            vstore_s<T, V>(z_out + id, zo);
+        }
+        if (prio == 2) __builtin_amdgcn_s_setprio(0);
+        // ---- the register roles move one plane on ----
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            zw[v] = zc[v];
+            zc[v] = ze[v];
+            ze[v] = zn[v];
+            xz[v] = xn[v];
+            rc[v] = rn[v];
+            pc[v] = pn_[v];
+            cc[v] = cn[v];
+            pkeep[v] = pnew[v];
+            rkeep[v] = rnew[v];
+            ckeep[v] = cnew[v];
+        }
+    }
+}
+
+// ------------------------------------------------------------------
 // q = A p with the direction update fused (PU 0: p given; PU 1:
 // p = z + beta p_old, stored) and the deferred x update, marching along i
 // like the Chebyshev pair: for plane t a workgroup forms p on its J rows and
@@ -2001,12 +2219,80 @@ hipError_t launch_cheb_pair_cd(hipStream_t s, const Dims& d, Stencil<T> A, const
     return hipGetLastError();
 }
 
+// The staged pair (k_cheb_pair_st): FP32 with the coded operator, rows
+// inside one wave, one 1024-thread workgroup per CU.
+#ifndef LES_PAIR_STAGED
+#define LES_PAIR_STAGED 1
+#endif
+#ifndef LES_PAIR_ST_GRID
+#define LES_PAIR_ST_GRID 256
+#endif
+constexpr int kPairStBlock = 1024;
+
+template <typename T>
+size_t pair_st_lds(const Dims& d, const PairGeom& g)
+{
+    return ((size_t)LES_PAIR_RING * (g.J + 2) + 2 * (size_t)(g.J + 4)) * d.nz * sizeof(T);
+}
+
+template <typename T>
+bool cheb_pair_st_geom(const Dims& d, PairGeom& g, int target_grid)
+{
+    constexpr int V = vec_width<T>();
+    if (sizeof(T) != 4 || d.nz % V) return false;
+    g.tpr = d.nz / V;
+    if (g.tpr > 64 || (g.tpr & (g.tpr - 1))) return false;  // rows inside one wave
+    g.rpp = kPairStBlock / g.tpr;
+    g.J = g.rpp - 2;
+    if (g.J < 2) return false;
+    if (pair_st_lds<T>(d, g) + sizeof(CodeTab<T>) > kPairMaxLds) return false;
+    g.njb = (d.ny + g.J - 1) / g.J;
+    if (target_grid <= 0) target_grid = LES_PAIR_ST_GRID;
+    int nxc = std::max(1, std::min(d.nx, target_grid / g.njb));
+    g.irange = (d.nx + nxc - 1) / nxc;
+    nxc = (d.nx + g.irange - 1) / g.irange;
+    g.grid = ((g.njb * nxc + kXcd - 1) / kXcd) * kXcd;
+    return true;
+}
+
+template <typename T>
+hipError_t launch_cheb_pair_st(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
+                               const T* z_in, T* z_out, const T* p_in, T* p_out, double alpha0,
+                               double beta_a, double alpha_a, double beta_b, double alpha_b,
+                               int first, SolveScalars* sc, const PairGeom& g)
+{
+    constexpr int V = vec_width<T>(), R = LES_PAIR_RING;
+    const size_t lds = pair_st_lds<T>(d, g);
+    const void* fn = first ? reinterpret_cast<const void*>(&k_cheb_pair_st<T, true, V, R>)
+                           : reinterpret_cast<const void*>(&k_cheb_pair_st<T, false, V, R>);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    if (first)
+        hipLaunchKernelGGL((k_cheb_pair_st<T, true, V, R>), dim3(g.grid), dim3(kPairStBlock), lds,
+                           s, d, g, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a,
+                           beta_b, alpha_b, sc);
+    else
+        hipLaunchKernelGGL((k_cheb_pair_st<T, false, V, R>), dim3(g.grid), dim3(kPairStBlock),
+                           lds, s, d, g, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a,
+                           beta_b, alpha_b, sc);
+    return hipGetLastError();
+}
+
 template <typename T>
 hipError_t launch_cheb_pair(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
                             const T* z_in, T* z_out, const T* p_in, T* p_out, double alpha0,
                             double beta_a, double alpha_a, double beta_b, double alpha_b,
                             int first, SolveScalars* sc, int target_grid)
 {
+    if (LES_PAIR_STAGED && A.code && sizeof(T) == 4) {
+        PairGeom g;
+        if (cheb_pair_st_geom<T>(d, g, target_grid))
+            return launch_cheb_pair_st<T>(s, d, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a,
+                                          alpha_a, beta_b, alpha_b, first, sc, g);
+    }
     if (A.code)
         return launch_cheb_pair_cd<T, 1>(s, d, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a,
                                          alpha_a, beta_b, alpha_b, first, sc, target_grid);
