@@ -54,6 +54,8 @@ struct les_ctx {
            *zb = nullptr, *pc = nullptr, *pc2 = nullptr, *t1 = nullptr, *t2 = nullptr;
     MixedWork mw{};
     RedScratch red{};
+    RedScratch red2{};                // a second partials buffer (a producer whose
+                                      // consumer reduces while writing its own partials)
     SolveScalars* sc = nullptr;       // device
     SolveScalars* sc_host = nullptr;  // pinned, two slots
     StatScalars* st = nullptr;        // device
@@ -408,6 +410,7 @@ int les_ctx_create(int nx, int ny, int nz, double dx, double dy, double dz, les_
         if (alloc(c, &c->f[q], N) != LES_OK) { g_create_err = c->err; les_ctx_destroy(c); return LES_ERR_NOMEM; }
     if (alloc(c, &c->solid, N) != LES_OK) { g_create_err = c->err; les_ctx_destroy(c); return LES_ERR_NOMEM; }
     if (alloc(c, &c->red.partials, (size_t)kMaxGrid * kMaxRed) != LES_OK ||
+        alloc(c, &c->red2.partials, (size_t)kMaxGrid * kMaxRed) != LES_OK ||
         alloc(c, &c->sc, 1) != LES_OK ||
         alloc(c, &c->st, 1) != LES_OK || alloc(c, &c->dsc, kScalarSlots) != LES_OK) {
         g_create_err = c->err;
@@ -454,6 +457,7 @@ int les_ctx_destroy(les_ctx* c)
     for (auto* p : c->shadow) fr(p);
     free_mixed(c->mw);
     fr(c->red.partials);
+    fr(c->red2.partials);
     fr(c->sc);
     fr(c->st);
     fr(c->dsc);
@@ -1105,6 +1109,7 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     int issued = 0, batch = 0;
     bool finished = prm->max_iter <= 0;
     bool near = false;  // replacement kernels on every iteration from then on
+    int pq_grid = 0;    // > 0: the matvec left its p.q partials in red2 for the update
     double last_err = 0.0;
     while (!finished) {
         const int n = std::min(check, prm->max_iter - issued);
@@ -1115,16 +1120,19 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
                     HIPCHK(c, launch_pcg_matvec<float>(s, d, A32, pb[cur], w.q, c->red, sc));
                 } else {
                     // p = z + beta p_old fused into q = A p, with the
-                    // previous iteration's x32 += alpha p_old
+                    // previous iteration's x32 += alpha p_old; its p.q
+                    // partials go to red2, and the r update forms alpha
                     HIPCHK(c, launch_pcg_matvec_pu<float>(s, d, A32, pb[cur], w.z, nullptr,
-                                                          pb[cur ^ 1], w.q, c->red, sc, nullptr,
-                                                          w.x));
+                                                          pb[cur ^ 1], w.q, c->red2, sc, nullptr,
+                                                          w.x, &pq_grid));
                     cur ^= 1;
                 }
             }
             {
                 KtScope kt(c, LES_KT_PCG_UPDATE, 4);
-                HIPCHK(c, launch_pcg_update_mixed(s, d, nullptr, w.r, nullptr, w.q, c->red, sc));
+                HIPCHK(c, launch_pcg_update_mixed(s, d, nullptr, w.r, nullptr, w.q, c->red, sc,
+                                                  c->red2, pq_grid));
+                pq_grid = 0;
             }
             // The replacement kernels run gated on the device's request; they
             // are enqueued on the iterations where a request can stand: the

@@ -129,10 +129,13 @@ hipError_t launch_pcg_matvec(hipStream_t s, const Dims& d, Stencil<T> A, const T
 // Jacobi), formed at every stencil point and written to p (distinct from
 // p_old); q = A p.  x non-null: also the previous iteration's deferred
 // x += alpha p_old.
+// grid_out non-null: no finalize runs; the launch's partials stay in red and
+// their count goes to *grid_out (the consumer forms alpha itself).
 template <typename T>
 hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, const T* p_old,
                                 const T* z, const T* diag, T* p, T* q, RedScratch red,
-                                SolveScalars* sc, const double* wgt = nullptr, T* x = nullptr);
+                                SolveScalars* sc, const double* wgt = nullptr, T* x = nullptr,
+                                int* grid_out = nullptr);
 // x += alpha p_last after the last iteration when x's update is deferred to
 // the matvec (p_last = p0 for even iterations, p1 for odd; nothing when no
 // iteration ran)
@@ -213,9 +216,13 @@ hipError_t launch_mixed_fold(hipStream_t s, const Dims& d, double* x64, float* x
 hipError_t launch_mixed_residual(hipStream_t s, const Dims& d, Stencil<double> A, const double* b,
                                  const double* x64, float* r32, int init, int force,
                                  RedScratch red, SolveScalars* sc, double* out);
-// FP32 update with the mixed convergence epilogue
+// FP32 update with the mixed convergence epilogue.  pq_grid > 0: alpha =
+// rho / (p.q) is formed here from the matvec's pq_grid partials in pq_red
+// (the sum and epilogue of its finalize, in every workgroup; workgroup 0
+// stores pq and alpha), so the matvec needs no finalize launch.
 hipError_t launch_pcg_update_mixed(hipStream_t s, const Dims& d, float* x, float* r, const float* p,
-                                   const float* q, RedScratch red, SolveScalars* sc);
+                                   const float* q, RedScratch red, SolveScalars* sc,
+                                   RedScratch pq_red = RedScratch{nullptr}, int pq_grid = 0);
 
 // ---- power iteration helpers (pressure.cpp:917-955) ----
 // v = w / sqrt(tot) where tot = ||w||^2 is read from device memory

@@ -935,6 +935,15 @@ struct PairGeom {
 #ifndef LES_MV_TAB
 #define LES_MV_TAB 1
 #endif
+// The staged forms of the pair and of the reducing last step
+// (k_cheb_pair_st) where they apply: FP32, coded operator, rows inside one
+// wave.
+#ifndef LES_PAIR_STAGED
+#define LES_PAIR_STAGED 1
+#endif
+#ifndef LES_LAST_STAGED
+#define LES_LAST_STAGED 1
+#endif
 // Wave priority during one phase of each plane (s_setprio 2, else 0):
 // 1 step A (the loads), 2 step B, 0 neither, -1 by element type (measured
 // at 256^3: A for FP32, 1.4 % faster; B for FP64, 1.7 % faster).
@@ -1168,18 +1177,22 @@ __global__ __launch_bounds__(TB, pair_waves_per_simd<TB>()) void k_cheb_pair(Dim
 // barrier per plane still orders all LDS use.  One 1024-thread workgroup per
 // CU.  Step B and every cell's arithmetic are k_cheb_pair's, so the results
 // are bit-identical.  Rows must lie inside one wave (k-edges by shuffle).
-template <typename T, bool FIRST, int V, int R>
+// SINGLE: one reducing step (an application's last; its p is never read):
+// the core rows' z_{k+1} goes to z_out and r.z_{k+1} to red's partials, with
+// no ring and no step B.
+template <typename T, bool FIRST, int V, int R, bool SINGLE>
 __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
     Dims d, PairGeom g, Stencil<T> A, const T* __restrict__ r, const T* __restrict__ z_in,
     T* __restrict__ z_out, const T* __restrict__ p_in, T* __restrict__ p_out, double alpha0,
-    double beta_a, double alpha_a, double beta_b, double alpha_b, const SolveScalars* sc)
+    double beta_a, double alpha_a, double beta_b, double alpha_b, const SolveScalars* sc,
+    RedScratch red)
 {
     if (is_done(sc)) return;
     extern __shared__ unsigned char les_smem[];
     const int arows = g.J + 2;  // A-rows j0-1 .. j0+J
     const int zrows = arows + 2;
     T* zs = reinterpret_cast<T*>(les_smem);        // ring of z_{k+1}: [R][arows][nz]
-    T* zi = zs + (size_t)R * arows * d.nz;         // plane t of z_k: [2][zrows][nz]
+    T* zi = zs + (SINGLE ? 0 : (size_t)R * arows * d.nz);  // plane t of z_k: [2][zrows][nz]
     __shared__ CodeTab<T> tab_s;
     code_tab_fill(tab_s, A);
     __syncthreads();
@@ -1188,7 +1201,11 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
     const int jb = lb % g.njb, ic = lb / g.njb;
     const int j0 = jb * g.J;
     const int ia = ic * g.irange, ib = min(ia + g.irange, d.nx);
-    if (ia >= d.nx) return;  // padding of the grid (the whole workgroup)
+    double acc[1] = {0.0};
+    if (ia >= d.nx) {  // padding of the grid (the whole workgroup)
+        if (SINGLE) block_partial<1>(acc, red.partials, lb);
+        return;
+    }
     const int a = (int)threadIdx.x / g.tpr;  // A-row
     const int kk = ((int)threadIdx.x - a * g.tpr) * V;
     const int j = j0 - 1 + a;
@@ -1196,7 +1213,7 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
     const bool core = a >= 1 && a <= g.J && j < d.ny;
     // the extra input row this thread also loads: j0-2 (A-row 0) or j0+J+1
     // (the last A-row); its LDS row index and whether it lies in the grid
-    const int xa = (a == 0) ? 0 : (a == arows - 1 ? zrows - 1 : -1);
+    const int xa = SINGLE ? -1 : (a == 0) ? 0 : (a == arows - 1 ? zrows - 1 : -1);
     const int xj = (a == 0) ? j0 - 2 : j0 + g.J + 1;
     const bool xin = xa >= 0 && xj >= 0 && xj < d.ny;
     const long sij = d.sij;
@@ -1244,7 +1261,8 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
         rc[v] = pc[v] = pkeep[v] = rkeep[v] = pnew[v] = rnew[v] = T(0);
         cc[v] = ckeep[v] = cnew[v] = 0;
     }
-    const int t0 = ia - 1;
+    const int t0 = SINGLE ? ia : ia - 1;
+    const int t1 = SINGLE ? ib - 1 : ib;
     load_z(t0 - 1, jin, col, zw);
     load_z(t0, jin, col, zc);
     load_z(t0 + 1, jin, col, ze);
@@ -1262,7 +1280,7 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
         vload<uint16_t, V>(A.code + t0 * sij + col, cc);
     }
     __syncthreads();
-    for (int t = t0; t <= ib; ++t) {
+    for (int t = t0; t <= t1; ++t) {
         if (prio == 1) __builtin_amdgcn_s_setprio(2);
         // ---- loads for the next planes ----
         T zn[V], xn[V], rn[V], pn_[V];
@@ -1284,7 +1302,7 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
         }
         // ---- A at plane t: z_{k+1} into the ring, p_k of core rows ----
         T* ring_t = zs + (t + R) % R * rplane + a * d.nz + kk;
-        if (t >= 0 && t < d.nx && jin) {
+        if (t >= 0 && t < d.nx && (SINGLE ? core : jin)) {
             Nbr<T, V> nb;
             zval(true, zc, nb.c);
             if (t > 0) zval(true, zw, nb.w);
@@ -1317,8 +1335,14 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
                 rnew[v] = rv[v];
                 cnew[v] = cc[v];
             }
-            vstore<T, V>(ring_t, zo);
-        } else {
+            if (SINGLE) {
+                vstore_s<T, V>(z_out + id, zo);
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[0] += (double)rv[v] * (double)zo[v];
+            } else {
+                vstore<T, V>(ring_t, zo);
+            }
+        } else if (!SINGLE) {
             vstore<T, V>(ring_t, pad);
         }
         // ---- plane t+1 of z_k into the other LDS buffer ----
@@ -1328,7 +1352,7 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
         if (prio == 2) __builtin_amdgcn_s_setprio(2);
         // ---- B at plane t-1 from the ring ----
         const int iB = t - 1;
-        if (core && iB >= ia && iB < ib) {
+        if (!SINGLE && core && iB >= ia && iB < ib) {
             const T* zc_pl = zs + (iB + R) % R * rplane;
             const T* zm_pl = zs + (iB + R - 1) % R * rplane;
             const T* zp_pl = zs + (iB + 1) % R * rplane;
@@ -1369,6 +1393,7 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
             ckeep[v] = cnew[v];
         }
     }
+    if (SINGLE) block_partial<1>(acc, red.partials, lb);
 }
 
 // ------------------------------------------------------------------
@@ -1600,19 +1625,37 @@ __global__ __launch_bounds__(kBlock) void k_mixed_residual(Dims d, Chunking ch, 
     if (red.partials) block_partial<2>(acc, red.partials, lb);
 }
 
-template <int V>
+template <int V, bool FOLD>
 __global__ __launch_bounds__(kBlock) void k_pcg_update_mixed(Dims d, Chunking ch,
                                                              float* __restrict__ x,
                                                              float* __restrict__ r,
                                                              const float* __restrict__ p,
                                                              const float* __restrict__ q,
                                                              RedScratch red,
-                                                             const SolveScalars* sc)
+                                                             const SolveScalars* sc,
+                                                             RedScratch pq_red, int pq_grid)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
-    const float ax = (float)sc->alpha;
-    const float ar = (float)(-sc->alpha);
+    double alpha;
+    if (FOLD) {
+        // the matvec's finalize (EPI_ALPHA) in every workgroup: the same
+        // fixed-order sum, the same division; workgroup 0 stores the result
+        // for the kernels that follow.  No thread of this launch reads the
+        // stored pq or alpha, and rho is not written here.
+        double tot[1];
+        reduce_partials<1, false>(pq_red.partials, pq_grid, tot);
+        alpha = sc->rho / tot[0];
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            SolveScalars* w = const_cast<SolveScalars*>(sc);
+            w->pq = tot[0];
+            w->alpha = alpha;
+        }
+    } else {
+        alpha = sc->alpha;
+    }
+    const float ax = (float)alpha;
+    const float ar = (float)(-alpha);
     double acc[1] = {0.0};
     FOR_VCELLS(d, ch, lb, V)
     {
@@ -1908,7 +1951,7 @@ bool mv_ring_geom(const Dims& d, PairGeom& g)
 template <typename T>
 hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, const T* p_old,
                                 const T* z, const T* diag, T* p, T* q, RedScratch red,
-                                SolveScalars* sc, const double* wgt, T* x)
+                                SolveScalars* sc, const double* wgt, T* x, int* grid_out)
 {
     constexpr int W = vec_width<T>();
     const int V = pick_width<T>(d);
@@ -1940,6 +1983,10 @@ hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, cons
         }
 #undef LES_MVR
         if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+        if (grid_out) {
+            *grid_out = g.grid;
+            return hipSuccess;
+        }
         return launch_finalize(s, g.grid, 1, false, EPI_ALPHA, GATE_DONE, red, sc, nullptr);
     }
     const Chunking ch = make_chunking(d, V);
@@ -1963,6 +2010,10 @@ hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, cons
 #undef LES_MV_V
 #undef LES_MV
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    if (grid_out) {
+        *grid_out = ch.grid;
+        return hipSuccess;
+    }
     return launch_finalize(s, ch.grid, 1, false, EPI_ALPHA, GATE_DONE, red, sc, nullptr);
 }
 
@@ -2042,11 +2093,22 @@ hipError_t launch_pcg_rz(hipStream_t s, const Dims& d, const T* r, const T* z, i
 }
 
 template <typename T>
+hipError_t launch_cheb_last_st(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
+                               const T* z_in, T* z_out, const T* p, double beta, double alpha,
+                               int epi, RedScratch red, SolveScalars* sc);
+
+template <typename T>
 hipError_t launch_cheb_step(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
                             const T* z_in, T* z_out, T* p, double alpha0, double beta,
                             double alpha, int first, int epi, RedScratch red, SolveScalars* sc,
                             const double* wgt)
 {
+    if (LES_PAIR_STAGED && LES_LAST_STAGED && A.code && sizeof(T) == 4 && !first && epi >= 0 &&
+        !wgt) {
+        const hipError_t e = launch_cheb_last_st<T>(s, d, A, r, z_in, z_out, p, beta, alpha, epi,
+                                                    red, sc);
+        if (e != hipErrorNotSupported) return e;
+    }
     constexpr int VW = vec_width<T>();
     const bool vec = pick_width<T>(d) == VW;
     const Chunking ch = make_chunking(d, vec ? VW : 1);
@@ -2220,19 +2282,18 @@ hipError_t launch_cheb_pair_cd(hipStream_t s, const Dims& d, Stencil<T> A, const
 }
 
 // The staged pair (k_cheb_pair_st): FP32 with the coded operator, rows
-// inside one wave, one 1024-thread workgroup per CU.
-#ifndef LES_PAIR_STAGED
-#define LES_PAIR_STAGED 1
-#endif
+// inside one wave, one 1024-thread workgroup per CU (LES_PAIR_STAGED and
+// LES_LAST_STAGED are defined with the pair's other switches).
 #ifndef LES_PAIR_ST_GRID
 #define LES_PAIR_ST_GRID 256
 #endif
 constexpr int kPairStBlock = 1024;
 
 template <typename T>
-size_t pair_st_lds(const Dims& d, const PairGeom& g)
+size_t pair_st_lds(const Dims& d, const PairGeom& g, bool single = false)
 {
-    return ((size_t)LES_PAIR_RING * (g.J + 2) + 2 * (size_t)(g.J + 4)) * d.nz * sizeof(T);
+    return ((single ? 0 : (size_t)LES_PAIR_RING * (g.J + 2)) + 2 * (size_t)(g.J + 4)) * d.nz *
+           sizeof(T);
 }
 
 template <typename T>
@@ -2263,22 +2324,48 @@ hipError_t launch_cheb_pair_st(hipStream_t s, const Dims& d, Stencil<T> A, const
 {
     constexpr int V = vec_width<T>(), R = LES_PAIR_RING;
     const size_t lds = pair_st_lds<T>(d, g);
-    const void* fn = first ? reinterpret_cast<const void*>(&k_cheb_pair_st<T, true, V, R>)
-                           : reinterpret_cast<const void*>(&k_cheb_pair_st<T, false, V, R>);
+    const void* fn = first ? reinterpret_cast<const void*>(&k_cheb_pair_st<T, true, V, R, false>)
+                           : reinterpret_cast<const void*>(&k_cheb_pair_st<T, false, V, R, false>);
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
                                            (int)lds);
         if (e != hipSuccess) return e;
     }
     if (first)
-        hipLaunchKernelGGL((k_cheb_pair_st<T, true, V, R>), dim3(g.grid), dim3(kPairStBlock), lds,
-                           s, d, g, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a,
-                           beta_b, alpha_b, sc);
+        hipLaunchKernelGGL((k_cheb_pair_st<T, true, V, R, false>), dim3(g.grid),
+                           dim3(kPairStBlock), lds, s, d, g, A, r, z_in, z_out, p_in, p_out,
+                           alpha0, beta_a, alpha_a, beta_b, alpha_b, sc, RedScratch{nullptr});
     else
-        hipLaunchKernelGGL((k_cheb_pair_st<T, false, V, R>), dim3(g.grid), dim3(kPairStBlock),
-                           lds, s, d, g, A, r, z_in, z_out, p_in, p_out, alpha0, beta_a, alpha_a,
-                           beta_b, alpha_b, sc);
+        hipLaunchKernelGGL((k_cheb_pair_st<T, false, V, R, false>), dim3(g.grid),
+                           dim3(kPairStBlock), lds, s, d, g, A, r, z_in, z_out, p_in, p_out,
+                           alpha0, beta_a, alpha_a, beta_b, alpha_b, sc, RedScratch{nullptr});
     return hipGetLastError();
+}
+
+// The reducing last Chebyshev step as the staged kernel's single mode (FP32,
+// coded, not the first step, no weights), then its finalize.  Returns
+// hipErrorNotSupported where it does not apply.
+template <typename T>
+hipError_t launch_cheb_last_st(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
+                               const T* z_in, T* z_out, const T* p, double beta, double alpha,
+                               int epi, RedScratch red, SolveScalars* sc)
+{
+    constexpr int V = vec_width<T>(), R = LES_PAIR_RING;
+    PairGeom g;
+    if (!cheb_pair_st_geom<T>(d, g, kMaxGrid) || g.grid > kMaxGrid) return hipErrorNotSupported;
+    const size_t lds = pair_st_lds<T>(d, g, true);
+    const void* fn = reinterpret_cast<const void*>(&k_cheb_pair_st<T, false, V, R, true>);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_cheb_pair_st<T, false, V, R, true>), dim3(g.grid), dim3(kPairStBlock),
+                       lds, s, d, g, A, r, z_in, z_out, p, nullptr, 0.0, beta, alpha, 0.0, 0.0,
+                       sc, red);
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    return launch_finalize(s, g.grid, 1, false, epi, sc ? GATE_DONE : GATE_NONE, red, sc,
+                           nullptr);
 }
 
 template <typename T>
@@ -2392,17 +2479,25 @@ hipError_t launch_mixed_residual(hipStream_t s, const Dims& d, Stencil<double> A
 }
 
 hipError_t launch_pcg_update_mixed(hipStream_t s, const Dims& d, float* x, float* r, const float* p,
-                                   const float* q, RedScratch red, SolveScalars* sc)
+                                   const float* q, RedScratch red, SolveScalars* sc,
+                                   RedScratch pq_red, int pq_grid)
 {
     constexpr int W = vec_width<float>();
     const int V = pick_width<float>(d);
     const Chunking ch = make_chunking(d, V);
-    if (V == W)
-        hipLaunchKernelGGL((k_pcg_update_mixed<W>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, r,
-                           p, q, red, sc);
-    else
-        hipLaunchKernelGGL((k_pcg_update_mixed<1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, r,
-                           p, q, red, sc);
+    const bool fold = pq_grid > 0;
+    if (fold && pq_red.partials == red.partials) return hipErrorInvalidValue;
+#define LES_UPM(VV, FF)                                                                           \
+    hipLaunchKernelGGL((k_pcg_update_mixed<VV, FF>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, \
+                       r, p, q, red, sc, pq_red, pq_grid)
+    if (V == W) {
+        if (fold) LES_UPM(W, true);
+        else LES_UPM(W, false);
+    } else {
+        if (fold) LES_UPM(1, true);
+        else LES_UPM(1, false);
+    }
+#undef LES_UPM
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     return launch_finalize(s, ch.grid, 1, false, EPI_CONV_MIXED, GATE_DONE, red, sc, nullptr);
 }
@@ -2438,7 +2533,7 @@ hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double
                                              T*, RedScratch, SolveScalars*, const double*);     \
     template hipError_t launch_pcg_matvec_pu<T>(hipStream_t, const Dims&, Stencil<T>, const T*, \
                                                 const T*, const T*, T*, T*, RedScratch,         \
-                                                SolveScalars*, const double*, T*);              \
+                                                SolveScalars*, const double*, T*, int*);        \
     template hipError_t launch_pcg_p_jacobi<T>(hipStream_t, const Dims&, const T*, const T*,    \
                                                T*, int, int, RedScratch, SolveScalars*,         \
                                                const double*);                                  \
