@@ -93,7 +93,7 @@ def spectral_velocity(n: int, seed: int):
     out = []
     for _c in range(3):
         coef = (rng.standard_normal(kk.shape) + 1j * rng.standard_normal(kk.shape)) * amp
-        f = np.fft.irfftn(coef, s=(n, n, n))
+        f = np.fft.irfftn(coef, s=(n, n, n), axes=(0, 1, 2))
         f /= np.sqrt(np.mean(f * f))
         out.append(np.ascontiguousarray(f.reshape(-1)))
     return out
