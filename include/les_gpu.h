@@ -41,6 +41,8 @@ extern "C" {
 #define LES_ERR_STATE (-3)
 #define LES_ERR_NOMEM (-4)
 #define LES_ERR_IO (-5)
+/* a wavefront sweep's wave waited longer than LES_OPT_SPIN_LIMIT allows */
+#define LES_ERR_STALL (-6)
 
 typedef struct les_ctx les_ctx;
 
@@ -89,8 +91,18 @@ int les_synchronize(les_ctx* ctx);
  * results are bit-identical either way.
  * LES_OPT_PAIR_GRID: about how many workgroups a fused pair launch uses
  * (0: the built-in default; at most 4096).  A tuning knob: the grid shape
- * changes only where the reduction-free pair launches split their work. */
-enum { LES_OPT_CHEB_PAIRS = 1, LES_OPT_PAIR_GRID = 2 };
+ * changes only where the reduction-free pair launches split their work.
+ * LES_OPT_WAVEFRONT: 1 (default) runs each lexicographic sweep -- the IC(0)
+ * factor, its forward and backward solves, a lexicographic SOR sweep -- as
+ * one launch (a wave per 8x8 tile of (i, j) columns, tiles handing their
+ * edge values to each other through memory); 0 as one launch per i+j+k
+ * hyperplane.  Bit-identical either way.
+ * LES_OPT_SPIN_LIMIT: how many times a wave of such a launch may poll for
+ * its neighbour tiles' values (default 4194304, about 4 s).  Past it the
+ * wave stops waiting, the launch completes with invalid results and the
+ * call that ran it returns LES_ERR_STALL. */
+enum { LES_OPT_CHEB_PAIRS = 1, LES_OPT_PAIR_GRID = 2, LES_OPT_WAVEFRONT = 3,
+       LES_OPT_SPIN_LIMIT = 4 };
 int les_set_option(les_ctx* ctx, int option, int value);
 
 /* ---- fields (Field, src/field.h:9-41) ---- */
@@ -342,7 +354,8 @@ int les_stats_samples(les_ctx* ctx, int* num_samples);
  * context's stream so its average duration can be read back.  Starting
  * resets the record; LES_KT_NONE stops. */
 enum { LES_KT_NONE = 0, LES_KT_CHEB_STEP = 1, LES_KT_PCG_MATVEC = 2, LES_KT_PCG_UPDATE = 3,
-       LES_KT_MOMENTUM = 4, LES_KT_IC_SOLVE = 5, LES_KT_CHEB_PAIR = 6, LES_KT_STATS = 7 };
+       LES_KT_MOMENTUM = 4, LES_KT_IC_SOLVE = 5, LES_KT_CHEB_PAIR = 6, LES_KT_STATS = 7,
+       LES_KT_IC_FACTOR = 8, LES_KT_SOR_SWEEP = 9 };
 int les_kernel_timing(les_ctx* ctx, int kernel_class);
 /* launches recorded, their summed duration in ms, and the bytes per cell the
  * class is charged (element size of the run) are returned; synchronises. */

@@ -26,6 +26,7 @@
 #include "les_io.hpp"
 #include "les_mixed.hpp"
 #include "les_physics.hpp"
+#include "les_wavefront.hpp"
 
 using namespace les;
 
@@ -80,6 +81,14 @@ struct les_ctx {
     double roll_dt = 0, roll_nu = 0, roll_rho = 0, roll_Cs = 0;
     bool cheb_pairs = true;           // fuse Chebyshev steps in pairs where the grid allows
     int pair_grid = 0;                // workgroups of a pair launch (0: built-in default)
+    // lexicographic sweeps (IC(0), SOR): one wavefront launch each, or one
+    // launch per i+j+k hyperplane
+    bool wavefront = true;
+    int spin_limit = kWfDefaultSpin;
+    WfState* wf = nullptr;
+    unsigned long long* wf_fx = nullptr;
+    unsigned long long* wf_fy = nullptr;
+    bool wf_ran = false;              // a wavefront launch since the last stall check
     // checkpoint, VTK output, statistics
     void* io_stage[2] = {};           // pinned staging chunks of the checkpoint streams
     hipEvent_t io_ev[2] = {};         // completion of each chunk's copy
@@ -249,6 +258,7 @@ void cheb_coeffs(double lmin, double lmax, int degree, double& a0, std::vector<d
 
 int last_level(const Dims& d) { return d.nx + d.ny + d.nz - 3; }
 
+
 // Event bracket around the launches of the timed kernel class.
 struct KtScope {
     les_ctx* c;
@@ -274,6 +284,62 @@ struct KtScope {
         if (on) (void)hipEventRecord(stop, c->stream);
     }
 };
+
+// Buffers of the wavefront sweeps: the ticket counters and the tiles' face
+// packets (zeroed once; every launch writes packets with a new tag).
+int ensure_wf(les_ctx* c, WfBufs* wb)
+{
+    int rc;
+    if ((rc = alloc(c, &c->wf, 1)) != LES_OK) return rc;
+    // at least one packet each: lanes with nothing to read load the first
+    if ((rc = alloc(c, &c->wf_fx, std::max<size_t>(wf_face_packets_x(c->d), 2))) != LES_OK)
+        return rc;
+    if ((rc = alloc(c, &c->wf_fy, std::max<size_t>(wf_face_packets_y(c->d), 2))) != LES_OK)
+        return rc;
+    *wb = WfBufs{c->wf, c->wf_fx, c->wf_fy, c->spin_limit};
+    c->wf_ran = true;
+    return LES_OK;
+}
+
+// After the stream has drained: a wave that ran out of its spin budget left
+// a flag (and results that are not to be trusted).  Clears it.
+int wf_collect(les_ctx* c)
+{
+    if (!c->wf_ran) return LES_OK;
+    c->wf_ran = false;
+    int stalled = 0;
+    HIPCHK(c, hipMemcpyAsync(&stalled, &c->wf->stalled, sizeof(int), hipMemcpyDeviceToHost,
+                             c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    if (!stalled) return LES_OK;
+    HIPCHK(c, hipMemsetAsync(&c->wf->stalled, 0, sizeof(int), c->stream));
+    return fail(c, LES_ERR_STALL,
+                "wavefront sweep: a wave polled its neighbour tiles more than the spin limit "
+                "(LES_OPT_SPIN_LIMIT) allows; the results of that launch are invalid");
+}
+
+// z = (L L^T)^-1 r by forward and backward substitution
+// (apply_ic_preconditioner, pressure.cpp:442-482).
+template <typename T>
+int ic_solve(les_ctx* c, const T* r, T* z, const T* const* L, const SolveScalars* sc)
+{
+    const Dims& d = c->d;
+    hipStream_t s = c->stream;
+    KtScope kt(c, LES_KT_IC_SOLVE, (int)sizeof(T));
+    if (c->wavefront) {
+        WfBufs wb;
+        int rc;
+        if ((rc = ensure_wf(c, &wb)) != LES_OK) return rc;
+        HIPCHK(c, launch_wf_ic_fwd<T>(s, d, r, z, L[SC], L[SW], L[SS], L[SB], sc, wb));
+        HIPCHK(c, launch_wf_ic_bwd<T>(s, d, z, L[SC], L[SW], L[SS], L[SB], sc, wb));
+        return LES_OK;
+    }
+    for (int l = 0; l <= last_level(d); ++l)
+        HIPCHK(c, launch_ic_fwd_level<T>(s, d, l, r, z, L[SC], L[SW], L[SS], L[SB], sc));
+    for (int l = last_level(d); l >= 0; --l)
+        HIPCHK(c, launch_ic_bwd_level<T>(s, d, l, z, L[SC], L[SW], L[SS], L[SB], sc));
+    return LES_OK;
+}
 
 double tridiag_eig(const std::vector<double>& a, const std::vector<double>& b, int which);
 
@@ -357,11 +423,8 @@ int precond_fp64(les_ctx* c, int pc, const double* r, double* zout, double lmin,
     case LES_PC_LEGACY_IC_A: {
         double* const* L = (pc == LES_PC_IC0) ? c->L : c->A;
         if (pc == LES_PC_IC0 && !c->have_L) return fail(c, LES_ERR_STATE, "IC(0) factor not built");
-        KtScope kt(c, LES_KT_IC_SOLVE, 8);
-        for (int l = 0; l <= last_level(d); ++l)
-            HIPCHK(c, launch_ic_fwd_level<double>(s, d, l, r, zout, L[SC], L[SW], L[SS], L[SB], sc));
-        for (int l = last_level(d); l >= 0; --l)
-            HIPCHK(c, launch_ic_bwd_level<double>(s, d, l, zout, L[SC], L[SW], L[SS], L[SB], sc));
+        int rc;
+        if ((rc = ic_solve<double>(c, r, zout, L, sc)) != LES_OK) return rc;
         break;
     }
     case LES_PC_CHEBYSHEV:
@@ -462,6 +525,9 @@ int les_ctx_destroy(les_ctx* c)
     fr(c->st);
     fr(c->dsc);
     fr(c->trace);
+    fr(c->wf);
+    fr(c->wf_fx);
+    fr(c->wf_fy);
     if (c->sc_host) hipHostFree(c->sc_host);
     if (c->st_host) hipHostFree(c->st_host);
     if (c->hsc) hipHostFree(c->hsc);
@@ -504,6 +570,13 @@ int les_set_option(les_ctx* c, int option, int value)
     case LES_OPT_PAIR_GRID:
         if (value < 0 || value > kPairMaxGrid) return fail(c, LES_ERR_ARG, "pair grid out of range");
         c->pair_grid = value;
+        return LES_OK;
+    case LES_OPT_WAVEFRONT:
+        c->wavefront = value != 0;
+        return LES_OK;
+    case LES_OPT_SPIN_LIMIT:
+        if (value < 0) return fail(c, LES_ERR_ARG, "spin limit < 0");
+        c->spin_limit = value;
         return LES_OK;
     default:
         return fail(c, LES_ERR_ARG, "unknown option");
@@ -753,9 +826,21 @@ int les_ic0_factor(les_ctx* c)
         HIPCHK(c, hipMemsetAsync(c->L[q], 0, nbytes(c, 8), c->stream));
     }
     const Stencil<double> A = stencil_of(c->A);
-    for (int l = 0; l <= last_level(c->d); ++l)
-        HIPCHK(c, launch_ic_factor_level(c->stream, c->d, l, A, c->L[SC], c->L[SW], c->L[SS],
-                                         c->L[SB]));
+    {
+        KtScope kt(c, LES_KT_IC_FACTOR, 8);
+        if (c->wavefront) {
+            WfBufs wb;
+            if ((rc = ensure_wf(c, &wb)) != LES_OK) return rc;
+            HIPCHK(c, launch_wf_ic_factor(c->stream, c->d, A, c->L[SC], c->L[SW], c->L[SS],
+                                          c->L[SB], wb));
+        } else {
+            for (int l = 0; l <= last_level(c->d); ++l)
+                HIPCHK(c, launch_ic_factor_level(c->stream, c->d, l, A, c->L[SC], c->L[SW],
+                                                 c->L[SS], c->L[SB]));
+        }
+    }
+    c->have_L = false;
+    if ((rc = wf_collect(c)) != LES_OK) return rc;
     c->have_L = true;
     c->have_L32 = false;
     return LES_OK;
@@ -785,7 +870,7 @@ int les_precond_apply(les_ctx* c, int pc, const double* r, double* z, int ptr_ki
         HIPCHK(c, hipMemcpyAsync(z, zd, nbytes(c, 8), kind_out(ptr_kind), c->stream));
         if (!ptr_kind) HIPCHK(c, hipStreamSynchronize(c->stream));
     }
-    return LES_OK;
+    return wf_collect(c);
 }
 
 int les_power_lambda_max(les_ctx* c, int n_iter, double* out)
@@ -1044,11 +1129,8 @@ int precond_fp32(les_ctx* c, int pc, const float* r, float* zout, double lmin, d
     case LES_PC_IC0:
     case LES_PC_LEGACY_IC_A: {
         float* const* L = (pc == LES_PC_IC0) ? c->L32 : c->A32;
-        KtScope kt(c, LES_KT_IC_SOLVE, 4);
-        for (int l = 0; l <= last_level(d); ++l)
-            HIPCHK(c, launch_ic_fwd_level<float>(s, d, l, r, zout, L[SC], L[SW], L[SS], L[SB], sc));
-        for (int l = last_level(d); l >= 0; --l)
-            HIPCHK(c, launch_ic_bwd_level<float>(s, d, l, zout, L[SC], L[SW], L[SS], L[SB], sc));
+        int rc;
+        if ((rc = ic_solve<float>(c, r, zout, L, sc)) != LES_OK) return rc;
         break;
     }
     case LES_PC_CHEBYSHEV:
@@ -1245,9 +1327,11 @@ int les_pcg(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
         return pcg_fp64(c, prm, res, nullptr, c->W);
     }
     if (prm->inner_product != LES_IP_EUCLID) return fail(c, LES_ERR_ARG, "unknown inner product");
-    if (prm->precision == LES_PREC_MIXED) return pcg_mixed(c, prm, res);
-    if (prm->precision != LES_PREC_FP64) return fail(c, LES_ERR_ARG, "unknown precision");
-    return pcg_fp64(c, prm, res);
+    if (prm->precision != LES_PREC_MIXED && prm->precision != LES_PREC_FP64)
+        return fail(c, LES_ERR_ARG, "unknown precision");
+    const int rc = prm->precision == LES_PREC_MIXED ? pcg_mixed(c, prm, res) : pcg_fp64(c, prm, res);
+    if (rc != LES_OK) return rc;
+    return wf_collect(c);
 }
 
 int les_pcg_trace(les_ctx* c, double* trace, int n)
@@ -1374,8 +1458,17 @@ int les_sor_solve(les_ctx* c, int ordering, double dt, double rho, double omega,
     const int lmax = (d.nx - 2) + (d.ny - 2) + (d.nz - 2);
     rc = stat_loop(c, max_sweeps, [&](int k) -> int {
         if (ordering == LES_SOLVER_SOR_LEX) {
-            for (int l = 3; l <= lmax; ++l)
-                HIPCHK(c, launch_sor_level(c->stream, d, l, p, rhs, omc, oc, dx2, dy2, dz2, c->st));
+            KtScope kt(c, LES_KT_SOR_SWEEP, 8);
+            if (c->wavefront) {
+                WfBufs wb;
+                int rc2;
+                if ((rc2 = ensure_wf(c, &wb)) != LES_OK) return rc2;
+                HIPCHK(c, launch_wf_sor(c->stream, d, p, rhs, omc, oc, dx2, dy2, dz2, c->st, wb));
+            } else {
+                for (int l = 3; l <= lmax; ++l)
+                    HIPCHK(c, launch_sor_level(c->stream, d, l, p, rhs, omc, oc, dx2, dy2, dz2,
+                                               c->st));
+            }
         } else {
             for (int colour = 0; colour < 2; ++colour)
                 HIPCHK(c, launch_sor_colour(c->stream, d, p, rhs, colour, omc, oc, dx2, dy2, dz2,
@@ -1385,6 +1478,7 @@ int les_sor_solve(les_ctx* c, int ordering, double dt, double rho, double omega,
         return LES_OK;
     });
     if (rc != LES_OK) return rc;
+    if ((rc = wf_collect(c)) != LES_OK) return rc;
     if (sweeps_out) *sweeps_out = c->st_host[0].sweeps;
     return LES_OK;
 }

@@ -28,9 +28,10 @@ PC_NONE, PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV = 0, 1, 2, 3, 4
 PREC_FP64, PREC_MIXED = 0, 1
 SOLVER_PCG, SOLVER_SOR_LEX, SOLVER_SOR_RB, SOLVER_JACOBI = 0, 1, 2, 3
 KT_NONE, KT_CHEB_STEP, KT_PCG_MATVEC, KT_PCG_UPDATE, KT_MOMENTUM, KT_IC_SOLVE = 0, 1, 2, 3, 4, 5
-KT_CHEB_PAIR, KT_STATS = 6, 7
+KT_CHEB_PAIR, KT_STATS, KT_IC_FACTOR, KT_SOR_SWEEP = 6, 7, 8, 9
 IP_EUCLID, IP_MIRROR = 0, 1
-OPT_CHEB_PAIRS, OPT_PAIR_GRID = 1, 2
+OPT_CHEB_PAIRS, OPT_PAIR_GRID, OPT_WAVEFRONT, OPT_SPIN_LIMIT = 1, 2, 3, 4
+ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_IO, ERR_STALL = -1, -2, -3, -4, -5, -6
 # statistics slots (statistics.h:15-31): means of u, v, w, p, then the sums
 # of the six products
 STATS = ("u", "v", "w", "p", "uu", "vv", "ww", "uv", "uw", "vw")
@@ -248,7 +249,9 @@ class LesGpu:
 
     def _chk(self, rc, what):
         if rc != 0:
-            raise LesError(f"{what}: {self.lib.les_last_error(self.h).decode()} ({rc})")
+            err = LesError(f"{what}: {self.lib.les_last_error(self.h).decode()} ({rc})")
+            err.code = rc
+            raise err
 
     # ---- fields ----
     def set_field(self, name, a):
