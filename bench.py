@@ -292,6 +292,67 @@ def adjoint_timing():
             bwd["unconverged"]}
 
 
+def sweep_timing(g, m):
+    """The lexicographic sweeps on the workload's grid and operator: the IC(0)
+    factor (pressure.cpp:495-556), its forward + backward solve in FP64 and
+    in FP32 (pressure.cpp:442-482, inside a fixed-iteration mixed PCG-IC(0))
+    and one lexicographic SOR sweep (pressure.cpp:216-234), each as one
+    wavefront launch and, for comparison, as one launch per i+j+k
+    hyperplane.  Times are HIP events around the launches of the class.
+    Algorithmic bytes per cell: factor 4 reads + 4 writes, solve pair 2 x (5
+    reads + 1 write), SOR sweep p and rhs read + p written (neighbour
+    values are re-reads of p)."""
+    import lesgpu
+    N = m["nx"] * m["ny"] * m["nz"]
+    levels = m["nx"] + m["ny"] + m["nz"] - 2
+    r = np.random.default_rng(5).uniform(-1.0, 1.0, N)
+    st = g.get_state()
+
+    def timed(cls, fn, reps):
+        g.kernel_timing(cls)
+        for _ in range(reps):
+            fn()
+        n_l, ms, _ = g.kernel_timing_read()
+        g.kernel_timing(lesgpu.KT_NONE)
+        return 1e3 * ms / max(n_l, 1)
+
+    def mixed_ic():
+        g.set_field("p", np.zeros(N))
+        g.pcg(precond=lesgpu.PC_IC0, max_iter=6, tol=0.0, precision=lesgpu.PREC_MIXED,
+              reliable_every=50)
+
+    def sor():
+        g.set_field("p", np.zeros(N))
+        g.sor_solve(1e-3, 1.0, 1.7, 2, lesgpu.SOLVER_SOR_LEX)
+
+    g.set_field("b", r)
+    cases = (("ic0_factor", lesgpu.KT_IC_FACTOR, g.ic0_factor, 8 * 8, "k_wf<double, FACTOR>"),
+             ("ic0_solve_fp64", lesgpu.KT_IC_SOLVE,
+              lambda: g.precond_apply(lesgpu.PC_IC0, r), 2 * 6 * 8,
+              "k_wf<double, FWD> + k_wf<double, BWD>"),
+             ("ic0_solve_fp32", lesgpu.KT_IC_SOLVE, mixed_ic, 2 * 6 * 4,
+              "k_wf<float, FWD> + k_wf<float, BWD>"),
+             ("sor_lex_sweep", lesgpu.KT_SOR_SWEEP, sor, 3 * 8, "k_wf<double, SOR>"))
+    out = {"levels": levels}
+    g.ic0_factor()
+    for wf in (1, 0):
+        g.set_option(lesgpu.OPT_WAVEFRONT, wf)
+        for name, cls, fn, bpc, kern in cases:
+            fn()  # warm
+            us = timed(cls, fn, 3 if wf else 1)
+            e = out.setdefault(name, {"kernel": kern, "bytes_per_cell": bpc})
+            if wf:
+                gbs = bpc * N / (us * 1e-6) / 1e9
+                e.update(avg_us=us, achieved_gbs=gbs, frac=gbs / HBM_PEAK_GBS,
+                         ns_per_level=1e3 * us / levels / (2 if "solve" in name else 1))
+            else:
+                e["hyperplane_launches_us"] = us
+                e["speedup_vs_hyperplanes"] = us / e["avg_us"]
+    g.set_option(lesgpu.OPT_WAVEFRONT, 1)
+    g.set_state(st)
+    return out
+
+
 def io_timing(g, m, ref_sample=True):
     """The next-row paths on the workload's grid, from the fields the timed
     run left on the device: the statistics update kernel (HIP events around
@@ -420,6 +481,8 @@ def main():
     ap.add_argument("--step0", type=int, default=21)
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the timing of the differentiation workload")
+    ap.add_argument("--no-sweeps", action="store_true",
+                    help="skip the timing of the IC(0) and lexicographic SOR sweeps")
     ap.add_argument("--no-io", action="store_true",
                     help="skip the statistics / checkpoint / VTK timings")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -573,6 +636,10 @@ def measure(args, out):
         out["adjoint"] = adjoint_timing()
         log(f"  rollout {out['adjoint']['rollout_s']:.2f} s, adjoint "
             f"{out['adjoint']['adjoint_s']:.2f} s")
+    if not args.no_sweeps:
+        log("lexicographic sweeps (IC(0), SOR) on the workload's grid")
+        out["lexicographic_sweeps"] = sweep_timing(g, m)
+        log(f"  {json.dumps(out['lexicographic_sweeps'])}")
     if not args.no_io:
         log("statistics, checkpoint and VTK output on the workload's grid")
         out["io"] = io_timing(g, m, ref_sample=not args.no_cpu_baseline)

@@ -36,6 +36,9 @@ struct WfState {
     unsigned long long ticket[WF_KINDS];  // waves ever started, per kind
     int stalled;                          // 1: a wave ran out of its spin budget
     int pad[3];
+    // where the lanes with nothing to write send their results, so that the
+    // stores of a chunk need no branch
+    alignas(16) unsigned long long sink[2];
 };
 
 struct WfBufs {
