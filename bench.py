@@ -292,7 +292,7 @@ def adjoint_timing():
             bwd["unconverged"]}
 
 
-def sweep_timing(g, m):
+def sweep_timing(g, m, hyperplanes=True):
     """The lexicographic sweeps on the workload's grid and operator: the IC(0)
     factor (pressure.cpp:495-556), its forward + backward solve in FP64 and
     in FP32 (pressure.cpp:442-482, inside a fixed-iteration mixed PCG-IC(0))
@@ -335,7 +335,7 @@ def sweep_timing(g, m):
              ("sor_lex_sweep", lesgpu.KT_SOR_SWEEP, sor, 3 * 8, "k_wf<double, SOR>"))
     out = {"levels": levels}
     g.ic0_factor()
-    for wf in (1, 0):
+    for wf in ((1, 0) if hyperplanes else (1,)):
         g.set_option(lesgpu.OPT_WAVEFRONT, wf)
         for name, cls, fn, bpc, kern in cases:
             fn()  # warm
@@ -483,6 +483,9 @@ def main():
                     help="skip the timing of the differentiation workload")
     ap.add_argument("--no-sweeps", action="store_true",
                     help="skip the timing of the IC(0) and lexicographic SOR sweeps")
+    ap.add_argument("--no-sweep-hyperplanes", action="store_true",
+                    help="time the sweeps' wavefront launches only, not also one launch per "
+                         "hyperplane (about 29000 small launches: slow under a profiler)")
     ap.add_argument("--no-io", action="store_true",
                     help="skip the statistics / checkpoint / VTK timings")
     ap.add_argument("--copy-calibration", type=int, default=0,
@@ -638,7 +641,7 @@ def measure(args, out):
             f"{out['adjoint']['adjoint_s']:.2f} s")
     if not args.no_sweeps:
         log("lexicographic sweeps (IC(0), SOR) on the workload's grid")
-        out["lexicographic_sweeps"] = sweep_timing(g, m)
+        out["lexicographic_sweeps"] = sweep_timing(g, m, not args.no_sweep_hyperplanes)
         log(f"  {json.dumps(out['lexicographic_sweeps'])}")
     if not args.no_io:
         log("statistics, checkpoint and VTK output on the workload's grid")
