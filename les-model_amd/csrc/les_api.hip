@@ -57,6 +57,7 @@ struct les_ctx {
     RedScratch red{};
     RedScratch red2{};                // a second partials buffer (a producer whose
                                       // consumer reduces while writing its own partials)
+    RedScratch red3{};                // a third: the r update's while red holds r.z
     SolveScalars* sc = nullptr;       // device
     SolveScalars* sc_host = nullptr;  // pinned, two slots
     StatScalars* st = nullptr;        // device
@@ -474,6 +475,7 @@ int les_ctx_create(int nx, int ny, int nz, double dx, double dy, double dz, les_
     if (alloc(c, &c->solid, N) != LES_OK) { g_create_err = c->err; les_ctx_destroy(c); return LES_ERR_NOMEM; }
     if (alloc(c, &c->red.partials, (size_t)kMaxGrid * kMaxRed) != LES_OK ||
         alloc(c, &c->red2.partials, (size_t)kMaxGrid * kMaxRed) != LES_OK ||
+        alloc(c, &c->red3.partials, (size_t)kMaxGrid * kMaxRed) != LES_OK ||
         alloc(c, &c->sc, 1) != LES_OK ||
         alloc(c, &c->st, 1) != LES_OK || alloc(c, &c->dsc, kScalarSlots) != LES_OK) {
         g_create_err = c->err;
@@ -521,6 +523,7 @@ int les_ctx_destroy(les_ctx* c)
     free_mixed(c->mw);
     fr(c->red.partials);
     fr(c->red2.partials);
+    fr(c->red3.partials);
     fr(c->sc);
     fr(c->st);
     fr(c->dsc);
@@ -1192,6 +1195,8 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     bool finished = prm->max_iter <= 0;
     bool near = false;  // replacement kernels on every iteration from then on
     int pq_grid = 0;    // > 0: the matvec left its p.q partials in red2 for the update
+    int rz_grid = 0;    // > 0: the preconditioner left its r.z partials in red for the
+                        // next matvec (beta) and r update (rho, EPI_BETA's bookkeeping)
     double last_err = 0.0;
     while (!finished) {
         const int n = std::min(check, prm->max_iter - issued);
@@ -1206,15 +1211,16 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
                     // partials go to red2, and the r update forms alpha
                     HIPCHK(c, launch_pcg_matvec_pu<float>(s, d, A32, pb[cur], w.z, nullptr,
                                                           pb[cur ^ 1], w.q, c->red2, sc, nullptr,
-                                                          w.x, &pq_grid));
+                                                          w.x, &pq_grid, c->red, rz_grid));
                     cur ^= 1;
                 }
             }
             {
                 KtScope kt(c, LES_KT_PCG_UPDATE, 4);
-                HIPCHK(c, launch_pcg_update_mixed(s, d, nullptr, w.r, nullptr, w.q, c->red, sc,
-                                                  c->red2, pq_grid));
+                HIPCHK(c, launch_pcg_update_mixed(s, d, nullptr, w.r, nullptr, w.q, c->red3, sc,
+                                                  c->red2, pq_grid, c->red, rz_grid));
                 pq_grid = 0;
+                rz_grid = 0;
             }
             // The replacement kernels run gated on the device's request; they
             // are enqueued on the iterations where a request can stand: the
@@ -1226,9 +1232,14 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
                 HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.r, 0, 0, c->red, sc,
                                                 nullptr));
             }
+            // r.z stays as partials: the next matvec forms beta and the
+            // next r update rho from them (one finalize launch fewer)
+            take_partials_grid();
             if ((rc = precond_fp32(c, pc, w.r, w.z, prm->lambda_min, prm->lambda_max, prm->degree,
-                                   EPI_BETA, sc, A32)) != LES_OK)
+                                   EPI_PARTIALS, sc, A32)) != LES_OK)
                 return rc;
+            rz_grid = take_partials_grid();
+            if (rz_grid <= 0) return fail(c, LES_ERR_STATE, "preconditioner left no r.z partials");
         }
         issued += n;
         const int slot = batch & 1;

@@ -94,7 +94,13 @@ enum Epilogue {
     EPI_INIT_MIXED = 9,  // tot = {rr64, bb}: as EPI_INIT, also true_err
     EPI_STORE2 = 10,     // out[0..1] = tot[0..1]
     EPI_STORE_SQRT = 11, // out[0] = sqrt(tot[0])
+    EPI_PARTIALS = 12,   // no finalize: the partials stay for a consumer that sums them
+                         // itself (take_partials_grid() returns their count)
 };
+
+// The workgroup count of the last reduction launched with EPI_PARTIALS on
+// this host thread (0 if none since the previous call); resets it.
+int take_partials_grid();
 
 // The one-workgroup tail of a reduction over `grid` workgroup partials.
 hipError_t launch_finalize(hipStream_t s, int grid, int nr, bool max, int epi, int gate,
@@ -135,7 +141,8 @@ template <typename T>
 hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, const T* p_old,
                                 const T* z, const T* diag, T* p, T* q, RedScratch red,
                                 SolveScalars* sc, const double* wgt = nullptr, T* x = nullptr,
-                                int* grid_out = nullptr);
+                                int* grid_out = nullptr, RedScratch rz_red = RedScratch{nullptr},
+                                int rz_grid = 0);
 // x += alpha p_last after the last iteration when x's update is deferred to
 // the matvec (p_last = p0 for even iterations, p1 for odd; nothing when no
 // iteration ran)
@@ -220,9 +227,13 @@ hipError_t launch_mixed_residual(hipStream_t s, const Dims& d, Stencil<double> A
 // rho / (p.q) is formed here from the matvec's pq_grid partials in pq_red
 // (the sum and epilogue of its finalize, in every workgroup; workgroup 0
 // stores pq and alpha), so the matvec needs no finalize launch.
+// With rz_grid > 0 the preconditioner's r.z reduction was left as rz_grid
+// partials in rz_red (EPI_PARTIALS): rho is their sum, and workgroup 0 runs
+// EPI_BETA's bookkeeping (the matvec formed beta from the same partials).
 hipError_t launch_pcg_update_mixed(hipStream_t s, const Dims& d, float* x, float* r, const float* p,
                                    const float* q, RedScratch red, SolveScalars* sc,
-                                   RedScratch pq_red = RedScratch{nullptr}, int pq_grid = 0);
+                                   RedScratch pq_red = RedScratch{nullptr}, int pq_grid = 0,
+                                   RedScratch rz_red = RedScratch{nullptr}, int rz_grid = 0);
 
 // ---- power iteration helpers (pressure.cpp:917-955) ----
 // v = w / sqrt(tot) where tot = ||w||^2 is read from device memory

@@ -494,6 +494,18 @@ __device__ void epilogue(int epi, const double* tot, SolveScalars* sc, double* o
 
 __device__ __forceinline__ bool is_done(const SolveScalars* sc) { return sc && sc->done; }
 
+// beta of a p update: EPI_BETA's, formed from the preconditioner's r.z
+// partials when they were left for this kernel (rz_grid > 0; every
+// workgroup sums them in the finalize's fixed order and divides as it
+// does), else the value its finalize stored.  Nothing here is written.
+__device__ __forceinline__ double fold_beta(const SolveScalars* sc, RedScratch rz, int rz_grid)
+{
+    if (rz_grid <= 0) return sc->beta;
+    double tot[1];
+    reduce_partials<1, false>(rz.partials, rz_grid, tot);
+    return sc->restart_pending ? 0.0 : tot[0] / sc->rho;
+}
+
 // Whether a kernel gated by `gate` runs.  Producer and finalize apply the
 // same test, and neither changes the scalars it reads.
 __device__ __forceinline__ bool gate_open(int gate, const SolveScalars* sc)
@@ -663,11 +675,12 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec(Dims d, Chunking ch, Sten
                                                        T* __restrict__ q, RedScratch red,
                                                        const SolveScalars* sc,
                                                        const double* __restrict__ wgt,
-                                                       T* __restrict__ x)
+                                                       T* __restrict__ x, RedScratch rz,
+                                                       int rz_grid)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
-    const T beta = (T)sc->beta;
+    const T beta = (T)fold_beta(sc, rz, rz_grid);
     const T ax = (T)sc->alpha;
     double acc[1] = {0.0};
     FOR_VCELLS(d, ch, lb, V)
@@ -1415,7 +1428,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec_ring(Dims d, PairGeom g, 
                                                             T* __restrict__ q, RedScratch red,
                                                             const SolveScalars* sc,
                                                             const double* __restrict__ wgt,
-                                                            T* __restrict__ x)
+                                                            T* __restrict__ x, RedScratch rz,
+                                                            int rz_grid)
 {
     if (sc->done) return;
     extern __shared__ unsigned char les_smem[];
@@ -1434,7 +1448,7 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec_ring(Dims d, PairGeom g, 
     const int sub = (int)threadIdx.x / g.tpr;
     const int kk = ((int)threadIdx.x - sub * g.tpr) * V;
     const int plane_sz = P * g.rpp * d.nz;
-    const T beta = (T)sc->beta;
+    const T beta = (T)fold_beta(sc, rz, rz_grid);
     const T ax = (T)sc->alpha;
     double acc[1] = {0.0};
     // p_old of the core rows, kept for the x update of the same cells one
@@ -1633,7 +1647,8 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_mixed(Dims d, Chunking ch
                                                              const float* __restrict__ q,
                                                              RedScratch red,
                                                              const SolveScalars* sc,
-                                                             RedScratch pq_red, int pq_grid)
+                                                             RedScratch pq_red, int pq_grid,
+                                                             RedScratch rz_red, int rz_grid)
 {
     if (sc->done) return;
     const int lb = logical_block(ch.grid);
@@ -1642,12 +1657,34 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update_mixed(Dims d, Chunking ch
         // the matvec's finalize (EPI_ALPHA) in every workgroup: the same
         // fixed-order sum, the same division; workgroup 0 stores the result
         // for the kernels that follow.  No thread of this launch reads the
-        // stored pq or alpha, and rho is not written here.
+        // stored pq or alpha.  With rz_grid > 0, rho (this iteration's r.z)
+        // is likewise the sum of the preconditioner's partials, and
+        // workgroup 0 also does what EPI_BETA's finalize did (the matvec
+        // already formed beta the same way); then no other thread reads
+        // rho, beta or the restart flag.
         double tot[1];
         reduce_partials<1, false>(pq_red.partials, pq_grid, tot);
-        alpha = sc->rho / tot[0];
+        double rho;
+        if (rz_grid > 0) {
+            double rz[1];
+            reduce_partials<1, false>(rz_red.partials, rz_grid, rz);
+            rho = rz[0];
+        } else {
+            rho = sc->rho;
+        }
+        alpha = rho / tot[0];
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             SolveScalars* w = const_cast<SolveScalars*>(sc);
+            if (rz_grid > 0) {
+                if (w->restart_pending) {
+                    w->beta = 0.0;
+                    w->restart_pending = 0;
+                    w->restarts += 1;
+                } else {
+                    w->beta = rho / w->rho;
+                }
+                w->rho = rho;
+            }
             w->pq = tot[0];
             w->alpha = alpha;
         }
@@ -1820,9 +1857,24 @@ int flat_grid(int n)
 // ------------------------------------------------------------------
 // launchers
 // ------------------------------------------------------------------
+namespace {
+thread_local int g_partials_grid = 0;
+}
+
+int take_partials_grid()
+{
+    const int g = g_partials_grid;
+    g_partials_grid = 0;
+    return g;
+}
+
 hipError_t launch_finalize(hipStream_t s, int grid, int nr, bool max, int epi, int gate,
                            RedScratch red, SolveScalars* sc, double* out)
 {
+    if (epi == EPI_PARTIALS) {
+        g_partials_grid = grid;
+        return hipSuccess;
+    }
 #define LES_FIN(NR, MX)                                                                         \
     hipLaunchKernelGGL((k_finalize<NR, MX>), dim3(1), dim3(kBlock), 0, s, grid, red.partials,  \
                        epi, gate, sc, out)
@@ -1951,8 +2003,11 @@ bool mv_ring_geom(const Dims& d, PairGeom& g)
 template <typename T>
 hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, const T* p_old,
                                 const T* z, const T* diag, T* p, T* q, RedScratch red,
-                                SolveScalars* sc, const double* wgt, T* x, int* grid_out)
+                                SolveScalars* sc, const double* wgt, T* x, int* grid_out,
+                                RedScratch rz_red, int rz_grid)
 {
+    if (rz_grid > 0 && (rz_red.partials == red.partials || p_old == nullptr))
+        return hipErrorInvalidValue;
     constexpr int W = vec_width<T>();
     const int V = pick_width<T>(d);
     const int pu = p_old == nullptr ? 0 : (diag ? 2 : 1);
@@ -1973,7 +2028,7 @@ hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, cons
         }
 #define LES_MVR(CD, PU)                                                                         \
     hipLaunchKernelGGL((k_pcg_matvec_ring<T, CD, W, PU, P, R>), dim3(g.grid), dim3(kBlock), lds, \
-                       s, d, g, A, p_old, z, p, q, red, sc, wgt, x)
+                       s, d, g, A, p_old, z, p, q, red, sc, wgt, x, rz_red, rz_grid)
         if (cd) {
             if (pu) LES_MVR(1, 1);
             else LES_MVR(1, 0);
@@ -1992,7 +2047,7 @@ hipError_t launch_pcg_matvec_pu(hipStream_t s, const Dims& d, Stencil<T> A, cons
     const Chunking ch = make_chunking(d, V);
 #define LES_MV(CD, VV, PU)                                                                      \
     hipLaunchKernelGGL((k_pcg_matvec<T, CD, VV, PU>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, \
-                       A, p_old, z, diag, p, q, red, sc, wgt, x)
+                       A, p_old, z, diag, p, q, red, sc, wgt, x, rz_red, rz_grid)
 #define LES_MV_V(CD, PU)                 \
     do {                                 \
         if (V == W) LES_MV(CD, W, PU);   \
@@ -2480,16 +2535,18 @@ hipError_t launch_mixed_residual(hipStream_t s, const Dims& d, Stencil<double> A
 
 hipError_t launch_pcg_update_mixed(hipStream_t s, const Dims& d, float* x, float* r, const float* p,
                                    const float* q, RedScratch red, SolveScalars* sc,
-                                   RedScratch pq_red, int pq_grid)
+                                   RedScratch pq_red, int pq_grid, RedScratch rz_red, int rz_grid)
 {
     constexpr int W = vec_width<float>();
     const int V = pick_width<float>(d);
     const Chunking ch = make_chunking(d, V);
     const bool fold = pq_grid > 0;
     if (fold && pq_red.partials == red.partials) return hipErrorInvalidValue;
+    // the partials read here must not be the ones this launch writes
+    if (rz_grid > 0 && (!fold || rz_red.partials == red.partials)) return hipErrorInvalidValue;
 #define LES_UPM(VV, FF)                                                                           \
     hipLaunchKernelGGL((k_pcg_update_mixed<VV, FF>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x, \
-                       r, p, q, red, sc, pq_red, pq_grid)
+                       r, p, q, red, sc, pq_red, pq_grid, rz_red, rz_grid)
     if (V == W) {
         if (fold) LES_UPM(W, true);
         else LES_UPM(W, false);
@@ -2533,7 +2590,8 @@ hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double
                                              T*, RedScratch, SolveScalars*, const double*);     \
     template hipError_t launch_pcg_matvec_pu<T>(hipStream_t, const Dims&, Stencil<T>, const T*, \
                                                 const T*, const T*, T*, T*, RedScratch,         \
-                                                SolveScalars*, const double*, T*, int*);        \
+                                                SolveScalars*, const double*, T*, int*,         \
+                                                RedScratch, int);                               \
     template hipError_t launch_pcg_p_jacobi<T>(hipStream_t, const Dims&, const T*, const T*,    \
                                                T*, int, int, RedScratch, SolveScalars*,         \
                                                const double*);                                  \
