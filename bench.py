@@ -228,17 +228,18 @@ def cpu_baseline(m, state, solid, bounds, threads, step, gpu_iters, nu, single_t
     # the shipped driver's order: A built before the mask (main.cpp:65-81)
     A = chk.build_stencil(np.zeros_like(solid), m)
 
-    def one_step(nthreads):
+    def one_step(nthreads, max_iter=None):
         chk.set_threads(nthreads)
         secs = []
         t0 = time.perf_counter()
+        kws = dict(kw, max_iter=max_iter) if max_iter else kw
         if kind == "reference":
             # the reference fixes Cs = 0.18 (main.cpp:425); the cost is the same
             _, its, dts = chk.run_steps(state, solid, m, A, 1, step0=step, solver=4,
-                                        step_seconds=secs, **kw)
+                                        step_seconds=secs, **kws)
         else:
             _, its, dts = chk.run_steps(state, solid, m, A, 1, step0=step, precond=PC_CHEBYSHEV,
-                                        Cs=0.17, **kw)
+                                        Cs=0.17, **kws)
             secs = [time.perf_counter() - t0]
         return float(secs[0]), int(its[0]), float(dts[0])
 
@@ -254,9 +255,24 @@ def cpu_baseline(m, state, solid, bounds, threads, step, gpu_iters, nu, single_t
                       f"(GPU: {gpu_iters}), dt {dt:.3g}, {t:.1f} s on {threads} OpenMP threads "
                       f"of {cpu_model()}")
     if single_thread:
-        t1, it1, _ = one_step(1)
-        out["single_thread"] = dict(value=1.0 / t1, unit="steps/s", cores=1, seconds_per_step=t1,
-                                    pcg_iters=it1)
+        # the same step on one thread, as two runs stopped after 20 and 60
+        # PCG iterations (a whole step takes about ten minutes): their
+        # difference is the time of 40 iterations, the rest is the step's
+        # physics and set-up; scaled to the step's iteration count above
+        n1, n2 = 20, 60
+        log(f"CPU baseline, one thread: the same step stopped after {n1} PCG iterations")
+        ta, _, _ = one_step(1, n1)
+        log(f"  {ta:.1f} s; stopped after {n2} iterations")
+        tb, _, _ = one_step(1, n2)
+        per_it = (tb - ta) / (n2 - n1)
+        other = ta - n1 * per_it
+        t1 = other + it * per_it
+        out["single_thread"] = dict(
+            value=1.0 / t1, unit="steps/s", cores=1, seconds_per_step=t1, pcg_iters=it,
+            seconds_per_pcg_iter=per_it, seconds_outside_pcg=other,
+            sample=f"the same step on one thread stopped after {n1} and {n2} PCG iterations "
+                   f"({ta:.1f} s and {tb:.1f} s): {per_it:.3f} s per iteration and {other:.1f} s "
+                   f"outside the solve, scaled to the step's {it} iterations")
     chk.set_threads(threads)
     return out
 
