@@ -47,6 +47,9 @@ sys.path.insert(0, os.path.join(ROOT, "tests"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
+# the Chebyshev launch of each fusion width and its mode suffix
+KERNEL_OF_WIDTH = {1: "k_cheb_step", 2: "k_cheb_pair"}
+WIDTH_NAME = {1: "single", 2: "pairs"}
 METRIC = "LES steps/s at 256^3, FP64 vs mixed PCG to 1e-8 rel.; PCG HBM GB/s vs peak"
 GROWTH_LIMIT = 4.0   # max |u| may grow to this multiple of its value at the first step
 
@@ -55,50 +58,52 @@ def log(*a):
     print("[bench]", *a, file=sys.stderr, flush=True)
 
 
-def cheb_pair_bytes_per_cell(degree, elem, coded=False):
-    """Algorithmic bytes per cell per timed fused two-step launch
-    (k_cheb_pair), averaged over the pair launches of one application that
-    the library times (all but a reducing last one).  First pair: r, the
-    operator -> p, z.  Later pairs: z, r, p, the operator -> p, z (the pair's
-    intermediate z and p stay on chip)."""
+def cheb_schedule(degree, width):
+    """The launches of one Chebyshev application as the library chains them
+    (les_api.hip cheb_chain): fused launches of `width` steps (1 single, 2
+    pairs) that never run the reducing last step, then single steps.
+    (steps, first, last) each."""
+    out, k = [], 1
+    while k < degree:
+        w = width
+        while w > 1 and (degree - k < w or k + w - 1 == degree - 1):
+            w -= 1
+        out.append((w, k == 1, k + w - 1 == degree - 1))
+        k += w
+    return out
+
+
+def _launch_bytes(first, last, elem, coded):
+    # first: r, the operator -> p, z.  Later: z, r, p, the operator -> p, z
+    # (a fused launch's intermediate z and p stay on chip).  The reducing
+    # last step stores z only.
     op = 2 if coded else 7 * elem
-    first = (1 + 2) * elem + op
-    later = (3 + 2) * elem + op
-    steps = degree - 1
-    m = steps // 2 - (1 if steps % 2 == 0 else 0)
-    return (first + (m - 1) * later) / m
+    if last:
+        return 4 * elem + op
+    return (3 if first else 5) * elem + op
 
 
-def cheb_bytes_per_cell(degree, elem, coded=False):
-    """Algorithmic bytes per cell per timed Chebyshev single-step launch,
-    averaged over the degree-2 launches of one application before the
-    reducing last one.  First step: r (stencil), the operator -> p, z.
-    Later steps: z (stencil), r, p, the operator -> p, z.  The operator is 7
-    coefficient arrays, or one 2-byte code."""
-    op = 2 if coded else 7 * elem
-    first = (1 + 2) * elem + op
-    later = (3 + 2) * elem + op
-    m = degree - 2
-    return (first + (m - 1) * later) / m
+def cheb_bytes_per_cell(degree, elem, coded=False, width=1):
+    """Algorithmic bytes per cell per timed Chebyshev launch of the fused
+    width (1: k_cheb_step, 2: k_cheb_pair), averaged
+    over that width's launches of one application that the library times
+    (all but a reducing last one).  The operator is 7 coefficient arrays,
+    or one 2-byte code."""
+    sched = cheb_schedule(degree, width)
+    timed = [_launch_bytes(f, l, elem, coded) for w, f, l in sched if w == width and not l]
+    return sum(timed) / len(timed)
 
 
-def pcg_bytes_per_iter(degree, elem, coded, pairs):
+def pcg_bytes_per_iter(degree, elem, coded, width):
     """Algorithmic bytes per cell of one PCG-Chebyshev iteration as the
-    library runs it: the preconditioner application (fused pairs where they
-    apply, the reducing last step single), the matvec with the direction
-    update and the previous iteration's x update fused in (p_old, z, the
-    operator, x -> p, q, x) and the r update (r, q -> r).  The finalize
-    kernels and the gated FP64 residual replacement (every 50 iterations)
-    are left out."""
+    library runs it: the preconditioner application (fused launches of the
+    given width where they apply, the reducing last step single), the matvec
+    with the direction update and the previous iteration's x update fused in
+    (p_old, z, the operator, x -> p, q, x) and the r update (r, q -> r).
+    The finalize kernels and the gated FP64 residual replacement (every 50
+    iterations) are left out."""
     op = 2 if coded else 7 * elem
-    first, later = 3 * elem + op, 5 * elem + op
-    last = 4 * elem + op  # the reducing last step stores z only
-    steps = degree - 1
-    if pairs:
-        npairs = steps // 2 - (1 if steps % 2 == 0 else 0)
-        cheb = first + (npairs - 1) * later + (steps - 2 * npairs - 1) * later + last
-    else:
-        cheb = first + (steps - 2) * later + last
+    cheb = sum(_launch_bytes(f, l, elem, coded) for _, f, l in cheb_schedule(degree, width))
     return cheb + (6 * elem + op) + 3 * elem
 
 
@@ -446,7 +451,7 @@ def pcg_hbm(r, elem, cells):
     """Whole-iteration PCG bandwidth of one mode: algorithmic bytes per
     iteration times iterations per second of the timed steps (their physics
     counted in the time), against the 8 TB/s peak."""
-    bpc = pcg_bytes_per_iter(8, elem, r["coded"], r["pairs"])
+    bpc = pcg_bytes_per_iter(8, elem, r["coded"], r["width"])
     gbs = bpc * cells * float(np.sum(r["iters"])) / r["seconds"] / 1e9
     return {"bytes_per_cell_per_iter": bpc, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
 
@@ -456,7 +461,7 @@ def parse_mode(mode):
     prec_s = parts[0]
     sten_s = parts[1] if len(parts) > 1 else "coded"
     kern_s = parts[2] if len(parts) > 2 else "pairs"
-    return prec_s, sten_s, kern_s != "single"
+    return prec_s, sten_s, {"single": 1, "pairs": 2}[kern_s]
 
 
 def base_line(args, m=None):
@@ -492,7 +497,8 @@ def main():
     ap.add_argument("--cpu-single-thread", action="store_true",
                     help="also time the CPU baseline step on one thread (about 1.5 min at 256^3)")
     ap.add_argument("--modes", default="fp64:arrays,mixed:arrays,fp64:coded,mixed:coded",
-                    help="precision:stencil pairs; precision fp64|mixed, stencil arrays|coded")
+                    help="precision:stencil[:fusion] entries; precision fp64|mixed, stencil "
+                         "arrays|coded, fusion single|pairs (default)")
     ap.add_argument("--check-every", type=int, default=0)
     ap.add_argument("--step0", type=int, default=21)
     ap.add_argument("--no-adjoint", action="store_true",
@@ -554,29 +560,29 @@ def measure(args, out):
     out["config"]["lanczos_bounds"] = list(bounds)
 
     modes = args.modes.split(",")
+    KT_OF_WIDTH = {1: lesgpu.KT_CHEB_STEP, 2: lesgpu.KT_CHEB_PAIR}
     res = {}
     pcg_kw = dict(check_every=args.check_every)
     fp64_for_cpu = next((md for md in modes if md.startswith("fp64")), None)
     for mode in modes:
-        prec_s, sten_s, pairs = parse_mode(mode)
+        prec_s, sten_s, width = parse_mode(mode)
         prec = lesgpu.PREC_MIXED if prec_s == "mixed" else lesgpu.PREC_FP64
         sten = lesgpu.STENCIL_CODED if sten_s == "coded" else lesgpu.STENCIL_ARRAYS
-        g.set_option(lesgpu.OPT_CHEB_PAIRS, 1 if pairs else 0)
+        g.set_option(lesgpu.OPT_CHEB_PAIRS, width - 1)
         log(f"mode {mode}: {args.warmup} warmup + {args.steps} timed steps")
-        r = run_mode(g, st0, prec, sten, bounds, args.steps, args.warmup,
-                     lesgpu.KT_CHEB_PAIR if pairs else lesgpu.KT_CHEB_STEP, pcg_kw, args.step0,
-                     args.nu, keep_start_state=(mode == fp64_for_cpu and not args.no_cpu_baseline))
-        r.update(elem=4 if prec_s == "mixed" else 8, coded=sten_s == "coded", pairs=pairs)
+        r = run_mode(g, st0, prec, sten, bounds, args.steps, args.warmup, KT_OF_WIDTH[width],
+                     pcg_kw, args.step0, args.nu,
+                     keep_start_state=(mode == fp64_for_cpu and not args.no_cpu_baseline))
+        r.update(elem=4 if prec_s == "mixed" else 8, coded=sten_s == "coded", width=width)
         res[mode] = r
         if "error" in r:
             log(f"  {mode}: REFUSED: {r['error']}")
             continue
         n_l, ms, _ = r["kt"]
-        r["cheb_bpc"] = (cheb_pair_bytes_per_cell if pairs else cheb_bytes_per_cell)(
-            8, r["elem"], r["coded"])
+        r["cheb_bpc"] = cheb_bytes_per_cell(8, r["elem"], r["coded"], width)
         r["cheb_gbs"] = (r["cheb_bpc"] * N) / (ms / n_l * 1e-3) / 1e9 if n_l else 0.0
         log(f"  {mode}: {args.steps / r['seconds']:.3f} steps/s, iters/step "
-            f"{np.mean(r['iters']):.1f}, {'pair' if pairs else 'step'} {n_l} launches avg "
+            f"{np.mean(r['iters']):.1f}, {KERNEL_OF_WIDTH[width]} {n_l} launches avg "
             f"{1e3 * ms / max(n_l, 1):.1f} us = {r['cheb_gbs']:.0f} GB/s algorithmic")
 
     def mode_entry(r):
@@ -587,7 +593,7 @@ def measure(args, out):
         e.update(steps_per_s=args.steps / r["seconds"],
                  pcg_iters_per_step=float(np.mean(r["iters"])),
                  pcg_hbm=pcg_hbm(r, r["elem"], N),
-                 cheb_kernel="k_cheb_pair" if r["pairs"] else "k_cheb_step",
+                 cheb_kernel=KERNEL_OF_WIDTH[r["width"]],
                  cheb_gbs=r["cheb_gbs"], cheb_frac=r["cheb_gbs"] / HBM_PEAK_GBS,
                  cheb_bytes_per_cell=r["cheb_bpc"],
                  final_component_max=r["final_component_max"])
@@ -617,13 +623,13 @@ def measure(args, out):
             "pcg_iters_per_step": float(np.mean(R["iters"])),
             "pcg_iters_per_s": float(np.sum(R["iters"]) / R["seconds"]),
             "roofline": {"bound": "hbm",
-                         "kernel": f"{'k_cheb_pair' if R['pairs'] else 'k_cheb_step'}"
+                         "kernel": f"{KERNEL_OF_WIDTH[R['width']]}"
                                    f"<{'float' if elem == 4 else 'double'}>"
                                    f"{' coded' if R['coded'] else ''}",
                          "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                          "frac": achieved / HBM_PEAK_GBS,
                          "traffic": measured_traffic(
-                             f"{head}:{'pairs' if R['pairs'] else 'single'}", m["nx"]),
+                             f"{head}:{WIDTH_NAME[R['width']]}", m["nx"]),
                          "algorithmic_bytes_per_launch": bpc * N,
                          "bytes_per_cell": bpc, "cells": N, "launches": n_launch,
                          "avg_us": 1e3 * kt_ms / n_launch if n_launch else None},

@@ -114,6 +114,58 @@ __device__ __forceinline__ void vstore(T* __restrict__ p, const T (&in)[V])
 #ifndef LES_NT
 #define LES_NT 1
 #endif
+// Buffer access: a resource over an array from a given element on (the
+// range check is left wide open), loads and stores with a byte offset in a
+// vector register (a thread's column, shared by every array) and one in a
+// scalar register (a plane).  NT: non-temporal, as vload_s / vstore_s.
+using Rsrc = __amdgpu_buffer_rsrc_t;
+constexpr int kRsrcWord3 = 0x00020000;  // raw 32-bit buffer, CDNA
+
+template <typename U>
+__device__ __forceinline__ Rsrc buf_rsrc(const U* base)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<U*>(base), 0, -1, kRsrcWord3);
+}
+
+template <typename U, int V, bool NT>
+__device__ __forceinline__ void bload(Rsrc rs, unsigned voff, unsigned soff, U (&o)[V])
+{
+    constexpr int B = V * (int)sizeof(U);
+    constexpr int aux = NT ? 2 : 0;
+    static_assert(B == 4 || B == 8 || B == 16, "buffer access of 4, 8 or 16 bytes");
+    if constexpr (B == 16) {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b128(rs, (int)voff, (int)soff, aux);
+        __builtin_memcpy(o, &x, B);
+    } else if constexpr (B == 8) {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b64(rs, (int)voff, (int)soff, aux);
+        __builtin_memcpy(o, &x, B);
+    } else {
+        const auto x = __builtin_amdgcn_raw_buffer_load_b32(rs, (int)voff, (int)soff, aux);
+        __builtin_memcpy(o, &x, B);
+    }
+}
+
+template <typename U, int V, bool NT>
+__device__ __forceinline__ void bstore(Rsrc rs, unsigned voff, unsigned soff, const U (&in)[V])
+{
+    constexpr int B = V * (int)sizeof(U);
+    constexpr int aux = NT ? 2 : 0;
+    static_assert(B == 4 || B == 8 || B == 16, "buffer access of 4, 8 or 16 bytes");
+    if constexpr (B == 16) {
+        int __attribute__((ext_vector_type(4))) x;
+        __builtin_memcpy(&x, in, B);
+        __builtin_amdgcn_raw_buffer_store_b128(x, rs, (int)voff, (int)soff, aux);
+    } else if constexpr (B == 8) {
+        int __attribute__((ext_vector_type(2))) x;
+        __builtin_memcpy(&x, in, B);
+        __builtin_amdgcn_raw_buffer_store_b64(x, rs, (int)voff, (int)soff, aux);
+    } else {
+        int x;
+        __builtin_memcpy(&x, in, B);
+        __builtin_amdgcn_raw_buffer_store_b32(x, rs, (int)voff, (int)soff, aux);
+    }
+}
+
 template <typename T, int V>
 __device__ __forceinline__ void vload_s(const T* __restrict__ p, T (&out)[V])
 {
@@ -155,9 +207,29 @@ struct Nbr {
     T km, kp;  // k-1 of the first cell, k+1 of the last
 };
 
+// x of lane i-1 (UP) or lane i+1 of the wave, by a DPP move (wave_shr:1 /
+// wave_shl:1: one VALU operation, no LDS crossbar); the first / last lane
+// gets 0.
+template <bool UP, typename T>
+__device__ __forceinline__ T wave_shift1(T x)
+{
+    constexpr int ctrl = UP ? 0x138 : 0x130;
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8, "32- or 64-bit values");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(
+            T, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, x), ctrl, 0xf, 0xf, false));
+    } else {
+        const unsigned long long u = __builtin_bit_cast(unsigned long long, x);
+        const int lo = __builtin_amdgcn_update_dpp(0, (int)(unsigned)u, ctrl, 0xf, 0xf, false);
+        const int hi = __builtin_amdgcn_update_dpp(0, (int)(unsigned)(u >> 32), ctrl, 0xf, 0xf,
+                                                   false);
+        return __builtin_bit_cast(T, ((unsigned long long)(unsigned)hi << 32) | (unsigned)lo);
+    }
+}
+
 // The k-1 / k+V neighbours of a thread's cells are the last / first cell of
 // the adjacent lanes in the same row.  LES_KSHFL takes them from those lanes
-// (ds_bpermute): when a row lies inside one wave (nz / V a power of two up to
+// (wave_shift1): when a row lies inside one wave (nz / V a power of two up to
 // 64) no memory access is left; when rows cross wave edges, SPAN kernels
 // load at the edge lanes only and the others keep the two scalar loads
 // (a divergent load between the unrolled passes of the fused Chebyshev pair
@@ -177,8 +249,8 @@ __device__ __forceinline__ void row_edges(const Dims& d, int id, int k, const T 
     const bool inwave = tpr <= 64 && (tpr & (tpr - 1)) == 0;
     if (LES_KSHFL && (inwave || SPAN)) {
         const int lane = (int)threadIdx.x & 63;
-        const T up = __shfl_up(c[V - 1], 1, 64);
-        const T dn = __shfl_down(c[0], 1, 64);
+        const T up = wave_shift1<true>(c[V - 1]);
+        const T dn = wave_shift1<false>(c[0]);
         if (inwave) {
             if (k > 0) km = up;
             if (k + V < d.nz) kp = dn;
@@ -1193,13 +1265,18 @@ __global__ __launch_bounds__(TB, pair_waves_per_simd<TB>()) void k_cheb_pair(Dim
 // SINGLE: one reducing step (an application's last; its p is never read):
 // the core rows' z_{k+1} goes to z_out and r.z_{k+1} to red's partials, with
 // no ring and no step B.
-template <typename T, bool FIRST, int V, int R, bool SINGLE>
+// NZ > 0: the kernel for grids with nz = NZ (row strides and LDS offsets
+// then fold into the instructions); 0: any nz.
+template <typename T, bool FIRST, int V, int R, bool SINGLE, int NZ>
 __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
-    Dims d, PairGeom g, Stencil<T> A, const T* __restrict__ r, const T* __restrict__ z_in,
+    Dims dn, PairGeom g, Stencil<T> A, const T* __restrict__ r, const T* __restrict__ z_in,
     T* __restrict__ z_out, const T* __restrict__ p_in, T* __restrict__ p_out, double alpha0,
     double beta_a, double alpha_a, double beta_b, double alpha_b, const SolveScalars* sc,
     RedScratch red)
 {
+    static_assert(SINGLE || R == 4, "the ring slot of a plane is its place in a four-plane pass");
+    Dims d = dn;
+    if constexpr (NZ > 0) d.nz = NZ;
     if (is_done(sc)) return;
     extern __shared__ unsigned char les_smem[];
     const int arows = g.J + 2;  // A-rows j0-1 .. j0+J
@@ -1229,8 +1306,26 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
     const int xa = SINGLE ? -1 : (a == 0) ? 0 : (a == arows - 1 ? zrows - 1 : -1);
     const int xj = (a == 0) ? j0 - 2 : j0 + g.J + 1;
     const bool xin = xa >= 0 && xj >= 0 && xj < d.ny;
-    const long sij = d.sij;
-    const long col = (long)j * d.nz + kk, xcol = (long)xj * d.nz + kk;
+    // 32-bit cell offsets (the grid holds fewer than 2^32 cells)
+    const unsigned usij = (unsigned)d.sij;
+    const unsigned col = (unsigned)(j * d.nz + kk), xcol = (unsigned)(xj * d.nz + kk);
+    // plane i's first cell, wave-uniform and opaque to the compiler's
+    // induction-variable rewriting, so it stays in a scalar register
+    auto pl = [&](int i) {
+        return (unsigned)__builtin_amdgcn_readfirstlane((int)((unsigned)i * usij));
+    };
+    auto off = [&](int i, unsigned c0) { return pl(i) + c0; };
+    // the arrays as buffer resources from the first plane the workgroup
+    // touches (ia - 2): plane i's byte offset from there is scalar, a
+    // thread's column is its fixed vector offset
+    const int pbase = max(ia - 2, 0);
+    const size_t cbase = (size_t)pbase * d.sij;
+    auto rsrc = [&](const auto* p) { return buf_rsrc(p ? p + cbase : p); };
+    const Rsrc rz = rsrc(FIRST ? r : z_in), rr = rsrc(r), rp = rsrc(p_in), rcd = rsrc(A.code),
+               rzo = rsrc(z_out), rpo = rsrc(p_out);
+    auto so = [&](int i, unsigned es) { return (unsigned)(i - pbase) * usij * es; };
+    constexpr unsigned ET = sizeof(T), EC = sizeof(uint16_t);
+    const unsigned colt = col * ET, colc = col * EC;
     const int zplane = zrows * d.nz, rplane = arows * d.nz;
     const T a0 = (T)alpha0, bta = (T)beta_a, ata = (T)alpha_a, btb = (T)beta_b, atb = (T)alpha_b;
     constexpr int prio = LES_PAIR_PRIO >= 0 ? LES_PAIR_PRIO : (sizeof(T) == 4 ? 1 : 2);
@@ -1239,9 +1334,8 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
     for (int v = 0; v < V; ++v) pad[v] = outside<T>();
     // z_k as stored (r for the first pair, scaled where used), outside<T>()
     // outside the grid
-    const T* zsrc = FIRST ? r : z_in;
-    auto load_z = [&](int i, bool in, long c0, T (&o)[V]) {
-        if (in && i >= 0 && i < d.nx) vload<T, V>(zsrc + i * sij + c0, o);
+    auto load_z = [&](int i, bool in, unsigned c0, T (&o)[V]) {
+        if (in && i >= 0 && i < d.nx) bload<T, V, false>(rz, c0 * ET, so(i, ET), o);
         else {
 #pragma unroll
             for (int v = 0; v < V; ++v) o[v] = pad[v];
@@ -1252,8 +1346,8 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
 #pragma unroll
         for (int v = 0; v < V; ++v) o[v] = (FIRST && loaded) ? T(0) + a0 * x[v] : x[v];
     };
-    auto write_zi = [&](int i, const T (&own)[V], const T (&ext)[V]) {
-        T* zt = zi + (i & 1) * zplane;
+    auto write_zi = [&](int i, int buf, const T (&own)[V], const T (&ext)[V]) {
+        T* zt = zi + buf * zplane;
         const bool pin = i >= 0 && i < d.nx;
         T o[V];
         zval(pin && jin, own, o);
@@ -1264,40 +1358,52 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
         }
     };
 
-    T zw[V], zc[V], ze[V], xz[V];            // z_k at t-1, t, t+1; extra row at t+1
-    T rc[V], pc[V];                          // r and p of plane t (non-first)
-    uint16_t cc[V];                          // code of plane t
-    T pkeep[V], rkeep[V], pnew[V], rnew[V];  // p_k, r of core rows for step B
-    uint16_t ckeep[V], cnew[V];
+    // Registers of one plane step; the roles rotate from plane to plane
+    // (z_k over four buffers, the rest over two), and the march runs four
+    // planes per loop pass with the roles renamed, so no copies are made.
+    T zb[4][V], xb[2][V];                    // z_k at t-1, t, t+1, t+2; extra row at t+1, t+2
+    T rb[2][V], pb[2][V];                    // r and p of plane t, t+1 (non-first)
+    uint16_t cb[2][V];                       // code of plane t, t+1
+    T pk[2][V], rk[2][V];                    // p_k, r of core rows for step B: plane t-1, t
+    uint16_t ck[2][V];
 #pragma unroll
     for (int v = 0; v < V; ++v) {
-        rc[v] = pc[v] = pkeep[v] = rkeep[v] = pnew[v] = rnew[v] = T(0);
-        cc[v] = ckeep[v] = cnew[v] = 0;
+        rb[0][v] = rb[1][v] = pb[0][v] = pb[1][v] = pk[0][v] = pk[1][v] = rk[0][v] = rk[1][v] =
+            T(0);
+        cb[0][v] = cb[1][v] = ck[0][v] = ck[1][v] = 0;
     }
     const int t0 = SINGLE ? ia : ia - 1;
     const int t1 = SINGLE ? ib - 1 : ib;
-    load_z(t0 - 1, jin, col, zw);
-    load_z(t0, jin, col, zc);
-    load_z(t0 + 1, jin, col, ze);
+    load_z(t0 - 1, jin, col, zb[0]);
+    load_z(t0, jin, col, zb[1]);
+    load_z(t0 + 1, jin, col, zb[2]);
     {
         T x0[V];
         load_z(t0, xin, xcol, x0);
-        load_z(t0 + 1, xin, xcol, xz);
-        write_zi(t0, zc, x0);
+        load_z(t0 + 1, xin, xcol, xb[0]);
+        write_zi(t0, 0, zb[1], x0);
     }
     if (jin && t0 >= 0) {
         if (!FIRST) {
-            vload_s<T, V>(r + t0 * sij + col, rc);
-            vload_s<T, V>(p_in + t0 * sij + col, pc);
+            bload<T, V, LES_NT>(rr, colt, so(t0, ET), rb[0]);
+            bload<T, V, LES_NT>(rp, colt, so(t0, ET), pb[0]);
         }
-        vload<uint16_t, V>(A.code + t0 * sij + col, cc);
+        bload<uint16_t, V, false>(rcd, colc, so(t0, EC), cb[0]);
     }
     __syncthreads();
-    for (int t = t0; t <= t1; ++t) {
+    // one plane: zw, zc, ze, zn are z_k at t-1 .. t+2 (zn loaded here), xz/xn
+    // the extra row at t+1/t+2, rc/rn, pc/pn, cc/cn plane t/t+1, pkeep etc.
+    // step B's inputs from plane t-1 and pnew etc. step A's outputs
+    // s: the plane's place in the four-plane pass, which is also its ring
+    // slot and (s & 1) its LDS copy of z_k
+    auto plane = [&](int t, int s, const T (&zw)[V], const T (&zc)[V], const T (&ze)[V], T (&zn)[V],
+                     const T (&xz)[V], T (&xn)[V], const T (&rc)[V], T (&rn)[V],
+                     const T (&pc)[V], T (&pn_)[V], const uint16_t (&cc)[V], uint16_t (&cn)[V],
+                     const T (&pkeep)[V], T (&pnew)[V], const T (&rkeep)[V], T (&rnew)[V],
+                     const uint16_t (&ckeep)[V], uint16_t (&cnew)[V])
+                     __attribute__((always_inline)) {
         if (prio == 1) __builtin_amdgcn_s_setprio(2);
         // ---- loads for the next planes ----
-        T zn[V], xn[V], rn[V], pn_[V];
-        uint16_t cn[V];
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             rn[v] = pn_[v] = T(0);
@@ -1308,13 +1414,13 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
         if (xa >= 0) load_z(t + 2, xin, xcol, xn);
         if (jin && t + 1 < d.nx) {
             if (!FIRST) {
-                vload_s<T, V>(r + (t + 1) * sij + col, rn);
-                vload_s<T, V>(p_in + (t + 1) * sij + col, pn_);
+                bload<T, V, LES_NT>(rr, colt, so(t + 1, ET), rn);
+                bload<T, V, LES_NT>(rp, colt, so(t + 1, ET), pn_);
             }
-            vload<uint16_t, V>(A.code + (t + 1) * sij + col, cn);
+            bload<uint16_t, V, false>(rcd, colc, so(t + 1, EC), cn);
         }
         // ---- A at plane t: z_{k+1} into the ring, p_k of core rows ----
-        T* ring_t = zs + (t + R) % R * rplane + a * d.nz + kk;
+        T* ring_t = zs + s * rplane + a * d.nz + kk;
         if (t >= 0 && t < d.nx && (SINGLE ? core : jin)) {
             Nbr<T, V> nb;
             zval(true, zc, nb.c);
@@ -1328,10 +1434,10 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
 #pragma unroll
                 for (int v = 0; v < V; ++v) nb.e[v] = pad[v];
             }
-            const T* zt = zi + (t & 1) * zplane;
+            const T* zt = zi + (s & 1) * zplane;
             vload<T, V>(zt + a * d.nz + kk, nb.s);
             vload<T, V>(zt + (a + 2) * d.nz + kk, nb.n);
-            const long id = t * sij + col;
+            const int id = (int)off(t, col);
             row_edges<T, V, false>(d, (int)id, kk, nb.c, nb.km, nb.kp,
                                    [&](int n) { return zt[(a + 1) * d.nz + kk + (n - (int)id)]; });
             T az[V];
@@ -1342,14 +1448,14 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 const T res = rv[v] - az[v];
-                const T pk = res + bta * pp[v];
-                zo[v] = nb.c[v] + ata * pk;
-                pnew[v] = pk;
+                const T pkv = res + bta * pp[v];
+                zo[v] = nb.c[v] + ata * pkv;
+                pnew[v] = pkv;
                 rnew[v] = rv[v];
                 cnew[v] = cc[v];
             }
             if (SINGLE) {
-                vstore_s<T, V>(z_out + id, zo);
+                bstore<T, V, LES_NT>(rzo, colt, so(t, ET), zo);
 #pragma unroll
                 for (int v = 0; v < V; ++v) acc[0] += (double)rv[v] * (double)zo[v];
             } else {
@@ -1359,23 +1465,23 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
             vstore<T, V>(ring_t, pad);
         }
         // ---- plane t+1 of z_k into the other LDS buffer ----
-        write_zi(t + 1, ze, xz);
+        write_zi(t + 1, (s + 1) & 1, ze, xz);
         if (prio == 1) __builtin_amdgcn_s_setprio(0);
         __syncthreads();
         if (prio == 2) __builtin_amdgcn_s_setprio(2);
         // ---- B at plane t-1 from the ring ----
         const int iB = t - 1;
         if (!SINGLE && core && iB >= ia && iB < ib) {
-            const T* zc_pl = zs + (iB + R) % R * rplane;
-            const T* zm_pl = zs + (iB + R - 1) % R * rplane;
-            const T* zp_pl = zs + (iB + 1) % R * rplane;
+            const T* zc_pl = zs + ((s + 3) & 3) * rplane;
+            const T* zm_pl = zs + ((s + 2) & 3) * rplane;
+            const T* zp_pl = zs + s * rplane;
             Nbr<T, V> nb;
             vload<T, V>(zc_pl + a * d.nz + kk, nb.c);
             vload<T, V>(zm_pl + a * d.nz + kk, nb.w);
             vload<T, V>(zp_pl + a * d.nz + kk, nb.e);
             vload<T, V>(zc_pl + (a - 1) * d.nz + kk, nb.s);
             vload<T, V>(zc_pl + (a + 1) * d.nz + kk, nb.n);
-            const long id = iB * sij + col;
+            const int id = (int)off(iB, col);
             row_edges<T, V, false>(d, (int)id, kk, nb.c, nb.km, nb.kp,
                                    [&](int n) { return zc_pl[a * d.nz + kk + (n - (int)id)]; });
             T az[V];
@@ -1387,24 +1493,23 @@ __global__ __launch_bounds__(1024, 1) void k_cheb_pair_st(
                 pn[v] = res + btb * pkeep[v];
                 zo[v] = nb.c[v] + atb * pn[v];
             }
-            vstore_s<T, V>(p_out + id, pn);
-            vstore_s<T, V>(z_out + id, zo);
+            bstore<T, V, LES_NT>(rpo, colt, so(iB, ET), pn);
+            bstore<T, V, LES_NT>(rzo, colt, so(iB, ET), zo);
         }
         if (prio == 2) __builtin_amdgcn_s_setprio(0);
-        // ---- the register roles move one plane on ----
-#pragma unroll
-        for (int v = 0; v < V; ++v) {
-            zw[v] = zc[v];
-            zc[v] = ze[v];
-            ze[v] = zn[v];
-            xz[v] = xn[v];
-            rc[v] = rn[v];
-            pc[v] = pn_[v];
-            cc[v] = cn[v];
-            pkeep[v] = pnew[v];
-            rkeep[v] = rnew[v];
-            ckeep[v] = cnew[v];
-        }
+    };
+    for (int t = t0; t <= t1; t += 4) {
+        plane(t, 0, zb[0], zb[1], zb[2], zb[3], xb[0], xb[1], rb[0], rb[1], pb[0], pb[1], cb[0],
+              cb[1], pk[0], pk[1], rk[0], rk[1], ck[0], ck[1]);
+        if (t + 1 > t1) break;
+        plane(t + 1, 1, zb[1], zb[2], zb[3], zb[0], xb[1], xb[0], rb[1], rb[0], pb[1], pb[0], cb[1],
+              cb[0], pk[1], pk[0], rk[1], rk[0], ck[1], ck[0]);
+        if (t + 2 > t1) break;
+        plane(t + 2, 2, zb[2], zb[3], zb[0], zb[1], xb[0], xb[1], rb[0], rb[1], pb[0], pb[1], cb[0],
+              cb[1], pk[0], pk[1], rk[0], rk[1], ck[0], ck[1]);
+        if (t + 3 > t1) break;
+        plane(t + 3, 3, zb[3], zb[0], zb[1], zb[2], xb[1], xb[0], rb[1], rb[0], pb[1], pb[0], cb[1],
+              cb[0], pk[1], pk[0], rk[1], rk[0], ck[1], ck[0]);
     }
     if (SINGLE) block_partial<1>(acc, red.partials, lb);
 }
@@ -2371,30 +2476,64 @@ bool cheb_pair_st_geom(const Dims& d, PairGeom& g, int target_grid)
     return true;
 }
 
+// The staged kernel for the grid's nz: specialised for nz = 32 .. 256 (row
+// strides and LDS offsets fold into the instructions), generic otherwise.
+template <typename T, bool FIRST, bool SINGLE, int NZ>
+hipError_t run_pair_st_nz(hipStream_t s, const Dims& d, const PairGeom& g, size_t lds,
+                          Stencil<T> A, const T* r, const T* z_in, T* z_out, const T* p_in,
+                          T* p_out, double alpha0, double beta_a, double alpha_a, double beta_b,
+                          double alpha_b, SolveScalars* sc, RedScratch red)
+{
+    constexpr int V = vec_width<T>(), R = LES_PAIR_RING;
+    const void* fn = reinterpret_cast<const void*>(&k_cheb_pair_st<T, FIRST, V, R, SINGLE, NZ>);
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL((k_cheb_pair_st<T, FIRST, V, R, SINGLE, NZ>), dim3(g.grid),
+                       dim3(kPairStBlock), lds, s, d, g, A, r, z_in, z_out, p_in, p_out, alpha0,
+                       beta_a, alpha_a, beta_b, alpha_b, sc, red);
+    return hipGetLastError();
+}
+
+template <typename T, bool FIRST, bool SINGLE>
+hipError_t run_pair_st(hipStream_t s, const Dims& d, const PairGeom& g, size_t lds,
+                       Stencil<T> A, const T* r, const T* z_in, T* z_out, const T* p_in,
+                       T* p_out, double alpha0, double beta_a, double alpha_a, double beta_b,
+                       double alpha_b, SolveScalars* sc, RedScratch red)
+{
+    if constexpr (sizeof(T) != 4) {
+        return hipErrorNotSupported;  // the staged kernels run FP32 only
+    } else {
+#define LES_RUN_NZ(NZ)                                                                         \
+    return run_pair_st_nz<T, FIRST, SINGLE, NZ>(s, d, g, lds, A, r, z_in, z_out, p_in, p_out, \
+                                                alpha0, beta_a, alpha_a, beta_b, alpha_b, sc,  \
+                                                red)
+        switch (d.nz) {
+        case 256: LES_RUN_NZ(256);
+        case 128: LES_RUN_NZ(128);
+        case 64: LES_RUN_NZ(64);
+        case 32: LES_RUN_NZ(32);
+        default: LES_RUN_NZ(0);
+        }
+#undef LES_RUN_NZ
+    }
+}
+
 template <typename T>
 hipError_t launch_cheb_pair_st(hipStream_t s, const Dims& d, Stencil<T> A, const T* r,
                                const T* z_in, T* z_out, const T* p_in, T* p_out, double alpha0,
                                double beta_a, double alpha_a, double beta_b, double alpha_b,
                                int first, SolveScalars* sc, const PairGeom& g)
 {
-    constexpr int V = vec_width<T>(), R = LES_PAIR_RING;
     const size_t lds = pair_st_lds<T>(d, g);
-    const void* fn = first ? reinterpret_cast<const void*>(&k_cheb_pair_st<T, true, V, R, false>)
-                           : reinterpret_cast<const void*>(&k_cheb_pair_st<T, false, V, R, false>);
-    if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds);
-        if (e != hipSuccess) return e;
-    }
     if (first)
-        hipLaunchKernelGGL((k_cheb_pair_st<T, true, V, R, false>), dim3(g.grid),
-                           dim3(kPairStBlock), lds, s, d, g, A, r, z_in, z_out, p_in, p_out,
-                           alpha0, beta_a, alpha_a, beta_b, alpha_b, sc, RedScratch{nullptr});
-    else
-        hipLaunchKernelGGL((k_cheb_pair_st<T, false, V, R, false>), dim3(g.grid),
-                           dim3(kPairStBlock), lds, s, d, g, A, r, z_in, z_out, p_in, p_out,
-                           alpha0, beta_a, alpha_a, beta_b, alpha_b, sc, RedScratch{nullptr});
-    return hipGetLastError();
+        return run_pair_st<T, true, false>(s, d, g, lds, A, r, z_in, z_out, p_in, p_out, alpha0,
+                                           beta_a, alpha_a, beta_b, alpha_b, sc,
+                                           RedScratch{nullptr});
+    return run_pair_st<T, false, false>(s, d, g, lds, A, r, z_in, z_out, p_in, p_out, alpha0,
+                                        beta_a, alpha_a, beta_b, alpha_b, sc, RedScratch{nullptr});
 }
 
 // The reducing last Chebyshev step as the staged kernel's single mode (FP32,
@@ -2405,20 +2544,12 @@ hipError_t launch_cheb_last_st(hipStream_t s, const Dims& d, Stencil<T> A, const
                                const T* z_in, T* z_out, const T* p, double beta, double alpha,
                                int epi, RedScratch red, SolveScalars* sc)
 {
-    constexpr int V = vec_width<T>(), R = LES_PAIR_RING;
     PairGeom g;
     if (!cheb_pair_st_geom<T>(d, g, kMaxGrid) || g.grid > kMaxGrid) return hipErrorNotSupported;
     const size_t lds = pair_st_lds<T>(d, g, true);
-    const void* fn = reinterpret_cast<const void*>(&k_cheb_pair_st<T, false, V, R, true>);
-    if (lds > 64 * 1024) {
-        hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                           (int)lds);
-        if (e != hipSuccess) return e;
-    }
-    hipLaunchKernelGGL((k_cheb_pair_st<T, false, V, R, true>), dim3(g.grid), dim3(kPairStBlock),
-                       lds, s, d, g, A, r, z_in, z_out, p, nullptr, 0.0, beta, alpha, 0.0, 0.0,
-                       sc, red);
-    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    const hipError_t e = run_pair_st<T, false, true>(s, d, g, lds, A, r, z_in, z_out, p, nullptr,
+                                                     0.0, beta, alpha, 0.0, 0.0, sc, red);
+    if (e != hipSuccess) return e == hipErrorNotSupported ? hipErrorLaunchFailure : e;
     return launch_finalize(s, g.grid, 1, false, epi, sc ? GATE_DONE : GATE_NONE, red, sc,
                            nullptr);
 }

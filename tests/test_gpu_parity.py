@@ -429,13 +429,15 @@ def test_mixed_precision_rhs_scale(orc, pc, shape):
 @pytest.mark.parametrize("precision", [lesgpu.PREC_FP64, lesgpu.PREC_MIXED])
 @pytest.mark.parametrize("fmt",[lesgpu.STENCIL_ARRAYS, lesgpu.STENCIL_CODED])
 @pytest.mark.parametrize("shape,degree", [((6, 10, 256, 8), 8), ((7, 23, 32, 9), 5),
-                                          ((5, 14, 256, 10), 3), ((64, 120, 256, 11), 8)])
+                                          ((5, 14, 256, 10), 3), ((64, 120, 256, 11), 8),
+                                          ((9, 13, 16, 12), 8), ((11, 9, 128, 13), 8)])
 def test_cheb_pairs_identical(orc, precision, fmt, shape, degree):
     """Fused pairs of Chebyshev steps reproduce single steps bit for bit:
     fixed-iteration solves and their residual histories are identical.  The
     last shape gives each workgroup a run of several i-planes; the pair runs
     at its default grid, at 8 workgroups (long i-runs) and at 4096 (one
-    plane per workgroup)."""
+    plane per workgroup).  nz = 256, 128 and 32 take the staged kernel's
+    specialised forms, nz = 16 its generic one."""
     m, solid, f = random_case(*shape, frac=0.05)
     A = orc.build_stencil(solid, m)
     lmax = orc.lambda_max(A, m, 20)
