@@ -216,12 +216,21 @@ def run_mode(g, st0, precision, stencil, bounds, steps, warmup, kt_class, pcg_kw
                 start_state=start_state, final_component_max=last_speed)
 
 
-def cpu_baseline(m, state, solid, bounds, threads, step, gpu_iters, nu, single_thread=False):
+def cpu_baseline(m, state, solid, bounds, threads, step, gpu_iters, nu, single_thread=False,
+                 full_step=False):
     """The reference solver built from its own sources (oracle/_ref), else
     the oracle, timed on the host cores on the step the GPU timed first:
     from the GPU's FP64 state at that step, with the same stencil order,
-    Chebyshev bounds and tolerance.  One whole step to 1e-8 (std::chrono
-    around the step inside the reference harness)."""
+    Chebyshev bounds and tolerance (std::chrono around the step inside the
+    reference harness).
+
+    A whole step takes about a minute on 16 threads, so the default is a
+    bounded sample: the same step run twice, stopped after N1 and N2 PCG
+    iterations.  The difference is the time of N2 - N1 iterations, the rest
+    is the step outside the solve (physics, set-up, projection); the two are
+    scaled to the iteration count the step takes to 1e-8 (the FP64 GPU
+    solve's: the reference's count equals it, see the C2 replay fixtures).
+    full_step times the whole step to 1e-8 instead."""
     from cpu_checkers import Reference, Oracle, REF_SO, PC_CHEBYSHEV
     lmin, lmax = bounds
     kw = dict(lmin=lmin, lmax=lmax, degree=8, tol=1e-8, tol_rel=True, nu=nu, rho=1.2,
@@ -248,36 +257,50 @@ def cpu_baseline(m, state, solid, bounds, threads, step, gpu_iters, nu, single_t
             secs = [time.perf_counter() - t0]
         return float(secs[0]), int(its[0]), float(dts[0])
 
-    t, it, dt = one_step(threads)
-    out = dict(value=1.0 / t, unit="steps/s", cores=threads, kind=kind,
-               nproc=os.cpu_count(), cpu_model=cpu_model(),
-               seconds_per_step=t, pcg_iters=it, gpu_pcg_iters=gpu_iters,
-               pcg_iters_per_s=it / t,
-               sample=f"step {step} of the {m['nx']}^3 workload (the first step the GPU timed) "
-                      f"from the GPU's FP64 state at that step: the reference's "
-                      f"solve_pressure_pcg_chebyshev(degree 8) on the same Lanczos bounds to "
-                      f"1e-8 relative, A built before the mask; {it} PCG iterations "
-                      f"(GPU: {gpu_iters}), dt {dt:.3g}, {t:.1f} s on {threads} OpenMP threads "
-                      f"of {cpu_model()}")
-    if single_thread:
-        # the same step on one thread, as two runs stopped after 20 and 60
-        # PCG iterations (a whole step takes about ten minutes): their
-        # difference is the time of 40 iterations, the rest is the step's
-        # physics and set-up; scaled to the step's iteration count above
-        n1, n2 = 20, 60
-        log(f"CPU baseline, one thread: the same step stopped after {n1} PCG iterations")
-        ta, _, _ = one_step(1, n1)
-        log(f"  {ta:.1f} s; stopped after {n2} iterations")
-        tb, _, _ = one_step(1, n2)
+    def sampled(nthreads, n1, n2, iters):
+        ta, _, _ = one_step(nthreads, n1)
+        tb, _, dt = one_step(nthreads, n2)
         per_it = (tb - ta) / (n2 - n1)
-        other = ta - n1 * per_it
-        t1 = other + it * per_it
+        other = max(ta - n1 * per_it, 0.0)
+        return other + iters * per_it, per_it, other, ta, tb, dt
+
+    common = dict(unit="steps/s", kind=kind, nproc=os.cpu_count(), cpu_model=cpu_model(),
+                  gpu_pcg_iters=gpu_iters)
+    if full_step:
+        t, it, dt = one_step(threads)
+        out = dict(common, value=1.0 / t, cores=threads, seconds_per_step=t, pcg_iters=it,
+                   pcg_iters_per_s=it / t,
+                   sample=f"step {step} of the {m['nx']}^3 workload (the first step the GPU "
+                          f"timed) from the GPU's FP64 state at that step: the reference's "
+                          f"solve_pressure_pcg_chebyshev(degree 8) on the same Lanczos bounds "
+                          f"to 1e-8 relative, A built before the mask; {it} PCG iterations "
+                          f"(GPU: {gpu_iters}), dt {dt:.3g}, {t:.1f} s on {threads} OpenMP "
+                          f"threads of {cpu_model()}")
+    else:
+        n1, n2 = 20, 100
+        t, per_it, other, ta, tb, dt = sampled(threads, n1, n2, gpu_iters)
+        out = dict(common, value=1.0 / t, cores=threads, seconds_per_step=t,
+                   pcg_iters=gpu_iters, pcg_iters_per_s=1.0 / per_it,
+                   seconds_per_pcg_iter=per_it, seconds_outside_pcg=other,
+                   sample=f"step {step} of the {m['nx']}^3 workload (the first step the GPU "
+                          f"timed) from the GPU's FP64 state at that step, the reference's "
+                          f"solve_pressure_pcg_chebyshev(degree 8) on the same Lanczos bounds, "
+                          f"A built before the mask, run twice on {threads} OpenMP threads of "
+                          f"{cpu_model()} and stopped after {n1} and {n2} PCG iterations "
+                          f"({ta:.1f} s and {tb:.1f} s): {per_it:.4f} s per iteration and "
+                          f"{other:.2f} s outside the solve, scaled to the {gpu_iters} "
+                          f"iterations the step takes to 1e-8 relative")
+    if single_thread:
+        # the same step on one thread, stopped after 20 and 60 iterations
+        n1, n2 = 20, 60
+        log(f"CPU baseline, one thread: the same step stopped after {n1} and {n2} iterations")
+        t1, per_it, other, ta, tb, _ = sampled(1, n1, n2, out["pcg_iters"])
         out["single_thread"] = dict(
-            value=1.0 / t1, unit="steps/s", cores=1, seconds_per_step=t1, pcg_iters=it,
-            seconds_per_pcg_iter=per_it, seconds_outside_pcg=other,
+            value=1.0 / t1, unit="steps/s", cores=1, seconds_per_step=t1,
+            pcg_iters=out["pcg_iters"], seconds_per_pcg_iter=per_it, seconds_outside_pcg=other,
             sample=f"the same step on one thread stopped after {n1} and {n2} PCG iterations "
                    f"({ta:.1f} s and {tb:.1f} s): {per_it:.3f} s per iteration and {other:.1f} s "
-                   f"outside the solve, scaled to the step's {it} iterations")
+                   f"outside the solve, scaled to the step's {out['pcg_iters']} iterations")
     chk.set_threads(threads)
     return out
 
@@ -496,6 +519,9 @@ def main():
                     default=int(os.environ.get("OMP_NUM_THREADS") or 0) or (os.cpu_count() or 1))
     ap.add_argument("--cpu-single-thread", action="store_true",
                     help="also time the CPU baseline step on one thread (about 1.5 min at 256^3)")
+    ap.add_argument("--cpu-full-step", action="store_true",
+                    help="time the CPU baseline's whole step to 1e-8 (about a minute on 16 "
+                         "threads at 256^3) instead of the bounded two-run sample")
     ap.add_argument("--modes", default="fp64:arrays,mixed:arrays,fp64:coded,mixed:coded",
                     help="precision:stencil[:fusion] entries; precision fp64|mixed, stencil "
                          "arrays|coded, fusion single|pairs (default)")
@@ -677,7 +703,8 @@ def measure(args, out):
         try:
             out["cpu_baseline"] = cpu_baseline(m, F["start_state"], solid, bounds,
                                                args.cpu_threads, step, F["iters"][0],
-                                               args.nu, single_thread=args.cpu_single_thread)
+                                               args.nu, single_thread=args.cpu_single_thread,
+                                               full_step=args.cpu_full_step)
             log(f"  {out['cpu_baseline']['sample']}")
         except Exception as e:  # reported, not hidden
             out["cpu_baseline"] = {"error": repr(e)}
