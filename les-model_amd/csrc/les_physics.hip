@@ -507,9 +507,12 @@ struct MarchGeom {
 };
 
 #ifndef LES_MARCH_GRID
-#define LES_MARCH_GRID 256
+#define LES_MARCH_GRID 512
 #endif
-constexpr int kMarchGrid = LES_MARCH_GRID;  // workgroups aimed at (one per CU)
+// workgroups aimed at: two 512-thread workgroups per CU (measured at 256^3
+// against one of 1024 threads: SGS 141 -> 131 us, momentum 253 -> 248 us,
+// host-timed; 384 workgroups or 768 threads were slower)
+constexpr int kMarchGrid = LES_MARCH_GRID;
 // cells per thread along k (2: 16-byte loads where nz is even)
 #ifndef LES_MARCH_V_SGS
 #define LES_MARCH_V_SGS 2
@@ -520,7 +523,7 @@ constexpr int kMarchGrid = LES_MARCH_GRID;  // workgroups aimed at (one per CU)
 constexpr int kMarchBlock = 1024;
 // threads of a march workgroup aimed at (rows R = this / threads per row)
 #ifndef LES_MARCH_THREADS
-#define LES_MARCH_THREADS 1024
+#define LES_MARCH_THREADS 512
 #endif
 
 template <int V>
