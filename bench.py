@@ -48,8 +48,10 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 # the Chebyshev launch of each fusion width and its mode suffix
-KERNEL_OF_WIDTH = {1: "k_cheb_step", 2: "k_cheb_pair"}
-WIDTH_NAME = {1: "single", 2: "pairs"}
+# the preconditioner's timed launch per mode suffix: Chebyshev as single
+# steps (1) or fused pairs (2), or the multigrid V-cycle's level-0 sweeps (0)
+KERNEL_OF_WIDTH = {0: "k_mg_smooth", 1: "k_cheb_step", 2: "k_cheb_pair"}
+WIDTH_NAME = {0: "mg", 1: "single", 2: "pairs"}
 METRIC = "LES steps/s at 256^3, FP64 vs mixed PCG to 1e-8 rel.; PCG HBM GB/s vs peak"
 GROWTH_LIMIT = 4.0   # max |u| may grow to this multiple of its value at the first step
 
@@ -88,7 +90,10 @@ def cheb_bytes_per_cell(degree, elem, coded=False, width=1):
     width (1: k_cheb_step, 2: k_cheb_pair), averaged
     over that width's launches of one application that the library times
     (all but a reducing last one).  The operator is 7 coefficient arrays,
-    or one 2-byte code."""
+    or one 2-byte code.  Width 0: the multigrid's level-0 damped-Jacobi
+    sweep (x, b, the operator -> x)."""
+    if width == 0:
+        return 3 * elem + (2 if coded else 7 * elem)
     sched = cheb_schedule(degree, width)
     timed = [_launch_bytes(f, l, elem, coded) for w, f, l in sched if w == width and not l]
     return sum(timed) / len(timed)
@@ -166,12 +171,17 @@ def trajectory_error(rec, speed0):
 
 
 def run_mode(g, st0, precision, stencil, bounds, steps, warmup, kt_class, pcg_kw, step0,
-             nu, keep_start_state=False):
+             nu, keep_start_state=False, precond=None):
     import torch
     import lesgpu
     g.set_state(st0)
     lmin, lmax = bounds
-    kw = dict(nu=nu, rho=1.2, Cs=0.17, precond=lesgpu.PC_CHEBYSHEV, lmin=lmin, lmax=lmax,
+    precond = lesgpu.PC_CHEBYSHEV if precond is None else precond
+    # multigrid runs in the mirror-weighted inner product (CG on W A, which is
+    # symmetric for the operator built before the mask)
+    if precond == lesgpu.PC_MG:
+        pcg_kw = dict(pcg_kw, inner_product=lesgpu.IP_MIRROR)
+    kw = dict(nu=nu, rho=1.2, Cs=0.17, precond=precond, lmin=lmin, lmax=lmax,
               degree=8, max_iter=20000, tol=1e-8, tol_rel=True, precision=precision,
               reliable_every=50, stencil_format=stencil, **pcg_kw)
     traj = []
@@ -474,6 +484,8 @@ def pcg_hbm(r, elem, cells):
     """Whole-iteration PCG bandwidth of one mode: algorithmic bytes per
     iteration times iterations per second of the timed steps (their physics
     counted in the time), against the 8 TB/s peak."""
+    if r["width"] == 0:
+        return None   # the V-cycle's bytes are not modelled
     bpc = pcg_bytes_per_iter(8, elem, r["coded"], r["width"])
     gbs = bpc * cells * float(np.sum(r["iters"])) / r["seconds"] / 1e9
     return {"bytes_per_cell_per_iter": bpc, "achieved_gbs": gbs, "frac": gbs / HBM_PEAK_GBS}
@@ -484,7 +496,7 @@ def parse_mode(mode):
     prec_s = parts[0]
     sten_s = parts[1] if len(parts) > 1 else "coded"
     kern_s = parts[2] if len(parts) > 2 else "pairs"
-    return prec_s, sten_s, {"single": 1, "pairs": 2}[kern_s]
+    return prec_s, sten_s, {"mg": 0, "single": 1, "pairs": 2}[kern_s]
 
 
 def base_line(args, m=None):
@@ -522,9 +534,12 @@ def main():
     ap.add_argument("--cpu-full-step", action="store_true",
                     help="time the CPU baseline's whole step to 1e-8 (about a minute on 16 "
                          "threads at 256^3) instead of the bounded two-run sample")
-    ap.add_argument("--modes", default="fp64:arrays,mixed:arrays,fp64:coded,mixed:coded",
-                    help="precision:stencil[:fusion] entries; precision fp64|mixed, stencil "
-                         "arrays|coded, fusion single|pairs (default)")
+    ap.add_argument("--modes", default="fp64:arrays,mixed:arrays,fp64:coded,mixed:coded,"
+                                       "fp64:arrays:mg,mixed:coded:mg",
+                    help="precision:stencil[:kind] entries; precision fp64|mixed, stencil "
+                         "arrays|coded, kind single|pairs (default): PCG-Chebyshev(8) with its "
+                         "steps single or fused in pairs, mg: PCG with one multigrid V-cycle "
+                         "(reported beside the headline, which is PCG-Chebyshev)")
     ap.add_argument("--check-every", type=int, default=0)
     ap.add_argument("--step0", type=int, default=21)
     ap.add_argument("--no-adjoint", action="store_true",
@@ -586,7 +601,7 @@ def measure(args, out):
     out["config"]["lanczos_bounds"] = list(bounds)
 
     modes = args.modes.split(",")
-    KT_OF_WIDTH = {1: lesgpu.KT_CHEB_STEP, 2: lesgpu.KT_CHEB_PAIR}
+    KT_OF_WIDTH = {0: lesgpu.KT_MG_SWEEP, 1: lesgpu.KT_CHEB_STEP, 2: lesgpu.KT_CHEB_PAIR}
     res = {}
     pcg_kw = dict(check_every=args.check_every)
     fp64_for_cpu = next((md for md in modes if md.startswith("fp64")), None)
@@ -594,11 +609,12 @@ def measure(args, out):
         prec_s, sten_s, width = parse_mode(mode)
         prec = lesgpu.PREC_MIXED if prec_s == "mixed" else lesgpu.PREC_FP64
         sten = lesgpu.STENCIL_CODED if sten_s == "coded" else lesgpu.STENCIL_ARRAYS
-        g.set_option(lesgpu.OPT_CHEB_PAIRS, width - 1)
+        g.set_option(lesgpu.OPT_CHEB_PAIRS, 1 if width != 1 else 0)
         log(f"mode {mode}: {args.warmup} warmup + {args.steps} timed steps")
         r = run_mode(g, st0, prec, sten, bounds, args.steps, args.warmup, KT_OF_WIDTH[width],
                      pcg_kw, args.step0, args.nu,
-                     keep_start_state=(mode == fp64_for_cpu and not args.no_cpu_baseline))
+                     keep_start_state=(mode == fp64_for_cpu and not args.no_cpu_baseline),
+                     precond=lesgpu.PC_MG if width == 0 else lesgpu.PC_CHEBYSHEV)
         r.update(elem=4 if prec_s == "mixed" else 8, coded=sten_s == "coded", width=width)
         res[mode] = r
         if "error" in r:
@@ -626,7 +642,9 @@ def measure(args, out):
         return e
 
     out["modes"] = {mname: mode_entry(r) for mname, r in res.items()}
-    ok = [md for md in res if "error" not in res[md]]
+    # the headline is the configured method, PCG-Chebyshev(8): the multigrid
+    # modes are reported beside it
+    ok = [md for md in res if "error" not in res[md] and res[md]["width"] != 0]
     mixed_ok = [md for md in ok if md.startswith("mixed")]
     pool = mixed_ok or ok
     head = max(pool, key=lambda md: args.steps / res[md]["seconds"]) if pool else None

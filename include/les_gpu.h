@@ -63,7 +63,11 @@ enum {
     LES_PC_JACOBI = 1,      /* apply_jacobi_preconditioner, pressure.cpp:420 */
     LES_PC_IC0 = 2,         /* apply_ic_preconditioner with L from build_ic_preconditioner */
     LES_PC_LEGACY_IC_A = 3, /* apply_ic_preconditioner(., ., A) as shipped, pressure.cpp:363,396 */
-    LES_PC_CHEBYSHEV = 4    /* apply_chebyshev_preconditioner, pressure.cpp:851 */
+    LES_PC_CHEBYSHEV = 4,   /* apply_chebyshev_preconditioner, pressure.cpp:851 */
+    LES_PC_MG = 5           /* one multigrid V-cycle: no reference counterpart (the
+                               reference's README.md:53 lists multigrid as a TODO);
+                               Galerkin aggregation over 2x2x2 blocks, damped-Jacobi
+                               sweeps (LES_OPT_MG_SWEEPS, LES_OPT_MG_COARSE_SWEEPS) */
 };
 
 enum { LES_PREC_FP64 = 0, LES_PREC_MIXED = 1 };
@@ -100,9 +104,14 @@ int les_synchronize(les_ctx* ctx);
  * LES_OPT_SPIN_LIMIT: how many times a wave of such a launch may poll for
  * its neighbour tiles' values (default 4194304, about 4 s).  Past it the
  * wave stops waiting, the launch completes with invalid results and the
- * call that ran it returns LES_ERR_STALL. */
+ * call that ran it returns LES_ERR_STALL.
+ * LES_OPT_MG_SWEEPS: damped-Jacobi sweeps (omega 0.85) of the multigrid
+ * preconditioner before and after each coarse correction (default 2,
+ * 1..16); the correction is added times 1.8.  LES_OPT_MG_COARSE_SWEEPS:
+ * sweeps on the coarsest level, at most 4 cells per direction (default 32,
+ * 1..4096). */
 enum { LES_OPT_CHEB_PAIRS = 1, LES_OPT_PAIR_GRID = 2, LES_OPT_WAVEFRONT = 3,
-       LES_OPT_SPIN_LIMIT = 4 };
+       LES_OPT_SPIN_LIMIT = 4, LES_OPT_MG_SWEEPS = 5, LES_OPT_MG_COARSE_SWEEPS = 6 };
 int les_set_option(les_ctx* ctx, int option, int value);
 
 /* ---- fields (Field, src/field.h:9-41) ---- */
@@ -355,7 +364,7 @@ int les_stats_samples(les_ctx* ctx, int* num_samples);
  * resets the record; LES_KT_NONE stops. */
 enum { LES_KT_NONE = 0, LES_KT_CHEB_STEP = 1, LES_KT_PCG_MATVEC = 2, LES_KT_PCG_UPDATE = 3,
        LES_KT_MOMENTUM = 4, LES_KT_IC_SOLVE = 5, LES_KT_CHEB_PAIR = 6, LES_KT_STATS = 7,
-       LES_KT_IC_FACTOR = 8, LES_KT_SOR_SWEEP = 9 };
+       LES_KT_IC_FACTOR = 8, LES_KT_SOR_SWEEP = 9, LES_KT_MG_SWEEP = 10 };
 int les_kernel_timing(les_ctx* ctx, int kernel_class);
 /* launches recorded, their summed duration in ms, and the bytes per cell the
  * class is charged (element size of the run) are returned; synchronises. */

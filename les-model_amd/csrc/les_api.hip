@@ -30,6 +30,15 @@
 
 using namespace les;
 
+// Multigrid preconditioner: levels while the largest dimension exceeds
+// kMgCoarseMax (so the coarsest has at most 64 cells, one workgroup's), the
+// damping and the coarse-correction factor (chosen on the CPU for PCG at
+// 64^3 and 128^3 with and without solid masks: 19-24 iterations).
+constexpr int kMgMaxLevels = 16;
+constexpr int kMgCoarseMax = 4;
+constexpr double kMgOmega = 0.85;
+constexpr double kMgScale = 1.8;
+
 struct les_ctx {
     int device = 0;
     hipStream_t own = nullptr;
@@ -80,6 +89,20 @@ struct les_ctx {
     bool have_roll = false;
     int roll_nsteps = 0, roll_every = 0;
     double roll_dt = 0, roll_nu = 0, roll_rho = 0, roll_Cs = 0;
+    // multigrid preconditioner: the levels below the grid and their P^T A P
+    // operators, rebuilt when A changes; x, t, b per level and precision
+    struct Mg {
+        int nlev = 0;
+        Dims dims[kMgMaxLevels] = {};
+        double* A64[kMgMaxLevels][7] = {};
+        float* A32[kMgMaxLevels][7] = {};
+        double* v64[kMgMaxLevels][3] = {};
+        float* v32[kMgMaxLevels][3] = {};
+        bool built = false, have32 = false;
+        bool weighted = false;  // level 1 is P^T W A P (mirror weights)
+    } mg;
+    int mg_sweeps = 2;                // damped-Jacobi sweeps before and after the coarse correction
+    int mg_coarse_sweeps = 32;        // sweeps on the coarsest level
     bool cheb_pairs = true;           // fuse Chebyshev steps in pairs where the grid allows
     int pair_grid = 0;                // workgroups of a pair launch (0: built-in default)
     // lexicographic sweeps (IC(0), SOR): one wavefront launch each, or one
@@ -407,6 +430,137 @@ int cheb_chain(les_ctx* c, const Stencil<T>& A, const T* r, T* zout, T* za, T* z
 
 // z = M^-1 r for FP64 preconditioners.  When epi >= 0 the final kernel also
 // reduces r.z into the solve scalars with that epilogue.
+// The multigrid hierarchy of the current A: level dimensions, the P^T A P
+// operators in FP64 (and FP32 for mixed solves), and each level's x, t, b.
+// W (the mirror weights, or null): level 1 is P^T W A P, for solves in the
+// W inner product.
+int mg_setup(les_ctx* c, bool fp32, const double* W)
+{
+    les_ctx::Mg& g = c->mg;
+    hipStream_t s = c->stream;
+    int rc;
+    if (g.built && g.weighted != (W != nullptr)) g.built = false;
+    if (!g.built) {
+        g.nlev = 1;
+        g.dims[0] = c->d;
+        while (g.nlev < kMgMaxLevels) {
+            const Dims& p = g.dims[g.nlev - 1];
+            if (std::max(p.nx, std::max(p.ny, p.nz)) <= kMgCoarseMax) break;
+            Dims q{};
+            q.nx = (p.nx + 1) / 2;
+            q.ny = (p.ny + 1) / 2;
+            q.nz = (p.nz + 1) / 2;
+            q.sij = q.ny * q.nz;
+            q.N = q.nx * q.sij;
+            g.dims[g.nlev++] = q;
+        }
+        if (g.dims[g.nlev - 1].N > kBlock)
+            return fail(c, LES_ERR_ARG, "multigrid: the coarsest level exceeds one workgroup");
+        for (int l = 1; l < g.nlev; ++l) {
+            for (int q = 0; q < 7; ++q)
+                if ((rc = alloc(c, &g.A64[l][q], g.dims[l].N, false)) != LES_OK) return rc;
+            HIPCHK(c, launch_mg_coarsen(s, g.dims[l - 1], g.dims[l],
+                                        stencil_of(l == 1 ? c->A : g.A64[l - 1]),
+                                        l == 1 ? W : nullptr, g.A64[l]));
+        }
+        g.built = true;
+        g.weighted = W != nullptr;
+        g.have32 = false;
+    }
+    for (int l = 1; l < g.nlev; ++l)
+        for (int q = 0; q < 3; ++q) {
+            if (fp32) {
+                if ((rc = alloc(c, &g.v32[l][q], g.dims[l].N)) != LES_OK) return rc;
+            } else {
+                if ((rc = alloc(c, &g.v64[l][q], g.dims[l].N)) != LES_OK) return rc;
+            }
+        }
+    if (fp32 && !g.have32) {
+        for (int l = 1; l < g.nlev; ++l)
+            for (int q = 0; q < 7; ++q) {
+                if ((rc = alloc(c, &g.A32[l][q], g.dims[l].N, false)) != LES_OK) return rc;
+                HIPCHK(c, (launch_convert<float, double>(s, g.dims[l].N, g.A64[l][q],
+                                                         g.A32[l][q])));
+            }
+        g.have32 = true;
+    }
+    return LES_OK;
+}
+
+template <typename T>
+T* mg_vec(les_ctx::Mg& g, int l, int q);
+template <>
+double* mg_vec<double>(les_ctx::Mg& g, int l, int q) { return g.v64[l][q]; }
+template <>
+float* mg_vec<float>(les_ctx::Mg& g, int l, int q) { return g.v32[l][q]; }
+template <typename T>
+Stencil<T> mg_op(les_ctx::Mg& g, int l);
+template <>
+Stencil<double> mg_op<double>(les_ctx::Mg& g, int l) { return stencil_of(g.A64[l]); }
+template <>
+Stencil<float> mg_op<float>(les_ctx::Mg& g, int l) { return stencil_of(g.A32[l]); }
+
+// z = one V-cycle on r (A0 the level-0 operator, arrays or coded; t0 a
+// level-0 work vector distinct from r and z).  Each level's 2 nu sweeps
+// alternate between two buffers, the first writing the one that is not the
+// level's result, so the last writes z on level 0.  epi >= 0: the last
+// sweep also reduces r.z with that epilogue.  W: the hierarchy's mirror
+// weights (the level-0 restriction and r.z are weighted).
+template <typename T>
+int mg_vcycle(les_ctx* c, const Stencil<T>& A0, const T* r, T* z, T* t0, int epi,
+              SolveScalars* sc, const double* W)
+{
+    les_ctx::Mg& g = c->mg;
+    hipStream_t s = c->stream;
+    const int L = g.nlev, nu = c->mg_sweeps;
+    auto op = [&](int l) { return l == 0 ? A0 : mg_op<T>(g, l); };
+    // the two buffers of level l (the result lands in [1]) and its b
+    auto buf = [&](int l, int k) -> T* { return l == 0 ? (k ? z : t0) : mg_vec<T>(g, l, k); };
+    auto bvec = [&](int l) -> const T* { return l == 0 ? r : mg_vec<T>(g, l, 2); };
+    if (L == 1) {  // no coarser level: the coarsest sweeps on the grid itself
+        const int sw = c->mg_coarse_sweeps;
+        T* cur = nullptr;
+        for (int k = 1; k <= sw; ++k) {
+            T* out = buf(0, (k + sw + 1) & 1);  // the last (k = sw) writes z
+            HIPCHK(c, launch_mg_smooth<T>(s, g.dims[0], A0, cur, r, nullptr, g.dims[0], out,
+                                          k == 1 ? MG_ZERO : MG_SWEEP, k == sw ? epi : -1,
+                                          kMgOmega, kMgScale, c->red, sc, W));
+            cur = out;
+        }
+        return LES_OK;
+    }
+    T* cur[kMgMaxLevels] = {};
+    for (int l = 0; l < L - 1; ++l) {
+        for (int k = 1; k <= nu; ++k) {
+            T* out = buf(l, (k + 1) & 1);
+            // the level-0 plain sweeps are the timed class
+            KtScope kt(c, l == 0 && k > 1 ? LES_KT_MG_SWEEP : -1, (int)sizeof(T));
+            HIPCHK(c, launch_mg_smooth<T>(s, g.dims[l], op(l), cur[l], bvec(l), nullptr, g.dims[l],
+                                          out, k == 1 ? MG_ZERO : MG_SWEEP, -1, kMgOmega,
+                                          kMgScale, c->red, sc));
+            cur[l] = out;
+        }
+        HIPCHK(c, launch_mg_restrict<T>(s, g.dims[l], g.dims[l + 1], op(l), cur[l], bvec(l),
+                                        mg_vec<T>(g, l + 1, 2), sc, l == 0 ? W : nullptr));
+    }
+    cur[L - 1] = buf(L - 1, 1);
+    HIPCHK(c, launch_mg_coarsest<T>(s, g.dims[L - 1], op(L - 1), bvec(L - 1), cur[L - 1],
+                                    c->mg_coarse_sweeps, kMgOmega, sc));
+    for (int l = L - 2; l >= 0; --l) {
+        for (int k = nu + 1; k <= 2 * nu; ++k) {
+            T* out = buf(l, (k + 1) & 1);
+            const bool last = l == 0 && k == 2 * nu;
+            KtScope kt(c, l == 0 && k > nu + 1 && !last ? LES_KT_MG_SWEEP : -1, (int)sizeof(T));
+            HIPCHK(c, launch_mg_smooth<T>(s, g.dims[l], op(l), cur[l], bvec(l), cur[l + 1],
+                                          g.dims[l + 1], out, k == nu + 1 ? MG_PROLONG : MG_SWEEP,
+                                          last ? epi : -1, kMgOmega, kMgScale, c->red, sc,
+                                          last ? W : nullptr));
+            cur[l] = out;
+        }
+    }
+    return LES_OK;
+}
+
 int precond_fp64(les_ctx* c, int pc, const double* r, double* zout, double lmin, double lmax,
                  int degree, int epi, SolveScalars* sc, const Stencil<double>& Aop,
                  const double* wgt = nullptr)
@@ -431,6 +585,11 @@ int precond_fp64(les_ctx* c, int pc, const double* r, double* zout, double lmin,
     case LES_PC_CHEBYSHEV:
         return cheb_chain<double>(c, Aop, r, zout, c->za, c->zb, c->pc, c->pc2, lmin, lmax,
                                   degree, epi, sc, wgt);
+    case LES_PC_MG: {
+        int rc;
+        if ((rc = mg_setup(c, false, wgt)) != LES_OK) return rc;
+        return mg_vcycle<double>(c, Aop, r, zout, c->za, epi, sc, wgt);
+    }
     default:
         return fail(c, LES_ERR_ARG, "unknown preconditioner");
     }
@@ -515,6 +674,16 @@ int les_ctx_destroy(les_ctx* c)
     }
     for (double* p : {c->r, c->z, c->p, c->p2, c->q, c->za, c->zb, c->pc, c->pc2, c->t1, c->t2})
         fr(p);
+    for (int l = 0; l < kMgMaxLevels; ++l) {
+        for (int q = 0; q < 7; ++q) {
+            fr(c->mg.A64[l][q]);
+            fr(c->mg.A32[l][q]);
+        }
+        for (int q = 0; q < 3; ++q) {
+            fr(c->mg.v64[l][q]);
+            fr(c->mg.v32[l][q]);
+        }
+    }
     fr(c->W);
     fr(c->ckpt);
     fr(c->seg);
@@ -580,6 +749,15 @@ int les_set_option(les_ctx* c, int option, int value)
     case LES_OPT_SPIN_LIMIT:
         if (value < 0) return fail(c, LES_ERR_ARG, "spin limit < 0");
         c->spin_limit = value;
+        return LES_OK;
+    case LES_OPT_MG_SWEEPS:
+        if (value < 1 || value > 16) return fail(c, LES_ERR_ARG, "multigrid sweeps out of 1..16");
+        c->mg_sweeps = value;
+        return LES_OK;
+    case LES_OPT_MG_COARSE_SWEEPS:
+        if (value < 1 || value > 4096)
+            return fail(c, LES_ERR_ARG, "multigrid coarsest sweeps out of 1..4096");
+        c->mg_coarse_sweeps = value;
         return LES_OK;
     default:
         return fail(c, LES_ERR_ARG, "unknown option");
@@ -659,6 +837,7 @@ int les_build_stencil(les_ctx* c, int ignore_mask)
     c->have_code = true;
     c->A_from_mask = !ignore_mask;
     c->have_A32 = false;
+    c->mg.built = false;
     c->have_L = false;
     c->have_L32 = false;
     c->have_W = false;
@@ -692,6 +871,7 @@ int les_set_stencil(les_ctx* c, int which, double* const coef[7], int ptr_kind)
         c->have_code = false;
         c->A_from_mask = false;
         c->have_A32 = false;
+        c->mg.built = false;
         c->have_W = false;
         c->have_roll = false;
     } else {
@@ -1118,8 +1298,12 @@ int ensure_fp32_ops(les_ctx* c, int pc, bool coded)
 
 // z = M^-1 r in FP32 (operator and vectors FP32, r.z accumulated in FP64)
 int precond_fp32(les_ctx* c, int pc, const float* r, float* zout, double lmin, double lmax,
-                 int degree, int epi, SolveScalars* sc, const Stencil<float>& Aop)
+                 int degree, int epi, SolveScalars* sc, const Stencil<float>& Aop,
+                 const double* wgt = nullptr)
 {
+    if (wgt && pc != LES_PC_MG)
+        return fail(c, LES_ERR_ARG, "mixed precision in the mirror-weighted inner product needs "
+                                    "the multigrid preconditioner");
     const Dims& d = c->d;
     hipStream_t s = c->stream;
     switch (pc) {
@@ -1139,6 +1323,11 @@ int precond_fp32(les_ctx* c, int pc, const float* r, float* zout, double lmin, d
     case LES_PC_CHEBYSHEV:
         return cheb_chain<float>(c, Aop, r, zout, c->mw.za, c->mw.zb, c->mw.pc, c->mw.pc2, lmin,
                                  lmax, degree, epi, sc, nullptr);
+    case LES_PC_MG: {
+        int rc;
+        if ((rc = mg_setup(c, true, wgt)) != LES_OK) return rc;
+        return mg_vcycle<float>(c, Aop, r, zout, c->mw.za, epi, sc, wgt);
+    }
     default:
         return fail(c, LES_ERR_ARG, "unknown preconditioner");
     }
@@ -1183,11 +1372,13 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     SolveScalars* sc = c->sc;
     float* pb[2] = {w.p, w.p2};
     int cur = 0;
+    // r.z and p.q in the mirror-weighted inner product (multigrid only)
+    const double* wgt = prm->inner_product == LES_IP_MIRROR ? c->W : nullptr;
 
     // r = b - A x0 in FP64, rounded to FP32; ||r||, ||b||
     HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.r, 1, 0, c->red, sc, nullptr));
     if ((rc = precond_fp32(c, pc, w.r, w.p, prm->lambda_min, prm->lambda_max, prm->degree, EPI_RHO,
-                           sc, A32)) != LES_OK)
+                           sc, A32, wgt)) != LES_OK)
         return rc;
 
     int check = prm->check_every > 0 ? prm->check_every : (d.N >= (1 << 22) ? 4 : 16);
@@ -1204,13 +1395,13 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
             {
                 KtScope kt(c, LES_KT_PCG_MATVEC, 4);
                 if (issued + it == 0) {
-                    HIPCHK(c, launch_pcg_matvec<float>(s, d, A32, pb[cur], w.q, c->red, sc));
+                    HIPCHK(c, launch_pcg_matvec<float>(s, d, A32, pb[cur], w.q, c->red, sc, wgt));
                 } else {
                     // p = z + beta p_old fused into q = A p, with the
                     // previous iteration's x32 += alpha p_old; its p.q
                     // partials go to red2, and the r update forms alpha
                     HIPCHK(c, launch_pcg_matvec_pu<float>(s, d, A32, pb[cur], w.z, nullptr,
-                                                          pb[cur ^ 1], w.q, c->red2, sc, nullptr,
+                                                          pb[cur ^ 1], w.q, c->red2, sc, wgt,
                                                           w.x, &pq_grid, c->red, rz_grid));
                     cur ^= 1;
                 }
@@ -1236,7 +1427,7 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
             // next r update rho from them (one finalize launch fewer)
             take_partials_grid();
             if ((rc = precond_fp32(c, pc, w.r, w.z, prm->lambda_min, prm->lambda_max, prm->degree,
-                                   EPI_PARTIALS, sc, A32)) != LES_OK)
+                                   EPI_PARTIALS, sc, A32, wgt)) != LES_OK)
                 return rc;
             rz_grid = take_partials_grid();
             if (rz_grid <= 0) return fail(c, LES_ERR_STATE, "preconditioner left no r.z partials");
@@ -1323,18 +1514,24 @@ int les_pcg(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
 {
     if (!c || !prm || !res) return fail(c, LES_ERR_ARG, "NULL argument");
     if (!c->have_A) return fail(c, LES_ERR_STATE, "stencil A not built");
-    if (prm->precond < LES_PC_NONE || prm->precond > LES_PC_CHEBYSHEV)
+    if (prm->precond < LES_PC_NONE || prm->precond > LES_PC_MG)
         return fail(c, LES_ERR_ARG, "unknown preconditioner");
     if (prm->max_iter < 0) return fail(c, LES_ERR_ARG, "max_iter < 0");
     std::memset(res, 0, sizeof(*res));
     if (prm->inner_product == LES_IP_MIRROR) {
         int rc;
-        if (prm->precision != LES_PREC_FP64)
-            return fail(c, LES_ERR_ARG, "the mirror-weighted inner product is FP64 only");
+        if (prm->precision != LES_PREC_FP64 &&
+            !(prm->precision == LES_PREC_MIXED && prm->precond == LES_PC_MG))
+            return fail(c, LES_ERR_ARG, "the mirror-weighted inner product is FP64 only (mixed "
+                                        "precision: with the multigrid preconditioner)");
         if (prm->precond == LES_PC_IC0 || prm->precond == LES_PC_LEGACY_IC_A)
             return fail(c, LES_ERR_ARG, "the mirror-weighted inner product needs a preconditioner "
-                                        "self-adjoint in it (NONE, JACOBI or CHEBYSHEV)");
+                                        "self-adjoint in it (NONE, JACOBI, CHEBYSHEV or MG)");
         if ((rc = ensure_mirror_weights(c)) != LES_OK) return rc;
+        if (prm->precision == LES_PREC_MIXED) {
+            if ((rc = pcg_mixed(c, prm, res)) != LES_OK) return rc;
+            return wf_collect(c);
+        }
         return pcg_fp64(c, prm, res, nullptr, c->W);
     }
     if (prm->inner_product != LES_IP_EUCLID) return fail(c, LES_ERR_ARG, "unknown inner product");

@@ -201,6 +201,34 @@ hipError_t launch_cheb_scale(hipStream_t s, const Dims& d, const T* r, T* z, dou
                              int epi, RedScratch red, SolveScalars* sc,
                              const double* wgt = nullptr);
 
+// ---- multigrid preconditioner (no reference counterpart: the reference's
+// README.md:53 lists it as a TODO) ----
+// Galerkin aggregation over 2x2x2 blocks; the level dimensions halve,
+// rounding up.  Ac = P^T A P (FP64; f the fine dimensions, cd the coarse);
+// with mirror weights W, P^T W A P.
+hipError_t launch_mg_coarsen(hipStream_t s, const Dims& f, const Dims& cd, Stencil<double> A,
+                             const double* W, double* const Ac[7]);
+// One damped-Jacobi sweep on a level (mode MG_ZERO: from x = 0; MG_SWEEP;
+// MG_PROLONG: x first corrected by scale * e of its block on the coarser
+// level cd).  epi >= 0: b.x_out reduced with that epilogue (the PCG's r.z,
+// in the wgt inner product when wgt is given).
+enum { MG_ZERO = 0, MG_SWEEP = 1, MG_PROLONG = 2 };
+template <typename T>
+hipError_t launch_mg_smooth(hipStream_t s, const Dims& d, Stencil<T> A, const T* x, const T* b,
+                            const T* e, const Dims& cd, T* x_out, int mode, int epi,
+                            double omega, double scale, RedScratch red, SolveScalars* sc,
+                            const double* wgt = nullptr);
+// bc = P^T (b - A x), or P^T W (b - A x)
+template <typename T>
+hipError_t launch_mg_restrict(hipStream_t s, const Dims& f, const Dims& cd, Stencil<T> A,
+                              const T* x, const T* b, T* bc, SolveScalars* sc,
+                              const double* W = nullptr);
+// the coarsest level's sweeps from x = 0 in one workgroup (d.N <= kBlock,
+// arrays format)
+template <typename T>
+hipError_t launch_mg_coarsest(hipStream_t s, const Dims& d, Stencil<T> A, const T* b, T* x,
+                              int sweeps, double omega, SolveScalars* sc);
+
 // ---- IC(0) (pressure.cpp:442-556), i+j+k hyperplane order ----
 hipError_t launch_ic_factor_level(hipStream_t s, const Dims& d, int level, Stencil<double> A,
                                   double* Lc, double* Lw, double* Ls, double* Lb);

@@ -1957,6 +1957,193 @@ int flat_grid(int n)
     return g < 1 ? 1 : g;
 }
 
+
+// ------------------------------------------------------------------
+// Multigrid preconditioner (no reference counterpart: the reference's
+// README.md:53 lists multigrid as a TODO).  Galerkin aggregation over 2x2x2
+// blocks: piecewise-constant prolongation P, restriction P^T (the block
+// sum), coarse operator P^T A P (again 7-point: a coupling inside a block
+// adds to the coarse centre, one across a face to that face's coefficient).
+// One V-cycle of damped-Jacobi sweeps from x = 0 is the preconditioner.
+// Every per-cell operation order below (children in (di, dj, dk) order, a
+// child's couplings in StencilMatrix member order, the sweep's apply7 order)
+// is the oracle's, so the FP64 V-cycle is bit-identical to it.
+// ------------------------------------------------------------------
+template <typename T>
+struct Arr7 {
+    T* a[7];
+};
+
+// Ac = P^T A P, one thread per coarse cell (f: the fine dimensions); with
+// W, P^T W A P (each fine coefficient times W of its row)
+__global__ __launch_bounds__(kBlock) void k_mg_coarsen(Dims f, Dims cd, Chunking ch,
+                                                       Stencil<double> A, const double* W,
+                                                       Arr7<double> Ac)
+{
+    const int lb = logical_block(ch.grid);
+    FOR_CELLS(cd, ch, lb)
+    {
+        double c = 0.0, cn = 0.0, cs = 0.0, ce = 0.0, cw = 0.0, ct = 0.0, cb = 0.0;
+        for (int di = 0; di < 2; ++di)
+            for (int dj = 0; dj < 2; ++dj)
+                for (int dk = 0; dk < 2; ++dk) {
+                    const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
+                    if (fi >= f.nx || fj >= f.ny || fk >= f.nz) continue;
+                    const int fid = (fi * f.ny + fj) * f.nz + fk;
+                    auto wa = [&](int q) { return W ? W[fid] * A.a[q][fid] : A.a[q][fid]; };
+                    c += wa(SC);
+                    if (fj + 1 < f.ny) { if (dj == 0) c += wa(SN); else cn += wa(SN); }
+                    if (fj > 0) { if (dj == 1) c += wa(SS); else cs += wa(SS); }
+                    if (fi + 1 < f.nx) { if (di == 0) c += wa(SE); else ce += wa(SE); }
+                    if (fi > 0) { if (di == 1) c += wa(SW); else cw += wa(SW); }
+                    if (fk + 1 < f.nz) { if (dk == 0) c += wa(ST); else ct += wa(ST); }
+                    if (fk > 0) { if (dk == 1) c += wa(SB); else cb += wa(SB); }
+                }
+        Ac.a[SC][id] = c;
+        Ac.a[SN][id] = cn;
+        Ac.a[SS][id] = cs;
+        Ac.a[SE][id] = ce;
+        Ac.a[SW][id] = cw;
+        Ac.a[ST][id] = ct;
+        Ac.a[SB][id] = cb;
+    }
+}
+
+// One damped-Jacobi sweep x_out = x + omega ((b - A x) / A_c) on a level.
+// MG_ZERO: from x = 0, x_out = omega (b / A_c).  MG_PROLONG: x is first
+// corrected by scale * e of the cell's block on the next coarser level (cd),
+// at the cell and at each neighbour the stencil reads.  RZ: b.x_out partials
+// into red (the PCG's r.z, b being the level-0 residual; W-weighted when
+// wgt is given).
+template <typename T, int CODED, int MODE, bool RZ>
+__global__ __launch_bounds__(kBlock) void k_mg_smooth(Dims d, Chunking ch, Stencil<T> A,
+                                                      const T* __restrict__ x,
+                                                      const T* __restrict__ b,
+                                                      const T* __restrict__ e, Dims cd,
+                                                      T* __restrict__ x_out, double omega,
+                                                      double scale, RedScratch red,
+                                                      const SolveScalars* sc,
+                                                      const double* __restrict__ wgt)
+{
+    if (is_done(sc)) return;
+    const int lb = logical_block(ch.grid);
+    const T om = (T)omega, sf = (T)scale;
+    double acc[1] = {0.0};
+    FOR_CELLS(d, ch, lb)
+    {
+        T c, w, ea, s, n, bb, t;
+        coefs7<CODED>(A, id, c, w, ea, s, n, bb, t);
+        T out;
+        if constexpr (MODE == MG_ZERO) {
+            out = om * (b[id] / c);
+        } else {
+            // x at (i+di, j+dj, k+dk), corrected in MG_PROLONG
+            auto xv = [&](int di, int dj, int dk) -> T {
+                const int nid = id + di * d.sij + dj * d.nz + dk;
+                if constexpr (MODE == MG_PROLONG) {
+                    const int pi = (i + di) >> 1, pj = (j + dj) >> 1, pk = (k + dk) >> 1;
+                    return x[nid] + sf * e[(pi * cd.ny + pj) * cd.nz + pk];
+                } else {
+                    return x[nid];
+                }
+            };
+            const T xc = xv(0, 0, 0);
+            T q = c * xc;
+            if (i > 0) q += w * xv(-1, 0, 0);
+            if (i < d.nx - 1) q += ea * xv(1, 0, 0);
+            if (j > 0) q += s * xv(0, -1, 0);
+            if (j < d.ny - 1) q += n * xv(0, 1, 0);
+            if (k > 0) q += bb * xv(0, 0, -1);
+            if (k < d.nz - 1) q += t * xv(0, 0, 1);
+            out = xc + om * ((b[id] - q) / c);
+        }
+        x_out[id] = out;
+        if (RZ) {
+            const double bz = (double)b[id] * (double)out;
+            acc[0] += wgt ? wgt[id] * bz : bz;
+        }
+    }
+    if (RZ) block_partial<1>(acc, red.partials, lb);
+}
+
+// b_c = P^T (b - A x), or P^T W (b - A x): one thread per coarse cell,
+// children in (di, dj, dk) order
+template <typename T, int CODED>
+__global__ __launch_bounds__(kBlock) void k_mg_restrict(Dims f, Dims cd, Chunking ch,
+                                                        Stencil<T> A, const T* __restrict__ x,
+                                                        const T* __restrict__ b,
+                                                        T* __restrict__ bc,
+                                                        const SolveScalars* sc,
+                                                        const double* __restrict__ W)
+{
+    if (is_done(sc)) return;
+    const int lb = logical_block(ch.grid);
+    FOR_CELLS(cd, ch, lb)
+    {
+        T acc = T(0);
+        for (int di = 0; di < 2; ++di)
+            for (int dj = 0; dj < 2; ++dj)
+                for (int dk = 0; dk < 2; ++dk) {
+                    const int fi = 2 * i + di, fj = 2 * j + dj, fk = 2 * k + dk;
+                    if (fi >= f.nx || fj >= f.ny || fk >= f.nz) continue;
+                    const int fid = (fi * f.ny + fj) * f.nz + fk;
+                    const T q = apply7<CODED>(A, f, fid, fi, fj, fk, [&](int m) { return x[m]; });
+                    const T res = b[fid] - q;
+                    acc += W ? (T)W[fid] * res : res;
+                }
+        bc[id] = acc;
+    }
+}
+
+// The coarsest level: `sweeps` damped-Jacobi sweeps from x = 0 in one
+// workgroup, the iterate in LDS (at most kBlock cells)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_coarsest(Dims d, Stencil<T> A,
+                                                        const T* __restrict__ b,
+                                                        T* __restrict__ x, int sweeps,
+                                                        double omega, const SolveScalars* sc)
+{
+    if (is_done(sc)) return;
+    __shared__ T xs[2][kBlock];
+    const int id = threadIdx.x;
+    const bool live = id < d.N;
+    const T om = (T)omega;
+    int i = 0, j = 0, k = 0;
+    T c = T(1), w = T(0), ea = T(0), s = T(0), n = T(0), bb = T(0), t = T(0), bv = T(0);
+    if (live) {
+        i = id / d.sij;
+        j = (id - i * d.sij) / d.nz;
+        k = id - i * d.sij - j * d.nz;
+        coefs7<0>(A, id, c, w, ea, s, n, bb, t);
+        bv = b[id];
+    }
+    int cur = 0;
+    for (int sw = 0; sw < sweeps; ++sw) {
+        T out = T(0);
+        if (live) {
+            if (sw == 0) {
+                out = om * (bv / c);
+            } else {
+                const T* xp = xs[cur];
+                const T xc = xp[id];
+                T q = c * xc;
+                if (i > 0) q += w * xp[id - d.sij];
+                if (i < d.nx - 1) q += ea * xp[id + d.sij];
+                if (j > 0) q += s * xp[id - d.nz];
+                if (j < d.ny - 1) q += n * xp[id + d.nz];
+                if (k > 0) q += bb * xp[id - 1];
+                if (k < d.nz - 1) q += t * xp[id + 1];
+                out = xc + om * ((bv - q) / c);
+            }
+        }
+        const int nxt = sw == 0 ? 0 : cur ^ 1;
+        xs[nxt][id] = out;
+        cur = nxt;
+        __syncthreads();
+    }
+    if (live) x[id] = sweeps > 0 ? xs[cur][id] : T(0);
+}
+
 }  // namespace
 
 // ------------------------------------------------------------------
@@ -2713,6 +2900,75 @@ hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double
 }
 
 // explicit instantiations
+
+// ---- multigrid preconditioner ----
+hipError_t launch_mg_coarsen(hipStream_t s, const Dims& f, const Dims& cd, Stencil<double> A,
+                             const double* W, double* const Ac[7])
+{
+    const Chunking ch = make_chunking(cd);
+    Arr7<double> out;
+    for (int q = 0; q < 7; ++q) out.a[q] = Ac[q];
+    hipLaunchKernelGGL(k_mg_coarsen, dim3(ch.grid), dim3(kBlock), 0, s, f, cd, ch, A, W, out);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_mg_smooth(hipStream_t s, const Dims& d, Stencil<T> A, const T* x, const T* b,
+                            const T* e, const Dims& cd, T* x_out, int mode, int epi,
+                            double omega, double scale, RedScratch red, SolveScalars* sc,
+                            const double* wgt)
+{
+    const Chunking ch = make_chunking(d);
+    const bool rz = epi >= 0;
+#define LES_MGS(CD, MD, RZ)                                                                      \
+    hipLaunchKernelGGL((k_mg_smooth<T, CD, MD, RZ>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, \
+                       x, b, e, cd, x_out, omega, scale, red, sc, wgt)
+#define LES_MGS_M(CD, RZ)                             \
+    do {                                              \
+        if (mode == MG_ZERO) LES_MGS(CD, MG_ZERO, RZ);     \
+        else if (mode == MG_SWEEP) LES_MGS(CD, MG_SWEEP, RZ); \
+        else LES_MGS(CD, MG_PROLONG, RZ);                 \
+    } while (0)
+    if (A.code) {
+        if (rz) LES_MGS_M(1, true);
+        else LES_MGS_M(1, false);
+    } else {
+        if (rz) LES_MGS_M(0, true);
+        else LES_MGS_M(0, false);
+    }
+#undef LES_MGS_M
+#undef LES_MGS
+    if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+    if (rz)
+        return launch_finalize(s, ch.grid, 1, false, epi, sc ? GATE_DONE : GATE_NONE, red, sc,
+                               nullptr);
+    return hipSuccess;
+}
+
+template <typename T>
+hipError_t launch_mg_restrict(hipStream_t s, const Dims& f, const Dims& cd, Stencil<T> A,
+                              const T* x, const T* b, T* bc, SolveScalars* sc, const double* W)
+{
+    const Chunking ch = make_chunking(cd);
+    if (A.code)
+        hipLaunchKernelGGL((k_mg_restrict<T, 1>), dim3(ch.grid), dim3(kBlock), 0, s, f, cd, ch, A,
+                           x, b, bc, sc, W);
+    else
+        hipLaunchKernelGGL((k_mg_restrict<T, 0>), dim3(ch.grid), dim3(kBlock), 0, s, f, cd, ch, A,
+                           x, b, bc, sc, W);
+    return hipGetLastError();
+}
+
+template <typename T>
+hipError_t launch_mg_coarsest(hipStream_t s, const Dims& d, Stencil<T> A, const T* b, T* x,
+                              int sweeps, double omega, SolveScalars* sc)
+{
+    if (d.N > kBlock || A.code) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mg_coarsest<T>, dim3(1), dim3(kBlock), 0, s, d, A, b, x, sweeps, omega,
+                       sc);
+    return hipGetLastError();
+}
+
 #define LES_INST(T)                                                                              \
     template hipError_t launch_matvec<T>(hipStream_t, const Dims&, Stencil<T>, const T*, T*);   \
     template hipError_t launch_pcg_init<T>(hipStream_t, const Dims&, Stencil<T>, const T*,      \
@@ -2744,7 +3000,16 @@ hipError_t launch_scale_by_invnorm(hipStream_t s, int n, const double* w, double
                                                const SolveScalars*);                            \
     template hipError_t launch_ic_bwd_level<T>(hipStream_t, const Dims&, int, T*, const T*,     \
                                                const T*, const T*, const T*,                    \
-                                               const SolveScalars*);
+                                               const SolveScalars*);                            \
+    template hipError_t launch_mg_smooth<T>(hipStream_t, const Dims&, Stencil<T>, const T*,     \
+                                            const T*, const T*, const Dims&, T*, int, int,      \
+                                            double, double, RedScratch, SolveScalars*,          \
+                                            const double*);                                     \
+    template hipError_t launch_mg_restrict<T>(hipStream_t, const Dims&, const Dims&, Stencil<T>, \
+                                              const T*, const T*, T*, SolveScalars*,            \
+                                              const double*);                                   \
+    template hipError_t launch_mg_coarsest<T>(hipStream_t, const Dims&, Stencil<T>, const T*,   \
+                                              T*, int, double, SolveScalars*);
 
 LES_INST(double)
 LES_INST(float)

@@ -24,13 +24,14 @@ FIELDS = ("u", "v", "w", "u_new", "v_new", "w_new", "p", "p_rhs", "nu_t", "b")
 FIELD_ID = {n: i for i, n in enumerate(FIELDS)}
 STENCIL_A, STENCIL_L = 0, 1
 STENCIL_AUTO, STENCIL_ARRAYS, STENCIL_CODED = 0, 1, 2
-PC_NONE, PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV = 0, 1, 2, 3, 4
+PC_NONE, PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV, PC_MG = 0, 1, 2, 3, 4, 5
 PREC_FP64, PREC_MIXED = 0, 1
 SOLVER_PCG, SOLVER_SOR_LEX, SOLVER_SOR_RB, SOLVER_JACOBI = 0, 1, 2, 3
 KT_NONE, KT_CHEB_STEP, KT_PCG_MATVEC, KT_PCG_UPDATE, KT_MOMENTUM, KT_IC_SOLVE = 0, 1, 2, 3, 4, 5
-KT_CHEB_PAIR, KT_STATS, KT_IC_FACTOR, KT_SOR_SWEEP = 6, 7, 8, 9
+KT_CHEB_PAIR, KT_STATS, KT_IC_FACTOR, KT_SOR_SWEEP, KT_MG_SWEEP = 6, 7, 8, 9, 10
 IP_EUCLID, IP_MIRROR = 0, 1
 OPT_CHEB_PAIRS, OPT_PAIR_GRID, OPT_WAVEFRONT, OPT_SPIN_LIMIT = 1, 2, 3, 4
+OPT_MG_SWEEPS, OPT_MG_COARSE_SWEEPS = 5, 6
 ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_IO, ERR_STALL = -1, -2, -3, -4, -5, -6
 # statistics slots (statistics.h:15-31): means of u, v, w, p, then the sums
 # of the six products

@@ -376,6 +376,60 @@ static double wdot(const double* a, const double* b, const double* W, long n)
     return sum;
 }
 
+/* The PCG loop of pressure.cpp:342-414, with the preconditioner as a
+ * callback: apply(ctx, z, r). */
+typedef void (*orc_precond_fn)(void* ctx, double* z, const double* r);
+
+struct std_pc {
+    int pc;
+    double* const* A;
+    double* const* L;
+    int nx, ny, nz, degree;
+    double lmin, lmax;
+};
+
+static void std_pc_apply(void* ctx, double* z, const double* r)
+{
+    const struct std_pc* c = (const struct std_pc*)ctx;
+    orc_precond(c->pc, z, r, c->A, c->L, c->nx, c->ny, c->nz, c->lmin, c->lmax, c->degree);
+}
+
+static int pcg_core(double* const A[7], const double* b, double* x, int nx, int ny, int nz,
+                    orc_precond_fn apply, void* ctx, int max_iter, double tol,
+                    double* res_trace, const double* W)
+{
+    const long n = (long)nx * ny * nz;
+    double* r = (double*)calloc(n, sizeof(double));
+    double* z = (double*)calloc(n, sizeof(double));
+    double* p = (double*)calloc(n, sizeof(double));
+    double* q = (double*)calloc(n, sizeof(double));
+    int ret = max_iter;
+    orc_matvec(q, x, A, nx, ny, nz);
+    orc_saxpy(r, 1.0, b, -1.0, q, n);
+    double err0 = orc_norm(r, n);
+    if (res_trace) res_trace[0] = err0;
+    if (err0 < tol) { ret = 0; goto out; }
+    apply(ctx, p, r);
+    double rho = wdot(r, p, W, n);
+    for (int k = 1; k <= max_iter; ++k) {
+        orc_matvec(q, p, A, nx, ny, nz);
+        double alpha = rho / wdot(p, q, W, n);
+        orc_axpy(x, alpha, p, n);
+        orc_axpy(r, -alpha, q, n);
+        double err = orc_norm(r, n);
+        if (res_trace) res_trace[k] = err;
+        if (err < tol) { ret = k; goto out; }
+        apply(ctx, z, r);
+        double rho_new = wdot(r, z, W, n);
+        double beta = rho_new / rho;
+        orc_saxpy(p, 1.0, z, beta, p, n);
+        rho = rho_new;
+    }
+out:
+    free(r); free(z); free(p); free(q);
+    return ret;
+}
+
 int orc_pcg(double* const A[7], const double* b, double* x, int nx, int ny, int nz,
             int precond, double* const L[7], double lmin, double lmax, int degree,
             int max_iter, double tol, double* res_trace)
@@ -391,36 +445,238 @@ int orc_pcg_w(double* const A[7], const double* b, double* x, int nx, int ny, in
               int precond, double* const L[7], double lmin, double lmax, int degree,
               int max_iter, double tol, double* res_trace, const double* W)
 {
-    const long n = (long)nx * ny * nz;
-    double* r = (double*)calloc(n, sizeof(double));
-    double* z = (double*)calloc(n, sizeof(double));
-    double* p = (double*)calloc(n, sizeof(double));
-    double* q = (double*)calloc(n, sizeof(double));
-    int ret = max_iter;
-    orc_matvec(q, x, A, nx, ny, nz);
-    orc_saxpy(r, 1.0, b, -1.0, q, n);
-    double err0 = orc_norm(r, n);
-    if (res_trace) res_trace[0] = err0;
-    if (err0 < tol) { ret = 0; goto out; }
-    orc_precond(precond, p, r, A, L, nx, ny, nz, lmin, lmax, degree);
-    double rho = wdot(r, p, W, n);
-    for (int k = 1; k <= max_iter; ++k) {
-        orc_matvec(q, p, A, nx, ny, nz);
-        double alpha = rho / wdot(p, q, W, n);
-        orc_axpy(x, alpha, p, n);
-        orc_axpy(r, -alpha, q, n);
-        double err = orc_norm(r, n);
-        if (res_trace) res_trace[k] = err;
-        if (err < tol) { ret = k; goto out; }
-        orc_precond(precond, z, r, A, L, nx, ny, nz, lmin, lmax, degree);
-        double rho_new = wdot(r, z, W, n);
-        double beta = rho_new / rho;
-        orc_saxpy(p, 1.0, z, beta, p, n);
-        rho = rho_new;
+    if (precond == ORC_PC_MG) {
+        const orc_mg_params prm = orc_mg_default_params();
+        return orc_pcg_mg(A, b, x, nx, ny, nz, &prm, max_iter, tol, res_trace, W);
     }
-out:
-    free(r); free(z); free(p); free(q);
-    return ret;
+    struct std_pc c = {precond, A, L, nx, ny, nz, degree, lmin, lmax};
+    return pcg_core(A, b, x, nx, ny, nz, std_pc_apply, &c, max_iter, tol, res_trace, W);
+}
+
+/* ------------------------------------------------------------------ */
+/* multigrid preconditioner (no reference counterpart; README.md:53)   */
+/* ------------------------------------------------------------------ */
+orc_mg_params orc_mg_default_params(void)
+{
+    orc_mg_params p;
+    p.sweeps = 2;
+    p.coarse_sweeps = 32;
+    p.coarse_max = 4;
+    p.omega = 0.85;
+    p.scale = 1.8;
+    return p;
+}
+
+int orc_mg_levels(int nx, int ny, int nz, int coarse_max, int* dims)
+{
+    int l = 0;
+    dims[0] = nx; dims[1] = ny; dims[2] = nz;
+    while (l + 1 < ORC_MG_MAX_LEVELS) {
+        const int* d = dims + 3 * l;
+        int mx = d[0] > d[1] ? d[0] : d[1];
+        if (d[2] > mx) mx = d[2];
+        if (mx <= coarse_max) break;
+        dims[3 * (l + 1) + 0] = (d[0] + 1) / 2;
+        dims[3 * (l + 1) + 1] = (d[1] + 1) / 2;
+        dims[3 * (l + 1) + 2] = (d[2] + 1) / 2;
+        ++l;
+    }
+    return l + 1;
+}
+
+void orc_mg_coarsen(double* const Ac[7], double* const A[7], const double* W, int nx, int ny,
+                    int nz)
+{
+    const int NX = (nx + 1) / 2, NY = (ny + 1) / 2, NZ = (nz + 1) / 2;
+    const int sij = ny * nz;
+    for (int I = 0; I < NX; ++I)
+        for (int J = 0; J < NY; ++J)
+            for (int K = 0; K < NZ; ++K) {
+                double c = 0.0, cn = 0.0, cs = 0.0, ce = 0.0, cw = 0.0, ct = 0.0, cb = 0.0;
+                for (int di = 0; di < 2; ++di)
+                    for (int dj = 0; dj < 2; ++dj)
+                        for (int dk = 0; dk < 2; ++dk) {
+                            const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
+                            if (i >= nx || j >= ny || k >= nz) continue;
+                            const int id = IDX(i, j, k);
+                            /* a fine coefficient, times W of its row when weighted */
+#define WA(q) (W ? W[id] * A[q][id] : A[q][id])
+                            c += WA(ORC_C);
+                            /* member order n, s, e, w, t, b: inside the block
+                             * to the centre, across its face to that face */
+                            if (j + 1 < ny) { if (dj == 0) c += WA(ORC_N); else cn += WA(ORC_N); }
+                            if (j > 0) { if (dj == 1) c += WA(ORC_S); else cs += WA(ORC_S); }
+                            if (i + 1 < nx) { if (di == 0) c += WA(ORC_E); else ce += WA(ORC_E); }
+                            if (i > 0) { if (di == 1) c += WA(ORC_W); else cw += WA(ORC_W); }
+                            if (k + 1 < nz) { if (dk == 0) c += WA(ORC_T); else ct += WA(ORC_T); }
+                            if (k > 0) { if (dk == 1) c += WA(ORC_B); else cb += WA(ORC_B); }
+#undef WA
+                        }
+                const int cid = (I * NY + J) * NZ + K;
+                Ac[ORC_C][cid] = c;
+                Ac[ORC_N][cid] = cn;
+                Ac[ORC_S][cid] = cs;
+                Ac[ORC_E][cid] = ce;
+                Ac[ORC_W][cid] = cw;
+                Ac[ORC_T][cid] = ct;
+                Ac[ORC_B][cid] = cb;
+            }
+}
+
+typedef struct {
+    int nlev;
+    int dims[3 * ORC_MG_MAX_LEVELS];
+    double* A[ORC_MG_MAX_LEVELS][7];  /* level 0 borrows the caller's */
+    double* x[ORC_MG_MAX_LEVELS];     /* level iterates (level 0: the output z) */
+    double* t[ORC_MG_MAX_LEVELS];     /* second buffer of a sweep */
+    double* b[ORC_MG_MAX_LEVELS];     /* right-hand sides of levels >= 1 */
+    const double* W;                  /* NULL, or the mirror weights of level 0 */
+    orc_mg_params prm;
+} orc_mg;
+
+static long mg_n(const orc_mg* g, int l)
+{
+    return (long)g->dims[3 * l] * g->dims[3 * l + 1] * g->dims[3 * l + 2];
+}
+
+static void mg_free(orc_mg* g)
+{
+    for (int l = 0; l < g->nlev; ++l) {
+        if (l > 0)
+            for (int q = 0; q < 7; ++q) free(g->A[l][q]);
+        free(g->t[l]);
+        if (l > 0) { free(g->x[l]); free(g->b[l]); }
+    }
+}
+
+static void mg_build(orc_mg* g, double* const A[7], int nx, int ny, int nz,
+                     const orc_mg_params* prm, const double* W)
+{
+    memset(g, 0, sizeof(*g));
+    g->prm = *prm;
+    g->W = W;
+    g->nlev = orc_mg_levels(nx, ny, nz, prm->coarse_max, g->dims);
+    for (int q = 0; q < 7; ++q) g->A[0][q] = A[q];
+    for (int l = 0; l < g->nlev; ++l) {
+        const long n = mg_n(g, l);
+        g->t[l] = (double*)calloc(n, sizeof(double));
+        if (l == 0) continue;
+        for (int q = 0; q < 7; ++q) g->A[l][q] = (double*)calloc(n, sizeof(double));
+        g->x[l] = (double*)calloc(n, sizeof(double));
+        g->b[l] = (double*)calloc(n, sizeof(double));
+        const int* d = g->dims + 3 * (l - 1);
+        orc_mg_coarsen(g->A[l], g->A[l - 1], l == 1 ? W : NULL, d[0], d[1], d[2]);
+    }
+}
+
+/* (A x)[id] in fast_apply_laplacian's order (pressure.cpp:709-746) */
+static double mg_ax(double* const A[7], const double* x, int nx, int ny, int nz, int i, int j,
+                    int k)
+{
+    const int sij = ny * nz, id = IDX(i, j, k);
+    double val = A[ORC_C][id] * x[id];
+    if (i > 0) val += A[ORC_W][id] * x[id - sij];
+    if (i < nx - 1) val += A[ORC_E][id] * x[id + sij];
+    if (j > 0) val += A[ORC_S][id] * x[id - nz];
+    if (j < ny - 1) val += A[ORC_N][id] * x[id + nz];
+    if (k > 0) val += A[ORC_B][id] * x[id - 1];
+    if (k < nz - 1) val += A[ORC_T][id] * x[id + 1];
+    return val;
+}
+
+/* `sweeps` damped-Jacobi sweeps on level l into g->x[l]; from x = 0 when
+ * zero (x = omega (b / A_c) first) */
+static void mg_smooth(orc_mg* g, int l, double* x, const double* b, int sweeps, int zero)
+{
+    const int nx = g->dims[3 * l], ny = g->dims[3 * l + 1], nz = g->dims[3 * l + 2];
+    const int sij = ny * nz;
+    const long n = mg_n(g, l);
+    double* const* A = g->A[l];
+    const double om = g->prm.omega;
+    if (sweeps <= 0 && zero) memset(x, 0, (size_t)n * sizeof(double));
+    for (int s = 0; s < sweeps; ++s) {
+        if (s == 0 && zero) {
+            for (long id = 0; id < n; ++id) x[id] = om * (b[id] / A[ORC_C][id]);
+            continue;
+        }
+        double* t = g->t[l];
+        for (int i = 0; i < nx; ++i)
+            for (int j = 0; j < ny; ++j)
+                for (int k = 0; k < nz; ++k) {
+                    const int id = IDX(i, j, k);
+                    const double q = mg_ax(A, x, nx, ny, nz, i, j, k);
+                    t[id] = x[id] + om * ((b[id] - q) / A[ORC_C][id]);
+                }
+        memcpy(x, t, (size_t)n * sizeof(double));
+    }
+}
+
+static void mg_vcycle(orc_mg* g, int l, double* x, const double* b)
+{
+    const orc_mg_params* p = &g->prm;
+    if (l == g->nlev - 1) {
+        mg_smooth(g, l, x, b, p->coarse_sweeps, 1);
+        return;
+    }
+    const int nx = g->dims[3 * l], ny = g->dims[3 * l + 1], nz = g->dims[3 * l + 2];
+    const int NX = g->dims[3 * l + 3], NY = g->dims[3 * l + 4], NZ = g->dims[3 * l + 5];
+    const int sij = ny * nz;
+    double* const* A = g->A[l];
+    mg_smooth(g, l, x, b, p->sweeps, 1);
+    /* restriction of the residual: the sum over each block, children in
+     * (di, dj, dk) order */
+    double* bc = g->b[l + 1];
+    for (int I = 0; I < NX; ++I)
+        for (int J = 0; J < NY; ++J)
+            for (int K = 0; K < NZ; ++K) {
+                double acc = 0.0;
+                for (int di = 0; di < 2; ++di)
+                    for (int dj = 0; dj < 2; ++dj)
+                        for (int dk = 0; dk < 2; ++dk) {
+                            const int i = 2 * I + di, j = 2 * J + dj, k = 2 * K + dk;
+                            if (i >= nx || j >= ny || k >= nz) continue;
+                            const int id = IDX(i, j, k);
+                            const double res = b[id] - mg_ax(A, x, nx, ny, nz, i, j, k);
+                            acc += (l == 0 && g->W) ? g->W[id] * res : res;
+                        }
+                bc[(I * NY + J) * NZ + K] = acc;
+            }
+    double* xc = g->x[l + 1];
+    mg_vcycle(g, l + 1, xc, bc);
+    /* prolongation of the correction, scaled */
+    for (int i = 0; i < nx; ++i)
+        for (int j = 0; j < ny; ++j)
+            for (int k = 0; k < nz; ++k) {
+                const int id = IDX(i, j, k);
+                x[id] = x[id] + p->scale * xc[((i / 2) * NY + j / 2) * NZ + k / 2];
+            }
+    mg_smooth(g, l, x, b, p->sweeps, 0);
+}
+
+void orc_mg_apply(double* z, const double* r, double* const A[7], int nx, int ny, int nz,
+                  const orc_mg_params* prm, const double* W)
+{
+    orc_mg g;
+    mg_build(&g, A, nx, ny, nz, prm, W);
+    mg_vcycle(&g, 0, z, r);
+    mg_free(&g);
+}
+
+static void mg_pc_apply(void* ctx, double* z, const double* r)
+{
+    orc_mg* g = (orc_mg*)ctx;
+    mg_vcycle(g, 0, z, r);
+}
+
+int orc_pcg_mg(double* const A[7], const double* b, double* x, int nx, int ny, int nz,
+               const orc_mg_params* prm, int max_iter, double tol, double* res_trace,
+               const double* W)
+{
+    orc_mg g;
+    mg_build(&g, A, nx, ny, nz, prm, W);
+    const int it = pcg_core(A, b, x, nx, ny, nz, mg_pc_apply, &g, max_iter, tol, res_trace, W);
+    mg_free(&g);
+    return it;
 }
 
 /* ------------------------------------------------------------------ */

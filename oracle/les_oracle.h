@@ -33,7 +33,8 @@ enum {
     ORC_PC_JACOBI = 1,     /* z = r / A_c (pressure.cpp:420) */
     ORC_PC_IC0 = 2,        /* z = (L L^T)^-1 r with L from build_ic (pressure.cpp:442, 495) */
     ORC_PC_LEGACY_IC_A = 3,/* apply_ic_preconditioner(., ., A) as shipped (pressure.cpp:363, 396) */
-    ORC_PC_CHEBYSHEV = 4   /* apply_chebyshev_preconditioner (pressure.cpp:851) */
+    ORC_PC_CHEBYSHEV = 4,  /* apply_chebyshev_preconditioner (pressure.cpp:851) */
+    ORC_PC_MG = 5          /* multigrid V-cycle (no reference counterpart; README.md:53 TODO) */
 };
 
 void orc_set_threads(int n);
@@ -75,6 +76,52 @@ int orc_pcg(double* const A[7], const double* b, double* x, int nx, int ny, int 
 int orc_pcg_w(double* const A[7], const double* b, double* x, int nx, int ny, int nz,
               int precond, double* const L[7], double lmin, double lmax, int degree,
               int max_iter, double tol, double* res_trace, const double* W);
+
+/* ---- multigrid preconditioner (no reference counterpart: the reference's
+ * README.md:53 lists it as a TODO) ----
+ * Galerkin aggregation multigrid on the 7-point operator.  Level l+1 merges
+ * 2x2x2 blocks of level l (ceil(n/2) cells per direction); prolongation P is
+ * piecewise constant, restriction is P^T (a sum over the block), and the
+ * coarse operator is P^T A P, again a 7-point stencil: a fine coupling inside
+ * a block adds to the coarse centre, one across a block face to that face's
+ * coefficient.  Levels are added while the largest dimension exceeds
+ * coarse_max.  The preconditioner is one V-cycle from x = 0: on every level
+ * but the coarsest, `sweeps` damped-Jacobi sweeps x += omega (b - A x) / A_c
+ * (the first from x = 0 is x = omega (b / A_c)), the residual restricted,
+ * the coarse correction times `scale` added, `sweeps` more sweeps; on the
+ * coarsest, coarse_sweeps sweeps from 0.  A fixed linear operator, so PCG
+ * can use it.  The summation orders (children in (di, dj, dk) order, a
+ * child's couplings in StencilMatrix member order) are part of the
+ * definition: the GPU follows them bit for bit.
+ * With mirror weights W (W A symmetric: A without rows mirrored around
+ * interior obstacles), the cycle is the symmetric one of S = W A applied to
+ * W r: the level-0 residual is restricted as P^T (W (b - A x)) and level 1
+ * is P^T W A P (each fine coefficient times W of its row); damped Jacobi on
+ * A is damped Jacobi on S.  PCG in the W inner product with it is then CG
+ * on S with a symmetric preconditioner. */
+enum { ORC_MG_MAX_LEVELS = 16 };
+typedef struct {
+    int sweeps, coarse_sweeps, coarse_max;
+    double omega, scale;
+} orc_mg_params;
+/* the built-in parameters (sweeps 2, coarse_sweeps 32, coarse_max 4,
+ * omega 0.85, scale 1.8) */
+orc_mg_params orc_mg_default_params(void);
+/* level count and the dimensions of every level (dims[3*l + 0..2]) */
+int orc_mg_levels(int nx, int ny, int nz, int coarse_max, int* dims);
+/* Ac = P^T A P of one level (nx, ny, nz the fine dimensions); with W
+ * (may be NULL) P^T W A P */
+void orc_mg_coarsen(double* const Ac[7], double* const A[7], const double* W, int nx, int ny,
+                    int nz);
+/* z = V-cycle(r) on the hierarchy built from A (built, applied, freed); W
+ * NULL or the mirror weights */
+void orc_mg_apply(double* z, const double* r, double* const A[7], int nx, int ny, int nz,
+                  const orc_mg_params* prm, const double* W);
+/* PCG with the multigrid preconditioner (orc_pcg's conventions), in the W
+ * inner product when W != NULL */
+int orc_pcg_mg(double* const A[7], const double* b, double* x, int nx, int ny, int nz,
+               const orc_mg_params* prm, int max_iter, double tol, double* res_trace,
+               const double* W);
 
 /* ---- stationary solvers (pressure.cpp:14-247) ---- */
 void orc_apply_pressure_bcs(double* p, const uint8_t* solid, const orc_mesh* m);

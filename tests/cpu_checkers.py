@@ -23,7 +23,7 @@ DP = C.POINTER(C.c_double)
 U8P = C.POINTER(C.c_uint8)
 IP = C.POINTER(C.c_int)
 
-PC_NONE, PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV = 0, 1, 2, 3, 4
+PC_NONE, PC_JACOBI, PC_IC0, PC_LEGACY_IC_A, PC_CHEBYSHEV, PC_MG = 0, 1, 2, 3, 4, 5
 
 
 def _p(a):
@@ -51,6 +51,11 @@ def _p10(arrs):
 class OrcMesh(C.Structure):
     _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
                 ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double)]
+
+
+class OrcMgParams(C.Structure):
+    _fields_ = [("sweeps", C.c_int), ("coarse_sweeps", C.c_int), ("coarse_max", C.c_int),
+                ("omega", C.c_double), ("scale", C.c_double)]
 
 
 class OrcFields(C.Structure):
@@ -116,6 +121,14 @@ class Oracle:
                                    C.c_double]
         L.orc_correct.argtypes = [DP, DP, DP, DP, C.POINTER(OrcMesh), C.c_double, C.c_double]
         L.orc_estimate_lambda_max.argtypes = [C.POINTER(DP), C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_mg_default_params.restype = OrcMgParams
+        L.orc_mg_levels.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_int)]
+        L.orc_mg_coarsen.argtypes = [C.POINTER(DP), C.POINTER(DP), DP, C.c_int, C.c_int,
+                                     C.c_int]
+        L.orc_mg_apply.argtypes = [DP, DP, C.POINTER(DP), C.c_int, C.c_int, C.c_int,
+                                   C.POINTER(OrcMgParams), DP]
+        L.orc_pcg_mg.argtypes = [C.POINTER(DP), DP, DP, C.c_int, C.c_int, C.c_int,
+                                 C.POINTER(OrcMgParams), C.c_int, C.c_double, DP, DP]
         L.orc_step.argtypes = [C.POINTER(OrcFields), C.POINTER(OrcMesh), C.POINTER(OrcStepParams),
                                C.POINTER(OrcStepStats)]
 
@@ -197,6 +210,39 @@ class Oracle:
                               max_iter, tol, _p(tr))
         return x, it, tr[: it + 1]
 
+    # ---- multigrid preconditioner (no reference counterpart) ----
+    def mg_params(self, **kw):
+        p = self.lib.orc_mg_default_params()
+        for k, v in kw.items():
+            setattr(p, k, v)
+        return p
+
+    def mg_levels(self, m, coarse_max=4):
+        dims = (C.c_int * 48)()
+        nl = self.lib.orc_mg_levels(*_dims(m), coarse_max, dims)
+        return [tuple(dims[3 * l:3 * l + 3]) for l in range(nl)]
+
+    def mg_coarsen(self, A, dims, W=None):
+        nx, ny, nz = dims
+        nc = ((nx + 1) // 2) * ((ny + 1) // 2) * ((nz + 1) // 2)
+        Ac = [np.zeros(nc) for _ in range(7)]
+        self.lib.orc_mg_coarsen(_p7(Ac), _p7(A), _p(W), nx, ny, nz)
+        return Ac
+
+    def mg_apply(self, r, A, m, W=None, **kw):
+        z = np.zeros_like(r)
+        prm = self.mg_params(**kw)
+        self.lib.orc_mg_apply(_p(z), _p(r), _p7(A), *_dims(m), C.byref(prm), _p(W))
+        return z
+
+    def pcg_mg(self, A, b, x0, m, max_iter=1000, tol=1e-6, W=None, **kw):
+        x = x0.copy()
+        tr = np.zeros(max_iter + 1)
+        prm = self.mg_params(**kw)
+        it = self.lib.orc_pcg_mg(_p7(A), _p(b), _p(x), *_dims(m), C.byref(prm), max_iter, tol,
+                                 _p(tr), _p(W))
+        return x, it, tr[: it + 1]
+
     def mirror_weights(self, A, m):
         """W = 1/2 per direction in which a row of A is mirrored."""
         W = np.zeros(_ncell(m))
@@ -272,7 +318,7 @@ class Oracle:
 
     def run_steps(self, state, solid, m, A, nsteps, step0=0, precond=PC_LEGACY_IC_A, nu=1e-3,
                   rho=1.2, Cs=0.18, dt_fixed=0.0, lmin=0.0, lmax=0.0, degree=5, max_iter=1000,
-                  tol=1e-6, tol_rel=False, L=None):
+                  tol=1e-6, tol_rel=False, L=None, W=None):
         st = {k: state[k].copy() for k in STATE_KEYS}
         solid = np.ascontiguousarray(solid)
         f = OrcFields(*[_p(st[k]) for k in STATE_KEYS], _p(solid))
@@ -281,7 +327,7 @@ class Oracle:
         iters, dts = [], []
         for s in range(nsteps):
             prm = OrcStepParams(nu, rho, Cs, step0 + s, dt_fixed, precond, degree, max_iter, lmin,
-                                lmax, tol, 1 if tol_rel else 0, L7, A7, None)
+                                lmax, tol, 1 if tol_rel else 0, L7, A7, _p(W))
             stats = OrcStepStats()
             self.lib.orc_step(C.byref(f), self._m(m), C.byref(prm), C.byref(stats))
             iters.append(stats.iters)
