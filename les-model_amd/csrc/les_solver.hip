@@ -1023,6 +1023,11 @@ struct PairGeom {
 // The staged forms of the pair and of the reducing last step
 // (k_cheb_pair_st) where they apply: FP32, coded operator, rows inside one
 // wave.
+// Multigrid sweeps and restriction with vector loads where the rows allow
+// (nz % V == 0), else one cell per thread.
+#ifndef LES_MG_VEC
+#define LES_MG_VEC 1
+#endif
 #ifndef LES_PAIR_STAGED
 #define LES_PAIR_STAGED 1
 #endif
@@ -2095,6 +2100,132 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(Dims f, Dims cd, Chunkin
     }
 }
 
+// The diagonal of V cells (the coded centre, or A_c)
+template <int CODED, typename T, int V>
+__device__ __forceinline__ void diag7v(const Stencil<T>& A, int id, T (&dg)[V])
+{
+    if constexpr (CODED == 1) {
+        uint16_t cd[V];
+        vload<uint16_t, V>(A.code + id, cd);
+#pragma unroll
+        for (int v = 0; v < V; ++v) dg[v] = (cd[v] & CODE_IDENTITY) ? T(1) : A.cfluid;
+    } else {
+        vload<T, V>(A.a[SC] + id, dg);
+    }
+}
+
+// k_mg_smooth with V consecutive cells of a row per thread (16-byte loads;
+// k-neighbours by lane shifts as in the Chebyshev step).  In MG_PROLONG each
+// neighbour vector is corrected by scale * e of its cells' blocks (cells
+// k..k+V-1 lie in blocks k/2 ..), the k-edge values through the shifted
+// corrected centres.  Each cell's arithmetic is k_mg_smooth's.  nz % V == 0.
+template <typename T, int CODED, int MODE, bool RZ, int V>
+__global__ __launch_bounds__(kBlock) void k_mg_smooth_v(Dims d, Chunking ch, Stencil<T> A,
+                                                        const T* __restrict__ x,
+                                                        const T* __restrict__ b,
+                                                        const T* __restrict__ e, Dims cd,
+                                                        T* __restrict__ x_out, double omega,
+                                                        double scale, RedScratch red,
+                                                        const SolveScalars* sc,
+                                                        const double* __restrict__ wgt)
+{
+    if (is_done(sc)) return;
+    const int lb = logical_block(ch.grid);
+    const T om = (T)omega, sf = (T)scale;
+    double acc[1] = {0.0};
+    FOR_VCELLS(d, ch, lb, V)
+    {
+        T bv[V], dg[V], out[V];
+        vload_s<T, V>(b + id, bv);
+        diag7v<CODED, T, V>(A, id, dg);
+        if constexpr (MODE == MG_ZERO) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) out[v] = om * (bv[v] / dg[v]);
+        } else {
+            Nbr<T, V> nb;
+            if constexpr (MODE == MG_PROLONG) {
+                // x + scale * e at the cells (ii, jj, kk..), e of their blocks
+                auto eat = [&](int ii, int jj, int kk) {
+                    return e[((ii >> 1) * cd.ny + (jj >> 1)) * cd.nz + (kk >> 1)];
+                };
+                auto corr = [&](int ii, int jj, T (&vals)[V]) {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) vals[v] = vals[v] + sf * eat(ii, jj, k + v);
+                };
+                load_nbr5<T, V>(x, d, id, i, j, nb);
+                corr(i, j, nb.c);
+                if (i > 0) corr(i - 1, j, nb.w);
+                if (i < d.nx - 1) corr(i + 1, j, nb.e);
+                if (j > 0) corr(i, j - 1, nb.s);
+                if (j < d.ny - 1) corr(i, j + 1, nb.n);
+                row_edges<T, V, true>(d, id, k, nb.c, nb.km, nb.kp, [&](int n) {
+                    return x[n] + sf * eat(i, j, k + (n - id));
+                });
+            } else {
+                load_nbr<T, V>(x, d, id, i, j, k, [](T v) { return v; }, nb);
+            }
+            T q[V];
+            apply7v<CODED, T, V>(A, d, id, i, j, k, nb, q);
+#pragma unroll
+            for (int v = 0; v < V; ++v) out[v] = nb.c[v] + om * ((bv[v] - q[v]) / dg[v]);
+        }
+        vstore_s<T, V>(x_out + id, out);
+        if (RZ) {
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const double bz = (double)bv[v] * (double)out[v];
+                acc[0] += wgt ? wgt[id + v] * bz : bz;
+            }
+        }
+    }
+    if (RZ) block_partial<1>(acc, red.partials, lb);
+}
+
+// k_mg_restrict with two coarse cells of a row per thread: the four fine
+// rows of their blocks are read four cells at a time (k-neighbours by lane
+// shifts), the residual formed per cell as apply7 forms it, and each coarse
+// cell sums its children in (di, dj, dk) order.  Fine nz % 4 == 0 (so the
+// coarse nz is even).
+template <typename T, int CODED>
+__global__ __launch_bounds__(kBlock) void k_mg_restrict_v(Dims f, Dims cd, Chunking ch,
+                                                          Stencil<T> A, const T* __restrict__ x,
+                                                          const T* __restrict__ b,
+                                                          T* __restrict__ bc,
+                                                          const SolveScalars* sc,
+                                                          const double* __restrict__ W)
+{
+    if (is_done(sc)) return;
+    const int lb = logical_block(ch.grid);
+    FOR_VCELLS(cd, ch, lb, 2)
+    {
+        T acc[2] = {T(0), T(0)};
+        const int fk = 2 * k;
+        for (int di = 0; di < 2; ++di)
+            for (int dj = 0; dj < 2; ++dj) {
+                const int fi = 2 * i + di, fj = 2 * j + dj;
+                // a missing fine row contributes nothing, but its lanes still
+                // take part in the k-edge shifts (row_edges reads neighbours'
+                // lanes); only the sums skip it
+                const bool live = fi < f.nx && fj < f.ny;
+                const int ci = live ? fi : 2 * i, cj = live ? fj : 2 * j;
+                const int fid = (ci * f.ny + cj) * f.nz + fk;
+                Nbr<T, 4> nb;
+                load_nbr<T, 4>(x, f, fid, ci, cj, fk, [](T v) { return v; }, nb);
+                T q[4], bv[4];
+                apply7v<CODED, T, 4>(A, f, fid, ci, cj, fk, nb, q);
+                vload<T, 4>(b + fid, bv);
+                if (!live) continue;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const T res = bv[v] - q[v];
+                    acc[v >> 1] += W ? (T)W[fid + v] * res : res;
+                }
+            }
+        T o[2] = {acc[0], acc[1]};
+        vstore<T, 2>(bc + id, o);
+    }
+}
+
 // The coarsest level: `sweeps` damped-Jacobi sweeps from x = 0 in one
 // workgroup, the iterate in LDS (at most kBlock cells)
 template <typename T>
@@ -2918,11 +3049,19 @@ hipError_t launch_mg_smooth(hipStream_t s, const Dims& d, Stencil<T> A, const T*
                             double omega, double scale, RedScratch red, SolveScalars* sc,
                             const double* wgt)
 {
-    const Chunking ch = make_chunking(d);
+    constexpr int VW = vec_width<T>();
+    const bool vec = LES_MG_VEC && pick_width<T>(d) == VW;
+    const Chunking ch = make_chunking(d, vec ? VW : 1);
     const bool rz = epi >= 0;
-#define LES_MGS(CD, MD, RZ)                                                                      \
-    hipLaunchKernelGGL((k_mg_smooth<T, CD, MD, RZ>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, \
-                       x, b, e, cd, x_out, omega, scale, red, sc, wgt)
+#define LES_MGS(CD, MD, RZ)                                                                       \
+    do {                                                                                          \
+        if (vec)                                                                                  \
+            hipLaunchKernelGGL((k_mg_smooth_v<T, CD, MD, RZ, VW>), dim3(ch.grid), dim3(kBlock), 0, \
+                               s, d, ch, A, x, b, e, cd, x_out, omega, scale, red, sc, wgt);      \
+        else                                                                                      \
+            hipLaunchKernelGGL((k_mg_smooth<T, CD, MD, RZ>), dim3(ch.grid), dim3(kBlock), 0, s, d, \
+                               ch, A, x, b, e, cd, x_out, omega, scale, red, sc, wgt);            \
+    } while (0)
 #define LES_MGS_M(CD, RZ)                             \
     do {                                              \
         if (mode == MG_ZERO) LES_MGS(CD, MG_ZERO, RZ);     \
@@ -2949,6 +3088,16 @@ template <typename T>
 hipError_t launch_mg_restrict(hipStream_t s, const Dims& f, const Dims& cd, Stencil<T> A,
                               const T* x, const T* b, T* bc, SolveScalars* sc, const double* W)
 {
+    if (LES_MG_VEC && f.nz % 4 == 0) {
+        const Chunking ch = make_chunking(cd, 2);
+        if (A.code)
+            hipLaunchKernelGGL((k_mg_restrict_v<T, 1>), dim3(ch.grid), dim3(kBlock), 0, s, f, cd,
+                               ch, A, x, b, bc, sc, W);
+        else
+            hipLaunchKernelGGL((k_mg_restrict_v<T, 0>), dim3(ch.grid), dim3(kBlock), 0, s, f, cd,
+                               ch, A, x, b, bc, sc, W);
+        return hipGetLastError();
+    }
     const Chunking ch = make_chunking(cd);
     if (A.code)
         hipLaunchKernelGGL((k_mg_restrict<T, 1>), dim3(ch.grid), dim3(kBlock), 0, s, f, cd, ch, A,
