@@ -2100,6 +2100,20 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(Dims f, Dims cd, Chunkin
     }
 }
 
+// The mirror weight of a coded cell: 1/2 for each axis whose two
+// coefficients are a doubled face and a missing one (k_mirror_weights' rule;
+// products of 1/2, exact)
+__device__ __forceinline__ double code_weight(unsigned cd)
+{
+    double wt = 1.0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const unsigned lo = (cd >> (4 * ax)) & 3u, hi = (cd >> (4 * ax + 2)) & 3u;
+        if ((lo == 2u && hi == 0u) || (hi == 2u && lo == 0u)) wt *= 0.5;
+    }
+    return wt;
+}
+
 // The diagonal of V cells (the coded centre, or A_c)
 template <int CODED, typename T, int V>
 __device__ __forceinline__ void diag7v(const Stencil<T>& A, int id, T (&dg)[V])
@@ -2148,9 +2162,23 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth_v(Dims d, Chunking ch, Ste
                 auto eat = [&](int ii, int jj, int kk) {
                     return e[((ii >> 1) * cd.ny + (jj >> 1)) * cd.nz + (kk >> 1)];
                 };
+                // cells k..k+V-1 lie in blocks k/2 .. (k+V-1)/2: one load
+                // for V = 2, an aligned pair for V = 4 (k % 4 == 0)
                 auto corr = [&](int ii, int jj, T (&vals)[V]) {
+                    const T* er = e + ((ii >> 1) * cd.ny + (jj >> 1)) * cd.nz + (k >> 1);
+                    if constexpr (V == 4) {
+                        T ev[2];
+                        vload<T, 2>(er, ev);
 #pragma unroll
-                    for (int v = 0; v < V; ++v) vals[v] = vals[v] + sf * eat(ii, jj, k + v);
+                        for (int v = 0; v < V; ++v) vals[v] = vals[v] + sf * ev[v >> 1];
+                    } else if constexpr (V == 2) {
+                        const T ev = er[0];
+#pragma unroll
+                        for (int v = 0; v < V; ++v) vals[v] = vals[v] + sf * ev;
+                    } else {
+#pragma unroll
+                        for (int v = 0; v < V; ++v) vals[v] = vals[v] + sf * eat(ii, jj, k + v);
+                    }
                 };
                 load_nbr5<T, V>(x, d, id, i, j, nb);
                 corr(i, j, nb.c);
@@ -2171,10 +2199,15 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth_v(Dims d, Chunking ch, Ste
         }
         vstore_s<T, V>(x_out + id, out);
         if (RZ) {
+            // the coded operator's weights come from its code (the array was
+            // made from the same code)
+            uint16_t cw[V];
+            if (CODED == 1 && wgt) vload<uint16_t, V>(A.code + id, cw);
 #pragma unroll
             for (int v = 0; v < V; ++v) {
                 const double bz = (double)bv[v] * (double)out[v];
-                acc[0] += wgt ? wgt[id + v] * bz : bz;
+                const double wv = !wgt ? 1.0 : (CODED == 1 ? code_weight(cw[v]) : wgt[id + v]);
+                acc[0] += wgt ? wv * bz : bz;
             }
         }
     }
@@ -2215,10 +2248,14 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict_v(Dims f, Dims cd, Chunk
                 apply7v<CODED, T, 4>(A, f, fid, ci, cj, fk, nb, q);
                 vload<T, 4>(b + fid, bv);
                 if (!live) continue;
+                uint16_t cw[4];
+                if (CODED == 1 && W) vload<uint16_t, 4>(A.code + fid, cw);
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const T res = bv[v] - q[v];
-                    acc[v >> 1] += W ? (T)W[fid + v] * res : res;
+                    // the coded operator's weights from its code
+                    const double wv = !W ? 1.0 : (CODED == 1 ? code_weight(cw[v]) : W[fid + v]);
+                    acc[v >> 1] += W ? (T)wv * res : res;
                 }
             }
         T o[2] = {acc[0], acc[1]};
