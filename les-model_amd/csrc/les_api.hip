@@ -38,6 +38,11 @@ constexpr int kMgMaxLevels = 16;
 constexpr int kMgCoarseMax = 4;
 constexpr double kMgOmega = 0.85;
 constexpr double kMgScale = 1.8;
+// 1: the levels of at most kMgTailCells cells below level 0 run as one
+// launch (k_mg_tail); 0: five launches per level
+#ifndef LES_MG_TAIL
+#define LES_MG_TAIL 1
+#endif
 
 struct les_ctx {
     int device = 0;
@@ -494,6 +499,12 @@ double* mg_vec<double>(les_ctx::Mg& g, int l, int q) { return g.v64[l][q]; }
 template <>
 float* mg_vec<float>(les_ctx::Mg& g, int l, int q) { return g.v32[l][q]; }
 template <typename T>
+T* const* mg_arr(les_ctx::Mg& g, int l);
+template <>
+double* const* mg_arr<double>(les_ctx::Mg& g, int l) { return g.A64[l]; }
+template <>
+float* const* mg_arr<float>(les_ctx::Mg& g, int l) { return g.A32[l]; }
+template <typename T>
 Stencil<T> mg_op(les_ctx::Mg& g, int l);
 template <>
 Stencil<double> mg_op<double>(les_ctx::Mg& g, int l) { return stencil_of(g.A64[l]); }
@@ -529,8 +540,13 @@ int mg_vcycle(les_ctx* c, const Stencil<T>& A0, const T* r, T* z, T* t0, int epi
         }
         return LES_OK;
     }
+    // the bottom levels (from lt on, each small enough) run in one launch
+    int lt = L - 1;
+    while (lt > 1 && g.dims[lt - 1].N <= kMgTailCells && L - (lt - 1) <= kMgTailLevels) --lt;
+    const bool tail = LES_MG_TAIL && g.dims[lt].N <= kMgTailCells && L - lt <= kMgTailLevels;
+    if (!tail) lt = L - 1;
     T* cur[kMgMaxLevels] = {};
-    for (int l = 0; l < L - 1; ++l) {
+    for (int l = 0; l < lt; ++l) {
         for (int k = 1; k <= nu; ++k) {
             T* out = buf(l, (k + 1) & 1);
             // the level-0 plain sweeps are the timed class
@@ -543,10 +559,17 @@ int mg_vcycle(les_ctx* c, const Stencil<T>& A0, const T* r, T* z, T* t0, int epi
         HIPCHK(c, launch_mg_restrict<T>(s, g.dims[l], g.dims[l + 1], op(l), cur[l], bvec(l),
                                         mg_vec<T>(g, l + 1, 2), sc, l == 0 ? W : nullptr));
     }
-    cur[L - 1] = buf(L - 1, 1);
-    HIPCHK(c, launch_mg_coarsest<T>(s, g.dims[L - 1], op(L - 1), bvec(L - 1), cur[L - 1],
-                                    c->mg_coarse_sweeps, kMgOmega, sc));
-    for (int l = L - 2; l >= 0; --l) {
+    cur[lt] = buf(lt, 1);
+    if (tail) {
+        T* const* Al[kMgTailLevels];
+        for (int l = lt; l < L; ++l) Al[l - lt] = mg_arr<T>(g, l);
+        HIPCHK(c, launch_mg_tail<T>(s, L - lt, g.dims + lt, Al, bvec(lt), cur[lt], nu,
+                                    c->mg_coarse_sweeps, kMgOmega, kMgScale, sc));
+    } else {
+        HIPCHK(c, launch_mg_coarsest<T>(s, g.dims[L - 1], op(L - 1), bvec(L - 1), cur[L - 1],
+                                        c->mg_coarse_sweeps, kMgOmega, sc));
+    }
+    for (int l = lt - 1; l >= 0; --l) {
         for (int k = nu + 1; k <= 2 * nu; ++k) {
             T* out = buf(l, (k + 1) & 1);
             const bool last = l == 0 && k == 2 * nu;

@@ -201,6 +201,10 @@ hipError_t launch_cheb_scale(hipStream_t s, const Dims& d, const T* r, T* z, dou
                              int epi, RedScratch red, SolveScalars* sc,
                              const double* wgt = nullptr);
 
+// the cycle's bottom levels run in one workgroup up to this many cells each
+// (one per thread)
+constexpr int kMgTailCells = 1024;
+constexpr int kMgTailLevels = 8;
 // ---- multigrid preconditioner (no reference counterpart: the reference's
 // README.md:53 lists it as a TODO) ----
 // Galerkin aggregation over 2x2x2 blocks; the level dimensions halve,
@@ -223,6 +227,14 @@ template <typename T>
 hipError_t launch_mg_restrict(hipStream_t s, const Dims& f, const Dims& cd, Stencil<T> A,
                               const T* x, const T* b, T* bc, SolveScalars* sc,
                               const double* W = nullptr);
+// Levels dims[0..nlev-1] of the cycle's bottom (each at most kMgTailCells cells,
+// arrays format A[l][7]) in one workgroup: from b0 (level 0 of the tail's
+// right-hand side) to x0 (its result); the same arithmetic as the
+// per-level launches.
+template <typename T>
+hipError_t launch_mg_tail(hipStream_t s, int nlev, const Dims* dims, T* const* const* A,
+                          const T* b0, T* x0, int sweeps, int coarse_sweeps, double omega,
+                          double scale, SolveScalars* sc);
 // the coarsest level's sweeps from x = 0 in one workgroup (d.N <= kBlock,
 // arrays format)
 template <typename T>

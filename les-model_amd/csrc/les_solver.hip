@@ -2263,6 +2263,116 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict_v(Dims f, Dims cd, Chunk
     }
 }
 
+// The bottom of the V-cycle in one workgroup: levels l0 .. L-1 (each with
+// at most kMgTailCells = kBlock cells, so one cell per thread; l0 >= 1,
+// arrays format), their iterates, right-hand sides and coefficients in LDS.
+// b of level l0 comes from global memory (the restriction from level
+// l0 - 1), x of level l0 goes back (for its prolongation).  Every phase is
+// one pass over the level's cells with a barrier after it; each cell's
+// arithmetic is k_mg_smooth's and k_mg_restrict's, in their order.  Saves
+// the 5 launches per level that the small levels spend mostly waiting.
+template <typename T>
+struct MgTail {
+    int nlev;                        // levels in the tail (the last is the coarsest)
+    Dims d[kMgTailLevels];
+    Arr7<T> A[kMgTailLevels];
+    int off[kMgTailLevels];          // LDS offset (elements) of level l: x, t, b, 7 coefficients
+    const T* b0;
+    T* x0;
+    int sweeps, coarse_sweeps;
+    double omega, scale;
+};
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_mg_tail(MgTail<T> g, const SolveScalars* sc)
+{
+    if (is_done(sc)) return;
+    extern __shared__ unsigned char les_mg_smem[];
+    T* lds = reinterpret_cast<T*>(les_mg_smem);
+    const T om = (T)g.omega, sf = (T)g.scale;
+    const int c = threadIdx.x;  // this thread's cell on every level
+    auto X = [&](int l) { return lds + g.off[l]; };
+    auto Tb = [&](int l) { return lds + g.off[l] + g.d[l].N; };
+    auto B = [&](int l) { return lds + g.off[l] + 2 * g.d[l].N; };
+    auto Ac = [&](int l, int q) { return lds + g.off[l] + (3 + q) * g.d[l].N; };
+    auto ijk = [&](const Dims& d, int n, int& i, int& j, int& k) {
+        i = n / d.sij;
+        j = (n - i * d.sij) / d.nz;
+        k = n - i * d.sij - j * d.nz;
+    };
+    // A x at cell n of level l (apply7's order and guards)
+    auto ax = [&](int l, const T* x, int n, int i, int j, int k) {
+        const Dims& d = g.d[l];
+        T q = Ac(l, SC)[n] * x[n];
+        if (i > 0) q += Ac(l, SW)[n] * x[n - d.sij];
+        if (i < d.nx - 1) q += Ac(l, SE)[n] * x[n + d.sij];
+        if (j > 0) q += Ac(l, SS)[n] * x[n - d.nz];
+        if (j < d.ny - 1) q += Ac(l, SN)[n] * x[n + d.nz];
+        if (k > 0) q += Ac(l, SB)[n] * x[n - 1];
+        if (k < d.nz - 1) q += Ac(l, ST)[n] * x[n + 1];
+        return q;
+    };
+    const int L = g.nlev;
+    for (int l = 0; l < L; ++l)
+        if (c < g.d[l].N)
+            for (int q = 0; q < 7; ++q) Ac(l, q)[c] = g.A[l].a[q][c];
+    if (c < g.d[0].N) B(0)[c] = g.b0[c];
+    __syncthreads();
+    // one sweep of level l: mode 0 from zero into x; 1 plain; 2 with the
+    // correction from level l+1 (x corrected in place first, then a plain
+    // sweep, as the separate prolongation leaves it); the result is in X(l)
+    auto sweep = [&](int l, int mode) {
+        const Dims& d = g.d[l];
+        const bool live = c < d.N;
+        int i = 0, j = 0, k = 0;
+        if (live) ijk(d, c, i, j, k);
+        T* x = X(l);
+        if (mode == 0) {
+            if (live) x[c] = om * (B(l)[c] / Ac(l, SC)[c]);
+            __syncthreads();
+            return;
+        }
+        if (mode == 2) {
+            const Dims& cd = g.d[l + 1];
+            if (live) x[c] = x[c] + sf * X(l + 1)[((i >> 1) * cd.ny + (j >> 1)) * cd.nz + (k >> 1)];
+            __syncthreads();
+        }
+        if (live) Tb(l)[c] = x[c] + om * ((B(l)[c] - ax(l, x, c, i, j, k)) / Ac(l, SC)[c]);
+        __syncthreads();
+        if (live) x[c] = Tb(l)[c];
+        __syncthreads();
+    };
+    for (int l = 0; l < L - 1; ++l) {
+        sweep(l, 0);
+        for (int s2 = 1; s2 < g.sweeps; ++s2) sweep(l, 1);
+        // restriction into level l+1's b, children in (di, dj, dk) order
+        const Dims& f = g.d[l];
+        const Dims& cd = g.d[l + 1];
+        if (c < cd.N) {
+            int I, J, K;
+            ijk(cd, c, I, J, K);
+            T acc = T(0);
+            for (int di = 0; di < 2; ++di)
+                for (int dj = 0; dj < 2; ++dj)
+                    for (int dk = 0; dk < 2; ++dk) {
+                        const int fi = 2 * I + di, fj = 2 * J + dj, fk = 2 * K + dk;
+                        if (fi >= f.nx || fj >= f.ny || fk >= f.nz) continue;
+                        const int fid = (fi * f.ny + fj) * f.nz + fk;
+                        acc += B(l)[fid] - ax(l, X(l), fid, fi, fj, fk);
+                    }
+            B(l + 1)[c] = acc;
+        }
+        __syncthreads();
+    }
+    sweep(L - 1, 0);
+    for (int s2 = 1; s2 < g.coarse_sweeps; ++s2) sweep(L - 1, 1);
+    for (int l = L - 2; l >= 0; --l) {
+        sweep(l, 2);
+        for (int s2 = 1; s2 < g.sweeps; ++s2) sweep(l, 1);
+    }
+    if (c < g.d[0].N) g.x0[c] = X(0)[c];
+}
+
 // The coarsest level: `sweeps` damped-Jacobi sweeps from x = 0 in one
 // workgroup, the iterate in LDS (at most kBlock cells)
 template <typename T>
@@ -3146,6 +3256,40 @@ hipError_t launch_mg_restrict(hipStream_t s, const Dims& f, const Dims& cd, Sten
 }
 
 template <typename T>
+hipError_t launch_mg_tail(hipStream_t s, int nlev, const Dims* dims, T* const* const* A,
+                          const T* b0, T* x0, int sweeps, int coarse_sweeps, double omega,
+                          double scale, SolveScalars* sc)
+{
+    if (nlev < 1 || nlev > kMgTailLevels) return hipErrorInvalidValue;
+    MgTail<T> g{};
+    g.nlev = nlev;
+    size_t off = 0;
+    for (int l = 0; l < nlev; ++l) {
+        if (dims[l].N > kMgTailCells) return hipErrorInvalidValue;
+        g.d[l] = dims[l];
+        for (int q = 0; q < 7; ++q) g.A[l].a[q] = A[l][q];
+        g.off[l] = (int)off;
+        off += 10 * (size_t)dims[l].N;
+    }
+    const size_t lds = off * sizeof(T);
+    if (lds > kPairMaxLds) return hipErrorInvalidValue;
+    g.b0 = b0;
+    g.x0 = x0;
+    g.sweeps = sweeps;
+    g.coarse_sweeps = coarse_sweeps;
+    g.omega = omega;
+    g.scale = scale;
+    const void* fn = reinterpret_cast<const void*>(&k_mg_tail<T>);
+    if (lds > 64 * 1024) {
+        const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                                 (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k_mg_tail<T>, dim3(1), dim3(kBlock), lds, s, g, sc);
+    return hipGetLastError();
+}
+
+template <typename T>
 hipError_t launch_mg_coarsest(hipStream_t s, const Dims& d, Stencil<T> A, const T* b, T* x,
                               int sweeps, double omega, SolveScalars* sc)
 {
@@ -3195,7 +3339,10 @@ hipError_t launch_mg_coarsest(hipStream_t s, const Dims& d, Stencil<T> A, const 
                                               const T*, const T*, T*, SolveScalars*,            \
                                               const double*);                                   \
     template hipError_t launch_mg_coarsest<T>(hipStream_t, const Dims&, Stencil<T>, const T*,   \
-                                              T*, int, double, SolveScalars*);
+                                              T*, int, double, SolveScalars*);                  \
+    template hipError_t launch_mg_tail<T>(hipStream_t, int, const Dims*, T* const* const*,      \
+                                          const T*, T*, int, int, double, double,              \
+                                          SolveScalars*);
 
 LES_INST(double)
 LES_INST(float)
