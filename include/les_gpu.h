@@ -109,9 +109,18 @@ int les_synchronize(les_ctx* ctx);
  * preconditioner before and after each coarse correction (default 2,
  * 1..16); the correction is added times 1.8.  LES_OPT_MG_COARSE_SWEEPS:
  * sweeps on the coarsest level, at most 4 cells per direction (default 32,
- * 1..4096). */
+ * 1..4096).
+ * LES_OPT_REPLACE_DECADES: mixed precision also replaces the residual by
+ * the FP64 one once the recursive residual has fallen this many decades
+ * below the last true residual (default 3, 0..15; 0: only every
+ * reliable_every iterations, near the tolerance and at max_iter).  The
+ * FP32 recursion's drift is proportional to the residual it started from;
+ * with a preconditioner that gains a decade in a few iterations (LES_PC_MG)
+ * the periodic replacement comes too late and the search direction
+ * restarts. */
 enum { LES_OPT_CHEB_PAIRS = 1, LES_OPT_PAIR_GRID = 2, LES_OPT_WAVEFRONT = 3,
-       LES_OPT_SPIN_LIMIT = 4, LES_OPT_MG_SWEEPS = 5, LES_OPT_MG_COARSE_SWEEPS = 6 };
+       LES_OPT_SPIN_LIMIT = 4, LES_OPT_MG_SWEEPS = 5, LES_OPT_MG_COARSE_SWEEPS = 6,
+       LES_OPT_REPLACE_DECADES = 7 };
 int les_set_option(les_ctx* ctx, int option, int value);
 
 /* ---- fields (Field, src/field.h:9-41) ---- */
@@ -179,8 +188,9 @@ typedef struct les_pcg_params {
     int stencil_format;   /* 0 auto: coded when A came from les_build_stencil;
                              1 the 7 coefficient arrays; 2 coded (16 bits per cell) */
     int inner_product;    /* LES_IP_EUCLID: r.z and p.q as the reference forms them;
-                             LES_IP_MIRROR: weighted by the mirror weights W (FP64;
-                             preconditioner NONE, JACOBI or CHEBYSHEV) */
+                             LES_IP_MIRROR: weighted by the mirror weights W
+                             (preconditioner NONE, JACOBI, CHEBYSHEV or MG in FP64;
+                             MG in mixed precision) */
 } les_pcg_params;
 
 /* Inner product of the CG scalars.  W = 1/2 per direction in which a row of

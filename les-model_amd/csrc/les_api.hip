@@ -108,6 +108,7 @@ struct les_ctx {
     } mg;
     int mg_sweeps = 2;                // damped-Jacobi sweeps before and after the coarse correction
     int mg_coarse_sweeps = 32;        // sweeps on the coarsest level
+    int replace_decades = 3;          // mixed: replacement after this many decades of drop (0: off)
     bool cheb_pairs = true;           // fuse Chebyshev steps in pairs where the grid allows
     int pair_grid = 0;                // workgroups of a pair launch (0: built-in default)
     // lexicographic sweeps (IC(0), SOR): one wavefront launch each, or one
@@ -782,6 +783,10 @@ int les_set_option(les_ctx* c, int option, int value)
             return fail(c, LES_ERR_ARG, "multigrid coarsest sweeps out of 1..4096");
         c->mg_coarse_sweeps = value;
         return LES_OK;
+    case LES_OPT_REPLACE_DECADES:
+        if (value < 0 || value > 15) return fail(c, LES_ERR_ARG, "replacement decades out of 0..15");
+        c->replace_decades = value;
+        return LES_OK;
     default:
         return fail(c, LES_ERR_ARG, "unknown option");
     }
@@ -1360,9 +1365,10 @@ int precond_fp32(les_ctx* c, int pc, const float* r, float* zout, double lmin, d
 
 // Mixed-precision PCG: the Krylov iteration runs on the FP32 copy of A with
 // FP32 vectors and FP64 dot accumulation; the iterate is held as
-// x64 + x32.  Every `reliable_every` iterations, whenever the FP32
-// recursive residual meets the tolerance, and at max_iter, x32 is folded
-// into x64 and r is replaced by the FP64 residual b - A x64 (FP64 stencil).
+// x64 + x32.  Every `reliable_every` iterations, once the recursive
+// residual has fallen `replace_decades` decades below the last true one,
+// whenever it meets the tolerance, and at max_iter, x32 is folded into x64
+// and r is replaced by the FP64 residual b - A x64 (FP64 stencil).
 // Convergence is declared only on that FP64 residual, so the result meets
 // the tolerance in FP64 (outer iterative refinement to the requested tol).
 int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
@@ -1383,8 +1389,14 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
     init.tol_rel = prm->tol_is_relative ? prm->tol : 0.0;
     init.max_iter = prm->max_iter;
     init.reliable_every = prm->reliable_every > 0 ? prm->reliable_every : 50;
+    init.reliable_drop = c->replace_decades > 0 ? std::pow(10.0, -c->replace_decades) : 0.0;
     init.trace = c->trace;
     c->sc_host[0] = init;
+    // A multigrid iteration takes the residual down about 2.5x, so a drop
+    // request can stand on any iteration: its gated replacement kernels are
+    // enqueued on every one (a few microseconds against the V-cycle).  The
+    // slower preconditioners serve it at the next periodic replacement.
+    const bool gate_every = pc == LES_PC_MG && init.reliable_drop > 0.0;
     HIPCHK(c, hipMemcpyAsync(c->sc, &c->sc_host[0], sizeof(SolveScalars), hipMemcpyHostToDevice, s));
     HIPCHK(c, hipMemsetAsync(w.x, 0, nbytes(c, 4), s));
 
@@ -1438,10 +1450,11 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
             }
             // The replacement kernels run gated on the device's request; they
             // are enqueued on the iterations where a request can stand: the
-            // periodic ones, max_iter, and every iteration once the recursive
-            // residual is near the tolerance or a request was seen pending.
+            // periodic ones, max_iter, every iteration once the recursive
+            // residual is near the tolerance or a request was seen pending,
+            // and every iteration of a multigrid solve.
             const int itn = issued + it + 1;
-            if (near || itn % init.reliable_every == 0 || itn >= prm->max_iter) {
+            if (near || gate_every || itn % init.reliable_every == 0 || itn >= prm->max_iter) {
                 HIPCHK(c, launch_mixed_fold(s, d, x64, w.x, pb[0], pb[1], 0, sc));
                 HIPCHK(c, launch_mixed_residual(s, d, A64, b, x64, w.r, 0, 0, c->red, sc,
                                                 nullptr));

@@ -54,6 +54,7 @@ struct SolveScalars {
     int fault;       // 1 if a non-finite scalar was met
     // mixed precision (FP32 iteration, FP64 reliable residual replacement)
     int reliable_every;  // replace r by the FP64 residual every this many iterations
+    double reliable_drop; // > 0: also once ||r|| < reliable_drop * the last true residual
     int reliable_now;    // 1: fold x32 into x64 and replace r in this iteration
     int stop_pending;    // 1: stop after the pending replacement (max_iter reached)
     int replacements;
@@ -89,7 +90,8 @@ enum Epilogue {
     EPI_CONV_RZ = 5,     // tot = {rr, rz}: EPI_CONV, then beta = rz/rho, rho = rz
     EPI_BETA = 6,        // tot = {rz}: beta = rz/rho, rho = rz
     EPI_CONV_MIXED = 7,  // tot = {rr}: iter++, err; schedule an FP64 replacement when
-                         // err < tol, iter % reliable_every == 0 or iter == max_iter
+                         // err < tol, iter % reliable_every == 0, err has fallen below
+                         // reliable_drop * true_err, or iter == max_iter
     EPI_RELIABLE = 8,    // tot = {rr64}: true residual; done if below tol
     EPI_INIT_MIXED = 9,  // tot = {rr64, bb}: as EPI_INIT, also true_err
     EPI_STORE2 = 10,     // out[0..1] = tot[0..1]

@@ -529,6 +529,10 @@ __device__ void epilogue(int epi, const double* tot, SolveScalars* sc, double* o
         }
         if (e < sc->tol || (sc->reliable_every > 0 && it % sc->reliable_every == 0))
             sc->reliable_now = 1;
+        // the FP32 recursion's drift grows with the residual it started
+        // from: once the residual has fallen by reliable_drop since the last
+        // true residual, that drift is no longer small against it
+        if (e < sc->reliable_drop * sc->true_err) sc->reliable_now = 1;
         if (it >= sc->max_iter) {
             sc->reliable_now = 1;
             sc->stop_pending = 1;

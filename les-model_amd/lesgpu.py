@@ -31,7 +31,7 @@ KT_NONE, KT_CHEB_STEP, KT_PCG_MATVEC, KT_PCG_UPDATE, KT_MOMENTUM, KT_IC_SOLVE = 
 KT_CHEB_PAIR, KT_STATS, KT_IC_FACTOR, KT_SOR_SWEEP, KT_MG_SWEEP = 6, 7, 8, 9, 10
 IP_EUCLID, IP_MIRROR = 0, 1
 OPT_CHEB_PAIRS, OPT_PAIR_GRID, OPT_WAVEFRONT, OPT_SPIN_LIMIT = 1, 2, 3, 4
-OPT_MG_SWEEPS, OPT_MG_COARSE_SWEEPS = 5, 6
+OPT_MG_SWEEPS, OPT_MG_COARSE_SWEEPS, OPT_REPLACE_DECADES = 5, 6, 7
 ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_IO, ERR_STALL = -1, -2, -3, -4, -5, -6
 # statistics slots (statistics.h:15-31): means of u, v, w, p, then the sums
 # of the six products
