@@ -90,10 +90,11 @@ def cheb_bytes_per_cell(degree, elem, coded=False, width=1):
     width (1: k_cheb_step, 2: k_cheb_pair), averaged
     over that width's launches of one application that the library times
     (all but a reducing last one).  The operator is 7 coefficient arrays,
-    or one 2-byte code.  Width 0: the multigrid's level-0 damped-Jacobi
-    sweep (x, b, the operator -> x)."""
+    or one 2-byte code.  Width 0: the multigrid's level-0 pre-smoothing,
+    its two damped-Jacobi sweeps from x = 0 in one pass (b, the operator ->
+    x)."""
     if width == 0:
-        return 3 * elem + (2 if coded else 7 * elem)
+        return 2 * elem + (2 if coded else 7 * elem)
     sched = cheb_schedule(degree, width)
     timed = [_launch_bytes(f, l, elem, coded) for w, f, l in sched if w == width and not l]
     return sum(timed) / len(timed)

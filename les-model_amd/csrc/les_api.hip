@@ -289,6 +289,44 @@ void cheb_coeffs(double lmin, double lmax, int degree, double& a0, std::vector<d
 int last_level(const Dims& d) { return d.nx + d.ny + d.nz - 3; }
 
 
+// Sizes the batches of PCG iterations the host issues between polls of the
+// device's scalars.  Away from the end the host checks batch b-1 while batch
+// b runs, so the GPU never waits for it; but a solve that has converged then
+// still has up to two batches of iterations queued, whose launches exit at
+// once and yet cost several microseconds of dispatch each (about 1.6 ms a
+// solve for a multigrid iteration's 30 launches).  From the residuals seen
+// at the polls the planner predicts the iteration that meets the tolerance:
+// the batch that would pass it stops there and is waited for directly, and
+// from then on single iterations are issued and waited for until the solve
+// is done (one host round trip each, a few tens of microseconds).
+struct BatchPlanner {
+    int check;                      // iterations per batch away from the end
+    long predict = 1L << 40;        // predicted iteration that meets the tolerance
+    int seen_iter = -1;
+    double seen_err = 0.0;
+    bool tight = false;             // the batch just issued is waited for directly
+    explicit BatchPlanner(int chk) : check(chk) {}
+    int next(int issued, int left)
+    {
+        int n = std::min(check, left);
+        tight = predict <= (long)issued + n;
+        if (tight) n = (int)std::max<long>(1, std::min<long>(n, predict - issued));
+        return n;
+    }
+    // a poll's state: iterations done, ||r||, the tolerance; issued so far
+    void observe(int iter, double err, double tol, int issued)
+    {
+        if (seen_iter >= 0 && iter > seen_iter && err > 0.0 && err < seen_err && tol > 0.0) {
+            const double rate = std::log(err / seen_err) / (iter - seen_iter);  // < 0
+            const double m = std::max(0.0, std::log(tol / err) / rate);
+            predict = iter + (long)std::ceil(std::min(m, 1e9));
+        }
+        if (tight) predict = std::max<long>(predict, (long)issued + 1);
+        seen_iter = iter;
+        seen_err = err;
+    }
+};
+
 // Event bracket around the launches of the timed kernel class.
 struct KtScope {
     les_ctx* c;
@@ -548,10 +586,21 @@ int mg_vcycle(les_ctx* c, const Stencil<T>& A0, const T* r, T* z, T* t0, int epi
     if (!tail) lt = L - 1;
     T* cur[kMgMaxLevels] = {};
     for (int l = 0; l < lt; ++l) {
-        for (int k = 1; k <= nu; ++k) {
+        int k = 1;
+        if (nu >= 2) {
+            // the first two sweeps in one pass (buf(l, 0) holds the first
+            // where the level runs the scalar kernels); the level-0 one is
+            // the timed class
+            T* out = buf(l, 1);
+            KtScope kt(c, l == 0 ? LES_KT_MG_SWEEP : -1, (int)sizeof(T));
+            HIPCHK(c, launch_mg_smooth<T>(s, g.dims[l], op(l), buf(l, 0), bvec(l), nullptr,
+                                          g.dims[l], out, MG_ZS, -1, kMgOmega, kMgScale, c->red,
+                                          sc));
+            cur[l] = out;
+            k = 3;
+        }
+        for (; k <= nu; ++k) {
             T* out = buf(l, (k + 1) & 1);
-            // the level-0 plain sweeps are the timed class
-            KtScope kt(c, l == 0 && k > 1 ? LES_KT_MG_SWEEP : -1, (int)sizeof(T));
             HIPCHK(c, launch_mg_smooth<T>(s, g.dims[l], op(l), cur[l], bvec(l), nullptr, g.dims[l],
                                           out, k == 1 ? MG_ZERO : MG_SWEEP, -1, kMgOmega,
                                           kMgScale, c->red, sc));
@@ -574,7 +623,6 @@ int mg_vcycle(les_ctx* c, const Stencil<T>& A0, const T* r, T* z, T* t0, int epi
         for (int k = nu + 1; k <= 2 * nu; ++k) {
             T* out = buf(l, (k + 1) & 1);
             const bool last = l == 0 && k == 2 * nu;
-            KtScope kt(c, l == 0 && k > nu + 1 && !last ? LES_KT_MG_SWEEP : -1, (int)sizeof(T));
             HIPCHK(c, launch_mg_smooth<T>(s, g.dims[l], op(l), cur[l], bvec(l), cur[l + 1],
                                           g.dims[l + 1], out, k == nu + 1 ? MG_PROLONG : MG_SWEEP,
                                           last ? epi : -1, kMgOmega, kMgScale, c->red, sc,
@@ -1220,11 +1268,11 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res,
                                 sc, A, wgt)) != LES_OK)
         return rc;
 
-    int check = prm->check_every > 0 ? prm->check_every : (d.N >= (1 << 22) ? 4 : 16);
+    BatchPlanner plan(prm->check_every > 0 ? prm->check_every : (d.N >= (1 << 22) ? 4 : 16));
     int issued = 0, batch = 0;
     bool finished = false;
     while (!finished) {
-        const int n = std::min(check, prm->max_iter - issued);
+        const int n = plan.next(issued, prm->max_iter - issued);
         for (int it = 0; it < n; ++it) {
             {
                 KtScope kt(c, LES_KT_PCG_MATVEC, 8);
@@ -1260,10 +1308,14 @@ int pcg_fp64(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res,
         if (issued >= prm->max_iter) {
             HIPCHK(c, hipEventSynchronize(c->ev[slot]));
             finished = true;
-        } else if (batch > 0) {
-            // check the previous batch while this one runs
-            HIPCHK(c, hipEventSynchronize(c->ev[slot ^ 1]));
-            if (c->sc_host[slot ^ 1].done) finished = true;
+        } else if (plan.tight || batch > 0) {
+            // the previous batch while this one runs, or this one when the
+            // tolerance is predicted within it
+            const int seen = plan.tight ? slot : slot ^ 1;
+            HIPCHK(c, hipEventSynchronize(c->ev[seen]));
+            const SolveScalars& h = c->sc_host[seen];
+            if (h.done) finished = true;
+            plan.observe(h.iter, h.err, h.tol, issued);
         }
         if (prm->max_iter <= 0) finished = true;
         ++batch;
@@ -1416,7 +1468,7 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
                            sc, A32, wgt)) != LES_OK)
         return rc;
 
-    int check = prm->check_every > 0 ? prm->check_every : (d.N >= (1 << 22) ? 4 : 16);
+    BatchPlanner plan(prm->check_every > 0 ? prm->check_every : (d.N >= (1 << 22) ? 4 : 16));
     int issued = 0, batch = 0;
     bool finished = prm->max_iter <= 0;
     bool near = false;  // replacement kernels on every iteration from then on
@@ -1425,7 +1477,10 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
                         // next matvec (beta) and r update (rho, EPI_BETA's bookkeeping)
     double last_err = 0.0;
     while (!finished) {
-        const int n = std::min(check, prm->max_iter - issued);
+        const int n = plan.next(issued, prm->max_iter - issued);
+        // a batch predicted to meet the tolerance carries the replacement
+        // that confirms it
+        if (plan.tight) near = true;
         for (int it = 0; it < n; ++it) {
             {
                 KtScope kt(c, LES_KT_PCG_MATVEC, 4);
@@ -1475,10 +1530,12 @@ int pcg_mixed(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
         if (issued >= prm->max_iter) {
             HIPCHK(c, hipEventSynchronize(c->ev[slot]));
             finished = true;
-        } else if (batch > 0) {
-            HIPCHK(c, hipEventSynchronize(c->ev[slot ^ 1]));
-            const SolveScalars& h = c->sc_host[slot ^ 1];
+        } else if (plan.tight || batch > 0) {
+            const int seen = plan.tight ? slot : slot ^ 1;
+            HIPCHK(c, hipEventSynchronize(c->ev[seen]));
+            const SolveScalars& h = c->sc_host[seen];
             if (h.done) finished = true;
+            plan.observe(h.iter, h.err, h.tol, issued);
             // near: the residual would reach the tolerance within about two
             // more polls at the rate of the last one (the host sees a request
             // up to two batches late)

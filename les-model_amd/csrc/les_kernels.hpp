@@ -216,9 +216,11 @@ hipError_t launch_mg_coarsen(hipStream_t s, const Dims& f, const Dims& cd, Stenc
                              const double* W, double* const Ac[7]);
 // One damped-Jacobi sweep on a level (mode MG_ZERO: from x = 0; MG_SWEEP;
 // MG_PROLONG: x first corrected by scale * e of its block on the coarser
-// level cd).  epi >= 0: b.x_out reduced with that epilogue (the PCG's r.z,
-// in the wgt inner product when wgt is given).
-enum { MG_ZERO = 0, MG_SWEEP = 1, MG_PROLONG = 2 };
+// level cd; MG_ZS: the first two sweeps from x = 0 in one pass, x unused
+// where the level takes vector loads, else the first sweep's output).
+// epi >= 0: b.x_out reduced with that epilogue (the PCG's r.z, in the wgt
+// inner product when wgt is given).
+enum { MG_ZERO = 0, MG_SWEEP = 1, MG_PROLONG = 2, MG_ZS = 3 };
 template <typename T>
 hipError_t launch_mg_smooth(hipStream_t s, const Dims& d, Stencil<T> A, const T* x, const T* b,
                             const T* e, const Dims& cd, T* x_out, int mode, int epi,

@@ -299,6 +299,20 @@ __device__ __forceinline__ void load_nbr(const T* __restrict__ x, const Dims& d,
     if (k + V < d.nz) nb.kp = f(nb.kp);
 }
 
+// The mirror weight of a coded cell: 1/2 for each axis whose two
+// coefficients are a doubled face and a missing one (k_mirror_weights' rule;
+// products of 1/2, exact)
+__device__ __forceinline__ double code_weight(unsigned cd)
+{
+    double wt = 1.0;
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const unsigned lo = (cd >> (4 * ax)) & 3u, hi = (cd >> (4 * ax + 2)) & 3u;
+        if ((lo == 2u && hi == 0u) || (hi == 2u && lo == 0u)) wt *= 0.5;
+    }
+    return wt;
+}
+
 // The coded coefficients as LDS lookup tables: (w, e) by code bits 0-3,
 // (s, n) by bits 4-7, (b, t, centre) by bits 8-12.  Filled with the same
 // operations as the decode in apply7v, so every coefficient is bit-identical;
@@ -1647,10 +1661,15 @@ __global__ __launch_bounds__(kBlock) void k_pcg_matvec_ring(Dims d, PairGeom g, 
                     for (int v = 0; v < V; ++v) xv[v] = xv[v] + ax * okeep[pp][v];
                     vstore_s<T, V>(x + id, xv);
                 }
+                // the coded operator's mirror weights come from its code
+                // (the array was made from the same code)
+                uint16_t cw[V];
+                if (CODED == 1 && wgt) vload<uint16_t, V>(A.code + id, cw);
 #pragma unroll
                 for (int v = 0; v < V; ++v) {
                     const double pq = (double)nb.c[v] * (double)qv[v];
-                    acc[0] += wgt ? wgt[id + v] * pq : pq;
+                    const double wv = !wgt ? 1.0 : (CODED == 1 ? code_weight(cw[v]) : wgt[id + v]);
+                    acc[0] += wgt ? wv * pq : pq;
                 }
             }
         }
@@ -1692,6 +1711,7 @@ __global__ __launch_bounds__(kBlock) void k_cheb_scale(Dims d, Chunking ch,
 // p0 for even iterations, p1 for odd ones).  x32 is left at
 // -(alpha p_last), so that next matvec brings it to exactly 0 -- the value
 // an eager update followed by this fold would leave.
+template <int V>
 __global__ __launch_bounds__(kBlock) void k_mixed_fold(Dims d, Chunking ch,
                                                        double* __restrict__ x64,
                                                        float* __restrict__ x32,
@@ -1705,12 +1725,22 @@ __global__ __launch_bounds__(kBlock) void k_mixed_fold(Dims d, Chunking ch,
     const float ax = (float)sc->alpha;
     const double unscale = 1.0 / sc->rscale;  // exact: a power of two
     const float* pl = ((it - 1) & 1) ? p1 : p0;
-    FOR_CELLS(d, ch, lb)
+    FOR_VCELLS(d, ch, lb, V)
     {
-        const float ap = it > 0 ? ax * pl[id] : 0.0f;
-        const float xs = it > 0 ? x32[id] + ap : x32[id];
-        x64[id] = x64[id] + (double)xs * unscale;
-        x32[id] = -ap;
+        float xs[V], pv[V];
+        double xd[V];
+        vload<float, V>(x32 + id, xs);
+        vload<double, V>(x64 + id, xd);
+        if (it > 0) vload<float, V>(pl + id, pv);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const float ap = it > 0 ? ax * pv[v] : 0.0f;
+            const float xv = it > 0 ? xs[v] + ap : xs[v];
+            xd[v] = xd[v] + (double)xv * unscale;
+            xs[v] = -ap;
+        }
+        vstore<double, V>(x64 + id, xd);
+        vstore<float, V>(x32 + id, xs);
     }
 }
 
@@ -1730,7 +1760,9 @@ __global__ __launch_bounds__(kBlock) void k_pcg_x_pending(Dims d, Chunking ch, T
     FOR_CELLS(d, ch, lb) x[id] = x[id] + ax * pl[id];
 }
 
-template <int CODED>
+// V cells of a row per thread (V = 1 where nz is not a multiple of the
+// vector width), the neighbours as the matvec loads them.
+template <int CODED, int V>
 __global__ __launch_bounds__(kBlock) void k_mixed_residual(Dims d, Chunking ch, Stencil<double> A,
                                                            const double* __restrict__ b,
                                                            const double* __restrict__ x,
@@ -1741,14 +1773,22 @@ __global__ __launch_bounds__(kBlock) void k_mixed_residual(Dims d, Chunking ch, 
     const int lb = logical_block(ch.grid);
     const double rs = r32 ? sc->rscale : 1.0;
     double acc[2] = {0.0, 0.0};
-    FOR_CELLS(d, ch, lb)
+    FOR_VCELLS(d, ch, lb, V)
     {
-        const double q = apply7<CODED>(A, d, id, i, j, k, [&](int n) { return x[n]; });
-        const double bv = b[id];
-        const double rv = bv - q;
-        if (r32) r32[id] = (float)(rv * rs);
-        acc[0] += rv * rv;
-        acc[1] += bv * bv;
+        Nbr<double, V> nb;
+        load_nbr<double, V>(x, d, id, i, j, k, [](double v) { return v; }, nb);
+        double q[V], bv[V];
+        apply7v<CODED, double, V>(A, d, id, i, j, k, nb, q);
+        vload<double, V>(b + id, bv);
+        float rv32[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            const double rv = bv[v] - q[v];
+            rv32[v] = (float)(rv * rs);
+            acc[0] += rv * rv;
+            acc[1] += bv[v] * bv[v];
+        }
+        if (r32) vstore_s<float, V>(r32 + id, rv32);
     }
     if (red.partials) block_partial<2>(acc, red.partials, lb);
 }
@@ -2104,20 +2144,6 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict(Dims f, Dims cd, Chunkin
     }
 }
 
-// The mirror weight of a coded cell: 1/2 for each axis whose two
-// coefficients are a doubled face and a missing one (k_mirror_weights' rule;
-// products of 1/2, exact)
-__device__ __forceinline__ double code_weight(unsigned cd)
-{
-    double wt = 1.0;
-#pragma unroll
-    for (int ax = 0; ax < 3; ++ax) {
-        const unsigned lo = (cd >> (4 * ax)) & 3u, hi = (cd >> (4 * ax + 2)) & 3u;
-        if ((lo == 2u && hi == 0u) || (hi == 2u && lo == 0u)) wt *= 0.5;
-    }
-    return wt;
-}
-
 // The diagonal of V cells (the coded centre, or A_c)
 template <int CODED, typename T, int V>
 __device__ __forceinline__ void diag7v(const Stencil<T>& A, int id, T (&dg)[V])
@@ -2193,6 +2219,31 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth_v(Dims d, Chunking ch, Ste
                 row_edges<T, V, true>(d, id, k, nb.c, nb.km, nb.kp, [&](int n) {
                     return x[n] + sf * eat(i, j, k + (n - id));
                 });
+            } else if constexpr (MODE == MG_ZS) {
+                // the first sweep's iterate om * (b / A_c) at the cells and
+                // their neighbours, formed where it is read (bit for bit the
+                // values an MG_ZERO launch would have stored)
+                auto diag1 = [&](int n) {
+                    if constexpr (CODED == 1)
+                        return (A.code[n] & CODE_IDENTITY) ? T(1) : A.cfluid;
+                    else
+                        return A.a[SC][n];
+                };
+                auto zero_at = [&](int n, T (&vals)[V]) {
+                    T g[V];
+                    diag7v<CODED, T, V>(A, n, g);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) vals[v] = om * (vals[v] / g[v]);
+                };
+                load_nbr5<T, V>(b, d, id, i, j, nb);
+#pragma unroll
+                for (int v = 0; v < V; ++v) nb.c[v] = om * (nb.c[v] / dg[v]);
+                if (i > 0) zero_at(id - d.sij, nb.w);
+                if (i < d.nx - 1) zero_at(id + d.sij, nb.e);
+                if (j > 0) zero_at(id - d.nz, nb.s);
+                if (j < d.ny - 1) zero_at(id + d.nz, nb.n);
+                row_edges<T, V, true>(d, id, k, nb.c, nb.km, nb.kp,
+                                      [&](int n) { return om * (b[n] / diag1(n)); });
             } else {
                 load_nbr<T, V>(x, d, id, i, j, k, [](T v) { return v; }, nb);
             }
@@ -3088,9 +3139,16 @@ hipError_t launch_ic_bwd_level(hipStream_t s, const Dims& d, int level, T* z, co
 hipError_t launch_mixed_fold(hipStream_t s, const Dims& d, double* x64, float* x32,
                              const float* p0, const float* p1, int force, SolveScalars* sc)
 {
-    const Chunking ch = make_chunking(d);
-    hipLaunchKernelGGL(k_mixed_fold, dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x64, x32, p0, p1,
-                       force ? (int)GATE_NONE : (int)GATE_RELIABLE, sc);
+    constexpr int W = vec_width<float>();
+    const int V = pick_width<float>(d);
+    const Chunking ch = make_chunking(d, V);
+    const int gate = force ? (int)GATE_NONE : (int)GATE_RELIABLE;
+    if (V == W)
+        hipLaunchKernelGGL((k_mixed_fold<W>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x64, x32, p0,
+                           p1, gate, sc);
+    else
+        hipLaunchKernelGGL((k_mixed_fold<1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, x64, x32, p0,
+                           p1, gate, sc);
     return hipGetLastError();
 }
 
@@ -3110,15 +3168,22 @@ hipError_t launch_mixed_residual(hipStream_t s, const Dims& d, Stencil<double> A
                                  const double* x64, float* r32, int init, int force,
                                  RedScratch red, SolveScalars* sc, double* out)
 {
-    const Chunking ch = make_chunking(d);
+    constexpr int W = vec_width<double>();
+    const int V = pick_width<double>(d);
+    const Chunking ch = make_chunking(d, V);
     const int gate = (init || force) ? (int)GATE_NONE : (int)GATE_RELIABLE;
     auto pass = [&](float* dst, RedScratch rd) {
-        if (A.code)
-            hipLaunchKernelGGL((k_mixed_residual<1>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A,
-                               b, x64, dst, gate, rd, sc);
-        else
-            hipLaunchKernelGGL((k_mixed_residual<0>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A,
-                               b, x64, dst, gate, rd, sc);
+#define LES_MRES(CC, VV)                                                                          \
+    hipLaunchKernelGGL((k_mixed_residual<CC, VV>), dim3(ch.grid), dim3(kBlock), 0, s, d, ch, A, b, \
+                       x64, dst, gate, rd, sc)
+        if (A.code) {
+            if (V == W) LES_MRES(1, W);
+            else LES_MRES(1, 1);
+        } else {
+            if (V == W) LES_MRES(0, W);
+            else LES_MRES(0, 1);
+        }
+#undef LES_MRES
         return hipGetLastError();
     };
     if (init) {
@@ -3204,6 +3269,15 @@ hipError_t launch_mg_smooth(hipStream_t s, const Dims& d, Stencil<T> A, const T*
     const bool vec = LES_MG_VEC && pick_width<T>(d) == VW;
     const Chunking ch = make_chunking(d, vec ? VW : 1);
     const bool rz = epi >= 0;
+    if (mode == MG_ZS && !vec) {
+        // the scalar kernels run the two sweeps apart, the first into x
+        if (!x || x == x_out) return hipErrorInvalidValue;
+        const hipError_t e0 = launch_mg_smooth<T>(s, d, A, nullptr, b, e, cd, const_cast<T*>(x),
+                                                  MG_ZERO, -1, omega, scale, red, sc, nullptr);
+        if (e0 != hipSuccess) return e0;
+        return launch_mg_smooth<T>(s, d, A, x, b, e, cd, x_out, MG_SWEEP, epi, omega, scale, red,
+                                   sc, wgt);
+    }
 #define LES_MGS(CD, MD, RZ)                                                                       \
     do {                                                                                          \
         if (vec)                                                                                  \
@@ -3213,10 +3287,14 @@ hipError_t launch_mg_smooth(hipStream_t s, const Dims& d, Stencil<T> A, const T*
             hipLaunchKernelGGL((k_mg_smooth<T, CD, MD, RZ>), dim3(ch.grid), dim3(kBlock), 0, s, d, \
                                ch, A, x, b, e, cd, x_out, omega, scale, red, sc, wgt);            \
     } while (0)
+#define LES_MGS_ZS(CD, RZ)                                                                        \
+    hipLaunchKernelGGL((k_mg_smooth_v<T, CD, MG_ZS, RZ, VW>), dim3(ch.grid), dim3(kBlock), 0, s, d, \
+                       ch, A, x, b, e, cd, x_out, omega, scale, red, sc, wgt)
 #define LES_MGS_M(CD, RZ)                             \
     do {                                              \
         if (mode == MG_ZERO) LES_MGS(CD, MG_ZERO, RZ);     \
         else if (mode == MG_SWEEP) LES_MGS(CD, MG_SWEEP, RZ); \
+        else if (mode == MG_ZS) LES_MGS_ZS(CD, RZ);       \
         else LES_MGS(CD, MG_PROLONG, RZ);                 \
     } while (0)
     if (A.code) {
@@ -3227,6 +3305,7 @@ hipError_t launch_mg_smooth(hipStream_t s, const Dims& d, Stencil<T> A, const T*
         else LES_MGS_M(0, false);
     }
 #undef LES_MGS_M
+#undef LES_MGS_ZS
 #undef LES_MGS
     if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
     if (rz)
