@@ -1039,8 +1039,8 @@ struct PairGeom {
 #define LES_MV_TAB 1
 #endif
 // The staged forms of the pair and of the reducing last step
-// (k_cheb_pair_st) where they apply: FP32, coded operator, rows inside one
-// wave.
+// (k_cheb_pair_st) where they apply: FP32, coded operator, rows of one or
+// two waves.
 // Multigrid sweeps and restriction with vector loads where the rows allow
 // (nz % V == 0), else one cell per thread.
 #ifndef LES_MG_VEC
@@ -1284,7 +1284,9 @@ __global__ __launch_bounds__(TB, pair_waves_per_simd<TB>()) void k_cheb_pair(Dim
 // buffers), so every global load has a plane of work to arrive and one
 // barrier per plane still orders all LDS use.  One 1024-thread workgroup per
 // CU.  Step B and every cell's arithmetic are k_cheb_pair's, so the results
-// are bit-identical.  Rows must lie inside one wave (k-edges by shuffle).
+// are bit-identical.  A row inside one wave takes its k-edges by lane
+// shuffle, a row of two waves (nz = 512) from the LDS copies of z_k and
+// z_{k+1}.
 // SINGLE: one reducing step (an application's last; its p is never read):
 // the core rows' z_{k+1} goes to z_out and r.z_{k+1} to red's partials, with
 // no ring and no step B.
@@ -2961,8 +2963,8 @@ hipError_t launch_cheb_pair_cd(hipStream_t s, const Dims& d, Stencil<T> A, const
     return hipGetLastError();
 }
 
-// The staged pair (k_cheb_pair_st): FP32 with the coded operator, rows
-// inside one wave, one 1024-thread workgroup per CU (LES_PAIR_STAGED and
+// The staged pair (k_cheb_pair_st): FP32 with the coded operator, rows of
+// one or two waves, one 1024-thread workgroup per CU (LES_PAIR_STAGED and
 // LES_LAST_STAGED are defined with the pair's other switches).
 #ifndef LES_PAIR_ST_GRID
 #define LES_PAIR_ST_GRID 256
@@ -2982,7 +2984,9 @@ bool cheb_pair_st_geom(const Dims& d, PairGeom& g, int target_grid)
     constexpr int V = vec_width<T>();
     if (sizeof(T) != 4 || d.nz % V) return false;
     g.tpr = d.nz / V;
-    if (g.tpr > 64 || (g.tpr & (g.tpr - 1))) return false;  // rows inside one wave
+    // rows of one wave take their k-edges by lane shuffle, rows of two
+    // waves from the LDS copies; a power of two fills the workgroup
+    if (g.tpr > 128 || (g.tpr & (g.tpr - 1))) return false;
     g.rpp = kPairStBlock / g.tpr;
     g.J = g.rpp - 2;
     if (g.J < 2) return false;
@@ -2996,7 +3000,7 @@ bool cheb_pair_st_geom(const Dims& d, PairGeom& g, int target_grid)
     return true;
 }
 
-// The staged kernel for the grid's nz: specialised for nz = 32 .. 256 (row
+// The staged kernel for the grid's nz: specialised for nz = 32 .. 512 (row
 // strides and LDS offsets fold into the instructions), generic otherwise.
 template <typename T, bool FIRST, bool SINGLE, int NZ>
 hipError_t run_pair_st_nz(hipStream_t s, const Dims& d, const PairGeom& g, size_t lds,
@@ -3031,6 +3035,7 @@ hipError_t run_pair_st(hipStream_t s, const Dims& d, const PairGeom& g, size_t l
                                                 alpha0, beta_a, alpha_a, beta_b, alpha_b, sc,  \
                                                 red)
         switch (d.nz) {
+        case 512: LES_RUN_NZ(512);
         case 256: LES_RUN_NZ(256);
         case 128: LES_RUN_NZ(128);
         case 64: LES_RUN_NZ(64);

@@ -430,14 +430,16 @@ def test_mixed_precision_rhs_scale(orc, pc, shape):
 @pytest.mark.parametrize("fmt",[lesgpu.STENCIL_ARRAYS, lesgpu.STENCIL_CODED])
 @pytest.mark.parametrize("shape,degree", [((6, 10, 256, 8), 8), ((7, 23, 32, 9), 5),
                                           ((5, 14, 256, 10), 3), ((64, 120, 256, 11), 8),
-                                          ((9, 13, 16, 12), 8), ((11, 9, 128, 13), 8)])
+                                          ((9, 13, 16, 12), 8), ((11, 9, 128, 13), 8),
+                                          ((5, 13, 512, 14), 8)])
 def test_cheb_pairs_identical(orc, precision, fmt, shape, degree):
     """Fused pairs of Chebyshev steps reproduce single steps bit for bit:
     fixed-iteration solves and their residual histories are identical.  The
     last shape gives each workgroup a run of several i-planes; the pair runs
     at its default grid, at 8 workgroups (long i-runs) and at 4096 (one
-    plane per workgroup).  nz = 256, 128 and 32 take the staged kernel's
-    specialised forms, nz = 16 its generic one."""
+    plane per workgroup).  nz = 512, 256, 128 and 32 take the staged
+    kernel's specialised forms (at 512 a row spans two waves and takes its
+    k-edges from LDS), nz = 16 its generic one."""
     m, solid, f = random_case(*shape, frac=0.05)
     A = orc.build_stencil(solid, m)
     lmax = orc.lambda_max(A, m, 20)
@@ -456,10 +458,15 @@ def test_cheb_pairs_identical(orc, precision, fmt, shape, degree):
                       stencil_format=fmt)
             launches = g.kernel_timing_read()[0]
             g.kernel_timing(lesgpu.KT_NONE)
-            # every shape here fits every pair geometry, so the fused path
-            # ran -- except at degree 3, whose only pair would end on the
-            # reducing last step, which always runs single
-            assert (launches > 0) == bool(pairs and degree >= 4), (pairs, launches)
+            # every shape here up to nz = 256 fits every pair geometry, so
+            # the fused path ran -- except at degree 3, whose only pair would
+            # end on the reducing last step, which always runs single; at
+            # nz = 512 the staged FP32 coded pair runs, and the others where
+            # their LDS tiles fit
+            fits = shape[2] <= 256 or (precision == lesgpu.PREC_MIXED and
+                                       fmt == lesgpu.STENCIL_CODED)
+            if fits or launches > 0:
+                assert (launches > 0) == bool(pairs and degree >= 4), (pairs, launches)
             outs.append((r, g.get_field("p"), g.pcg_trace(13)))
     ra, xa, ta = outs[0]
     for rb, xb, tb in outs[1:]:
