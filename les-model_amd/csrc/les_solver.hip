@@ -1046,6 +1046,15 @@ struct PairGeom {
 #ifndef LES_MG_VEC
 #define LES_MG_VEC 1
 #endif
+// The multigrid's two-sweep pre-smoothing pass marching along i with the
+// first sweep's iterate in an LDS ring (k_mg_zs_ring) where the ring
+// matvec's geometry fits; 0: k_mg_smooth_v forms it at every neighbour.
+#ifndef LES_MG_ZS_RING
+#define LES_MG_ZS_RING 1
+#endif
+#ifndef LES_MG_ZS_GRID
+#define LES_MG_ZS_GRID 256
+#endif
 #ifndef LES_PAIR_STAGED
 #define LES_PAIR_STAGED 1
 #endif
@@ -2271,6 +2280,166 @@ __global__ __launch_bounds__(kBlock) void k_mg_smooth_v(Dims d, Chunking ch, Ste
     if (RZ) block_partial<1>(acc, red.partials, lb);
 }
 
+// The first two sweeps from x = 0 (MG_ZS) marching along i like the ring
+// matvec: for plane t a workgroup forms the first sweep's iterate
+// om * (b / A_c) on its J rows and one halo row on each side into an R-plane
+// LDS ring (once per cell, where k_mg_smooth_v forms it again at each of a
+// cell's four i/j neighbours), then applies the second sweep at plane t-1
+// from the ring.  b of the core rows is carried from plane to plane.  Every
+// cell's arithmetic is k_mg_smooth_v's, so x_out is bit-identical to it.
+template <typename T, int CODED, int V, int P, int R>
+__global__ __launch_bounds__(kBlock) void k_mg_zs_ring(Dims d, PairGeom g, Stencil<T> A,
+                                                       const T* __restrict__ b,
+                                                       T* __restrict__ x_out, double omega,
+                                                       const SolveScalars* sc)
+{
+    if (is_done(sc)) return;
+    extern __shared__ unsigned char les_smem[];
+    T* xs = reinterpret_cast<T*>(les_smem);  // [R][P*rpp][nz]
+    const CodeTab<T>* tab = nullptr;
+    if constexpr (CODED == 1 && LES_MV_TAB) {
+        __shared__ CodeTab<T> tab_s;
+        code_tab_fill(tab_s, A);
+        __syncthreads();
+        tab = &tab_s;
+    }
+    const int lb = logical_block(g.grid);
+    const int jb = lb % g.njb, ic = lb / g.njb;
+    const int j0 = jb * g.J;
+    const int ia = ic * g.irange, ib = min(ia + g.irange, d.nx);
+    const int sub = (int)threadIdx.x / g.tpr;
+    const int kk = ((int)threadIdx.x - sub * g.tpr) * V;
+    const int plane_sz = P * g.rpp * d.nz;
+    const T om = (T)omega;
+    // b and the diagonal's source (the code, or A_c) of this thread's rows:
+    // plane t-1 (b only, for the second sweep), t, t+1 and t+2 (loaded
+    // while plane t is computed, so the loads have two planes of work to
+    // arrive)
+    using DS = std::conditional_t<CODED == 1, uint16_t, T>;
+    T bkeep[P][V], bcur[P][V], bnx[P][V], bnx2[P][V];
+    DS dkeep[P][V], dcur[P][V], dnx[P][V], dnx2[P][V];
+    T pad[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) pad[v] = outside<T>();
+    auto row_of = [&](int pp, int& a, int& j) {
+        const int qr = pp * g.rpp + sub;
+        a = qr < g.J ? qr + 1 : (qr == g.J ? 0 : g.J + 1);
+        j = j0 - 1 + a;
+    };
+    auto load_plane = [&](int t, T (&bo)[P][V], DS (&dso)[P][V]) {
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp) {
+            int a, j;
+            row_of(pp, a, j);
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                bo[pp][v] = T(0);
+                dso[pp][v] = DS(0);
+            }
+            if (t < 0 || t >= d.nx || j < 0 || j >= d.ny) continue;
+            const int id = (t * d.ny + j) * d.nz + kk;
+            vload_s<T, V>(b + id, bo[pp]);
+            if constexpr (CODED == 1) vload<uint16_t, V>(A.code + id, dso[pp]);
+            else vload<T, V>(A.a[SC] + id, dso[pp]);
+        }
+    };
+#pragma unroll
+    for (int pp = 0; pp < P; ++pp)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            bkeep[pp][v] = T(0);
+            dkeep[pp][v] = DS(0);
+        }
+    if (ia < d.nx) {
+        load_plane(ia - 1, bcur, dcur);
+        load_plane(ia, bnx, dnx);
+    }
+    for (int t = ia - 1; t <= ib && ia < d.nx; ++t) {
+        load_plane(t + 2, bnx2, dnx2);
+        // ---- the first sweep's iterate at plane t into the ring (the
+        // coded operator reads rows and planes outside the grid as
+        // outside<T>() from it) ----
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp) {
+            int a, j;
+            row_of(pp, a, j);
+            T* slot = xs + (t + R) % R * plane_sz + a * d.nz + kk;
+            if (t < 0 || t >= d.nx || j < 0 || j >= d.ny) {
+                if (CODED == 1) vstore<T, V>(slot, pad);
+                continue;
+            }
+            T xv[V];
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                T dg;
+                if constexpr (CODED == 1) dg = (dcur[pp][v] & CODE_IDENTITY) ? T(1) : A.cfluid;
+                else dg = dcur[pp][v];
+                xv[v] = om * (bcur[pp][v] / dg);
+            }
+            vstore<T, V>(slot, xv);
+        }
+        __syncthreads();
+        // ---- the second sweep at plane t-1 from the ring ----
+        const int iB = t - 1;
+        if (iB >= ia && iB < ib) {
+            const T* pc_pl = xs + (iB + R) % R * plane_sz;
+            const T* pm_pl = xs + (iB + R - 1) % R * plane_sz;
+            const T* pp_pl = xs + (iB + 1) % R * plane_sz;
+#pragma unroll
+            for (int pp = 0; pp < P; ++pp) {
+                const int qr = pp * g.rpp + sub;
+                if (qr >= g.J) continue;
+                const int a = qr + 1, j = j0 + qr;
+                if (j >= d.ny) continue;
+                const int i = iB, k = kk, id = (i * d.ny + j) * d.nz + kk;
+                Nbr<T, V> nb;
+                vload<T, V>(pc_pl + a * d.nz + kk, nb.c);
+                if (CODED == 1) {
+                    vload<T, V>(pm_pl + a * d.nz + kk, nb.w);
+                    vload<T, V>(pp_pl + a * d.nz + kk, nb.e);
+                    vload<T, V>(pc_pl + (a - 1) * d.nz + kk, nb.s);
+                    vload<T, V>(pc_pl + (a + 1) * d.nz + kk, nb.n);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) nb.w[v] = nb.e[v] = nb.s[v] = nb.n[v] = T(0);
+                    if (i > 0) vload<T, V>(pm_pl + a * d.nz + kk, nb.w);
+                    if (i < d.nx - 1) vload<T, V>(pp_pl + a * d.nz + kk, nb.e);
+                    if (j > 0) vload<T, V>(pc_pl + (a - 1) * d.nz + kk, nb.s);
+                    if (j < d.ny - 1) vload<T, V>(pc_pl + (a + 1) * d.nz + kk, nb.n);
+                }
+                row_edges<T, V, false>(d, id, k, nb.c, nb.km, nb.kp,
+                                       [&](int n) { return pc_pl[a * d.nz + kk + (n - id)]; });
+                T qv[V], dg[V], out[V];
+                if constexpr (CODED == 1) {
+                    // the code of plane t-1, carried from its step A
+                    apply7v<1, T, V, LES_MV_TAB>(A, d, id, i, j, k, nb, qv, dkeep[pp], nullptr,
+                                                 tab);
+#pragma unroll
+                    for (int v = 0; v < V; ++v)
+                        dg[v] = (dkeep[pp][v] & CODE_IDENTITY) ? T(1) : A.cfluid;
+                } else {
+                    apply7v<0, T, V>(A, d, id, i, j, k, nb, qv);
+                    diag7v<0, T, V>(A, id, dg);
+                }
+#pragma unroll
+                for (int v = 0; v < V; ++v) out[v] = nb.c[v] + om * ((bkeep[pp][v] - qv[v]) / dg[v]);
+                vstore_s<T, V>(x_out + id, out);
+            }
+        }
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp)
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                bkeep[pp][v] = bcur[pp][v];
+                dkeep[pp][v] = dcur[pp][v];
+                bcur[pp][v] = bnx[pp][v];
+                dcur[pp][v] = dnx[pp][v];
+                bnx[pp][v] = bnx2[pp][v];
+                dnx[pp][v] = dnx2[pp][v];
+            }
+    }
+}
+
 // k_mg_restrict with two coarse cells of a row per thread: the four fine
 // rows of their blocks are read four cells at a time (k-neighbours by lane
 // shifts), the residual formed per cell as apply7 forms it, and each coarse
@@ -2606,9 +2775,10 @@ size_t mv_ring_lds(const Dims& d, const PairGeom& g)
     return (size_t)MvRingCfg<T>::R * MvRingCfg<T>::P * g.rpp * d.nz * sizeof(T);
 }
 
-// Workgroups: njb j-blocks times i-runs, at most kMaxGrid in all.
+// Workgroups: njb j-blocks times i-runs, at most target_grid in all (a
+// multiple of kXcd; kMaxGrid for the kernels that leave partials).
 template <typename T>
-bool mv_ring_geom(const Dims& d, PairGeom& g)
+bool mv_ring_geom(const Dims& d, PairGeom& g, int target_grid = kMaxGrid)
 {
     constexpr int V = vec_width<T>(), P = MvRingCfg<T>::P;
     if (d.nz % V) return false;
@@ -2620,10 +2790,10 @@ bool mv_ring_geom(const Dims& d, PairGeom& g)
     if (mv_ring_lds<T>(d, g) + (LES_MV_TAB ? sizeof(CodeTab<T>) : 0) > kPairMaxLds) return false;
     g.njb = (d.ny + g.J - 1) / g.J;
     if (g.njb > kMaxGrid) return false;
-    int nxc = std::max(1, std::min(d.nx, kMaxGrid / g.njb));
+    int nxc = std::max(1, std::min(d.nx, target_grid / g.njb));
     g.irange = (d.nx + nxc - 1) / nxc;
     nxc = (d.nx + g.irange - 1) / g.irange;
-    g.grid = ((g.njb * nxc + kXcd - 1) / kXcd) * kXcd;  // <= kMaxGrid, a multiple of kXcd
+    g.grid = ((g.njb * nxc + kXcd - 1) / kXcd) * kXcd;  // <= target_grid, a multiple of kXcd
     return true;
 }
 
@@ -3274,6 +3444,25 @@ hipError_t launch_mg_smooth(hipStream_t s, const Dims& d, Stencil<T> A, const T*
     const bool vec = LES_MG_VEC && pick_width<T>(d) == VW;
     const Chunking ch = make_chunking(d, vec ? VW : 1);
     const bool rz = epi >= 0;
+    PairGeom g;
+    if (LES_MG_ZS_RING && mode == MG_ZS && vec && !rz && mv_ring_geom<T>(d, g, LES_MG_ZS_GRID)) {
+        constexpr int P = MvRingCfg<T>::P, R = MvRingCfg<T>::R;
+        const size_t lds = mv_ring_lds<T>(d, g);
+        const void* fn = A.code ? reinterpret_cast<const void*>(&k_mg_zs_ring<T, 1, VW, P, R>)
+                                : reinterpret_cast<const void*>(&k_mg_zs_ring<T, 0, VW, P, R>);
+        if (lds > 64 * 1024) {
+            const hipError_t e = hipFuncSetAttribute(
+                fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        if (A.code)
+            hipLaunchKernelGGL((k_mg_zs_ring<T, 1, VW, P, R>), dim3(g.grid), dim3(kBlock), lds, s,
+                               d, g, A, b, x_out, omega, sc);
+        else
+            hipLaunchKernelGGL((k_mg_zs_ring<T, 0, VW, P, R>), dim3(g.grid), dim3(kBlock), lds, s,
+                               d, g, A, b, x_out, omega, sc);
+        return hipGetLastError();
+    }
     if (mode == MG_ZS && !vec) {
         // the scalar kernels run the two sweeps apart, the first into x
         if (!x || x == x_out) return hipErrorInvalidValue;

@@ -240,3 +240,48 @@ def test_mg_options_range():
         with pytest.raises(lesgpu.LesError, match="multigrid"):
             g.pcg(precond=lesgpu.PC_CHEBYSHEV, max_iter=3, inner_product=lesgpu.IP_MIRROR,
                   precision=lesgpu.PREC_MIXED)
+
+
+@pytest.mark.parametrize("precision", [lesgpu.PREC_FP64, lesgpu.PREC_MIXED])
+@pytest.mark.parametrize("precond", [PC_MG, lesgpu.PC_CHEBYSHEV])
+def test_host_batches_change_nothing(orc, precision, precond):
+    """How many iterations the host issues between its polls (fixed batches
+    of 1 or 3, or the default batches that stop at the predicted last
+    iteration) changes neither the iterates nor the counts: the device
+    decides when the solve is done.  Mixed precision with Chebyshev is the
+    exception in its last bits: the host enqueues the gated FP64
+    replacement only periodically until its polls see the residual near the
+    tolerance, so a pending replacement runs at an iteration that depends on
+    the poll schedule (reproducibly for a given schedule); the counts stay
+    the same and the results agree to well inside the tolerance.  A
+    multigrid solve enqueues the replacement on every iteration."""
+    m, solid, f = random_case(40, 36, 64, 5, frac=0.05)
+    A = orc.build_stencil(np.zeros_like(solid), m)
+    b = orc.build_b(orc.divergence(f[0], f[1], f[2], solid, m, 1e-3, 1.2), solid, m)
+    lmax = orc.lambda_max(A, m, 20)
+    kw = dict(precond=precond, max_iter=2000, tol=1e-9, tol_rel=True, precision=precision,
+              reliable_every=50, lmin=3 * np.pi ** 2, lmax=1.05 * lmax, degree=8,
+              stencil_format=lesgpu.STENCIL_CODED,
+              inner_product=lesgpu.IP_MIRROR if precond == PC_MG else lesgpu.IP_EUCLID)
+    outs = []
+    with ctx_for(m, solid) as g:
+        g.build_stencil(ignore_mask=True)
+        g.set_field("b", b)
+        for check in (0, 1, 3):
+            g.set_field("p", np.zeros_like(b))
+            r = g.pcg(check_every=check, **kw)
+            assert r["converged"] == 1 and r["fault"] == 0, (check, r)
+            outs.append((r, g.get_field("p")))
+    print(f"[mg] batches precision {precision} precond {precond}: {outs[0][0]['iters']} iterations",
+          flush=True)
+    exact = precision == lesgpu.PREC_FP64 or precond == PC_MG
+    r0, x0 = outs[0]
+    for r, x in outs[1:]:
+        if exact:
+            assert r == r0
+            assert bits_equal(x, x0)
+        else:
+            assert (r["iters"], r["replacements"], r["restarts"]) == \
+                (r0["iters"], r0["replacements"], r0["restarts"])
+            assert r["true_res_norm"] < 1e-9 * r["b_norm"]
+            assert rel_err(x, x0) < 1e-5
