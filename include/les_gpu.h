@@ -184,7 +184,11 @@ typedef struct les_pcg_params {
     int degree;           /* Chebyshev degree (pressure.h:49: default 5) */
     int precision;        /* LES_PREC_FP64 or LES_PREC_MIXED */
     int reliable_every;   /* mixed: FP64 residual replacement period (0: 50) */
-    int check_every;      /* host polls convergence every this many iterations (0: auto) */
+    int check_every;      /* host polls convergence every this many iterations (0: auto);
+                             near the iteration its polls predict to meet the
+                             tolerance it polls after each one.  No result
+                             depends on it except, in the last bits, a mixed
+                             solve's replacement near the tolerance */
     int stencil_format;   /* 0 auto: coded when A came from les_build_stencil;
                              1 the 7 coefficient arrays; 2 coded (16 bits per cell) */
     int inner_product;    /* LES_IP_EUCLID: r.z and p.q as the reference forms them;
