@@ -585,17 +585,21 @@ int mg_vcycle(les_ctx* c, const Stencil<T>& A0, const T* r, T* z, T* t0, int epi
     const bool tail = LES_MG_TAIL && g.dims[lt].N <= kMgTailCells && L - lt <= kMgTailLevels;
     if (!tail) lt = L - 1;
     T* cur[kMgMaxLevels] = {};
+    // levels whose two post-smoothing sweeps run as one pass: their
+    // pre-smoothing lands in buf(l, 0), so that pass writes buf(l, 1)
+    bool post[kMgMaxLevels] = {};
+    for (int l = 0; l < lt; ++l) post[l] = nu == 2 && mg_post_fits<T>(g.dims[l]);
     for (int l = 0; l < lt; ++l) {
         int k = 1;
         if (nu >= 2) {
-            // the first two sweeps in one pass (buf(l, 0) holds the first
-            // where the level runs the scalar kernels); the level-0 one is
-            // the timed class
-            T* out = buf(l, 1);
+            // the first two sweeps in one pass (the other buffer holds the
+            // first where the level runs the scalar kernels); the level-0
+            // one is the timed class
+            T* out = buf(l, post[l] ? 0 : 1);
             KtScope kt(c, l == 0 ? LES_KT_MG_SWEEP : -1, (int)sizeof(T));
-            HIPCHK(c, launch_mg_smooth<T>(s, g.dims[l], op(l), buf(l, 0), bvec(l), nullptr,
-                                          g.dims[l], out, MG_ZS, -1, kMgOmega, kMgScale, c->red,
-                                          sc));
+            HIPCHK(c, launch_mg_smooth<T>(s, g.dims[l], op(l), buf(l, post[l] ? 1 : 0), bvec(l),
+                                          nullptr, g.dims[l], out, MG_ZS, -1, kMgOmega, kMgScale,
+                                          c->red, sc));
             cur[l] = out;
             k = 3;
         }
@@ -620,6 +624,13 @@ int mg_vcycle(les_ctx* c, const Stencil<T>& A0, const T* r, T* z, T* t0, int epi
                                         c->mg_coarse_sweeps, kMgOmega, sc));
     }
     for (int l = lt - 1; l >= 0; --l) {
+        if (post[l]) {
+            HIPCHK(c, launch_mg_post<T>(s, g.dims[l], op(l), cur[l], bvec(l), cur[l + 1],
+                                        g.dims[l + 1], buf(l, 1), l == 0 ? epi : -1, kMgOmega,
+                                        kMgScale, c->red, sc, l == 0 ? W : nullptr));
+            cur[l] = buf(l, 1);
+            continue;
+        }
         for (int k = nu + 1; k <= 2 * nu; ++k) {
             T* out = buf(l, (k + 1) & 1);
             const bool last = l == 0 && k == 2 * nu;

@@ -226,6 +226,16 @@ hipError_t launch_mg_smooth(hipStream_t s, const Dims& d, Stencil<T> A, const T*
                             const T* e, const Dims& cd, T* x_out, int mode, int epi,
                             double omega, double scale, RedScratch red, SolveScalars* sc,
                             const double* wgt = nullptr);
+// The two post-smoothing sweeps (MG_PROLONG, then a plain sweep) in one
+// pass where mg_post_fits(d) (FP32 levels whose rows take vector loads);
+// x_out must differ from x.  epi as launch_mg_smooth's.
+template <typename T>
+bool mg_post_fits(const Dims& d);
+template <typename T>
+hipError_t launch_mg_post(hipStream_t s, const Dims& d, Stencil<T> A, const T* x, const T* b,
+                          const T* e, const Dims& cd, T* x_out, int epi, double omega,
+                          double scale, RedScratch red, SolveScalars* sc,
+                          const double* wgt = nullptr);
 // bc = P^T (b - A x), or P^T W (b - A x)
 template <typename T>
 hipError_t launch_mg_restrict(hipStream_t s, const Dims& f, const Dims& cd, Stencil<T> A,

@@ -1055,6 +1055,12 @@ struct PairGeom {
 #ifndef LES_MG_ZS_GRID
 #define LES_MG_ZS_GRID 256
 #endif
+// The two post-smoothing sweeps as one pass with two LDS rings
+// (k_mg_post_ring) on FP32 levels where its geometry fits.
+#ifndef LES_MG_POST_RING
+#define LES_MG_POST_RING 1
+#endif
+constexpr int kMgPostR = 4;
 #ifndef LES_PAIR_STAGED
 #define LES_PAIR_STAGED 1
 #endif
@@ -2440,6 +2446,209 @@ __global__ __launch_bounds__(kBlock) void k_mg_zs_ring(Dims d, PairGeom g, Stenc
     }
 }
 
+// The two post-smoothing sweeps of a level in one pass (MG_PROLONG, then a
+// plain sweep), marching along i with two LDS rings: for plane t+1 the
+// corrected iterate x + scale * e of its block goes into ring 0 (the
+// workgroup's J rows and two halo rows on each side, once per cell, where
+// the separate prolongation sweep forms it again at every neighbour); the
+// first sweep runs at plane t from ring 0 into ring 1 (J rows and one halo
+// row on each side); the second at plane t-1 from ring 1 for the J rows,
+// with b.x_out reduced when RZ.  b and the diagonal's source of the rows
+// are loaded with ring 0's plane and carried.  Every cell's arithmetic is
+// k_mg_smooth_v's, so x_out is bit-identical to the two sweeps apart; the
+// r.z partials are summed in another order.
+template <typename T, int CODED, int V, int P, int R, bool RZ>
+__global__ __launch_bounds__(kBlock) void k_mg_post_ring(Dims d, PairGeom g, Stencil<T> A,
+                                                         const T* __restrict__ x,
+                                                         const T* __restrict__ b,
+                                                         const T* __restrict__ e, Dims cd,
+                                                         T* __restrict__ x_out, double omega,
+                                                         double scale, RedScratch red,
+                                                         const SolveScalars* sc,
+                                                         const double* __restrict__ wgt)
+{
+    static_assert(V == 2 || V == 4, "the block corrections of V cells come as one or two loads");
+    if (is_done(sc)) return;
+    extern __shared__ unsigned char les_smem[];
+    const int Q = P * g.rpp;  // rows per plane: J + 4
+    const int plane_sz = Q * d.nz;
+    T* r0 = reinterpret_cast<T*>(les_smem);  // x + scale e: [R][Q][nz]
+    T* r1 = r0 + (size_t)R * plane_sz;       // first sweep:  [R][Q][nz]
+    const CodeTab<T>* tab = nullptr;
+    if constexpr (CODED == 1 && LES_MV_TAB) {
+        __shared__ CodeTab<T> tab_s;
+        code_tab_fill(tab_s, A);
+        __syncthreads();
+        tab = &tab_s;
+    }
+    const int lb = logical_block(g.grid);
+    const int jb = lb % g.njb, ic = lb / g.njb;
+    const int j0 = jb * g.J;
+    const int ia = ic * g.irange, ib = min(ia + g.irange, d.nx);
+    const int sub = (int)threadIdx.x / g.tpr;
+    const int kk = ((int)threadIdx.x - sub * g.tpr) * V;
+    const T om = (T)omega, sf = (T)scale;
+    using DS = std::conditional_t<CODED == 1, uint16_t, T>;
+    // b and the diagonal's source of this thread's rows at plane t+1 (just
+    // loaded), t (the first sweep) and t-1 (the second)
+    T bn[P][V], bA[P][V], bB[P][V];
+    DS dn[P][V], dA[P][V], dB[P][V];
+    double acc[1] = {0.0};
+    T pad[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) pad[v] = outside<T>();
+#pragma unroll
+    for (int pp = 0; pp < P; ++pp)
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            bn[pp][v] = bA[pp][v] = bB[pp][v] = T(0);
+            dn[pp][v] = dA[pp][v] = dB[pp][v] = DS(0);
+        }
+    auto dgv = [&](const DS (&ds)[V], T (&dg)[V]) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            if constexpr (CODED == 1) dg[v] = (ds[v] & CODE_IDENTITY) ? T(1) : A.cfluid;
+            else dg[v] = ds[v];
+        }
+    };
+    // ---- plane u of x + scale * e into ring 0; b, diagonal of plane u ----
+    auto stage0 = [&](int u) {
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp) {
+            const int a = pp * g.rpp + sub;
+            const int j = j0 - 2 + a;
+            T* slot = r0 + (u + R) % R * plane_sz + a * d.nz + kk;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                bn[pp][v] = T(0);
+                dn[pp][v] = DS(0);
+            }
+            if (u < 0 || u >= d.nx || j < 0 || j >= d.ny) {
+                if (CODED == 1) vstore<T, V>(slot, pad);
+                continue;
+            }
+            const int id = (u * d.ny + j) * d.nz + kk;
+            T xv[V];
+            vload<T, V>(x + id, xv);
+            const T* er = e + ((u >> 1) * cd.ny + (j >> 1)) * cd.nz + (kk >> 1);
+            if constexpr (V == 4) {
+                T ev[2];
+                vload<T, 2>(er, ev);
+#pragma unroll
+                for (int v = 0; v < V; ++v) xv[v] = xv[v] + sf * ev[v >> 1];
+            } else {
+                const T ev = er[0];
+#pragma unroll
+                for (int v = 0; v < V; ++v) xv[v] = xv[v] + sf * ev;
+            }
+            vstore<T, V>(slot, xv);
+            if (a >= 1 && a <= Q - 2) {
+                vload_s<T, V>(b + id, bn[pp]);
+                if constexpr (CODED == 1) vload<uint16_t, V>(A.code + id, dn[pp]);
+                else vload<T, V>(A.a[SC] + id, dn[pp]);
+            }
+        }
+    };
+    auto shift = [&]() {
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp)
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                bB[pp][v] = bA[pp][v];
+                dB[pp][v] = dA[pp][v];
+                bA[pp][v] = bn[pp][v];
+                dA[pp][v] = dn[pp][v];
+            }
+    };
+    // one sweep at plane i for row a from ring rg (planes i-1, i, i+1)
+    auto sweep = [&](const T* rg, int i, int a, int j, const T (&bv)[V], const DS (&ds)[V],
+                     T (&out)[V]) {
+        const T* pc_pl = rg + (i + R) % R * plane_sz;
+        const T* pm_pl = rg + (i + R - 1) % R * plane_sz;
+        const T* pp_pl = rg + (i + 1) % R * plane_sz;
+        const int k = kk, id = (i * d.ny + j) * d.nz + kk;
+        Nbr<T, V> nb;
+        vload<T, V>(pc_pl + a * d.nz + kk, nb.c);
+        if (CODED == 1) {
+            vload<T, V>(pm_pl + a * d.nz + kk, nb.w);
+            vload<T, V>(pp_pl + a * d.nz + kk, nb.e);
+            vload<T, V>(pc_pl + (a - 1) * d.nz + kk, nb.s);
+            vload<T, V>(pc_pl + (a + 1) * d.nz + kk, nb.n);
+        } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v) nb.w[v] = nb.e[v] = nb.s[v] = nb.n[v] = T(0);
+            if (i > 0) vload<T, V>(pm_pl + a * d.nz + kk, nb.w);
+            if (i < d.nx - 1) vload<T, V>(pp_pl + a * d.nz + kk, nb.e);
+            if (j > 0) vload<T, V>(pc_pl + (a - 1) * d.nz + kk, nb.s);
+            if (j < d.ny - 1) vload<T, V>(pc_pl + (a + 1) * d.nz + kk, nb.n);
+        }
+        row_edges<T, V, false>(d, id, k, nb.c, nb.km, nb.kp,
+                               [&](int n) { return pc_pl[a * d.nz + kk + (n - id)]; });
+        T q[V], dg[V];
+        if constexpr (CODED == 1)
+            apply7v<1, T, V, LES_MV_TAB>(A, d, id, i, j, k, nb, q, ds, nullptr, tab);
+        else
+            apply7v<0, T, V>(A, d, id, i, j, k, nb, q);
+        dgv(ds, dg);
+#pragma unroll
+        for (int v = 0; v < V; ++v) out[v] = nb.c[v] + om * ((bv[v] - q[v]) / dg[v]);
+    };
+    if (ia < d.nx) {
+        stage0(ia - 2);
+        shift();
+        stage0(ia - 1);
+        shift();
+    }
+    for (int t = ia - 1; t <= ib && ia < d.nx; ++t) {
+        stage0(t + 1);
+        __syncthreads();
+        // ---- the first sweep at plane t, rows 1 .. Q-2, into ring 1 ----
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp) {
+            const int a = pp * g.rpp + sub;
+            const int j = j0 - 2 + a;
+            if (a < 1 || a > Q - 2) continue;
+            T* slot = r1 + (t + R) % R * plane_sz + a * d.nz + kk;
+            if (t < 0 || t >= d.nx || j < 0 || j >= d.ny) {
+                if (CODED == 1) vstore<T, V>(slot, pad);
+                continue;
+            }
+            T o[V];
+            sweep(r0, t, a, j, bA[pp], dA[pp], o);
+            vstore<T, V>(slot, o);
+        }
+        __syncthreads();
+        // ---- the second sweep at plane t-1, rows 2 .. Q-3 ----
+        const int iB = t - 1;
+        if (iB >= ia && iB < ib) {
+#pragma unroll
+            for (int pp = 0; pp < P; ++pp) {
+                const int a = pp * g.rpp + sub;
+                const int j = j0 - 2 + a;
+                if (a < 2 || a > Q - 3 || j >= d.ny) continue;
+                T o[V];
+                sweep(r1, iB, a, j, bB[pp], dB[pp], o);
+                const int id = (iB * d.ny + j) * d.nz + kk;
+                vstore_s<T, V>(x_out + id, o);
+                if (RZ) {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) {
+                        const double bz = (double)bB[pp][v] * (double)o[v];
+                        double wv = 1.0;
+                        if (wgt) {
+                            if constexpr (CODED == 1) wv = code_weight(dB[pp][v]);
+                            else wv = wgt[id + v];
+                        }
+                        acc[0] += wgt ? wv * bz : bz;
+                    }
+                }
+            }
+        }
+        shift();
+    }
+    if (RZ) block_partial<1>(acc, red.partials, lb);
+}
+
 // k_mg_restrict with two coarse cells of a row per thread: the four fine
 // rows of their blocks are read four cells at a time (k-neighbours by lane
 // shifts), the residual formed per cell as apply7 forms it, and each coarse
@@ -3508,6 +3717,81 @@ hipError_t launch_mg_smooth(hipStream_t s, const Dims& d, Stencil<T> A, const T*
     return hipSuccess;
 }
 
+// The post-smoothing ring: FP32, rows of V cells, J = rows per plane - 4
+// core rows (at least 8), two four-plane rings in LDS; at most kMaxGrid
+// workgroups (the r.z partials).
+template <typename T>
+bool post_ring_geom(const Dims& d, PairGeom& g)
+{
+    constexpr int V = vec_width<T>();
+    if (sizeof(T) != 4 || !LES_MG_POST_RING || !LES_MG_VEC || d.nz % V) return false;
+    g.tpr = d.nz / V;
+    if (g.tpr > kBlock || kBlock % g.tpr) return false;
+    g.rpp = kBlock / g.tpr;
+    g.J = g.rpp - 4;
+    if (g.J < 8) return false;
+    if (2 * (size_t)kMgPostR * g.rpp * d.nz * sizeof(T) + sizeof(CodeTab<T>) > kPairMaxLds)
+        return false;
+    g.njb = (d.ny + g.J - 1) / g.J;
+    if (g.njb > kMaxGrid) return false;
+    int nxc = std::max(1, std::min(d.nx, kMaxGrid / g.njb));
+    g.irange = (d.nx + nxc - 1) / nxc;
+    nxc = (d.nx + g.irange - 1) / g.irange;
+    g.grid = ((g.njb * nxc + kXcd - 1) / kXcd) * kXcd;
+    return true;
+}
+
+template <typename T>
+bool mg_post_fits(const Dims& d)
+{
+    PairGeom g;
+    return post_ring_geom<T>(d, g);
+}
+
+template <typename T>
+hipError_t launch_mg_post(hipStream_t s, const Dims& d, Stencil<T> A, const T* x, const T* b,
+                          const T* e, const Dims& cd, T* x_out, int epi, double omega,
+                          double scale, RedScratch red, SolveScalars* sc, const double* wgt)
+{
+    PairGeom g;
+    if (!post_ring_geom<T>(d, g)) return hipErrorNotSupported;
+    if (x == x_out) return hipErrorInvalidValue;
+    if constexpr (sizeof(T) != 4) {
+        return hipErrorNotSupported;
+    } else {
+        constexpr int V = vec_width<T>(), R = kMgPostR;
+        const size_t lds = 2 * (size_t)R * g.rpp * d.nz * sizeof(T);
+        const bool rz = epi >= 0;
+        const bool cdd = A.code != nullptr;
+        const void* fn =
+            cdd ? (rz ? reinterpret_cast<const void*>(&k_mg_post_ring<T, 1, V, 1, R, true>)
+                      : reinterpret_cast<const void*>(&k_mg_post_ring<T, 1, V, 1, R, false>))
+                : (rz ? reinterpret_cast<const void*>(&k_mg_post_ring<T, 0, V, 1, R, true>)
+                      : reinterpret_cast<const void*>(&k_mg_post_ring<T, 0, V, 1, R, false>));
+        if (lds > 64 * 1024) {
+            const hipError_t er = hipFuncSetAttribute(
+                fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (er != hipSuccess) return er;
+        }
+#define LES_MGP(CD, RZ)                                                                          \
+    hipLaunchKernelGGL((k_mg_post_ring<T, CD, V, 1, R, RZ>), dim3(g.grid), dim3(kBlock), lds, s, \
+                       d, g, A, x, b, e, cd, x_out, omega, scale, red, sc, wgt)
+        if (cdd) {
+            if (rz) LES_MGP(1, true);
+            else LES_MGP(1, false);
+        } else {
+            if (rz) LES_MGP(0, true);
+            else LES_MGP(0, false);
+        }
+#undef LES_MGP
+        if (hipGetLastError() != hipSuccess) return hipErrorLaunchFailure;
+        if (rz)
+            return launch_finalize(s, g.grid, 1, false, epi, sc ? GATE_DONE : GATE_NONE, red, sc,
+                                   nullptr);
+        return hipSuccess;
+    }
+}
+
 template <typename T>
 hipError_t launch_mg_restrict(hipStream_t s, const Dims& f, const Dims& cd, Stencil<T> A,
                               const T* x, const T* b, T* bc, SolveScalars* sc, const double* W)
@@ -3612,6 +3896,10 @@ hipError_t launch_mg_coarsest(hipStream_t s, const Dims& d, Stencil<T> A, const 
                                             const T*, const T*, const Dims&, T*, int, int,      \
                                             double, double, RedScratch, SolveScalars*,          \
                                             const double*);                                     \
+    template bool mg_post_fits<T>(const Dims&);                                                \
+    template hipError_t launch_mg_post<T>(hipStream_t, const Dims&, Stencil<T>, const T*,       \
+                                          const T*, const T*, const Dims&, T*, int, double,     \
+                                          double, RedScratch, SolveScalars*, const double*);    \
     template hipError_t launch_mg_restrict<T>(hipStream_t, const Dims&, const Dims&, Stencil<T>, \
                                               const T*, const T*, T*, SolveScalars*,            \
                                               const double*);                                   \
