@@ -117,10 +117,14 @@ int les_synchronize(les_ctx* ctx);
  * FP32 recursion's drift is proportional to the residual it started from;
  * with a preconditioner that gains a decade in a few iterations (LES_PC_MG)
  * the periodic replacement comes too late and the search direction
- * restarts. */
+ * restarts.
+ * LES_OPT_MG_RINGS: 1 (default) runs the multigrid's pre- and
+ * post-smoothing sweep pairs as one pass each with the intermediate sweep
+ * in LDS rings where the level's shape allows; 0 as separate sweeps.
+ * Bit-identical either way (r.z summed in another order). */
 enum { LES_OPT_CHEB_PAIRS = 1, LES_OPT_PAIR_GRID = 2, LES_OPT_WAVEFRONT = 3,
        LES_OPT_SPIN_LIMIT = 4, LES_OPT_MG_SWEEPS = 5, LES_OPT_MG_COARSE_SWEEPS = 6,
-       LES_OPT_REPLACE_DECADES = 7 };
+       LES_OPT_REPLACE_DECADES = 7, LES_OPT_MG_RINGS = 8 };
 int les_set_option(les_ctx* ctx, int option, int value);
 
 /* ---- fields (Field, src/field.h:9-41) ---- */
@@ -162,6 +166,13 @@ int les_ic0_factor(les_ctx* ctx);
  * apply_chebyshev_preconditioner (pressure.cpp:851). */
 int les_precond_apply(les_ctx* ctx, int precond, const double* r, double* z, int ptr_kind,
                       double lambda_min, double lambda_max, int degree);
+/* (no reference counterpart) z = M^-1 r in the arithmetic a mixed-precision
+ * PCG applies it: FP32 vectors and FP32 copy of A (stencil_format as in
+ * les_pcg_params); LES_PC_MG in inner_product's cycle (LES_IP_MIRROR: the
+ * mirror-weighted one).  Sizes: N floats each. */
+int les_precond_apply_fp32(les_ctx* ctx, int precond, const float* r, float* z, int ptr_kind,
+                           double lambda_min, double lambda_max, int degree, int stencil_format,
+                           int inner_product);
 /* estimate_lambda_max (pressure.cpp:917): power iteration from the
  * mt19937(1337) U(-1,1) start vector drawn in sequence. */
 int les_power_lambda_max(les_ctx* ctx, int n_iter, double* out);

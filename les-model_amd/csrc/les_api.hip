@@ -109,6 +109,7 @@ struct les_ctx {
     int mg_sweeps = 2;                // damped-Jacobi sweeps before and after the coarse correction
     int mg_coarse_sweeps = 32;        // sweeps on the coarsest level
     int replace_decades = 3;          // mixed: replacement after this many decades of drop (0: off)
+    bool mg_rings = true;             // multigrid sweep pairs as one pass (LDS rings) where they fit
     bool cheb_pairs = true;           // fuse Chebyshev steps in pairs where the grid allows
     int pair_grid = 0;                // workgroups of a pair launch (0: built-in default)
     // lexicographic sweeps (IC(0), SOR): one wavefront launch each, or one
@@ -588,7 +589,7 @@ int mg_vcycle(les_ctx* c, const Stencil<T>& A0, const T* r, T* z, T* t0, int epi
     // levels whose two post-smoothing sweeps run as one pass: their
     // pre-smoothing lands in buf(l, 0), so that pass writes buf(l, 1)
     bool post[kMgMaxLevels] = {};
-    for (int l = 0; l < lt; ++l) post[l] = nu == 2 && mg_post_fits<T>(g.dims[l]);
+    for (int l = 0; l < lt; ++l) post[l] = c->mg_rings && nu == 2 && mg_post_fits<T>(g.dims[l]);
     for (int l = 0; l < lt; ++l) {
         int k = 1;
         if (nu >= 2) {
@@ -599,7 +600,7 @@ int mg_vcycle(les_ctx* c, const Stencil<T>& A0, const T* r, T* z, T* t0, int epi
             KtScope kt(c, l == 0 ? LES_KT_MG_SWEEP : -1, (int)sizeof(T));
             HIPCHK(c, launch_mg_smooth<T>(s, g.dims[l], op(l), buf(l, post[l] ? 1 : 0), bvec(l),
                                           nullptr, g.dims[l], out, MG_ZS, -1, kMgOmega, kMgScale,
-                                          c->red, sc));
+                                          c->red, sc, nullptr, c->mg_rings));
             cur[l] = out;
             k = 3;
         }
@@ -841,6 +842,9 @@ int les_set_option(les_ctx* c, int option, int value)
         if (value < 1 || value > 4096)
             return fail(c, LES_ERR_ARG, "multigrid coarsest sweeps out of 1..4096");
         c->mg_coarse_sweeps = value;
+        return LES_OK;
+    case LES_OPT_MG_RINGS:
+        c->mg_rings = value != 0;
         return LES_OK;
     case LES_OPT_REPLACE_DECADES:
         if (value < 0 || value > 15) return fail(c, LES_ERR_ARG, "replacement decades out of 0..15");
@@ -1613,6 +1617,34 @@ double tridiag_eig(const std::vector<double>& a, const std::vector<double>& b, i
 }  // namespace
 
 extern "C" {
+
+int les_precond_apply_fp32(les_ctx* c, int pc, const float* r, float* z, int ptr_kind, double lmin,
+                           double lmax, int degree, int stencil_format, int inner_product)
+{
+    if (!c || !r || !z) return fail(c, LES_ERR_ARG, "NULL vector");
+    if (!c->have_A) return fail(c, LES_ERR_STATE, "stencil A not built");
+    if (pc < LES_PC_NONE || pc > LES_PC_MG) return fail(c, LES_ERR_ARG, "unknown preconditioner");
+    if (inner_product != LES_IP_EUCLID && inner_product != LES_IP_MIRROR)
+        return fail(c, LES_ERR_ARG, "unknown inner product");
+    int rc;
+    if ((rc = ensure_krylov(c)) != LES_OK) return rc;
+    bool coded = false;
+    if ((rc = pick_coded(c, stencil_format, coded)) != LES_OK) return rc;
+    if ((rc = ensure_fp32_ops(c, pc, coded)) != LES_OK) return rc;
+    const double* wgt = nullptr;
+    if (inner_product == LES_IP_MIRROR) {
+        if ((rc = ensure_mirror_weights(c)) != LES_OK) return rc;
+        wgt = c->W;
+    }
+    const Stencil<float> A32 = coded ? coded_stencil<float>(c) : stencil_of(c->A32);
+    MixedWork& w = c->mw;
+    HIPCHK(c, hipMemcpyAsync(w.r, r, nbytes(c, 4), kind_in(ptr_kind), c->stream));
+    if ((rc = precond_fp32(c, pc, w.r, w.z, lmin, lmax, degree, -1, nullptr, A32, wgt)) != LES_OK)
+        return rc;
+    HIPCHK(c, hipMemcpyAsync(z, w.z, nbytes(c, 4), kind_out(ptr_kind), c->stream));
+    if (!ptr_kind) HIPCHK(c, hipStreamSynchronize(c->stream));
+    return wf_collect(c);
+}
 
 int les_pcg(les_ctx* c, const les_pcg_params* prm, les_pcg_result* res)
 {

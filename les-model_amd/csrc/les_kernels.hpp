@@ -221,11 +221,12 @@ hipError_t launch_mg_coarsen(hipStream_t s, const Dims& f, const Dims& cd, Stenc
 // epi >= 0: b.x_out reduced with that epilogue (the PCG's r.z, in the wgt
 // inner product when wgt is given).
 enum { MG_ZERO = 0, MG_SWEEP = 1, MG_PROLONG = 2, MG_ZS = 3 };
+// ring: MG_ZS may run as k_mg_zs_ring where its geometry fits.
 template <typename T>
 hipError_t launch_mg_smooth(hipStream_t s, const Dims& d, Stencil<T> A, const T* x, const T* b,
                             const T* e, const Dims& cd, T* x_out, int mode, int epi,
                             double omega, double scale, RedScratch red, SolveScalars* sc,
-                            const double* wgt = nullptr);
+                            const double* wgt = nullptr, bool ring = true);
 // The two post-smoothing sweeps (MG_PROLONG, then a plain sweep) in one
 // pass where mg_post_fits(d) (FP32 levels whose rows take vector loads);
 // x_out must differ from x.  epi as launch_mg_smooth's.

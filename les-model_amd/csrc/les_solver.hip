@@ -3647,14 +3647,15 @@ template <typename T>
 hipError_t launch_mg_smooth(hipStream_t s, const Dims& d, Stencil<T> A, const T* x, const T* b,
                             const T* e, const Dims& cd, T* x_out, int mode, int epi,
                             double omega, double scale, RedScratch red, SolveScalars* sc,
-                            const double* wgt)
+                            const double* wgt, bool ring)
 {
     constexpr int VW = vec_width<T>();
     const bool vec = LES_MG_VEC && pick_width<T>(d) == VW;
     const Chunking ch = make_chunking(d, vec ? VW : 1);
     const bool rz = epi >= 0;
     PairGeom g;
-    if (LES_MG_ZS_RING && mode == MG_ZS && vec && !rz && mv_ring_geom<T>(d, g, LES_MG_ZS_GRID)) {
+    if (LES_MG_ZS_RING && ring && mode == MG_ZS && vec && !rz &&
+        mv_ring_geom<T>(d, g, LES_MG_ZS_GRID)) {
         constexpr int P = MvRingCfg<T>::P, R = MvRingCfg<T>::R;
         const size_t lds = mv_ring_lds<T>(d, g);
         const void* fn = A.code ? reinterpret_cast<const void*>(&k_mg_zs_ring<T, 1, VW, P, R>)
@@ -3895,7 +3896,7 @@ hipError_t launch_mg_coarsest(hipStream_t s, const Dims& d, Stencil<T> A, const 
     template hipError_t launch_mg_smooth<T>(hipStream_t, const Dims&, Stencil<T>, const T*,     \
                                             const T*, const T*, const Dims&, T*, int, int,      \
                                             double, double, RedScratch, SolveScalars*,          \
-                                            const double*);                                     \
+                                            const double*, bool);                               \
     template bool mg_post_fits<T>(const Dims&);                                                \
     template hipError_t launch_mg_post<T>(hipStream_t, const Dims&, Stencil<T>, const T*,       \
                                           const T*, const T*, const Dims&, T*, int, double,     \

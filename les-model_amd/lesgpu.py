@@ -31,7 +31,7 @@ KT_NONE, KT_CHEB_STEP, KT_PCG_MATVEC, KT_PCG_UPDATE, KT_MOMENTUM, KT_IC_SOLVE = 
 KT_CHEB_PAIR, KT_STATS, KT_IC_FACTOR, KT_SOR_SWEEP, KT_MG_SWEEP = 6, 7, 8, 9, 10
 IP_EUCLID, IP_MIRROR = 0, 1
 OPT_CHEB_PAIRS, OPT_PAIR_GRID, OPT_WAVEFRONT, OPT_SPIN_LIMIT = 1, 2, 3, 4
-OPT_MG_SWEEPS, OPT_MG_COARSE_SWEEPS, OPT_REPLACE_DECADES = 5, 6, 7
+OPT_MG_SWEEPS, OPT_MG_COARSE_SWEEPS, OPT_REPLACE_DECADES, OPT_MG_RINGS = 5, 6, 7, 8
 ERR_ARG, ERR_HIP, ERR_STATE, ERR_NOMEM, ERR_IO, ERR_STALL = -1, -2, -3, -4, -5, -6
 # statistics slots (statistics.h:15-31): means of u, v, w, p, then the sums
 # of the six products
@@ -129,6 +129,8 @@ def load_library(path: str = LIB_PATH):
     L.les_ic0_factor.argtypes = [vp]
     L.les_precond_apply.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_double, C.c_double,
                                     C.c_int]
+    L.les_precond_apply_fp32.argtypes = [vp, C.c_int, vp, vp, C.c_int, C.c_double, C.c_double,
+                                         C.c_int, C.c_int, C.c_int]
     L.les_power_lambda_max.argtypes = [vp, C.c_int, DP]
     L.les_lanczos_bounds.argtypes = [vp, C.c_int, C.c_uint64, DP, DP]
     L.les_pcg.argtypes = [vp, C.POINTER(PcgParams), C.POINTER(PcgResult)]
@@ -379,6 +381,17 @@ class LesGpu:
         self._chk(self.lib.les_precond_apply(self.h, pc, _host_ptr(r, np.float64),
                                              _host_ptr(z, np.float64), 0, lmin, lmax, degree),
                   "les_precond_apply")
+        return z
+
+    def precond_apply_fp32(self, pc, r, lmin=0.0, lmax=0.0, degree=5,
+                           stencil_format=STENCIL_AUTO, inner_product=IP_EUCLID):
+        """z = M^-1 r in FP32, as a mixed-precision PCG applies it."""
+        r = np.ascontiguousarray(r, dtype=np.float32)
+        z = np.empty(self.N, dtype=np.float32)
+        self._chk(self.lib.les_precond_apply_fp32(self.h, pc, _host_ptr(r, np.float32),
+                                                  _host_ptr(z, np.float32), 0, lmin, lmax, degree,
+                                                  stencil_format, inner_product),
+                  "les_precond_apply_fp32")
         return z
 
     def power_lambda_max(self, n_iter=20):

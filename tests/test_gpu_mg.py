@@ -285,3 +285,32 @@ def test_host_batches_change_nothing(orc, precision, precond):
                 (r0["iters"], r0["replacements"], r0["restarts"])
             assert r["true_res_norm"] < 1e-9 * r["b_norm"]
             assert rel_err(x, x0) < 1e-5
+
+
+@pytest.mark.parametrize("shape", [(20, 24, 64, 21), (9, 40, 128, 22), (12, 30, 256, 23),
+                                   (17, 13, 16, 24), (7, 9, 36, 25)])
+@pytest.mark.parametrize("ip", [lesgpu.IP_EUCLID, lesgpu.IP_MIRROR])
+def test_vcycle_fp32_rings_identical(orc, shape, ip):
+    """The FP32 V-cycle, as a mixed-precision PCG applies it: the one-pass
+    pre- and post-smoothing rings give bit for bit the separate sweeps'
+    result, with the code and with the arrays, in the Euclidean and the
+    mirror-weighted cycle; and it follows the FP64 V-cycle to FP32 rounding.
+    nz = 64 .. 256 take the rings at level 0 with a few rows per workgroup,
+    nz = 16 with hundreds, nz = 36 (not a multiple of 8) the scalar kernels
+    below level 0."""
+    m, solid, f = random_case(*shape, frac=0.05)
+    r = f[3].astype(np.float32)
+    print(f"[mg] FP32 V-cycle {shape} ip {ip}", flush=True)
+    with ctx_for(m, solid) as g:
+        g.build_stencil(ignore_mask=True)
+        outs = []
+        for rings in (1, 0):
+            g.set_option(lesgpu.OPT_MG_RINGS, rings)
+            for fmt in (lesgpu.STENCIL_CODED, lesgpu.STENCIL_ARRAYS):
+                outs.append(g.precond_apply_fp32(PC_MG, r, stencil_format=fmt, inner_product=ip))
+        for z in outs[1:]:
+            assert bits_equal(z, outs[0])
+        assert np.all(np.isfinite(outs[0]))
+        if ip == lesgpu.IP_EUCLID:
+            z64 = g.precond_apply(PC_MG, r.astype(np.float64))
+            assert rel_err(outs[0].astype(np.float64), z64) < 1e-4
