@@ -237,11 +237,12 @@ hipError_t launch_mg_post(hipStream_t s, const Dims& d, Stencil<T> A, const T* x
                           const T* e, const Dims& cd, T* x_out, int epi, double omega,
                           double scale, RedScratch red, SolveScalars* sc,
                           const double* wgt = nullptr);
-// bc = P^T (b - A x), or P^T W (b - A x)
+// bc = P^T (b - A x), or P^T W (b - A x); ring: as k_mg_restrict_ring where
+// its geometry fits
 template <typename T>
 hipError_t launch_mg_restrict(hipStream_t s, const Dims& f, const Dims& cd, Stencil<T> A,
                               const T* x, const T* b, T* bc, SolveScalars* sc,
-                              const double* W = nullptr);
+                              const double* W = nullptr, bool ring = true);
 // Levels dims[0..nlev-1] of the cycle's bottom (each at most kMgTailCells cells,
 // arrays format A[l][7]) in one workgroup: from b0 (level 0 of the tail's
 // right-hand side) to x0 (its result); the same arithmetic as the

@@ -1055,6 +1055,11 @@ struct PairGeom {
 #ifndef LES_MG_ZS_GRID
 #define LES_MG_ZS_GRID 256
 #endif
+// The restriction marching along i with x in an LDS ring
+// (k_mg_restrict_ring) where its geometry fits.
+#ifndef LES_MG_RESTRICT_RING
+#define LES_MG_RESTRICT_RING 1
+#endif
 // The two post-smoothing sweeps as one pass with two LDS rings
 // (k_mg_post_ring) on FP32 levels where its geometry fits.
 #ifndef LES_MG_POST_RING
@@ -2649,6 +2654,172 @@ __global__ __launch_bounds__(kBlock) void k_mg_post_ring(Dims d, PairGeom g, Ste
     if (RZ) block_partial<1>(acc, red.partials, lb);
 }
 
+// The restriction bc = P^T (b - A x) (or P^T W (b - A x)) marching along i
+// like the ring matvec: x of plane t goes into an LDS ring (J rows and one
+// halo row on each side), the residual of plane t-1 is formed for the J
+// rows from it, and each coarse cell sums its eight children in (di, dj, dk)
+// order: the thread of the even fine row holds the sum over the plane pair,
+// adds its own two children of a plane and then its partner row's, which
+// come through LDS.  J and the i-runs are even, so a block of 2 x 2 x 2
+// children lies in one workgroup.  Bit-identical to k_mg_restrict_v.
+template <typename T, int CODED, int V, int P, int R>
+__global__ __launch_bounds__(kBlock) void k_mg_restrict_ring(Dims f, Dims cd, PairGeom g,
+                                                             Stencil<T> A,
+                                                             const T* __restrict__ x,
+                                                             const T* __restrict__ b,
+                                                             T* __restrict__ bc,
+                                                             const SolveScalars* sc,
+                                                             const double* __restrict__ W)
+{
+    static_assert(V == 2 || V == 4, "a thread's cells are whole coarse k-cells");
+    constexpr int VC = V / 2;  // coarse cells of a thread's V fine cells
+    if (is_done(sc)) return;
+    extern __shared__ unsigned char les_smem[];
+    const int Q = P * g.rpp;  // J + 2 rows per plane
+    const int plane_sz = Q * f.nz;
+    T* xs = reinterpret_cast<T*>(les_smem);  // [R][Q][nz]
+    T* rs = xs + (size_t)R * plane_sz;       // the odd rows' residuals: [Q][nz]
+    const CodeTab<T>* tab = nullptr;
+    if constexpr (CODED == 1 && LES_MV_TAB) {
+        __shared__ CodeTab<T> tab_s;
+        code_tab_fill(tab_s, A);
+        __syncthreads();
+        tab = &tab_s;
+    }
+    const int lb = logical_block(g.grid);
+    const int jb = lb % g.njb, ic = lb / g.njb;
+    const int j0 = jb * g.J;
+    const int ia = ic * g.irange, ib = min(ia + g.irange, f.nx);
+    const int sub = (int)threadIdx.x / g.tpr;
+    const int kk = ((int)threadIdx.x - sub * g.tpr) * V;
+    // b and the code of the core rows: plane t (loaded with x) and t-1 (its
+    // residual)
+    T bcur[P][V], bkeep[P][V];
+    uint16_t ccur[P][V], ckeep[P][V];
+    T acc[P][VC];
+    T pad[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) pad[v] = outside<T>();
+#pragma unroll
+    for (int pp = 0; pp < P; ++pp) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            bcur[pp][v] = bkeep[pp][v] = T(0);
+            ccur[pp][v] = ckeep[pp][v] = 0;
+        }
+#pragma unroll
+        for (int u = 0; u < VC; ++u) acc[pp][u] = T(0);
+    }
+    for (int t = ia - 1; t <= ib && ia < f.nx; ++t) {
+        // ---- x of plane t into the ring; b and the code of its core rows ----
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp) {
+            const int qr = pp * g.rpp + sub;
+            const int a = qr < g.J ? qr + 1 : (qr == g.J ? 0 : g.J + 1);
+            const int j = j0 - 1 + a;
+            T* slot = xs + (t + R) % R * plane_sz + a * f.nz + kk;
+            if (t < 0 || t >= f.nx || j < 0 || j >= f.ny) {
+                if (CODED == 1) vstore<T, V>(slot, pad);
+                continue;
+            }
+            const int id = (t * f.ny + j) * f.nz + kk;
+            T xv[V];
+            vload<T, V>(x + id, xv);
+            vstore<T, V>(slot, xv);
+            if (qr < g.J && t >= ia && t < ib) {
+                vload_s<T, V>(b + id, bcur[pp]);
+                if (CODED == 1) vload<uint16_t, V>(A.code + id, ccur[pp]);
+            }
+        }
+        __syncthreads();
+        // ---- residuals of plane t-1; the odd rows' into LDS ----
+        const int iB = t - 1;
+        const bool act = iB >= ia && iB < ib;
+        T res[P][V];
+        if (act) {
+            const T* pc_pl = xs + (iB + R) % R * plane_sz;
+            const T* pm_pl = xs + (iB + R - 1) % R * plane_sz;
+            const T* pp_pl = xs + (iB + 1) % R * plane_sz;
+#pragma unroll
+            for (int pp = 0; pp < P; ++pp) {
+                const int qr = pp * g.rpp + sub;
+                const int a = qr + 1, j = j0 + qr;
+#pragma unroll
+                for (int v = 0; v < V; ++v) res[pp][v] = T(0);
+                if (qr >= g.J || j >= f.ny) continue;
+                const int i = iB, k = kk, id = (i * f.ny + j) * f.nz + kk;
+                Nbr<T, V> nb;
+                vload<T, V>(pc_pl + a * f.nz + kk, nb.c);
+                if (CODED == 1) {
+                    vload<T, V>(pm_pl + a * f.nz + kk, nb.w);
+                    vload<T, V>(pp_pl + a * f.nz + kk, nb.e);
+                    vload<T, V>(pc_pl + (a - 1) * f.nz + kk, nb.s);
+                    vload<T, V>(pc_pl + (a + 1) * f.nz + kk, nb.n);
+                } else {
+#pragma unroll
+                    for (int v = 0; v < V; ++v) nb.w[v] = nb.e[v] = nb.s[v] = nb.n[v] = T(0);
+                    if (i > 0) vload<T, V>(pm_pl + a * f.nz + kk, nb.w);
+                    if (i < f.nx - 1) vload<T, V>(pp_pl + a * f.nz + kk, nb.e);
+                    if (j > 0) vload<T, V>(pc_pl + (a - 1) * f.nz + kk, nb.s);
+                    if (j < f.ny - 1) vload<T, V>(pc_pl + (a + 1) * f.nz + kk, nb.n);
+                }
+                row_edges<T, V, false>(f, id, k, nb.c, nb.km, nb.kp,
+                                       [&](int n) { return pc_pl[a * f.nz + kk + (n - id)]; });
+                T q[V];
+                if constexpr (CODED == 1)
+                    apply7v<1, T, V, LES_MV_TAB>(A, f, id, i, j, k, nb, q, ckeep[pp], nullptr, tab);
+                else
+                    apply7v<0, T, V>(A, f, id, i, j, k, nb, q);
+#pragma unroll
+                for (int v = 0; v < V; ++v) {
+                    const T r = bkeep[pp][v] - q[v];
+                    double wv = 1.0;
+                    if (W) {
+                        if constexpr (CODED == 1) wv = code_weight(ckeep[pp][v]);
+                        else wv = W[id + v];
+                    }
+                    res[pp][v] = W ? (T)wv * r : r;
+                }
+                if (qr & 1) vstore<T, V>(rs + qr * f.nz + kk, res[pp]);
+            }
+        }
+        __syncthreads();
+        // ---- the even rows sum their children and their partner's ----
+        if (act) {
+            const bool last = (iB & 1) || iB + 1 >= f.nx;  // the block's last fine plane
+#pragma unroll
+            for (int pp = 0; pp < P; ++pp) {
+                const int qr = pp * g.rpp + sub;
+                const int j = j0 + qr;
+                if (qr >= g.J || (qr & 1) || j >= f.ny) continue;
+                if ((iB & 1) == 0) {
+#pragma unroll
+                    for (int u = 0; u < VC; ++u) acc[pp][u] = T(0);
+                }
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[pp][v >> 1] += res[pp][v];
+                if (j + 1 < f.ny) {
+                    T rp[V];
+                    vload<T, V>(rs + (qr + 1) * f.nz + kk, rp);
+#pragma unroll
+                    for (int v = 0; v < V; ++v) acc[pp][v >> 1] += rp[v];
+                }
+                if (last) {
+                    const int cid = ((iB >> 1) * cd.ny + (j >> 1)) * cd.nz + (kk >> 1);
+                    vstore<T, VC>(bc + cid, acc[pp]);
+                }
+            }
+        }
+#pragma unroll
+        for (int pp = 0; pp < P; ++pp)
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                bkeep[pp][v] = bcur[pp][v];
+                ckeep[pp][v] = ccur[pp][v];
+            }
+    }
+}
+
 // k_mg_restrict with two coarse cells of a row per thread: the four fine
 // rows of their blocks are read four cells at a time (k-neighbours by lane
 // shifts), the residual formed per cell as apply7 forms it, and each coarse
@@ -3793,10 +3964,55 @@ hipError_t launch_mg_post(hipStream_t s, const Dims& d, Stencil<T> A, const T* x
     }
 }
 
+// The restriction ring: the ring matvec's rows (J = rows per plane - 2 core
+// rows, even), even i-runs, the ring and one plane of residuals in LDS.
+template <typename T>
+bool restrict_ring_geom(const Dims& f, PairGeom& g)
+{
+    constexpr int V = vec_width<T>(), P = MvRingCfg<T>::P, R = MvRingCfg<T>::R;
+    if (!LES_MG_RESTRICT_RING || !LES_MG_VEC || f.nz % V) return false;
+    g.tpr = f.nz / V;
+    if (g.tpr > kBlock || kBlock % g.tpr) return false;
+    g.rpp = kBlock / g.tpr;
+    g.J = P * g.rpp - 2;
+    if (g.J < 2 || (g.J & 1)) return false;
+    if ((size_t)(R + 1) * P * g.rpp * f.nz * sizeof(T) + sizeof(CodeTab<T>) > kPairMaxLds)
+        return false;
+    g.njb = (f.ny + g.J - 1) / g.J;
+    if (g.njb > kMaxGrid) return false;
+    int nxc = std::max(1, std::min(f.nx, kMaxGrid / g.njb));
+    g.irange = (f.nx + nxc - 1) / nxc;
+    g.irange += g.irange & 1;
+    nxc = (f.nx + g.irange - 1) / g.irange;
+    g.grid = ((g.njb * nxc + kXcd - 1) / kXcd) * kXcd;
+    return true;
+}
+
 template <typename T>
 hipError_t launch_mg_restrict(hipStream_t s, const Dims& f, const Dims& cd, Stencil<T> A,
-                              const T* x, const T* b, T* bc, SolveScalars* sc, const double* W)
+                              const T* x, const T* b, T* bc, SolveScalars* sc, const double* W,
+                              bool ring)
 {
+    PairGeom g;
+    if (ring && restrict_ring_geom<T>(f, g)) {
+        constexpr int V = vec_width<T>(), P = MvRingCfg<T>::P, R = MvRingCfg<T>::R;
+        const size_t lds = (size_t)(R + 1) * P * g.rpp * f.nz * sizeof(T);
+        const void* fn = A.code
+                             ? reinterpret_cast<const void*>(&k_mg_restrict_ring<T, 1, V, P, R>)
+                             : reinterpret_cast<const void*>(&k_mg_restrict_ring<T, 0, V, P, R>);
+        if (lds > 64 * 1024) {
+            const hipError_t e = hipFuncSetAttribute(
+                fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        if (A.code)
+            hipLaunchKernelGGL((k_mg_restrict_ring<T, 1, V, P, R>), dim3(g.grid), dim3(kBlock),
+                               lds, s, f, cd, g, A, x, b, bc, sc, W);
+        else
+            hipLaunchKernelGGL((k_mg_restrict_ring<T, 0, V, P, R>), dim3(g.grid), dim3(kBlock),
+                               lds, s, f, cd, g, A, x, b, bc, sc, W);
+        return hipGetLastError();
+    }
     if (LES_MG_VEC && f.nz % 4 == 0) {
         const Chunking ch = make_chunking(cd, 2);
         if (A.code)
@@ -3903,7 +4119,7 @@ hipError_t launch_mg_coarsest(hipStream_t s, const Dims& d, Stencil<T> A, const 
                                           double, RedScratch, SolveScalars*, const double*);    \
     template hipError_t launch_mg_restrict<T>(hipStream_t, const Dims&, const Dims&, Stencil<T>, \
                                               const T*, const T*, T*, SolveScalars*,            \
-                                              const double*);                                   \
+                                              const double*, bool);                             \
     template hipError_t launch_mg_coarsest<T>(hipStream_t, const Dims&, Stencil<T>, const T*,   \
                                               T*, int, double, SolveScalars*);                  \
     template hipError_t launch_mg_tail<T>(hipStream_t, int, const Dims*, T* const* const*,      \
