@@ -92,9 +92,10 @@ def cheb_bytes_per_cell(degree, elem, coded=False, width=1):
     (all but a reducing last one).  The operator is 7 coefficient arrays,
     or one 2-byte code.  Width 0: the multigrid's level-0 pre-smoothing,
     its two damped-Jacobi sweeps from x = 0 in one pass (b, the operator ->
-    x)."""
+    x), in FP32 with the seven arrays with the restriction in the same pass
+    (and b of the next level, an eighth of the cells)."""
     if width == 0:
-        return 2 * elem + (2 if coded else 7 * elem)
+        return 2 * elem + (2 if coded else 7 * elem) + (elem / 8 if elem == 4 and not coded else 0)
     sched = cheb_schedule(degree, width)
     timed = [_launch_bytes(f, l, elem, coded) for w, f, l in sched if w == width and not l]
     return sum(timed) / len(timed)

@@ -391,7 +391,8 @@ enum { LES_KT_NONE = 0, LES_KT_CHEB_STEP = 1, LES_KT_PCG_MATVEC = 2, LES_KT_PCG_
        LES_KT_MOMENTUM = 4, LES_KT_IC_SOLVE = 5, LES_KT_CHEB_PAIR = 6, LES_KT_STATS = 7,
        LES_KT_IC_FACTOR = 8, LES_KT_SOR_SWEEP = 9, LES_KT_MG_SWEEP = 10 };
 /* LES_KT_MG_SWEEP: the multigrid's level-0 pre-smoothing launch (its first
- * two sweeps in one pass; none with LES_OPT_MG_SWEEPS = 1). */
+ * two sweeps in one pass, with the restriction in FP32 with the seven
+ * arrays; none with LES_OPT_MG_SWEEPS = 1). */
 int les_kernel_timing(les_ctx* ctx, int kernel_class);
 /* launches recorded, their summed duration in ms, and the bytes per cell the
  * class is charged (element size of the run) are returned; synchronises. */

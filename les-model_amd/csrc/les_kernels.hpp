@@ -237,6 +237,15 @@ hipError_t launch_mg_post(hipStream_t s, const Dims& d, Stencil<T> A, const T* x
                           const T* e, const Dims& cd, T* x_out, int epi, double omega,
                           double scale, RedScratch red, SolveScalars* sc,
                           const double* wgt = nullptr);
+// The two pre-smoothing sweeps from x = 0 (into x_out) and the restriction
+// bc = P^T (b - A x_out), or P^T W (b - A x_out), in one pass where
+// mg_pre_fits(f) (FP32 levels whose rows take vector loads).
+template <typename T>
+bool mg_pre_fits(const Dims& f);
+template <typename T>
+hipError_t launch_mg_pre(hipStream_t s, const Dims& f, const Dims& cd, Stencil<T> A, const T* b,
+                         T* x_out, T* bc, double omega, SolveScalars* sc,
+                         const double* W = nullptr);
 // bc = P^T (b - A x), or P^T W (b - A x); ring: as k_mg_restrict_ring where
 // its geometry fits
 template <typename T>

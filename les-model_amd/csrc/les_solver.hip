@@ -1055,6 +1055,11 @@ struct PairGeom {
 #ifndef LES_MG_ZS_GRID
 #define LES_MG_ZS_GRID 256
 #endif
+// The pre-smoothing sweeps and the restriction as one pass with LDS rings
+// (k_mg_pre_ring) on FP32 levels where its geometry fits.
+#ifndef LES_MG_PRE_RING
+#define LES_MG_PRE_RING 1
+#endif
 // The restriction marching along i with x in an LDS ring
 // (k_mg_restrict_ring) where its geometry fits.
 #ifndef LES_MG_RESTRICT_RING
@@ -2820,6 +2825,207 @@ __global__ __launch_bounds__(kBlock) void k_mg_restrict_ring(Dims f, Dims cd, Pa
     }
 }
 
+// The pre-smoothing and the restriction of an FP32 level in one pass,
+// marching along i with three LDS buffers: the first sweep's iterate
+// om * (b / A_c) of plane t into ring 1 (the J rows and two halo rows on
+// each side), the second sweep at plane t-1 from it into ring 2 (and x_out
+// for the J rows), the residual b - A x of plane t-2 from ring 2 for the J
+// rows, whose 2 x 2 x 2 blocks the even rows sum in (di, dj, dk) order with
+// their partner rows' residuals from the third buffer.  b and the code (or
+// A_c) of the rows are loaded a plane ahead and carried.  J and the i-runs
+// are even.  Bit-identical to k_mg_zs_ring followed by k_mg_restrict_v.
+template <typename T, int CODED, int V, int R>
+__global__ __launch_bounds__(kBlock) void k_mg_pre_ring(Dims f, Dims cd, PairGeom g,
+                                                        Stencil<T> A, const T* __restrict__ b,
+                                                        T* __restrict__ x_out,
+                                                        T* __restrict__ bc, double omega,
+                                                        const SolveScalars* sc,
+                                                        const double* __restrict__ W)
+{
+    static_assert(V == 4, "a thread's cells are two whole coarse k-cells");
+    constexpr int VC = V / 2;
+    if (is_done(sc)) return;
+    extern __shared__ unsigned char les_smem[];
+    const int Q = g.rpp;  // J + 4 rows per plane
+    const int plane_sz = Q * f.nz;
+    T* r1 = reinterpret_cast<T*>(les_smem);  // first sweep:  [R][Q][nz]
+    T* r2 = r1 + (size_t)R * plane_sz;       // second sweep: [R][Q][nz]
+    T* rs = r2 + (size_t)R * plane_sz;       // the odd rows' residuals: [Q][nz]
+    const CodeTab<T>* tab = nullptr;
+    if constexpr (CODED == 1 && LES_MV_TAB) {
+        __shared__ CodeTab<T> tab_s;
+        code_tab_fill(tab_s, A);
+        __syncthreads();
+        tab = &tab_s;
+    }
+    const int lb = logical_block(g.grid);
+    const int jb = lb % g.njb, ic = lb / g.njb;
+    const int j0 = jb * g.J;
+    const int ia = ic * g.irange, ib = min(ia + g.irange, f.nx);
+    const int a = (int)threadIdx.x / g.tpr;  // this thread's row: j0 - 2 + a
+    const int kk = ((int)threadIdx.x - a * g.tpr) * V;
+    const int j = j0 - 2 + a;
+    const bool jin = j >= 0 && j < f.ny;
+    const int c = a - 2;  // core row index (0 .. J-1 for the J rows)
+    const T om = (T)omega;
+    using DS = std::conditional_t<CODED == 1, uint16_t, T>;
+    // b and the diagonal's source of this row: plane t+1 (loaded), t (first
+    // sweep), t-1 (second), t-2 (residual)
+    T bn[V], bA[V], bB[V], bC[V];
+    DS dn[V], dA[V], dB[V], dC[V];
+    T acc[VC];
+    T pad[V];
+#pragma unroll
+    for (int v = 0; v < V; ++v) {
+        pad[v] = outside<T>();
+        bn[v] = bA[v] = bB[v] = bC[v] = T(0);
+        dn[v] = dA[v] = dB[v] = dC[v] = DS(0);
+    }
+#pragma unroll
+    for (int u = 0; u < VC; ++u) acc[u] = T(0);
+    auto in_plane = [&](int i) { return i >= 0 && i < f.nx; };
+    auto load = [&](int i) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            bn[v] = T(0);
+            dn[v] = DS(0);
+        }
+        if (!jin || !in_plane(i)) return;
+        const int id = (i * f.ny + j) * f.nz + kk;
+        vload_s<T, V>(b + id, bn);
+        if constexpr (CODED == 1) vload<uint16_t, V>(A.code + id, dn);
+        else vload<T, V>(A.a[SC] + id, dn);
+    };
+    auto shift = [&]() {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            bC[v] = bB[v];
+            dC[v] = dB[v];
+            bB[v] = bA[v];
+            dB[v] = dA[v];
+            bA[v] = bn[v];
+            dA[v] = dn[v];
+        }
+    };
+    auto dgv = [&](const DS (&ds)[V], T (&dg)[V]) {
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            if constexpr (CODED == 1) dg[v] = (ds[v] & CODE_IDENTITY) ? T(1) : A.cfluid;
+            else dg[v] = ds[v];
+        }
+    };
+    // A y at plane i of this row from ring rg (planes i-1, i, i+1)
+    auto apply_ring = [&](const T* rg, int i, const DS (&ds)[V], Nbr<T, V>& nb, T (&q)[V]) {
+        const T* pc_pl = rg + (i + R) % R * plane_sz;
+        const T* pm_pl = rg + (i + R - 1) % R * plane_sz;
+        const T* pp_pl = rg + (i + 1) % R * plane_sz;
+        const int id = (i * f.ny + j) * f.nz + kk;
+        vload<T, V>(pc_pl + a * f.nz + kk, nb.c);
+        if (CODED == 1) {
+            vload<T, V>(pm_pl + a * f.nz + kk, nb.w);
+            vload<T, V>(pp_pl + a * f.nz + kk, nb.e);
+            vload<T, V>(pc_pl + (a - 1) * f.nz + kk, nb.s);
+            vload<T, V>(pc_pl + (a + 1) * f.nz + kk, nb.n);
+        } else {
+#pragma unroll
+            for (int v = 0; v < V; ++v) nb.w[v] = nb.e[v] = nb.s[v] = nb.n[v] = T(0);
+            if (i > 0) vload<T, V>(pm_pl + a * f.nz + kk, nb.w);
+            if (i < f.nx - 1) vload<T, V>(pp_pl + a * f.nz + kk, nb.e);
+            if (j > 0) vload<T, V>(pc_pl + (a - 1) * f.nz + kk, nb.s);
+            if (j < f.ny - 1) vload<T, V>(pc_pl + (a + 1) * f.nz + kk, nb.n);
+        }
+        row_edges<T, V, false>(f, id, kk, nb.c, nb.km, nb.kp,
+                               [&](int n) { return pc_pl[a * f.nz + kk + (n - id)]; });
+        if constexpr (CODED == 1)
+            apply7v<1, T, V, LES_MV_TAB>(A, f, id, i, j, kk, nb, q, ds, nullptr, tab);
+        else
+            apply7v<0, T, V>(A, f, id, i, j, kk, nb, q);
+    };
+    if (ia < f.nx) {
+        load(ia - 2);
+        shift();
+    }
+    for (int t = ia - 2; t <= ib + 1 && ia < f.nx; ++t) {
+        load(t + 1);
+        // ---- the first sweep at plane t, all rows, into ring 1 ----
+        {
+            T* slot = r1 + (t + R) % R * plane_sz + a * f.nz + kk;
+            if (jin && in_plane(t)) {
+                T dg[V], xv[V];
+                dgv(dA, dg);
+#pragma unroll
+                for (int v = 0; v < V; ++v) xv[v] = om * (bA[v] / dg[v]);
+                vstore<T, V>(slot, xv);
+            } else if (CODED == 1) {
+                vstore<T, V>(slot, pad);
+            }
+        }
+        __syncthreads();
+        // ---- the second sweep at plane t-1, rows 1 .. Q-2, into ring 2 ----
+        const int iB = t - 1;
+        if (a >= 1 && a <= Q - 2 && iB >= ia - 1 && iB <= ib) {
+            T* slot = r2 + (iB + R) % R * plane_sz + a * f.nz + kk;
+            if (jin && in_plane(iB)) {
+                Nbr<T, V> nb;
+                T q[V], dg[V], o[V];
+                apply_ring(r1, iB, dB, nb, q);
+                dgv(dB, dg);
+#pragma unroll
+                for (int v = 0; v < V; ++v) o[v] = nb.c[v] + om * ((bB[v] - q[v]) / dg[v]);
+                vstore<T, V>(slot, o);
+                if (c >= 0 && c < g.J && iB >= ia && iB < ib)
+                    vstore_s<T, V>(x_out + (iB * f.ny + j) * f.nz + kk, o);
+            } else if (CODED == 1) {
+                vstore<T, V>(slot, pad);
+            }
+        }
+        __syncthreads();
+        // ---- the residual at plane t-2 for the J rows; the odd rows' into LDS ----
+        const int iC = t - 2;
+        const bool act = iC >= ia && iC < ib;
+        const bool live = act && c >= 0 && c < g.J && jin;
+        T res[V];
+#pragma unroll
+        for (int v = 0; v < V; ++v) res[v] = T(0);
+        if (live) {
+            Nbr<T, V> nb;
+            T q[V];
+            apply_ring(r2, iC, dC, nb, q);
+            const int id = (iC * f.ny + j) * f.nz + kk;
+#pragma unroll
+            for (int v = 0; v < V; ++v) {
+                const T r = bC[v] - q[v];
+                double wv = 1.0;
+                if (W) {
+                    if constexpr (CODED == 1) wv = code_weight(dC[v]);
+                    else wv = W[id + v];
+                }
+                res[v] = W ? (T)wv * r : r;
+            }
+            if (c & 1) vstore<T, V>(rs + a * f.nz + kk, res);
+        }
+        __syncthreads();
+        // ---- the even rows sum their children and their partner's ----
+        if (live && !(c & 1)) {
+            if ((iC & 1) == 0) {
+#pragma unroll
+                for (int u = 0; u < VC; ++u) acc[u] = T(0);
+            }
+#pragma unroll
+            for (int v = 0; v < V; ++v) acc[v >> 1] += res[v];
+            if (j + 1 < f.ny) {
+                T rp[V];
+                vload<T, V>(rs + (a + 1) * f.nz + kk, rp);
+#pragma unroll
+                for (int v = 0; v < V; ++v) acc[v >> 1] += rp[v];
+            }
+            if ((iC & 1) || iC + 1 >= f.nx)
+                vstore<T, VC>(bc + ((iC >> 1) * cd.ny + (j >> 1)) * cd.nz + (kk >> 1), acc);
+        }
+        shift();
+    }
+}
+
 // k_mg_restrict with two coarse cells of a row per thread: the four fine
 // rows of their blocks are read four cells at a time (k-neighbours by lane
 // shifts), the residual formed per cell as apply7 forms it, and each coarse
@@ -3964,6 +4170,65 @@ hipError_t launch_mg_post(hipStream_t s, const Dims& d, Stencil<T> A, const T* x
     }
 }
 
+// The pre-smoothing + restriction ring: FP32, four cells per thread, J =
+// rows per plane - 4 core rows (at least 8, even), even i-runs, two rings
+// and one plane of residuals in LDS.
+template <typename T>
+bool pre_ring_geom(const Dims& f, PairGeom& g)
+{
+    if (sizeof(T) != 4 || !LES_MG_PRE_RING || !LES_MG_VEC || f.nz % 4) return false;
+    g.tpr = f.nz / 4;
+    if (g.tpr > kBlock || kBlock % g.tpr) return false;
+    g.rpp = kBlock / g.tpr;
+    g.J = g.rpp - 4;
+    if (g.J < 8 || (g.J & 1)) return false;
+    if ((size_t)(2 * kMgPostR + 1) * g.rpp * f.nz * sizeof(T) + sizeof(CodeTab<T>) > kPairMaxLds)
+        return false;
+    g.njb = (f.ny + g.J - 1) / g.J;
+    if (g.njb > kMaxGrid) return false;
+    int nxc = std::max(1, std::min(f.nx, kMaxGrid / g.njb));
+    g.irange = (f.nx + nxc - 1) / nxc;
+    g.irange += g.irange & 1;
+    nxc = (f.nx + g.irange - 1) / g.irange;
+    g.grid = ((g.njb * nxc + kXcd - 1) / kXcd) * kXcd;
+    return true;
+}
+
+template <typename T>
+bool mg_pre_fits(const Dims& f)
+{
+    PairGeom g;
+    return pre_ring_geom<T>(f, g);
+}
+
+template <typename T>
+hipError_t launch_mg_pre(hipStream_t s, const Dims& f, const Dims& cd, Stencil<T> A, const T* b,
+                         T* x_out, T* bc, double omega, SolveScalars* sc, const double* W)
+{
+    PairGeom g;
+    if (!pre_ring_geom<T>(f, g)) return hipErrorNotSupported;
+    if constexpr (sizeof(T) != 4) {
+        return hipErrorNotSupported;
+    } else {
+        constexpr int R = kMgPostR;
+        const size_t lds = (size_t)(2 * R + 1) * g.rpp * f.nz * sizeof(T);
+        const void* fn = A.code ? reinterpret_cast<const void*>(&k_mg_pre_ring<T, 1, 4, R>)
+                                : reinterpret_cast<const void*>(&k_mg_pre_ring<T, 0, 4, R>);
+        if (lds > 64 * 1024) {
+            const hipError_t e = hipFuncSetAttribute(
+                fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+            if (e != hipSuccess) return e;
+        }
+        if (A.code)
+            hipLaunchKernelGGL((k_mg_pre_ring<T, 1, 4, R>), dim3(g.grid), dim3(kBlock), lds, s, f,
+                               cd, g, A, b, x_out, bc, omega, sc, W);
+        else
+            hipLaunchKernelGGL((k_mg_pre_ring<T, 0, 4, R>), dim3(g.grid), dim3(kBlock), lds, s, f,
+                               cd, g, A, b, x_out, bc, omega, sc, W);
+        return hipGetLastError();
+    }
+}
+
 // The restriction ring: the ring matvec's rows (J = rows per plane - 2 core
 // rows, even), even i-runs, the ring and one plane of residuals in LDS.
 template <typename T>
@@ -4114,6 +4379,10 @@ hipError_t launch_mg_coarsest(hipStream_t s, const Dims& d, Stencil<T> A, const 
                                             double, double, RedScratch, SolveScalars*,          \
                                             const double*, bool);                               \
     template bool mg_post_fits<T>(const Dims&);                                                \
+    template bool mg_pre_fits<T>(const Dims&);                                                 \
+    template hipError_t launch_mg_pre<T>(hipStream_t, const Dims&, const Dims&, Stencil<T>,     \
+                                         const T*, T*, T*, double, SolveScalars*,              \
+                                         const double*);                                       \
     template hipError_t launch_mg_post<T>(hipStream_t, const Dims&, Stencil<T>, const T*,       \
                                           const T*, const T*, const Dims&, T*, int, double,     \
                                           double, RedScratch, SolveScalars*, const double*);    \

@@ -288,7 +288,7 @@ def test_host_batches_change_nothing(orc, precision, precond):
 
 
 @pytest.mark.parametrize("shape", [(20, 24, 64, 21), (9, 40, 128, 22), (12, 30, 256, 23),
-                                   (17, 13, 16, 24), (7, 9, 36, 25)])
+                                   (17, 13, 16, 24), (7, 9, 36, 25), (4, 6, 2048, 26)])
 @pytest.mark.parametrize("ip", [lesgpu.IP_EUCLID, lesgpu.IP_MIRROR])
 def test_vcycle_fp32_rings_identical(orc, shape, ip):
     """The FP32 V-cycle, as a mixed-precision PCG applies it: the one-pass
@@ -297,7 +297,8 @@ def test_vcycle_fp32_rings_identical(orc, shape, ip):
     mirror-weighted cycle; and it follows the FP64 V-cycle to FP32 rounding.
     nz = 64 .. 256 take the rings at level 0 with a few rows per workgroup,
     nz = 16 with hundreds, nz = 36 (not a multiple of 8) the scalar kernels
-    below level 0."""
+    below level 0; nz = 2048 has rows too long for the two-ring passes, and
+    its level 1 (nz = 1024) takes the restriction ring with two core rows."""
     m, solid, f = random_case(*shape, frac=0.05)
     r = f[3].astype(np.float32)
     print(f"[mg] FP32 V-cycle {shape} ip {ip}", flush=True)
