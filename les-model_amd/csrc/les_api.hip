@@ -39,7 +39,7 @@ constexpr int kMgCoarseMax = 4;
 constexpr double kMgOmega = 0.85;
 constexpr double kMgScale = 1.8;
 // 1: the levels of at most kMgTailCells cells below level 0 run as one
-// launch (k_mg_tail); 0: five launches per level
+// launch (k_mg_tail); 0: the per-level launches
 #ifndef LES_MG_TAIL
 #define LES_MG_TAIL 1
 #endif
