@@ -2507,12 +2507,18 @@ __global__ __launch_bounds__(kBlock) void k_mg_post_ring(Dims d, PairGeom g, Ste
     T pad[V];
 #pragma unroll
     for (int v = 0; v < V; ++v) pad[v] = outside<T>();
+    // this thread's own row of x + scale e at planes t-1, t, t+1 and of the
+    // first sweep at t-2, t-1, t: the i-neighbours come from registers, only
+    // the j-neighbours (and k-edges across waves) from the rings
+    T xo[P][3][V], yo[P][3][V];
 #pragma unroll
     for (int pp = 0; pp < P; ++pp)
 #pragma unroll
         for (int v = 0; v < V; ++v) {
             bn[pp][v] = bA[pp][v] = bB[pp][v] = T(0);
             dn[pp][v] = dA[pp][v] = dB[pp][v] = DS(0);
+#pragma unroll
+            for (int w = 0; w < 3; ++w) xo[pp][w][v] = yo[pp][w][v] = pad[v];
         }
     auto dgv = [&](const DS (&ds)[V], T (&dg)[V]) {
 #pragma unroll
@@ -2533,6 +2539,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_post_ring(Dims d, PairGeom g, Ste
                 bn[pp][v] = T(0);
                 dn[pp][v] = DS(0);
             }
+#pragma unroll
+            for (int v = 0; v < V; ++v) xo[pp][2][v] = pad[v];
             if (u < 0 || u >= d.nx || j < 0 || j >= d.ny) {
                 if (CODED == 1) vstore<T, V>(slot, pad);
                 continue;
@@ -2552,6 +2560,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_post_ring(Dims d, PairGeom g, Ste
                 for (int v = 0; v < V; ++v) xv[v] = xv[v] + sf * ev;
             }
             vstore<T, V>(slot, xv);
+#pragma unroll
+            for (int v = 0; v < V; ++v) xo[pp][2][v] = xv[v];
             if (a >= 1 && a <= Q - 2) {
                 vload_s<T, V>(b + id, bn[pp]);
                 if constexpr (CODED == 1) vload<uint16_t, V>(A.code + id, dn[pp]);
@@ -2568,27 +2578,30 @@ __global__ __launch_bounds__(kBlock) void k_mg_post_ring(Dims d, PairGeom g, Ste
                 dB[pp][v] = dA[pp][v];
                 bA[pp][v] = bn[pp][v];
                 dA[pp][v] = dn[pp][v];
+                xo[pp][0][v] = xo[pp][1][v];
+                xo[pp][1][v] = xo[pp][2][v];
+                yo[pp][0][v] = yo[pp][1][v];
+                yo[pp][1][v] = yo[pp][2][v];
             }
     };
     // one sweep at plane i for row a from ring rg (planes i-1, i, i+1)
-    auto sweep = [&](const T* rg, int i, int a, int j, const T (&bv)[V], const DS (&ds)[V],
-                     T (&out)[V]) {
+    auto sweep = [&](const T* rg, int i, int a, int j, const T (&own)[3][V], const T (&bv)[V],
+                     const DS (&ds)[V], T (&out)[V]) {
         const T* pc_pl = rg + (i + R) % R * plane_sz;
-        const T* pm_pl = rg + (i + R - 1) % R * plane_sz;
-        const T* pp_pl = rg + (i + 1) % R * plane_sz;
         const int k = kk, id = (i * d.ny + j) * d.nz + kk;
         Nbr<T, V> nb;
-        vload<T, V>(pc_pl + a * d.nz + kk, nb.c);
+#pragma unroll
+        for (int v = 0; v < V; ++v) {
+            nb.w[v] = own[0][v];
+            nb.c[v] = own[1][v];
+            nb.e[v] = own[2][v];
+        }
         if (CODED == 1) {
-            vload<T, V>(pm_pl + a * d.nz + kk, nb.w);
-            vload<T, V>(pp_pl + a * d.nz + kk, nb.e);
             vload<T, V>(pc_pl + (a - 1) * d.nz + kk, nb.s);
             vload<T, V>(pc_pl + (a + 1) * d.nz + kk, nb.n);
         } else {
 #pragma unroll
-            for (int v = 0; v < V; ++v) nb.w[v] = nb.e[v] = nb.s[v] = nb.n[v] = T(0);
-            if (i > 0) vload<T, V>(pm_pl + a * d.nz + kk, nb.w);
-            if (i < d.nx - 1) vload<T, V>(pp_pl + a * d.nz + kk, nb.e);
+            for (int v = 0; v < V; ++v) nb.s[v] = nb.n[v] = T(0);
             if (j > 0) vload<T, V>(pc_pl + (a - 1) * d.nz + kk, nb.s);
             if (j < d.ny - 1) vload<T, V>(pc_pl + (a + 1) * d.nz + kk, nb.n);
         }
@@ -2617,6 +2630,8 @@ __global__ __launch_bounds__(kBlock) void k_mg_post_ring(Dims d, PairGeom g, Ste
         for (int pp = 0; pp < P; ++pp) {
             const int a = pp * g.rpp + sub;
             const int j = j0 - 2 + a;
+#pragma unroll
+            for (int v = 0; v < V; ++v) yo[pp][2][v] = pad[v];
             if (a < 1 || a > Q - 2) continue;
             T* slot = r1 + (t + R) % R * plane_sz + a * d.nz + kk;
             if (t < 0 || t >= d.nx || j < 0 || j >= d.ny) {
@@ -2624,8 +2639,10 @@ __global__ __launch_bounds__(kBlock) void k_mg_post_ring(Dims d, PairGeom g, Ste
                 continue;
             }
             T o[V];
-            sweep(r0, t, a, j, bA[pp], dA[pp], o);
+            sweep(r0, t, a, j, xo[pp], bA[pp], dA[pp], o);
             vstore<T, V>(slot, o);
+#pragma unroll
+            for (int v = 0; v < V; ++v) yo[pp][2][v] = o[v];
         }
         __syncthreads();
         // ---- the second sweep at plane t-1, rows 2 .. Q-3 ----
@@ -2637,7 +2654,7 @@ __global__ __launch_bounds__(kBlock) void k_mg_post_ring(Dims d, PairGeom g, Ste
                 const int j = j0 - 2 + a;
                 if (a < 2 || a > Q - 3 || j >= d.ny) continue;
                 T o[V];
-                sweep(r1, iB, a, j, bB[pp], dB[pp], o);
+                sweep(r1, iB, a, j, yo[pp], bB[pp], dB[pp], o);
                 const int id = (iB * d.ny + j) * d.nz + kk;
                 vstore_s<T, V>(x_out + id, o);
                 if (RZ) {
